@@ -1,0 +1,970 @@
+// efl_api.hip - host scheduler and C ABI of libefl (include/efl.h).
+//
+// The scheduler replays EulerFluidNode::run (src/euler_fluid/render_node.rs
+// :193-476) for a batch of instances: every stage is one (or a few) batched
+// kernel launches covering all instances that take part in it, in the
+// reference's order, on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/efl.h"
+#include "efl_internal.h"
+#include "efl_launch.h"
+
+using efl::Batch;
+using efl::DevInst;
+
+static_assert(sizeof(efl_projection) == 32, "efl_projection layout");
+static_assert(sizeof(efl_reinit) == 8, "efl_reinit layout");
+static_assert(sizeof(efl_settings) == 128, "efl_settings layout");
+static_assert(sizeof(efl_source) == 32, "efl_source layout");
+static_assert(sizeof(efl_force) == 16, "efl_force layout");
+static_assert(sizeof(efl_obstacle) == 176, "efl_obstacle layout (std430 SolidObstacle)");
+static_assert(sizeof(efl_solid_force) == 16, "efl_solid_force layout");
+static_assert(sizeof(efl_stats) == 32, "efl_stats layout");
+
+struct efl_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  efl_obstacle* d_obs = nullptr;
+  uint32_t nobs = 0;
+  uint64_t obs_version = 0;
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* h_scal = nullptr;  // pinned, 16 doubles per batch slot
+  std::vector<efl_instance*> insts;
+};
+
+struct efl_instance {
+  efl_ctx* ctx = nullptr;
+  efl_settings s{};
+  DevInst h{};
+  DevInst* d = nullptr;
+  void* mem = nullptr;
+  size_t bytes = 0;
+  efl_force* d_forces = nullptr;
+  uint32_t forces_cap = 0;
+  bool dirty = true;
+  uint64_t obs_version = ~0ull;
+  bool initialized = false;
+  bool diag = false;
+  std::vector<float> custom_ls;
+  float* d_custom_ls = nullptr;
+  efl_stats stats{};
+  int jfa_max_power = 0;
+};
+
+namespace {
+
+int fail(efl_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) return fail(ctx, EFL_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- batches ---------------------------------------------------------------
+enum Extent { EX_CENTER, EX_XMAJOR, EX_YMAJOR, EX_FULL, EX_LEVEL, EX_COARSE };
+
+void extent_of(const efl_instance* in, Extent k, int lvl, int& ex, int& ey) {
+  int W = (int)in->s.width, H = (int)in->s.height;
+  switch (k) {
+    case EX_CENTER: ex = (W / 8) * 8; ey = (H / 8) * 8; break;      // pipeline.rs:176-178
+    case EX_XMAJOR: ex = W + 1; ey = (H / 64) * 64; break;          // pipeline.rs:180-182
+    case EX_YMAJOR: ex = (W / 64) * 64; ey = H + 1; break;          // pipeline.rs:184-186
+    case EX_FULL: ex = W; ey = H; break;
+    case EX_LEVEL: ex = in->h.lev[lvl].w; ey = in->h.lev[lvl].h; break;      // multi_grid.rs:405
+    case EX_COARSE: ex = in->h.lev[lvl + 1].w; ey = in->h.lev[lvl + 1].h; break;
+  }
+}
+
+// levels: per-instance Batch.level values (or -1 = use `lvl`)
+Batch make_batch(const std::vector<efl_instance*>& v, Extent k, int lvl = 0, const std::vector<int>* levels = nullptr) {
+  Batch b;
+  std::memset(&b, 0, sizeof(b));
+  int tw = efl::tile_w(), th = efl::tile_h();
+  int n = 0, start = 0;
+  for (size_t i = 0; i < v.size(); ++i) {
+    int l = levels ? (*levels)[i] : lvl;
+    int ex, ey;
+    extent_of(v[i], k, l, ex, ey);
+    int tx = (ex + tw - 1) / tw, ty = (ey + th - 1) / th;
+    if (ex <= 0 || ey <= 0) tx = ty = 0;
+    b.start[n] = start;
+    b.tiles_x[n] = tx > 0 ? tx : 1;
+    b.ex[n] = ex;
+    b.ey[n] = ey;
+    b.level[n] = l;
+    b.inst[n] = v[i]->d;
+    start += tx * ty;
+    ++n;
+  }
+  b.n = n;
+  b.start[n] = start;
+  return b;
+}
+
+// ---- uploads ----------------------------------------------------------------
+int upload(efl_instance* in) {
+  efl_ctx* c = in->ctx;
+  if (in->obs_version != c->obs_version) {
+    in->h.obs = c->d_obs;
+    in->h.nobs = c->nobs;
+    in->obs_version = c->obs_version;
+    in->dirty = true;
+  }
+  if (!in->dirty) return EFL_OK;
+  HIPCHK(c, hipMemcpyAsync(in->d, &in->h, sizeof(DevInst), hipMemcpyHostToDevice, c->stream));
+  in->dirty = false;
+  return EFL_OK;
+}
+
+// ---- stages ------------------------------------------------------------------
+using IV = std::vector<efl_instance*>;
+
+template <class F>
+IV filter(const IV& v, F f) {
+  IV o;
+  for (auto* i : v)
+    if (f(i)) o.push_back(i);
+  return o;
+}
+
+void stage_init(hipStream_t st, const IV& v) {  // render_node.rs:219-277
+  efl::launch_init_center(st, make_batch(v, EX_CENTER));
+  efl::launch_init_edge(st, make_batch(v, EX_XMAJOR));
+  efl::launch_fluid_source(st, make_batch(v, EX_CENTER), 1);
+}
+
+void stage_reinit(hipStream_t st, const IV& v) {  // reinitialize_levelset.rs:162-243
+  IV jfa = filter(v, [](efl_instance* i) { return i->s.reinit.kind == EFL_REINIT_JUMP_FLOODING; });
+  IV fim = filter(v, [](efl_instance* i) { return i->s.reinit.kind != EFL_REINIT_JUMP_FLOODING; });
+  if (!jfa.empty()) {
+    efl::launch_jfa_init(st, make_batch(jfa, EX_CENTER));
+    int M = 0;
+    for (auto* i : jfa) M = std::max(M, i->jfa_max_power);
+    for (int j = M; j >= 0; --j) {  // steps 2^M .. 1 (jump_flooding.rs:200-209)
+      IV part;
+      std::vector<int> src;
+      for (auto* i : jfa)
+        if (i->jfa_max_power >= j) {
+          part.push_back(i);
+          src.push_back((i->jfa_max_power - j) & 1);
+        }
+      efl::launch_jfa_step(st, make_batch(part, EX_CENTER, 0, &src), 1 << j);
+    }
+    std::vector<int> fin;
+    for (auto* i : jfa) fin.push_back((i->jfa_max_power + 1) & 1);
+    efl::launch_jfa_sdf(st, make_batch(jfa, EX_CENTER, 0, &fin));
+  }
+  if (!fim.empty()) {
+    efl::launch_fim_init(st, make_batch(fim, EX_CENTER));
+    efl::launch_fim_active(st, make_batch(fim, EX_CENTER));
+    uint32_t M = 0;
+    for (auto* i : fim) M = std::max(M, i->s.reinit.fim_iterations);
+    for (uint32_t k = 0; k < M; ++k) {
+      IV part = filter(fim, [k](efl_instance* i) { return i->s.reinit.fim_iterations > k; });
+      Batch b = make_batch(part, EX_CENTER);
+      efl::launch_fim_phase_a(st, b);
+      efl::launch_fim_phase_b(st, b);
+    }
+  }
+}
+
+void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_instance*), float weight, int pcg,
+               bool reverse_colour, Extent ext = EX_LEVEL) {
+  uint32_t M = 0;
+  for (auto* i : v) M = std::max(M, count(i));
+  for (uint32_t k = 0; k < M; ++k) {
+    IV part = filter(v, [&](efl_instance* i) { return count(i) > k; });
+    Batch b = make_batch(part, ext, lvl);
+    efl::launch_gs_color(st, b, reverse_colour ? 0 : 1, weight, pcg);
+    efl::launch_gs_color(st, b, reverse_colour ? 1 : 0, weight, pcg);
+  }
+}
+
+uint32_t cnt_pre(efl_instance* i) { return i->s.projection.mg_pre_iterations; }
+uint32_t cnt_post(efl_instance* i) { return i->s.projection.mg_post_iterations; }
+uint32_t cnt_coarse(efl_instance* i) { return i->s.projection.mg_coarsest_iterations; }
+uint32_t cnt_gs(efl_instance* i) { return i->s.projection.gs_iterations; }
+
+// v_cycle, multi_grid.rs:390-475 (symmetric: see the oracle's v_cycle)
+void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
+  if (v.empty()) return;
+  IV coarse = filter(v, [l](efl_instance* i) { return l == i->h.nlev - 1; });
+  IV rest = filter(v, [l](efl_instance* i) { return l < i->h.nlev - 1; });
+  if (!coarse.empty()) {
+    gs_sweeps(st, coarse, l, cnt_coarse, 1.0f, pcg, false);
+    if (symmetric) gs_sweeps(st, coarse, l, cnt_coarse, 1.0f, pcg, true);
+  }
+  if (rest.empty()) return;
+  gs_sweeps(st, rest, l, cnt_pre, 1.0f, pcg, false);
+  efl::launch_residual(st, make_batch(rest, EX_LEVEL, l), pcg);
+  efl::launch_restrict(st, make_batch(rest, EX_COARSE, l), pcg);
+  v_cycle(st, rest, l + 1, pcg, symmetric);
+  efl::launch_prolong(st, make_batch(rest, EX_COARSE, l), pcg);
+  gs_sweeps(st, rest, l, cnt_post, 1.0f, pcg, symmetric);
+}
+
+int read_scalars(efl_ctx* c, const IV& v) {
+  for (size_t i = 0; i < v.size(); ++i)
+    HIPCHK(c, hipMemcpyAsync(c->h_scal + 16 * i, v[i]->h.scal, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+double rel_of(const double* sc) {
+  double rr = sc[2], bb = sc[3];
+  return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+}
+
+int stage_projection(efl_ctx* c, const IV& v) {  // projection.rs:89-164
+  hipStream_t st = c->stream;
+  for (auto* i : v) {
+    i->stats.cycles = 0;
+    i->stats.converged = 1;
+    i->stats.rel_residual = -1.0f;
+  }
+  IV gs = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_GAUSS_SEIDEL; });
+  IV jac = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_JACOBI; });
+  IV mg = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_MULTIGRID; });
+  IV cg = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_MGPCG; });
+  if (!gs.empty()) {  // gauss_seidel.rs:120-148 with weight 1.9
+    gs_sweeps(st, gs, 0, cnt_gs, 1.9f, 0, false, EX_CENTER);
+    for (auto* i : gs) i->stats.cycles = i->s.projection.gs_iterations;
+  }
+  if (!jac.empty()) {  // projection.rs:135-143
+    Batch b = make_batch(jac, EX_CENTER);
+    for (int k = 0; k < 50; ++k) {
+      efl::launch_jacobi(st, b, 0);
+      efl::launch_jacobi(st, b, 1);
+    }
+    for (auto* i : jac) i->stats.cycles = 100;
+  }
+  if (!mg.empty()) {  // one V-cycle per cycle; repeat to the residual target
+    IV act = mg;
+    uint32_t k = 0;
+    while (!act.empty()) {
+      v_cycle(st, act, 0, 0, false);
+      ++k;
+      for (auto* i : act) i->stats.cycles = k;
+      IV keep;
+      IV chk;
+      for (auto* i : act) {
+        uint32_t maxc = i->s.projection.max_cycles ? i->s.projection.max_cycles : 1;
+        uint32_t every = i->s.projection.check_every ? i->s.projection.check_every : 1;
+        bool tol = i->s.projection.rel_tol > 0.0f;
+        if (tol && (k % every == 0 || k >= maxc)) chk.push_back(i);
+        else if (k < maxc) keep.push_back(i);
+      }
+      if (!chk.empty()) {
+        efl::launch_residual_norm(st, make_batch(chk, EX_FULL), 0);
+        int rc = read_scalars(c, chk);
+        if (rc) return rc;
+        for (size_t j = 0; j < chk.size(); ++j) {
+          efl_instance* i = chk[j];
+          double rel = rel_of(c->h_scal + 16 * j);
+          i->stats.rel_residual = (float)rel;
+          uint32_t maxc = i->s.projection.max_cycles ? i->s.projection.max_cycles : 1;
+          bool ok = rel <= i->s.projection.rel_tol;
+          i->stats.converged = ok;
+          if (!ok && k < maxc) keep.push_back(i);
+        }
+      }
+      act.swap(keep);
+    }
+  }
+  if (!cg.empty()) {  // flexible-free PCG with one symmetric V-cycle as M
+    Batch full = make_batch(cg, EX_FULL);
+    efl::launch_mask_p(st, full);
+    efl::launch_residual_norm(st, full, 1);
+    int rc = read_scalars(c, cg);
+    if (rc) return rc;
+    IV act;
+    for (size_t j = 0; j < cg.size(); ++j) {
+      efl_instance* i = cg[j];
+      double rel = rel_of(c->h_scal + 16 * j);
+      i->stats.rel_residual = (float)rel;
+      bool tol = i->s.projection.rel_tol > 0.0f;
+      if (tol && rel <= i->s.projection.rel_tol) i->stats.converged = 1;
+      else act.push_back(i);
+    }
+    if (!act.empty()) {
+      Batch b = make_batch(act, EX_FULL);
+      efl::launch_pcg_first(st, b);
+      efl::launch_zero_z(st, b);
+      v_cycle(st, act, 0, 1, true);
+      efl::launch_pcg_rz(st, b);
+      efl::launch_pcg_update_p(st, b);
+    }
+    uint32_t k = 0;
+    while (!act.empty()) {
+      Batch b = make_batch(act, EX_FULL);
+      efl::launch_pcg_apply(st, b);
+      efl::launch_pcg_axpy(st, b);
+      ++k;
+      for (auto* i : act) i->stats.cycles = k;
+      IV keep, chk;
+      for (auto* i : act) {
+        uint32_t maxc = i->s.projection.max_cycles ? i->s.projection.max_cycles : 1;
+        uint32_t every = i->s.projection.check_every ? i->s.projection.check_every : 1;
+        bool tol = i->s.projection.rel_tol > 0.0f;
+        if (tol && (k % every == 0 || k >= maxc)) chk.push_back(i);
+        else if (k < maxc) keep.push_back(i);
+      }
+      if (!chk.empty()) {
+        int rc2 = read_scalars(c, chk);
+        if (rc2) return rc2;
+        for (size_t j = 0; j < chk.size(); ++j) {
+          efl_instance* i = chk[j];
+          double rel = rel_of(c->h_scal + 16 * j);
+          i->stats.rel_residual = (float)rel;
+          uint32_t maxc = i->s.projection.max_cycles ? i->s.projection.max_cycles : 1;
+          bool ok = rel <= i->s.projection.rel_tol;
+          i->stats.converged = ok;
+          if (!ok && k < maxc) keep.push_back(i);
+        }
+      }
+      act.swap(keep);
+      if (!act.empty()) {
+        Batch bb = make_batch(act, EX_FULL);
+        efl::launch_zero_z(st, bb);
+        v_cycle(st, act, 0, 1, true);
+        efl::launch_pcg_rz(st, bb);
+        efl::launch_pcg_update_p(st, bb);
+      }
+    }
+  }
+  return EFL_OK;
+}
+
+void stage_extrapolate(hipStream_t st, const IV& v) {  // render_node.rs:540-608
+  Batch bu = make_batch(v, EX_XMAJOR), bv = make_batch(v, EX_YMAJOR);
+  efl::launch_valid_u(st, bu);
+  efl::launch_valid_v(st, bv);
+  for (int k = 0; k < 10 / 2; ++k) {
+    efl::launch_extrapolate(st, bu, 0, 0);
+    efl::launch_extrapolate(st, bu, 0, 1);
+    efl::launch_extrapolate(st, bv, 1, 0);
+    efl::launch_extrapolate(st, bv, 1, 1);
+  }
+}
+
+void stage_fluid_to_solid(hipStream_t st, const IV& v) {  // render_node.rs:610-641
+  Batch per;
+  std::memset(&per, 0, sizeof(per));
+  per = make_batch(v, EX_CENTER);
+  efl::launch_clear_bins(st, per);
+  efl::launch_sample_forces(st, per);
+  efl::launch_accumulate_forces(st, per);
+}
+
+int measure_after_projection(efl_ctx* c, const IV& v) {
+  IV d = filter(v, [](efl_instance* i) { return i->diag && i->stats.rel_residual < 0.0f; });
+  if (d.empty()) return EFL_OK;
+  efl::launch_residual_norm(c->stream, make_batch(d, EX_FULL), 0);
+  int rc = read_scalars(c, d);
+  if (rc) return rc;
+  for (size_t j = 0; j < d.size(); ++j) d[j]->stats.rel_residual = (float)rel_of(c->h_scal + 16 * j);
+  return EFL_OK;
+}
+
+int measure_divergence(efl_ctx* c, const IV& v) {
+  IV d = filter(v, [](efl_instance* i) { return i->diag; });
+  if (d.empty()) return EFL_OK;
+  Batch b = make_batch(d, EX_FULL);
+  efl::launch_clear_diag(c->stream, b);
+  efl::launch_max_divergence(c->stream, b);
+  std::vector<float> host(4 * d.size());
+  for (size_t j = 0; j < d.size(); ++j)
+    HIPCHK(c, hipMemcpyAsync(&host[4 * j], d[j]->h.diag, 4 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t j = 0; j < d.size(); ++j) d[j]->stats.max_norm_divergence = d[j]->h.dt * host[4 * j];
+  return EFL_OK;
+}
+
+int run_init(efl_ctx* c, const IV& v) {
+  hipStream_t st = c->stream;
+  stage_init(st, v);
+  for (auto* i : v) {
+    if (!i->custom_ls.empty()) {
+      // min-combine the staged level set into levelset_air1 (host round trip
+      // at initialisation only)
+      size_t n = (size_t)i->s.width * i->s.height;
+      std::vector<float> cur(n);
+      HIPCHK(c, hipMemcpyAsync(cur.data(), i->h.ls1, n * 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+      for (size_t k = 0; k < n; ++k) cur[k] = std::fmin(cur[k], i->custom_ls[k]);
+      HIPCHK(c, hipMemcpyAsync(i->h.ls1, cur.data(), n * 4, hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+    }
+  }
+  stage_reinit(st, v);
+  for (auto* i : v) {
+    i->initialized = true;
+    i->stats.steps = 0;
+  }
+  return EFL_OK;
+}
+
+int run_step(efl_ctx* c, const IV& v) {  // render_node.rs:278-466
+  hipStream_t st = c->stream;
+  bool timed = c->timing && v.size() == 1;
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[0], st));
+  Batch center = make_batch(v, EX_CENTER), bx = make_batch(v, EX_XMAJOR), by = make_batch(v, EX_YMAJOR);
+  efl::launch_update_solid(st, center);
+  efl::launch_area_fraction(st, center);
+  efl::launch_advect_u(st, bx);
+  efl::launch_advect_v(st, by);
+  efl::launch_forces_u(st, bx);
+  efl::launch_forces_v(st, by);
+  efl::launch_divergence(st, center);
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[1], st));
+  int rc = stage_projection(c, v);
+  if (rc) return rc;
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[2], st));
+  rc = measure_after_projection(c, v);
+  if (rc) return rc;
+  efl::launch_solve_u(st, bx);
+  efl::launch_solve_v(st, by);
+  rc = measure_divergence(c, v);
+  if (rc) return rc;
+  stage_extrapolate(st, v);
+  efl::launch_advect_levelset(st, center);
+  efl::launch_fluid_source(st, center, 0);
+  stage_reinit(st, v);
+  efl::launch_ls_gradient(st, center);
+  stage_fluid_to_solid(st, v);
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[3], st));
+  for (auto* i : v) {
+    i->stats.steps += 1;
+    if (i->h.nforces) {  // apply_forces_and_clear (apply_forces.rs:157-167)
+      i->h.nforces = 0;
+      i->dirty = true;
+    }
+  }
+  if (timed) {
+    HIPCHK(c, hipEventSynchronize(c->ev[3]));
+    float a = 0, b = 0;
+    hipEventElapsedTime(&a, c->ev[1], c->ev[2]);
+    hipEventElapsedTime(&b, c->ev[0], c->ev[3]);
+    v[0]->stats.projection_ms = a;
+    v[0]->stats.step_ms = b;
+  }
+  return EFL_OK;
+}
+
+size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct FieldDesc { void* p; size_t n; uint32_t w, h, c; };
+
+FieldDesc field_desc(efl_instance* in, int f) {
+  DevInst& h = in->h;
+  uint32_t W = in->s.width, H = in->s.height;
+  switch (f) {
+    case EFL_FIELD_U0: return {h.u0, (size_t)(W + 1) * H, W + 1, H, 1};
+    case EFL_FIELD_U1: return {h.u1, (size_t)(W + 1) * H, W + 1, H, 1};
+    case EFL_FIELD_V0: return {h.v0, (size_t)W * (H + 1), W, H + 1, 1};
+    case EFL_FIELD_V1: return {h.v1, (size_t)W * (H + 1), W, H + 1, 1};
+    case EFL_FIELD_U_SOLID: return {h.us, (size_t)(W + 1) * H, W + 1, H, 1};
+    case EFL_FIELD_V_SOLID: return {h.vs, (size_t)W * (H + 1), W, H + 1, 1};
+    case EFL_FIELD_SOLID_ID: return {h.sid, (size_t)W * H, W, H, 1};
+    case EFL_FIELD_U_VALID0: return {h.uval0, (size_t)(W + 1) * H, W + 1, H, 1};
+    case EFL_FIELD_U_VALID1: return {h.uval1, (size_t)(W + 1) * H, W + 1, H, 1};
+    case EFL_FIELD_V_VALID0: return {h.vval0, (size_t)W * (H + 1), W, H + 1, 1};
+    case EFL_FIELD_V_VALID1: return {h.vval1, (size_t)W * (H + 1), W, H + 1, 1};
+    case EFL_FIELD_DIV: return {h.div, (size_t)W * H, W, H, 1};
+    case EFL_FIELD_P0: return {h.p0, (size_t)W * H, W, H, 1};
+    case EFL_FIELD_P1: return {h.p1, (size_t)W * H, W, H, 1};
+    case EFL_FIELD_LS_AIR0: return {h.ls0, (size_t)W * H, W, H, 1};
+    case EFL_FIELD_LS_AIR1: return {h.ls1, (size_t)W * H, W, H, 1};
+    case EFL_FIELD_GRAD_LS: return {h.grad, (size_t)W * H * 2, W, H, 2};
+    case EFL_FIELD_LS_SOLID: return {h.lss, (size_t)W * H, W, H, 1};
+    case EFL_FIELD_AREA_FRACTION: return {h.afs, (size_t)W * H * 4, W, H, 4};
+    case EFL_FIELD_FIM_LABELS0: return {h.lab0, (size_t)W * H, W, H, 1};
+    case EFL_FIELD_FIM_LABELS: return {h.lab, (size_t)W * H, W, H, 1};
+    case EFL_FIELD_JFA_SEEDS0: return {h.seed0, (size_t)W * H * 2, W, H, 2};
+    case EFL_FIELD_JFA_SEEDS1: return {h.seed1, (size_t)W * H * 2, W, H, 2};
+    case EFL_FIELD_SOLID_BINS: return {h.bins, (size_t)3 * EFL_MAX_SOLIDS, EFL_MAX_SOLIDS, 3, 1};
+    case EFL_FIELD_SOLID_FORCES: return {h.fts, (size_t)4 * EFL_MAX_SOLIDS, EFL_MAX_SOLIDS, 1, 4};
+    case EFL_FIELD_MG_R0: return {h.lev[0].r, (size_t)W * H, W, H, 1};
+    default: return {nullptr, 0, 0, 0, 0};
+  }
+}
+
+bool valid_projection(const efl_projection& p) {
+  return p.kind >= EFL_PROJECTION_JACOBI && p.kind <= EFL_PROJECTION_MGPCG && p.rel_tol >= 0.0f;
+}
+bool valid_reinit(const efl_reinit& r) {
+  return r.kind == EFL_REINIT_JUMP_FLOODING || r.kind == EFL_REINIT_FAST_ITERATIVE;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+int efl_ctx_create(int device, efl_ctx** out) {
+  if (!out) return EFL_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return EFL_E_HIP;
+  if (device < 0 || device >= n) return EFL_E_INVALID;
+  efl_ctx* c = new (std::nothrow) efl_ctx();
+  if (!c) return EFL_E_OOM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return EFL_E_HIP;
+  }
+  if (hipMalloc(&c->d_obs, sizeof(efl_obstacle) * EFL_MAX_SOLIDS) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_scal, sizeof(double) * 16 * EFL_MAX_BATCH, hipHostMallocDefault) != hipSuccess) {
+    if (c->d_obs) hipFree(c->d_obs);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return EFL_E_OOM;
+  }
+  for (auto& e : c->ev) hipEventCreate(&e);
+  *out = c;
+  return EFL_OK;
+}
+
+int efl_ctx_destroy(efl_ctx* c) {
+  if (!c) return EFL_E_INVALID;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  std::vector<efl_instance*> left = c->insts;
+  for (auto* i : left) efl_instance_destroy(i);
+  for (auto& e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->d_obs) hipFree(c->d_obs);
+  if (c->h_scal) hipHostFree(c->h_scal);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return EFL_OK;
+}
+
+const char* efl_last_error(const efl_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int efl_sync(efl_ctx* c) {
+  if (!c) return EFL_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_set_timing(efl_ctx* c, int enabled) {
+  if (!c) return EFL_E_INVALID;
+  c->timing = enabled != 0;
+  return EFL_OK;
+}
+
+int efl_set_obstacles(efl_ctx* c, const efl_obstacle* obs, uint32_t n) {
+  if (!c || (n && !obs)) return EFL_E_INVALID;
+  if (n > EFL_MAX_SOLIDS) return fail(c, EFL_E_RANGE, "%u obstacles > EFL_MAX_SOLIDS", n);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (n) HIPCHK(c, hipMemcpyAsync(c->d_obs, obs, sizeof(efl_obstacle) * n, hipMemcpyHostToDevice, c->stream));
+  c->nobs = n;
+  c->obs_version++;
+  return EFL_OK;
+}
+
+int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
+  if (!c || !s || !out) return EFL_E_INVALID;
+  *out = nullptr;
+  if (s->width < 8 || s->height < 8 || s->width > 16384 || s->height > 16384)
+    return fail(c, EFL_E_INVALID, "grid size %ux%u outside [8, 16384]", s->width, s->height);
+  if (!(s->length_unit > 0.0f)) return fail(c, EFL_E_INVALID, "length_unit must be positive");  // euler_fluid.rs:49-51
+  if (!valid_projection(s->projection) || !valid_reinit(s->reinit))
+    return fail(c, EFL_E_INVALID, "invalid projection or reinitialisation method");
+  HIPCHK(c, hipSetDevice(c->device));
+  efl_instance* in = new (std::nothrow) efl_instance();
+  if (!in) return EFL_E_OOM;
+  in->ctx = c;
+  in->s = *s;
+  size_t W = s->width, H = s->height, n = W * H, nu = (W + 1) * H, nv = W * (H + 1);
+  // multigrid levels, multi_grid.rs:95-97
+  int mn = (int)std::min(W, H);
+  int nl = std::max(1, (int)std::log2((float)mn) - 2);
+  nl = std::min(nl, efl::kMaxLevels);
+  std::vector<std::pair<int, int>> lv(nl);
+  lv[0] = {(int)W, (int)H};
+  for (int l = 1; l < nl; ++l) lv[l] = {lv[l - 1].first / 2, lv[l - 1].second / 2};
+  int nblk = (int)(((W + efl::tile_w() - 1) / efl::tile_w()) * ((H + efl::tile_h() - 1) / efl::tile_h()));
+  // byte layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += al(bytes); return o; };
+  size_t o_u0 = take(nu * 4), o_u1 = take(nu * 4), o_us = take(nu * 4);
+  size_t o_v0 = take(nv * 4), o_v1 = take(nv * 4), o_vs = take(nv * 4);
+  size_t o_uv0 = take(nu * 4), o_uv1 = take(nu * 4), o_vv0 = take(nv * 4), o_vv1 = take(nv * 4);
+  size_t o_sid = take(n * 4), o_div = take(n * 4), o_p0 = take(n * 4), o_p1 = take(n * 4);
+  size_t o_ls0 = take(n * 4), o_ls1 = take(n * 4), o_lss = take(n * 4), o_phit = take(n * 4);
+  size_t o_lab0 = take(n * 4), o_lab = take(n * 4), o_labt = take(n * 4);
+  size_t o_grad = take(n * 8), o_s0 = take(n * 8), o_s1 = take(n * 8), o_afs = take(n * 16);
+  size_t o_bins = take(3 * EFL_MAX_SOLIDS * 4), o_fts = take(EFL_MAX_SOLIDS * 16);
+  size_t o_r0 = take(n * 4);
+  std::vector<size_t> o_lx(nl), o_lb(nl), o_lls(nl), o_laf(nl), o_lr(nl);
+  for (int l = 1; l < nl; ++l) {
+    size_t m = (size_t)lv[l].first * lv[l].second;
+    o_lx[l] = take(m * 4); o_lb[l] = take(m * 4); o_lls[l] = take(m * 4); o_laf[l] = take(m * 16); o_lr[l] = take(m * 4);
+  }
+  size_t o_pr = take(n * 4), o_pz = take(n * 4), o_pp = take(n * 4), o_pq = take(n * 4);
+  size_t o_red = take((size_t)3 * nblk * 8), o_scal = take(16 * 8), o_cnt = take(16), o_diag = take(16);
+  size_t o_src0 = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES), o_src = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES);
+  size_t o_desc = take(sizeof(DevInst));
+  in->bytes = off;
+  if (hipMalloc(&in->mem, off) != hipSuccess) {
+    delete in;
+    return fail(c, EFL_E_OOM, "hipMalloc(%zu) failed", off);
+  }
+  if (hipMemsetAsync(in->mem, 0, off, c->stream) != hipSuccess) {
+    hipFree(in->mem);
+    delete in;
+    return fail(c, EFL_E_HIP, "hipMemsetAsync failed");
+  }
+  char* base = (char*)in->mem;
+  DevInst& h = in->h;
+  std::memset(&h, 0, sizeof(h));
+  h.W = (int)W;
+  h.H = (int)H;
+  h.dx = 1.0f / s->length_unit;
+  h.dt = 0.0f;
+  h.rho = s->rho;
+  h.gx = s->gravity[0];
+  h.gy = s->gravity[1];
+  bool zero = true;
+  for (int k = 0; k < 16; ++k) zero = zero && s->transform[k] == 0.0f;
+  for (int k = 0; k < 16; ++k) h.T[k] = zero ? ((k % 5) == 0 ? 1.0f : 0.0f) : s->transform[k];
+  h.sizex = (float)W;
+  h.sizey = (float)H;
+  h.u0 = (float*)(base + o_u0); h.u1 = (float*)(base + o_u1); h.us = (float*)(base + o_us);
+  h.v0 = (float*)(base + o_v0); h.v1 = (float*)(base + o_v1); h.vs = (float*)(base + o_vs);
+  h.uval0 = (int32_t*)(base + o_uv0); h.uval1 = (int32_t*)(base + o_uv1);
+  h.vval0 = (int32_t*)(base + o_vv0); h.vval1 = (int32_t*)(base + o_vv1);
+  h.sid = (int32_t*)(base + o_sid); h.div = (float*)(base + o_div); h.p0 = (float*)(base + o_p0); h.p1 = (float*)(base + o_p1);
+  h.ls0 = (float*)(base + o_ls0); h.ls1 = (float*)(base + o_ls1); h.lss = (float*)(base + o_lss);
+  h.phi_tmp = (float*)(base + o_phit);
+  h.lab0 = (uint32_t*)(base + o_lab0); h.lab = (uint32_t*)(base + o_lab); h.lab_tmp = (uint32_t*)(base + o_labt);
+  h.grad = (float2*)(base + o_grad); h.seed0 = (float2*)(base + o_s0); h.seed1 = (float2*)(base + o_s1);
+  h.afs = (float4*)(base + o_afs);
+  h.bins = (int32_t*)(base + o_bins); h.fts = (float4*)(base + o_fts);
+  h.nlev = nl;
+  h.lev[0] = {(int)W, (int)H, h.p0, h.div, h.ls0, h.afs, (float*)(base + o_r0)};
+  for (int l = 1; l < nl; ++l)
+    h.lev[l] = {lv[l].first, lv[l].second, (float*)(base + o_lx[l]), (float*)(base + o_lb[l]), (float*)(base + o_lls[l]),
+                (float4*)(base + o_laf[l]), (float*)(base + o_lr[l])};
+  h.pcg_r = (float*)(base + o_pr); h.pcg_z = (float*)(base + o_pz); h.pcg_p = (float*)(base + o_pp); h.pcg_q = (float*)(base + o_pq);
+  h.lev_pcg0 = {(int)W, (int)H, h.pcg_z, h.pcg_r, h.ls0, h.afs, (float*)(base + o_r0)};
+  h.red = (double*)(base + o_red);
+  h.red_stride = nblk;
+  h.scal = (double*)(base + o_scal);
+  h.counter = (uint32_t*)(base + o_cnt);
+  h.diag = (float*)(base + o_diag);
+  h.src_init = (efl_source*)(base + o_src0);
+  h.src = (efl_source*)(base + o_src);
+  h.obs = c->d_obs;
+  h.nobs = c->nobs;
+  in->obs_version = c->obs_version;
+  in->d = (DevInst*)(base + o_desc);
+  in->jfa_max_power = (int)std::floor(std::log2((float)std::max(W, H)) - 1.0f);  // jump_flooding.rs:202
+  in->dirty = true;
+  int rc = upload(in);
+  if (rc) {
+    hipFree(in->mem);
+    delete in;
+    return rc;
+  }
+  c->insts.push_back(in);
+  *out = in;
+  return EFL_OK;
+}
+
+int efl_instance_destroy(efl_instance* in) {
+  if (!in) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  if (in->d_forces) hipFree(in->d_forces);
+  if (in->mem) hipFree(in->mem);
+  c->insts.erase(std::remove(c->insts.begin(), c->insts.end(), in), c->insts.end());
+  delete in;
+  return EFL_OK;
+}
+
+int efl_set_projection(efl_instance* in, const efl_projection* p) {
+  if (!in || !p) return EFL_E_INVALID;
+  if (!valid_projection(*p)) return fail(in->ctx, EFL_E_INVALID, "invalid projection");
+  in->s.projection = *p;
+  return EFL_OK;
+}
+
+int efl_set_reinit(efl_instance* in, const efl_reinit* r) {
+  if (!in || !r) return EFL_E_INVALID;
+  if (!valid_reinit(*r)) return fail(in->ctx, EFL_E_INVALID, "invalid reinitialisation method");
+  in->s.reinit = *r;
+  return EFL_OK;
+}
+
+int efl_set_uniforms(efl_instance* in, float rho, const float g[2], const float T[16]) {
+  if (!in) return EFL_E_INVALID;
+  in->h.rho = rho;
+  in->s.rho = rho;
+  if (g) { in->h.gx = g[0]; in->h.gy = g[1]; in->s.gravity[0] = g[0]; in->s.gravity[1] = g[1]; }
+  if (T) for (int k = 0; k < 16; ++k) in->h.T[k] = T[k];
+  in->dirty = true;
+  return EFL_OK;
+}
+
+int efl_set_sources(efl_instance* in, const efl_source* src, uint32_t n, int oneshot) {
+  if (!in || (n && !src)) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  if (n > EFL_MAX_FLUID_SOURCES)  // fluid_source_uniform.rs:92-97 warns and drops the rest
+    return fail(c, EFL_E_RANGE, "%u sources > EFL_MAX_FLUID_SOURCES", n);
+  HIPCHK(c, hipSetDevice(c->device));
+  efl_source* dst = oneshot ? in->h.src_init : in->h.src;
+  if (n) HIPCHK(c, hipMemcpyAsync(dst, src, sizeof(efl_source) * n, hipMemcpyHostToDevice, c->stream));
+  if (oneshot) in->h.nsrc_init = n;
+  else in->h.nsrc = n;
+  in->dirty = true;
+  return EFL_OK;
+}
+
+int efl_set_forces(efl_instance* in, const efl_force* f, uint32_t n) {
+  if (!in || (n && !f)) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (n > in->forces_cap) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (in->d_forces) hipFree(in->d_forces);
+    in->d_forces = nullptr;
+    uint32_t cap = std::max<uint32_t>(64, n);
+    if (hipMalloc(&in->d_forces, sizeof(efl_force) * cap) != hipSuccess) {
+      in->forces_cap = 0;
+      in->h.nforces = 0;
+      in->dirty = true;
+      return fail(c, EFL_E_OOM, "forces buffer");
+    }
+    in->forces_cap = cap;
+  }
+  if (n) HIPCHK(c, hipMemcpyAsync(in->d_forces, f, sizeof(efl_force) * n, hipMemcpyHostToDevice, c->stream));
+  in->h.forces = in->d_forces;
+  in->h.nforces = n;
+  in->dirty = true;
+  return EFL_OK;
+}
+
+int efl_init_levelset(efl_instance* in, const float* ls) {
+  if (!in) return EFL_E_INVALID;
+  if (!ls) in->custom_ls.clear();
+  else in->custom_ls.assign(ls, ls + (size_t)in->s.width * in->s.height);
+  return EFL_OK;
+}
+
+int efl_reset(efl_instance* in) {
+  if (!in) return EFL_E_INVALID;
+  in->initialized = false;
+  return EFL_OK;
+}
+
+int efl_initialize(efl_instance* in) {
+  if (!in) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = upload(in);
+  if (rc) return rc;
+  return run_init(c, IV{in});
+}
+
+int efl_set_diagnostics(efl_instance* in, int enabled) {
+  if (!in) return EFL_E_INVALID;
+  in->diag = enabled != 0;
+  return EFL_OK;
+}
+
+int efl_step(efl_ctx* c, efl_instance* const* insts, uint32_t n, float dt) {
+  if (!c || (n && !insts)) return EFL_E_INVALID;
+  if (n > EFL_MAX_BATCH) return fail(c, EFL_E_RANGE, "%u instances > EFL_MAX_BATCH", n);
+  if (!(dt > 0.0f)) return fail(c, EFL_E_INVALID, "dt must be positive");
+  HIPCHK(c, hipSetDevice(c->device));
+  IV init, step;
+  for (uint32_t k = 0; k < n; ++k) {
+    efl_instance* i = insts[k];
+    if (!i || i->ctx != c) return fail(c, EFL_E_INVALID, "instance %u is null or belongs to another context", k);
+    if (std::find(init.begin(), init.end(), i) != init.end() || std::find(step.begin(), step.end(), i) != step.end())
+      return fail(c, EFL_E_INVALID, "instance %u listed twice", k);
+    if (i->h.dt != dt) {
+      i->h.dt = dt;
+      i->dirty = true;
+    }
+    int rc = upload(i);
+    if (rc) return rc;
+    (i->initialized ? step : init).push_back(i);
+  }
+  if (!init.empty()) {
+    int rc = run_init(c, init);
+    if (rc) return rc;
+  }
+  if (!step.empty()) {
+    int rc = run_step(c, step);
+    if (rc) return rc;
+    for (auto* i : step) {  // forces were cleared after use
+      rc = upload(i);
+      if (rc) return rc;
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, EFL_E_HIP, "kernel launch: %s", hipGetErrorString(e));
+  return EFL_OK;
+}
+
+int efl_run_stage(efl_instance* in, int stage, float dt) {
+  if (!in) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (in->h.dt != dt) {
+    in->h.dt = dt;
+    in->dirty = true;
+  }
+  int rc = upload(in);
+  if (rc) return rc;
+  hipStream_t st = c->stream;
+  IV v{in};
+  Batch center = make_batch(v, EX_CENTER), bx = make_batch(v, EX_XMAJOR), by = make_batch(v, EX_YMAJOR);
+  switch (stage) {
+    case EFL_STAGE_INIT_GRID_CENTER: efl::launch_init_center(st, center); break;
+    case EFL_STAGE_INIT_GRID_EDGE: efl::launch_init_edge(st, bx); break;
+    case EFL_STAGE_SOURCE_ONESHOT: efl::launch_fluid_source(st, center, 1); break;
+    case EFL_STAGE_UPDATE_SOLID: efl::launch_update_solid(st, center); break;
+    case EFL_STAGE_AREA_FRACTION: efl::launch_area_fraction(st, center); break;
+    case EFL_STAGE_ADVECT_VELOCITY: efl::launch_advect_u(st, bx); efl::launch_advect_v(st, by); break;
+    case EFL_STAGE_APPLY_FORCES: efl::launch_forces_u(st, bx); efl::launch_forces_v(st, by); break;
+    case EFL_STAGE_DIVERGENCE: efl::launch_divergence(st, center); break;
+    case EFL_STAGE_PROJECTION: rc = stage_projection(c, v); break;
+    case EFL_STAGE_SOLVE_VELOCITY: efl::launch_solve_u(st, bx); efl::launch_solve_v(st, by); break;
+    case EFL_STAGE_EXTRAPOLATE_VELOCITY: stage_extrapolate(st, v); break;
+    case EFL_STAGE_ADVECT_LEVELSET: efl::launch_advect_levelset(st, center); break;
+    case EFL_STAGE_SOURCE: efl::launch_fluid_source(st, center, 0); break;
+    case EFL_STAGE_REINITIALIZE: stage_reinit(st, v); break;
+    case EFL_STAGE_LEVELSET_GRADIENT: efl::launch_ls_gradient(st, center); break;
+    case EFL_STAGE_FLUID_TO_SOLID: stage_fluid_to_solid(st, v); break;
+    case EFL_STAGE_GS_SWEEP: {
+      efl::launch_gs_color(st, center, 1, 1.9f, 0);
+      efl::launch_gs_color(st, center, 0, 1.9f, 0);
+      break;
+    }
+    case EFL_STAGE_MG_VCYCLE: v_cycle(st, v, 0, 0, false); break;
+    case EFL_STAGE_MG_RESIDUAL: efl::launch_residual(st, make_batch(v, EX_LEVEL, 0), 0); break;
+    default: return fail(c, EFL_E_INVALID, "unknown stage %d", stage);
+  }
+  if (rc) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, EFL_E_HIP, "kernel launch: %s", hipGetErrorString(e));
+  return EFL_OK;
+}
+
+int efl_field_info(const efl_instance* in, int field, size_t* count, uint32_t* w, uint32_t* h, uint32_t* ch) {
+  if (!in) return EFL_E_INVALID;
+  FieldDesc d = field_desc(const_cast<efl_instance*>(in), field);
+  if (!d.p) return EFL_E_INVALID;
+  if (count) *count = d.n;
+  if (w) *w = d.w;
+  if (h) *h = d.h;
+  if (ch) *ch = d.c;
+  return EFL_OK;
+}
+
+int efl_get_field(efl_instance* in, int field, void* host, size_t count) {
+  if (!in || !host) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  FieldDesc d = field_desc(in, field);
+  if (!d.p) return fail(c, EFL_E_INVALID, "unknown field %d", field);
+  if (count != d.n) return fail(c, EFL_E_INVALID, "field %d holds %zu elements, not %zu", field, d.n, count);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(host, d.p, d.n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_set_field(efl_instance* in, int field, const void* host, size_t count) {
+  if (!in || !host) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  FieldDesc d = field_desc(in, field);
+  if (!d.p) return fail(c, EFL_E_INVALID, "unknown field %d", field);
+  if (count != d.n) return fail(c, EFL_E_INVALID, "field %d holds %zu elements, not %zu", field, d.n, count);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(d.p, host, d.n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_field_device_ptr(efl_instance* in, int field, void** dptr) {
+  if (!in || !dptr) return EFL_E_INVALID;
+  FieldDesc d = field_desc(in, field);
+  if (!d.p) return EFL_E_INVALID;
+  *dptr = d.p;
+  return EFL_OK;
+}
+
+int efl_get_solid_forces(efl_instance* in, efl_solid_force* out, uint32_t n) {
+  if (!in || !out) return EFL_E_INVALID;
+  if (n > EFL_MAX_SOLIDS) return fail(in->ctx, EFL_E_RANGE, "n > EFL_MAX_SOLIDS");
+  efl_ctx* c = in->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(out, in->h.fts, sizeof(efl_solid_force) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_measure(efl_instance* in, float dt, float* rel, float* mdiv) {
+  if (!in) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (in->h.dt != dt) {
+    in->h.dt = dt;
+    in->dirty = true;
+  }
+  int rc = upload(in);
+  if (rc) return rc;
+  IV v{in};
+  efl::launch_residual_norm(c->stream, make_batch(v, EX_FULL), 0);
+  rc = read_scalars(c, v);
+  if (rc) return rc;
+  if (rel) *rel = (float)rel_of(c->h_scal);
+  bool saved = in->diag;
+  in->diag = true;
+  rc = measure_divergence(c, v);
+  in->diag = saved;
+  if (rc) return rc;
+  if (mdiv) *mdiv = in->stats.max_norm_divergence;
+  return EFL_OK;
+}
+
+int efl_get_stats(efl_instance* in, efl_stats* out) {
+  if (!in || !out) return EFL_E_INVALID;
+  *out = in->stats;
+  return EFL_OK;
+}
+
+}  // extern "C"

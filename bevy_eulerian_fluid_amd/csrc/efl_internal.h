@@ -1,0 +1,84 @@
+// efl_internal.h - device-side data layout shared by the HIP kernels and the
+// host scheduler of libefl.
+//
+// One instance = one fluid entity (setup_components.rs:38-330).  Its grids
+// live in one device allocation, row-major [y][x]; the MAC face grids are
+// u: (W+1) x H and v: W x (H+1).  Multi-channel textures are interleaved
+// (area fractions as float4, seeds as float2), matching the rgba32f / rg32f
+// texel order of the reference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/efl.h"
+
+namespace efl {
+
+constexpr int kMaxLevels = 14;
+constexpr int kMaxBatch = EFL_MAX_BATCH;
+
+struct Level {  // one multigrid level (multi_grid.rs:99-118)
+  int w, h;
+  float* x;
+  float* b;
+  float* ls;
+  float4* af;
+  float* r;
+};
+
+// Per-instance descriptor, copied to device memory; kernels read it through
+// a pointer (uniform across a block, so it stays in scalar registers).
+struct DevInst {
+  int W, H;
+  // SimulationUniform (fluid_uniform.wgsl:3-10)
+  float dx, dt, rho, gx, gy;
+  float T[16];
+  float sizex, sizey;
+  // grids
+  float *u0, *u1, *v0, *v1, *us, *vs;
+  int32_t* sid;
+  int32_t *uval0, *uval1, *vval0, *vval1;
+  float *div, *p0, *p1, *ls0, *ls1, *lss;
+  float2* grad;
+  float4* afs;
+  uint32_t *lab0, *lab;
+  float *phi_tmp;      // FIM phase buffer
+  uint32_t* lab_tmp;   // FIM phase labels
+  float2 *seed0, *seed1;
+  int32_t* bins;       // 3 x 256
+  float4* fts;         // 256 x (fx, fy, torque, 0)
+  // sources (uniform-sized), forces, obstacles
+  uint32_t nsrc_init, nsrc;
+  efl_source* src_init;
+  efl_source* src;
+  uint32_t nforces;
+  efl_force* forces;
+  uint32_t nobs;
+  const efl_obstacle* obs;
+  // multigrid pyramid; level 0 aliases p0/div/ls0/afs.  lev_pcg0 is level 0
+  // with x = pcg_z, b = pcg_r, used when the V-cycle preconditions CG.
+  int nlev;
+  Level lev[kMaxLevels];
+  Level lev_pcg0;
+  // PCG vectors (W x H) and scalars
+  float *pcg_r, *pcg_z, *pcg_p, *pcg_q;
+  double* red;         // reduction partials: 3 x red_stride doubles
+  int red_stride;      // >= blocks of one full-grid launch
+  double* scal;        // scalars: [0] rz, [1] pq, [2] rr, [3] bb, [4] alpha, [5] beta, [6] done
+  uint32_t* counter;   // last-block counters
+  float* diag;         // [0] max |D| (as float bits via atomic max), ...
+};
+
+// Launch map of a batched kernel: the flat grid of blocks is split into
+// per-instance ranges; inside a range, blocks tile the instance's extent.
+struct Batch {
+  int n;
+  int start[kMaxBatch + 1];
+  int tiles_x[kMaxBatch];
+  int ex[kMaxBatch], ey[kMaxBatch];
+  int level[kMaxBatch];
+  const DevInst* inst[kMaxBatch];
+};
+
+}  // namespace efl

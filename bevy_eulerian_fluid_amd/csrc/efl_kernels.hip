@@ -1,0 +1,1194 @@
+// efl_kernels.hip - HIP kernels for gfx950 of the per-step fluid pipeline.
+//
+// Each kernel computes what one WGSL entry point of bevy_eulerian_fluid
+// computes (the file:line it answers to is given at the kernel), with the
+// same floating-point operation order so that results match the reference
+// semantics bit for bit on IEEE hardware (the library is built with
+// -ffp-contract=off; division and sqrt are correctly rounded).
+//
+// Execution model: every kernel is batched.  A launch covers a list of
+// instances (efl::Batch); blockIdx.x is split into per-instance ranges and,
+// inside a range, 64 x 4 tiles (one wavefront per row, so every global access
+// of a row is one coalesced 256-byte transaction).  Extents per instance are
+// the dispatch coverage of the reference (pipeline.rs:175-187), so grids that
+// are not multiples of 8 / 64 leave exactly the cells the reference leaves.
+//
+// Out-of-bounds texture loads read 0 and out-of-bounds stores are dropped,
+// as under Vulkan robustImageAccess2.
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "efl_internal.h"
+#include "efl_launch.h"
+
+namespace efl {
+
+constexpr int TX = 64;
+constexpr int TY = 4;
+constexpr int NT = TX * TY;
+
+// ---------------------------------------------------------------------------
+// indexing helpers
+// ---------------------------------------------------------------------------
+struct Site {
+  const DevInst* I;
+  int x, y;
+  int ex, ey;
+  int lvl;
+  int slot;
+  bool ok;
+};
+
+__device__ __forceinline__ Site locate(const Batch& b) {
+  int blk = blockIdx.x;
+  int i = 0;
+  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  int lb = blk - b.start[i];
+  int tx = lb % b.tiles_x[i], ty = lb / b.tiles_x[i];
+  Site s;
+  s.I = b.inst[i];
+  s.x = tx * TX + (int)(threadIdx.x % TX);
+  s.y = ty * TY + (int)(threadIdx.x / TX);
+  s.ex = b.ex[i];
+  s.ey = b.ey[i];
+  s.lvl = b.level[i];
+  s.slot = i;
+  s.ok = s.x < s.ex && s.y < s.ey;
+  return s;
+}
+
+__device__ __forceinline__ bool inb(int x, int y, int w, int h) {
+  return (unsigned)x < (unsigned)w && (unsigned)y < (unsigned)h;
+}
+template <class T>
+__device__ __forceinline__ T ldz(const T* p, int w, int h, int x, int y) {
+  return inb(x, y, w, h) ? p[(size_t)y * w + x] : T(0);
+}
+__device__ __forceinline__ float4 ldz4(const float4* p, int w, int h, int x, int y) {
+  return inb(x, y, w, h) ? p[(size_t)y * w + x] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ float2 ldz2(const float2* p, int w, int h, int x, int y) {
+  return inb(x, y, w, h) ? p[(size_t)y * w + x] : make_float2(0.f, 0.f);
+}
+template <class T>
+__device__ __forceinline__ void stz(T* p, int w, int h, int x, int y, T v) {
+  if (inb(x, y, w, h)) p[(size_t)y * w + x] = v;
+}
+
+// WGSL built-ins with the conventions of the reference pipeline
+__device__ __forceinline__ float sat01(float v) { return fminf(fmaxf(v, 0.0f), 1.0f); }
+__device__ __forceinline__ float lerp_w(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float sgn(float v) { return v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f); }
+__device__ __forceinline__ float hyp(float a, float b) { return sqrtf(a * a + b * b); }
+__device__ __forceinline__ int to_i32(float v) {
+  if (isnan(v)) return 0;
+  if (v >= 2147483647.0f) return 2147483647;
+  if (v <= -2147483648.0f) return (-2147483647 - 1);
+  return (int)v;
+}
+// mat4 (column-major) * (x, y, 0, 1)
+__device__ __forceinline__ float2 xform(const float* m, float x, float y) {
+  float ox = ((m[0] * x + m[4] * y) + m[8] * 0.0f) + m[12] * 1.0f;
+  float oy = ((m[1] * x + m[5] * y) + m[9] * 0.0f) + m[13] * 1.0f;
+  return make_float2(ox, oy);
+}
+
+// area_fraction / area_fractions, utils/area_fraction.wgsl:10-56
+__device__ __forceinline__ float face_open(float a, float b) { return sat01(fmaxf(a, b) / fabsf(a - b)); }
+__device__ __forceinline__ float4 fractions_at(const float* ls, int w, int h, int x, int y) {
+  float a0 = ldz(ls, w, h, x - 1, y - 1), a1 = ldz(ls, w, h, x, y - 1), a2 = ldz(ls, w, h, x + 1, y - 1);
+  float a3 = ldz(ls, w, h, x - 1, y), a4 = ldz(ls, w, h, x, y), a5 = ldz(ls, w, h, x + 1, y);
+  float a6 = ldz(ls, w, h, x - 1, y + 1), a7 = ldz(ls, w, h, x, y + 1), a8 = ldz(ls, w, h, x + 1, y + 1);
+  float mm = 0.25f * (((a0 + a1) + a3) + a4);
+  float pm = 0.25f * (((a1 + a2) + a4) + a5);
+  float mp = 0.25f * (((a3 + a4) + a6) + a7);
+  float pp = 0.25f * (((a4 + a5) + a7) + a8);
+  return make_float4(face_open(mm, mp), face_open(pm, pp), face_open(mm, pm), face_open(mp, pp));
+}
+__device__ __forceinline__ bool all_closed(float4 f) { return f.x == 0.0f && f.y == 0.0f && f.z == 0.0f && f.w == 0.0f; }
+
+// utils/coordinate.wgsl:51-94
+__device__ __forceinline__ float sample_u(const float* u, int w, int h, float px, float py) {
+  int i = to_i32(rintf(px)), j = to_i32(floorf(py));
+  float fi = (px + 0.5f) - (float)i, fj = py - (float)j;
+  float a = lerp_w(ldz(u, w, h, i, j), ldz(u, w, h, i + 1, j), fi);
+  float b = lerp_w(ldz(u, w, h, i, j + 1), ldz(u, w, h, i + 1, j + 1), fi);
+  return lerp_w(a, b, fj);
+}
+__device__ __forceinline__ float sample_v(const float* v, int w, int h, float px, float py) {
+  int i = to_i32(floorf(px)), j = to_i32(rintf(py));
+  float fi = px - (float)i, fj = (py + 0.5f) - (float)j;
+  float a = lerp_w(ldz(v, w, h, i, j), ldz(v, w, h, i + 1, j), fi);
+  float b = lerp_w(ldz(v, w, h, i, j + 1), ldz(v, w, h, i + 1, j + 1), fi);
+  return lerp_w(a, b, fj);
+}
+__device__ __forceinline__ float2 backtrace(const DevInst* I, float px, float py, float dt) {
+  int W = I->W, H = I->H;
+  float vx = sample_u(I->u0, W + 1, H, px, py), vy = sample_v(I->v0, W, H + 1, px, py);
+  float hdt = 0.5f * dt;
+  float mx = px - hdt * vx, my = py - hdt * vy;
+  float wx = sample_u(I->u0, W + 1, H, mx, my), wy = sample_v(I->v0, W, H + 1, mx, my);
+  return make_float2(px - dt * wx, py - dt * wy);
+}
+
+// ---------------------------------------------------------------------------
+// K1 initialize_grid_center (initialize_grid_center.wgsl:5-15)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_init_center(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  size_t id = (size_t)s.y * I->W + s.x;
+  float level = (float)(uint32_t)(I->H - s.y);
+  I->ls0[id] = level;
+  I->ls1[id] = level;
+  I->grad[id] = make_float2(0.0f, -1.0f);
+}
+
+// K2 initialize_grid_edge (initialize_grid_edge.wgsl:6-17): u at (x, y), v at (y, x)
+__global__ __launch_bounds__(NT) void k_init_edge(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H;
+  stz(I->u0, W + 1, H, s.x, s.y, 0.0f);
+  stz(I->u1, W + 1, H, s.x, s.y, 0.0f);
+  stz(I->v0, W, H + 1, s.y, s.x, 0.0f);
+  stz(I->v1, W, H + 1, s.y, s.x, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// K3 update_fluid_source (update_fluid_source.wgsl:30-107).  Two cells write
+// each shared face in the reference; here each face is gathered from its two
+// cells with the later (raster-order) cell taking precedence.
+// ---------------------------------------------------------------------------
+struct SrcOut { float level; float vx, vy; bool upd; };
+
+__device__ __forceinline__ float box_level(float hx, float hy, float cx, float cy, float x, float y) {
+  float dx = fabsf(cx - x) - hx, dy = fabsf(cy - y) - hy;
+  if (dx < 0.0f) return dy < 0.0f ? fmaxf(dx, dy) : dy;
+  return dy < 0.0f ? dx : hyp(dx, dy);
+}
+
+__device__ SrcOut eval_sources(const DevInst* I, const efl_source* src, uint32_t n, int x, int y, float level_in) {
+  float dimx = (float)I->W, dimy = (float)I->H;
+  float ux = (float)x / dimx, uy = (float)y / dimy;
+  float nx = ux * 2.0f + (-1.0f), ny = uy * (-2.0f) + 1.0f;
+  float px = (0.5f * dimx) * nx, py = (0.5f * dimy) * ny;
+  SrcOut o{level_in, 0.0f, 0.0f, false};
+  bool has_source = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    const efl_source& q = src[i];
+    float sdf;
+    if (q.shape == 0u) sdf = hyp(q.center[0] - px, q.center[1] - py) - q.data[0];
+    else if (q.shape == 1u) sdf = box_level(q.data[0], q.data[1], q.center[0], q.center[1], px, py);
+    else sdf = 0.0f;
+    if (q.mode == 0u) {
+      o.level = fminf(o.level, sdf);
+      has_source = true;
+      if (sdf < 0.0f) {
+        o.vx = o.vx + q.velocity[0];
+        o.vy = o.vy + q.velocity[1];
+        o.upd = true;
+      }
+    } else if (!has_source) {
+      o.level = fmaxf(o.level, -sdf);
+    }
+  }
+  return o;
+}
+
+__global__ __launch_bounds__(NT) void k_fluid_source(Batch b, int oneshot) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  uint32_t n = oneshot ? I->nsrc_init : I->nsrc;
+  const efl_source* src = oneshot ? I->src_init : I->src;
+  int W = I->W;
+  int x = s.x, y = s.y;
+  size_t id = (size_t)y * W + x;
+  SrcOut me = eval_sources(I, src, n, x, y, I->ls1[id]);
+  I->ls1[id] = me.level;
+  if (n == 0) return;
+  // u face (x, y): cell (x, y) wins over cell (x - 1, y)
+  if (me.upd) {
+    I->u0[(size_t)y * (W + 1) + x] = me.vx;
+  } else if (x > 0) {
+    SrcOut l = eval_sources(I, src, n, x - 1, y, 0.0f);
+    if (l.upd) I->u0[(size_t)y * (W + 1) + x] = l.vx;
+  }
+  if (x == s.ex - 1 && me.upd) I->u0[(size_t)y * (W + 1) + x + 1] = me.vx;
+  // v face (x, y): cell (x, y) wins over cell (x, y - 1)
+  if (me.upd) {
+    I->v0[(size_t)y * W + x] = me.vy;
+  } else if (y > 0) {
+    SrcOut t = eval_sources(I, src, n, x, y - 1, 0.0f);
+    if (t.upd) I->v0[(size_t)y * W + x] = t.vy;
+  }
+  if (y == s.ey - 1 && me.upd) I->v0[(size_t)(y + 1) * W + x] = me.vy;
+}
+
+// ---------------------------------------------------------------------------
+// K4 update_solid (update_solid.wgsl:15-175, solid_obstacle.wgsl:43-89)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float seg_dist(float x, float y, float ax, float ay, float bx, float by) {
+  float abx = bx - ax, aby = by - ay;
+  float t = sat01(((x - ax) * abx + (y - ay) * aby) / (abx * abx + aby * aby));
+  float qx = ax + t * abx, qy = ay + t * aby;
+  return hyp(x - qx, y - qy);
+}
+__device__ __forceinline__ float side(float x, float y, float ax, float ay, float bx, float by) {
+  float px = x - ax, py = y - ay, abx = bx - ax, aby = by - ay;
+  return sgn(py * abx - px * aby);
+}
+__device__ __forceinline__ float det3(float a, float b, float c, float d, float e, float f, float g, float h, float i) {
+  return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+}
+__device__ float det_mat4(const float* m) {  // Laplace expansion along the first row
+  auto A = [&](int r, int c) { return m[c * 4 + r]; };
+  float d0 = det3(A(1, 1), A(1, 2), A(1, 3), A(2, 1), A(2, 2), A(2, 3), A(3, 1), A(3, 2), A(3, 3));
+  float d1 = det3(A(1, 0), A(1, 2), A(1, 3), A(2, 0), A(2, 2), A(2, 3), A(3, 0), A(3, 2), A(3, 3));
+  float d2 = det3(A(1, 0), A(1, 1), A(1, 3), A(2, 0), A(2, 1), A(2, 3), A(3, 0), A(3, 1), A(3, 3));
+  float d3 = det3(A(1, 0), A(1, 1), A(1, 2), A(2, 0), A(2, 1), A(2, 2), A(3, 0), A(3, 1), A(3, 2));
+  return A(0, 0) * d0 - A(0, 1) * d1 + A(0, 2) * d2 - A(0, 3) * d3;
+}
+__device__ float obstacle_level(const efl_obstacle& o, float x, float y) {
+  switch (o.shape) {
+    case 0u:
+      return hyp(x - o.transform[12], y - o.transform[13]) - o.values[0];
+    case 1u: {
+      if (det_mat4(o.transform) == 0.0f) return 1.0e6f;
+      float2 q = xform(o.inverse_transform, x, y);
+      float hx = o.values[0], hy = o.values[1];
+      float ax = fabsf(q.x) - hx, ay = fabsf(q.y) - hy;
+      bool inx = fabsf(q.x) < hx, iny = fabsf(q.y) < hy;
+      if (inx) return iny ? fmaxf(ax, ay) : ay;
+      return iny ? ax : hyp(ax, ay);
+    }
+    case 2u: {
+      float2 a = xform(o.transform, o.values[0], o.values[1]);
+      float2 c = xform(o.transform, o.values[2], o.values[3]);
+      return seg_dist(x, y, a.x, a.y, c.x, c.y) - o.values[4];
+    }
+    case 4u: {
+      float2 p0 = xform(o.transform, o.values[0], o.values[1]);
+      float2 p1 = xform(o.transform, o.values[2], o.values[3]);
+      float2 p2 = xform(o.transform, o.values[4], o.values[5]);
+      float d0 = seg_dist(x, y, p0.x, p0.y, p1.x, p1.y);
+      float d1 = seg_dist(x, y, p1.x, p1.y, p2.x, p2.y);
+      float d2 = seg_dist(x, y, p2.x, p2.y, p0.x, p0.y);
+      float m = fminf(fminf(d0, d1), d2);
+      bool inside = side(x, y, p0.x, p0.y, p1.x, p1.y) > 0.0f && side(x, y, p1.x, p1.y, p2.x, p2.y) > 0.0f &&
+                    side(x, y, p2.x, p2.y, p0.x, p0.y) > 0.0f;
+      return inside ? -m : m;
+    }
+    default:
+      return 1.0e6f;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  size_t id = (size_t)y * W + x;
+  if (x == 0 || y == 0 || x == W - 1 || y == H - 1) {
+    I->lss[id] = 0.0f;
+    I->us[(size_t)y * (W + 1) + x] = 0.0f;
+    I->vs[id] = 0.0f;
+    I->sid[id] = -1;
+    return;
+  }
+  auto world = [&](float fx, float fy) {
+    float ux = fx / (float)W, uy = fy / (float)H;
+    return xform(I->T, (ux - 0.5f) * I->sizex, (-uy + 0.5f) * I->sizey);
+  };
+  float2 pc = world((float)x, (float)y);
+  float2 pu = world((float)x - 0.5f, (float)y - 0.0f);
+  float2 pv = world((float)x - 0.0f, (float)y - 0.5f);
+  int tx = min(x, abs(W - 1 - x)), ty = min(y, abs(H - 1 - y));
+  float level = (float)min(tx, ty);
+  float uu = 0.0f, vv = 0.0f;
+  int32_t id_s = -1;
+  for (uint32_t k = 0; k < I->nobs; ++k) {
+    const efl_obstacle& o = I->obs[k];
+    float lc = obstacle_level(o, pc.x, pc.y);
+    if (lc < level) level = lc;
+    if (lc < 0.5f) id_s = (int32_t)o.entity_id;
+    if (obstacle_level(o, pu.x, pu.y) < 0.5f) {
+      float ry = pu.y - o.transform[13];
+      uu = o.linear_velocity[0] + (-o.angular_velocity * ry);
+    }
+    if (obstacle_level(o, pv.x, pv.y) < 0.5f) {
+      float rx = pv.x - o.transform[12];
+      vv = -(o.linear_velocity[1] + o.angular_velocity * rx);
+    }
+  }
+  I->us[(size_t)y * (W + 1) + x] = uu;
+  I->vs[id] = vv;
+  I->lss[id] = level;
+  I->sid[id] = id_s;
+}
+
+// K5 update_area_fraction (update_area_fraction.wgsl:6-14)
+__global__ __launch_bounds__(NT) void k_area_fraction(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  I->afs[(size_t)s.y * I->W + s.x] = fractions_at(I->lss, I->W, I->H, s.x, s.y);
+}
+
+// ---------------------------------------------------------------------------
+// K6 / K7 advect_u, advect_v (advect_velocity.wgsl:11-52)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_advect_u(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H;
+  float2 q = backtrace(I, (float)s.x + (-0.5f), (float)s.y + 0.0f, I->dt);
+  bool inside = q.x >= 0.0f && q.y >= 0.0f && q.x <= (float)(W + 1) - 1.0f && q.y <= (float)H - 1.0f;
+  size_t id = (size_t)s.y * (W + 1) + s.x;
+  I->u1[id] = inside ? sample_u(I->u0, W + 1, H, q.x, q.y) : I->u0[id];
+}
+__global__ __launch_bounds__(NT) void k_advect_v(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H;
+  float2 q = backtrace(I, (float)s.x + 0.0f, (float)s.y + (-0.5f), I->dt);
+  bool inside = q.x >= 0.0f && q.y >= 0.0f && q.x <= (float)W - 1.0f && q.y <= (float)(H + 1) - 1.0f;
+  size_t id = (size_t)s.y * W + s.x;
+  I->v1[id] = inside ? sample_v(I->v0, W, H + 1, q.x, q.y) : I->v0[id];
+}
+
+// ---------------------------------------------------------------------------
+// K8 / K9 apply_forces_u, apply_forces_v (apply_forces.wgsl:17-76)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float local_force(const DevInst* I, int comp, int x, int y) {
+  const float b = -1.0f / ((2.0f * 10.0f) * 10.0f);
+  float net = comp == 0 ? I->gx : I->gy;
+  for (int n = (int)I->nforces - 1; n >= 0; --n) {
+    const efl_force& f = I->forces[n];
+    float dx = (float)x - f.position[0], dy = (float)y - f.position[1];
+    net = net + f.force[comp] * expf(b * (dx * dx + dy * dy));
+  }
+  return net;
+}
+__global__ __launch_bounds__(NT) void k_forces_u(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H;
+  size_t id = (size_t)s.y * (W + 1) + s.x;
+  float4 fa = fractions_at(I->ls0, W, H, s.x, s.y);
+  float fs = ldz4(I->afs, W, H, s.x, s.y).x;
+  if (fa.x == 1.0f || fs == 0.0f) { I->u1[id] = 0.0f; return; }
+  float net = local_force(I, 0, s.x, s.y);
+  I->u1[id] = I->u1[id] + (net * I->dt) / I->dx;
+}
+__global__ __launch_bounds__(NT) void k_forces_v(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H;
+  size_t id = (size_t)s.y * W + s.x;
+  float4 fa = fractions_at(I->ls0, W, H, s.x, s.y);
+  float fs = ldz4(I->afs, W, H, s.x, s.y).z;
+  if (fa.z == 1.0f || fs == 0.0f) { I->v1[id] = 0.0f; return; }
+  float net = local_force(I, 1, s.x, s.y);
+  I->v1[id] = I->v1[id] + (net * I->dt) / I->dx;
+}
+
+// ---------------------------------------------------------------------------
+// K10 divergence (divergence.wgsl:16-42)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_divergence(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  float4 f = fractions_at(I->lss, W, H, x, y);
+  float du_f = f.y * ldz(I->u1, W + 1, H, x + 1, y) - f.x * ldz(I->u1, W + 1, H, x, y);
+  float dv_f = f.w * ldz(I->v1, W, H + 1, x, y + 1) - f.z * ldz(I->v1, W, H + 1, x, y);
+  float du_s = (1.0f - f.y) * ldz(I->us, W + 1, H, x + 1, y) - (1.0f - f.x) * ldz(I->us, W + 1, H, x, y);
+  float dv_s = (1.0f - f.w) * ldz(I->vs, W, H + 1, x, y + 1) - (1.0f - f.z) * ldz(I->vs, W, H + 1, x, y);
+  I->div[(size_t)y * W + x] = -(((du_f + dv_f) + du_s) + dv_s) / I->dx;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure operator (gauss_seidel.wgsl:44-88, residual.wgsl:12-57)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const Level& level_of(const DevInst* I, int l, int pcg) {
+  return (pcg && l == 0) ? I->lev_pcg0 : I->lev[l];
+}
+
+__device__ __forceinline__ float gs_value(const DevInst* I, const Level& L, int x, int y, float weight, float scale) {
+  size_t id = (size_t)y * L.w + x;
+  float lc = L.ls[id];
+  if (lc >= 0.0f) return 0.0f;
+  float4 f = L.af[id];
+  float p_old = L.x[id];
+  if (all_closed(f)) return 0.0f;
+  float dx = scale * I->dx;
+  float denom = 0.0f;
+  float nume = dx * dx * I->rho / I->dt * L.b[id];
+  const float fv[4] = {f.x, f.y, f.z, f.w};
+  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int jx = x + ox[k], jy = y + oy[k];
+    if (inb(jx, jy, L.w, L.h)) {
+      size_t jd = (size_t)jy * L.w + jx;
+      float lv = L.ls[jd];
+      if (lv < 0.0f) {
+        denom += fv[k];
+        nume += fv[k] * L.x[jd];
+      } else {
+        denom += fv[k] * (1.0f - lv / lc);
+      }
+    }
+  }
+  if (fabsf(denom) < 1.0e-6f) return 0.0f;
+  float pn = nume / denom;
+  return weight * pn + (1.0f - weight) * p_old;
+}
+
+// gauss_seidel_red (parity 1) / gauss_seidel_black (parity 0), :12-42
+__global__ __launch_bounds__(NT) void k_gs_color(Batch b, int parity, float weight, int pcg) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  const Level& L = level_of(I, s.lvl, pcg);
+  if (s.x >= L.w || s.y >= L.h) return;
+  if (((s.x + s.y) & 1) != parity) return;
+  float v = gs_value(I, L, s.x, s.y, weight, (float)(1 << s.lvl));
+  L.x[(size_t)s.y * L.w + s.x] = v;
+}
+
+__device__ __forceinline__ float residual_value(const DevInst* I, const Level& L, int x, int y, float scale) {
+  size_t id = (size_t)y * L.w + x;
+  float phi = L.ls[id];
+  if (phi >= 0.0f) return 0.0f;
+  float4 f = L.af[id];
+  if (all_closed(f)) return 0.0f;
+  float xc = L.x[id];
+  float dx = I->dx * scale;
+  float factor = I->dt / I->rho / dx / dx;
+  float res = L.b[id];
+  const float fv[4] = {f.x, f.y, f.z, f.w};
+  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int jx = x + ox[k], jy = y + oy[k];
+    if (inb(jx, jy, L.w, L.h)) {
+      size_t jd = (size_t)jy * L.w + jx;
+      float pn = L.ls[jd];
+      if (pn < 0.0f) res -= fv[k] * (xc - L.x[jd]) * factor;
+      else res -= fv[k] * (1.0f - pn / phi) * xc * factor;
+    }
+  }
+  return res;
+}
+
+// residual.wgsl:12-57
+__global__ __launch_bounds__(NT) void k_residual(Batch b, int pcg) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  const Level& L = level_of(I, s.lvl, pcg);
+  L.r[(size_t)s.y * L.w + s.x] = residual_value(I, L, s.x, s.y, (float)(1 << s.lvl));
+}
+
+// restriction.wgsl:10-60, level lvl -> lvl + 1 (extent = coarse grid)
+__global__ __launch_bounds__(NT) void k_restrict(Batch b, int pcg) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  const Level& F = level_of(I, s.lvl, pcg);
+  const Level& Cc = I->lev[s.lvl + 1];
+  int x = s.x, y = s.y, fx = 2 * x, fy = 2 * y;
+  float br = 0.25f * (((ldz(F.r, F.w, F.h, fx, fy) + ldz(F.r, F.w, F.h, fx, fy + 1)) + ldz(F.r, F.w, F.h, fx + 1, fy)) +
+                      ldz(F.r, F.w, F.h, fx + 1, fy + 1));
+  float q0 = ldz(F.ls, F.w, F.h, fx, fy), q1 = ldz(F.ls, F.w, F.h, fx, fy + 1);
+  float q2 = ldz(F.ls, F.w, F.h, fx + 1, fy), q3 = ldz(F.ls, F.w, F.h, fx + 1, fy + 1);
+  float pr = q0;
+  if (fabsf(q1) < fabsf(pr)) pr = q1;
+  if (fabsf(q2) < fabsf(pr)) pr = q2;
+  if (fabsf(q3) < fabsf(pr)) pr = q3;
+  float4 a00 = ldz4(F.af, F.w, F.h, fx, fy), a10 = ldz4(F.af, F.w, F.h, fx + 1, fy);
+  float4 a01 = ldz4(F.af, F.w, F.h, fx, fy + 1), a11 = ldz4(F.af, F.w, F.h, fx + 1, fy + 1);
+  size_t id = (size_t)y * Cc.w + x;
+  Cc.af[id] = make_float4(0.5f * (a00.x + a01.x), 0.5f * (a10.y + a11.y), 0.5f * (a00.z + a10.z), 0.5f * (a01.w + a11.w));
+  Cc.b[id] = br;
+  Cc.ls[id] = pr;
+  Cc.x[id] = 0.0f;
+}
+
+// prolongation.wgsl:5-34, level lvl + 1 -> lvl (extent = coarse grid)
+__global__ __launch_bounds__(NT) void k_prolong(Batch b, int pcg) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  const Level& F = level_of(I, s.lvl, pcg);
+  const Level& Cc = I->lev[s.lvl + 1];
+  float corr = Cc.x[(size_t)s.y * Cc.w + s.x];
+  const int ox[4] = {0, 1, 0, 1}, oy[4] = {0, 0, 1, 1};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int fx = 2 * s.x + ox[k], fy = 2 * s.y + oy[k];
+    if (fx < F.w && fy < F.h) {
+      size_t fd = (size_t)fy * F.w + fx;
+      if (F.ls[fd] < 0.0f) F.x[fd] = corr + F.x[fd];
+    }
+  }
+}
+
+// K16 / K17 jacobi_iteration (+ REVERSE), jacobi_iteration.wgsl:18-91
+__global__ __launch_bounds__(NT) void k_jacobi(Batch b, int reverse) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  const float* src = reverse ? I->p1 : I->p0;
+  float* dst = reverse ? I->p0 : I->p1;
+  size_t id = (size_t)y * W + x;
+  float lc = I->ls0[id];
+  if (lc >= 0.0f) { dst[id] = 0.0f; return; }
+  float4 f = fractions_at(I->lss, W, H, x, y);
+  if (all_closed(f)) { dst[id] = src[id]; return; }
+  float pm = ldz(src, W, H, x - 1, y), pp = ldz(src, W, H, x + 1, y);
+  float qm = ldz(src, W, H, x, y - 1), qp = ldz(src, W, H, x, y + 1);
+  float ffm = fminf(0.0f, ldz(I->ls0, W, H, x - 1, y) / lc);
+  float ffp = fminf(0.0f, ldz(I->ls0, W, H, x + 1, y) / lc);
+  float fgm = fminf(0.0f, ldz(I->ls0, W, H, x, y - 1) / lc);
+  float fgp = fminf(0.0f, ldz(I->ls0, W, H, x, y + 1) / lc);
+  float coef = ((f.x * (1.0f - ffm) + f.y * (1.0f - ffp)) + f.z * (1.0f - fgm)) + f.w * (1.0f - fgp);
+  if (fabsf(coef) < 1.0e-6f) { dst[id] = 0.0f; return; }
+  float factor = I->dx * I->rho / I->dt;
+  auto stepf = [](float v) { return v >= 0.0f ? 1.0f : 0.0f; };
+  float a = f.x * stepf(ffm) * pm, bb = f.y * stepf(ffp) * pp;
+  float c = f.z * stepf(fgm) * qm, d = f.w * stepf(fgp) * qp;
+  dst[id] = ((((a + bb) + c) + d) - factor * I->div[id]) / coef;
+}
+
+// ---------------------------------------------------------------------------
+// K18 / K19 solve_velocity_u, solve_velocity_v
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_solve_u(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  size_t id = (size_t)y * (W + 1) + x;
+  float factor = I->dt / (I->dx * I->rho);
+  if (ldz4(I->afs, W, H, x, y).x == 0.0f) { I->u0[id] = I->us[id]; return; }
+  float pij = ldz(I->p0, W, H, x, y), pim = ldz(I->p0, W, H, x - 1, y);
+  float lp = ldz(I->ls0, W, H, x, y), lm = ldz(I->ls0, W, H, x - 1, y);
+  if (lm >= 0.0f && lp < 0.0f) pim = lm / lp * pij;
+  else if (lm < 0.0f && lp >= 0.0f) pij = lp / lm * pim;
+  else if (lm >= 0.0f && lp >= 0.0f) { I->u0[id] = 0.0f; return; }
+  I->u0[id] = I->u1[id] - factor * (pij - pim);
+}
+__global__ __launch_bounds__(NT) void k_solve_v(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  size_t id = (size_t)y * W + x;
+  float factor = I->dt / (I->dx * I->rho);
+  if (ldz4(I->afs, W, H, x, y).z == 0.0f) { I->v0[id] = I->vs[id]; return; }
+  float pij = ldz(I->p0, W, H, x, y), pjm = ldz(I->p0, W, H, x, y - 1);
+  float lp = ldz(I->ls0, W, H, x, y), lm = ldz(I->ls0, W, H, x, y - 1);
+  if (lm >= 0.0f && lp < 0.0f) pjm = lm / lp * pij;
+  else if (lm < 0.0f && lp >= 0.0f) pij = lp / lm * pjm;
+  I->v0[id] = I->v1[id] - factor * (pij - pjm);
+}
+
+// ---------------------------------------------------------------------------
+// K20 - K23 velocity extrapolation (extrapolate/*.wgsl)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_valid_u(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  const float* L = I->ls0;
+  float c0 = ldz(L, W, H, x - 1, y - 1), c1 = ldz(L, W, H, x, y - 1), c2 = ldz(L, W, H, x - 1, y);
+  float c3 = ldz(L, W, H, x, y), c4 = ldz(L, W, H, x - 1, y + 1), c5 = ldz(L, W, H, x, y + 1);
+  float vm = 0.25f * (((c0 + c1) + c2) + c3), vp = 0.25f * (((c2 + c3) + c4) + c5);
+  I->uval0[(size_t)y * (W + 1) + x] = face_open(vm, vp) > 0.0f ? 0 : 1;
+}
+__global__ __launch_bounds__(NT) void k_valid_v(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  const float* L = I->ls0;
+  float c0 = ldz(L, W, H, x - 1, y - 1), c1 = ldz(L, W, H, x - 1, y), c2 = ldz(L, W, H, x, y - 1);
+  float c3 = ldz(L, W, H, x, y), c4 = ldz(L, W, H, x + 1, y - 1), c5 = ldz(L, W, H, x + 1, y);
+  float vm = 0.25f * (((c0 + c1) + c2) + c3), vp = 0.25f * (((c2 + c3) + c4) + c5);
+  I->vval0[(size_t)y * W + x] = face_open(vm, vp) > 0.0f ? 0 : 1;
+}
+// extrapolate_u / extrapolate_v (identical bodies): comp 0 = u, 1 = v;
+// dir 0 = valid0 -> valid1, dir 1 = valid1 -> valid0 (reverse bind group)
+__global__ __launch_bounds__(NT) void k_extrapolate(Batch b, int comp, int dir) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int w = comp == 0 ? I->W + 1 : I->W;
+  int h = comp == 0 ? I->H : I->H + 1;
+  float* f = comp == 0 ? I->u0 : I->v0;
+  const int32_t* vin = comp == 0 ? (dir ? I->uval1 : I->uval0) : (dir ? I->vval1 : I->vval0);
+  int32_t* vout = comp == 0 ? (dir ? I->uval0 : I->uval1) : (dir ? I->vval0 : I->vval1);
+  int x = s.x, y = s.y;
+  size_t id = (size_t)y * w + x;
+  if (vin[id] == 1) { vout[id] = 1; return; }
+  int count = 0;
+  float nu = 0.0f;
+  if (x + 1 < w && y < h && vin[id + 1] == 1) { nu += f[id + 1]; count += 1; }
+  if (x - 1 > 0 && y > 0 && vin[id - 1] == 1) { nu += f[id - 1]; count += 1; }
+  if (x < w && y + 1 < h && vin[id + w] == 1) { nu += f[id + w]; count += 1; }
+  if (x > 0 && y - 1 > 0 && vin[id - w] == 1) { nu += f[id - w]; count += 1; }
+  if (count > 0) {
+    nu /= (float)count;
+    f[id] = nu;
+    vout[id] = 1;
+  } else {
+    vout[id] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K24 advect_levelset with CUBIC (advect_levelset.wgsl:11-74)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float catmull(float y0, float y1, float y2, float y3, float t) {
+  float d1 = 0.5f * (y2 - y0), d2 = 0.5f * (y3 - y1);
+  float a2 = ((-2.0f * d1) - d2) + 3.0f * (y2 - y1);
+  float a3 = (d1 + d2) - 2.0f * (y2 - y1);
+  return (((a3 * t * t * t) + (a2 * t * t)) + (d1 * t)) + y1;
+}
+__global__ __launch_bounds__(NT) void k_advect_levelset(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H;
+  float2 q = backtrace(I, (float)s.x, (float)s.y, I->dt);
+  float nx = fminf(fmaxf(q.x, 0.0f), (float)W - 1.0f);
+  float ny = fminf(fmaxf(q.y, 0.0f), (float)H - 1.0f);
+  int bx = to_i32(nx), by = to_i32(ny);
+  float tx = nx - (float)bx, ty = ny - (float)by;
+  float r[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int yy = by - 1 + k;
+    r[k] = catmull(ldz(I->ls0, W, H, bx - 1, yy), ldz(I->ls0, W, H, bx, yy), ldz(I->ls0, W, H, bx + 1, yy),
+                   ldz(I->ls0, W, H, bx + 2, yy), tx);
+  }
+  float nl = catmull(r[0], r[1], r[2], r[3], ty);
+  if (fabsf(nl) < 1000.0f) I->ls1[(size_t)s.y * W + s.x] = nl;
+}
+
+// ---------------------------------------------------------------------------
+// K25 - K27 jump flooding (reinitialize_levelset/shaders/jump_flooding/*.wgsl)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_jfa_init(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  float level = I->ls1[(size_t)y * W + x];
+  float md = 10.0f;
+  float2 seed = make_float2(-1.0f, -1.0f);
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j) {
+      if (i == 0 && j == 0) continue;
+      int nx = x + i, ny = y + j;
+      if (!inb(nx, ny, W, H)) continue;
+      float nl = I->ls1[(size_t)ny * W + nx];
+      if (nl >= 0.0f && !(level >= 0.0f)) {
+        float d = level / (level - nl);
+        if (fabsf(d) < md) {
+          md = fabsf(d);
+          seed = make_float2((float)x + d * (float)i, (float)y + d * (float)j);
+        }
+      }
+    }
+  I->seed0[(size_t)y * W + x] = seed;
+}
+__device__ __forceinline__ bool seed_ok(float2 q) { return q.x != -1.0f && q.y != -1.0f; }
+__global__ __launch_bounds__(NT) void k_jfa_step(Batch b, int step) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  const int src_idx = s.lvl;  // per-instance ping-pong index
+  const float2* si = src_idx ? I->seed1 : I->seed0;
+  float2* so = src_idx ? I->seed0 : I->seed1;
+  float2 cur = si[(size_t)y * W + x];
+  float2 best = cur;
+  float bd = 1e10f;
+  if (seed_ok(cur)) bd = hyp(cur.x - (float)x, cur.y - (float)y);
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j) {
+      if (i == 0 && j == 0) continue;
+      int nx = x + i * step, ny = y + j * step;
+      if (!inb(nx, ny, W, H)) continue;
+      float2 q = si[(size_t)ny * W + nx];
+      if (!seed_ok(q)) continue;
+      float d = hyp(q.x - (float)x, q.y - (float)y);
+      if (d < bd) { best = q; bd = d; }
+    }
+  so[(size_t)y * W + x] = best;
+}
+__global__ __launch_bounds__(NT) void k_jfa_sdf(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  size_t id = (size_t)s.y * I->W + s.x;
+  float2 q = (s.lvl ? I->seed1 : I->seed0)[id];
+  float sdf = hyp(q.x - (float)s.x, q.y - (float)s.y);
+  I->ls0[id] = sdf * (I->ls1[id] < 0.0f ? -1.0f : 1.0f);
+}
+
+// ---------------------------------------------------------------------------
+// K28 - K30 fast iterative method (reinitialize_levelset/shaders/fast_iterative_method)
+// Labels: 0 NONE, 1 SOURCE, 2 ACTIVE.  The update kernel of the reference
+// races on neighbours; it runs here as two phases per iteration with every
+// read in a phase seeing the state at the start of the phase:
+//   A: ACTIVE cells solve their quadratic; converged ones are marked (3).
+//   B: NONE cells next to a cell marked in A (first in -x, +x, -y, +y order)
+//      solve on the phase-A values and become ACTIVE if that lowers |phi|.
+// ---------------------------------------------------------------------------
+constexpr uint32_t L_NONE = 0u, L_SOURCE = 1u, L_ACTIVE = 2u, L_CONVERGED = 3u;
+constexpr float FIM_FAR = 1e30f;
+
+__global__ __launch_bounds__(NT) void k_fim_init(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  size_t id = (size_t)y * W + x;
+  float level = I->ls1[id];
+  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+  bool src = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int nx = x + ox[k], ny = y + oy[k];
+    if (!src && inb(nx, ny, W, H) && level * I->ls1[(size_t)ny * W + nx] <= 0.0f) src = true;
+  }
+  I->ls0[id] = src ? level : sgn(level) * FIM_FAR;
+  I->lab0[id] = src ? L_SOURCE : L_NONE;
+}
+__global__ __launch_bounds__(NT) void k_fim_active(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  size_t id = (size_t)y * W + x;
+  if (I->lab0[id] == L_SOURCE) { I->lab[id] = L_SOURCE; return; }
+  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+  uint32_t out = L_NONE;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int nx = x + ox[k], ny = y + oy[k];
+    if (inb(nx, ny, W, H) && I->lab0[(size_t)ny * W + nx] == L_SOURCE) out = L_ACTIVE;
+  }
+  I->lab[id] = out;
+}
+__device__ __forceinline__ float far_abs(const float* phi, int W, int H, int x, int y) {
+  return inb(x, y, W, H) ? fabsf(phi[(size_t)y * W + x]) : FIM_FAR;
+}
+__device__ __forceinline__ float eikonal(const float* phi, int W, int H, int x, int y) {
+  const float R2 = 1.41421354f;  // sqrt(2.0) rounded to f32
+  float xm = fminf(far_abs(phi, W, H, x - 1, y), far_abs(phi, W, H, x + 1, y));
+  float ym = fminf(far_abs(phi, W, H, x, y - 1), far_abs(phi, W, H, x, y + 1));
+  float d = xm - ym;
+  if (d > R2) return ym + 1.0f;
+  if (d < -R2) return xm + 1.0f;
+  return 0.5f * ((xm + ym) + sqrtf(2.0f - d * d));
+}
+// phase A: ls0/lab -> phi_tmp/lab_tmp
+__global__ __launch_bounds__(NT) void k_fim_phase_a(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  size_t id = (size_t)y * W + x;
+  uint32_t l = I->lab[id];
+  float p = I->ls0[id];
+  if (l != L_ACTIVE) {
+    I->phi_tmp[id] = p;
+    I->lab_tmp[id] = l;
+    return;
+  }
+  float q = eikonal(I->ls0, W, H, x, y) * sgn(p);
+  I->phi_tmp[id] = q;
+  I->lab_tmp[id] = (fabsf(p - q) > 1e-6f) ? L_ACTIVE : L_CONVERGED;
+}
+// phase B: phi_tmp/lab_tmp (+ ls0 for the trigger's sign) -> ls0/lab
+__global__ __launch_bounds__(NT) void k_fim_phase_b(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  size_t id = (size_t)y * W + x;
+  uint32_t l = I->lab_tmp[id];
+  if (l == L_CONVERGED) { I->lab[id] = L_NONE; I->ls0[id] = I->phi_tmp[id]; return; }
+  if (l != L_NONE) { I->ls0[id] = I->phi_tmp[id]; return; }
+  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+  int trig = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int nx = x + ox[k], ny = y + oy[k];
+    if (trig < 0 && inb(nx, ny, W, H) && I->lab_tmp[(size_t)ny * W + nx] == L_CONVERGED) trig = k;
+  }
+  float pa = I->phi_tmp[id];
+  if (trig < 0) { I->ls0[id] = pa; return; }
+  float qn = eikonal(I->phi_tmp, W, H, x, y);
+  if (fabsf(pa) > qn) {
+    // The trigger's phase-A value has the sign of its pre-A value (the
+    // eikonal solution is > 0), so phi_tmp gives sign(p) without reading
+    // ls0, which this phase is writing.
+    float sg = sgn(I->phi_tmp[(size_t)(y + oy[trig]) * W + (x + ox[trig])]);
+    I->ls0[id] = qn * sg;
+    I->lab[id] = L_ACTIVE;
+  } else {
+    I->ls0[id] = pa;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K31 levelset_gradient (levelset_gradient.wgsl:4-21)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_ls_gradient(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  if (x == 0 || y == 0) return;
+  float gx = 0.5f * (ldz(I->ls0, W, H, x + 1, y) - ldz(I->ls0, W, H, x - 1, y));
+  float gy = 0.5f * (ldz(I->ls0, W, H, x, y + 1) - ldz(I->ls0, W, H, x, y - 1));
+  I->grad[(size_t)y * W + x] = make_float2(gx, gy);
+}
+
+// ---------------------------------------------------------------------------
+// K32 / K33 fluid -> solid forces (fluid_to_solid/*.wgsl)
+// ---------------------------------------------------------------------------
+__global__ void k_clear_bins(Batch b) {
+  const DevInst* I = b.inst[blockIdx.x];
+  for (int i = threadIdx.x; i < 3 * EFL_MAX_SOLIDS; i += blockDim.x) I->bins[i] = 0;
+}
+__device__ __forceinline__ int fixed_point(float v) { return to_i32(v * (1.0f / 0.1f)); }
+__global__ __launch_bounds__(NT) void k_sample_forces(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  float4 f = fractions_at(I->lss, W, H, x, y);
+  if (all_closed(f)) return;
+  size_t id = (size_t)y * W + x;
+  int sid = I->sid[id];
+  if (sid == -1) return;
+  float p = I->p0[id];
+  float fx = ((-(f.y - f.x)) * p) * I->dx;
+  float fy = ((-(f.w - f.z)) * p) * I->dx;
+  float cx = 0.0f, cy = 0.0f;
+  if (sid >= 0 && (uint32_t)sid < I->nobs) {
+    const efl_obstacle& o = I->obs[sid];
+    if (o.shape == 4u) {
+      float lx = ((o.values[0] + o.values[2]) + o.values[4]) / 3.0f;
+      float ly = ((o.values[1] + o.values[3]) + o.values[5]) / 3.0f;
+      float2 c = xform(o.transform, lx, ly);
+      cx = c.x; cy = c.y;
+    } else if (o.shape <= 2u) {
+      cx = o.transform[12]; cy = o.transform[13];
+    }
+  }
+  float ux = (float)x / I->sizex, uy = (float)y / I->sizey;
+  float2 w = xform(I->T, (ux - 0.5f) * I->sizex, (-uy + 0.5f) * I->sizey);
+  float rx = w.x - cx, ry = w.y - cy;
+  float tq = rx * fy - ry * fx;
+  if (sid >= 0 && sid < EFL_MAX_SOLIDS) {
+    atomicAdd(&I->bins[sid], fixed_point(fx));
+    atomicAdd(&I->bins[EFL_MAX_SOLIDS + sid], fixed_point(fy));
+    atomicAdd(&I->bins[2 * EFL_MAX_SOLIDS + sid], fixed_point(tq));
+  }
+}
+__global__ void k_accumulate_forces(Batch b) {
+  const DevInst* I = b.inst[blockIdx.x];
+  for (int i = threadIdx.x; i < EFL_MAX_SOLIDS; i += blockDim.x)
+    I->fts[i] = make_float4((float)I->bins[i] * 0.1f, (float)I->bins[EFL_MAX_SOLIDS + i] * 0.1f,
+                            (float)I->bins[2 * EFL_MAX_SOLIDS + i] * 0.1f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (deterministic: fixed per-block tree, then the last block sums
+// the block partials in index order).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool active_at(const DevInst* I, size_t id) {
+  return I->ls0[id] < 0.0f && !all_closed(I->afs[id]);
+}
+
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double (*sh)[NT]) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = sh[k][0];
+}
+
+// Sum N partials of this instance's blocks; the last block to finish folds
+// them (in block order) into scal[dst..dst+N) and runs `fin`.
+template <int N, class Fin>
+__device__ void finish_sums(const DevInst* I, int nblk, int lb, double (&v)[N], int dst, Fin fin) {
+  __shared__ double sh[N][NT];
+  __shared__ bool last;
+  block_sum<N>(v, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) I->red[(size_t)k * I->red_stride + lb] = v[k];
+    __threadfence();
+    uint32_t prev = atomicAdd(I->counter, 1u);
+    last = (prev == (uint32_t)nblk - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double acc[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = 0.0;
+  // threads sum strided chunks, then a fixed tree: deterministic for fixed nblk
+  for (int i = threadIdx.x; i < nblk; i += NT) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] += ((volatile double*)I->red)[(size_t)k * I->red_stride + i];
+  }
+  block_sum<N>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) I->scal[dst + k] = acc[k];
+    fin(I);
+    *I->counter = 0u;
+  }
+}
+
+__device__ __forceinline__ int inst_blocks(const Batch& b, int slot) { return b.start[slot + 1] - b.start[slot]; }
+__device__ __forceinline__ int local_block(const Batch& b, int slot) { return (int)blockIdx.x - b.start[slot]; }
+
+// rr = ||r||^2 and bb = ||b||^2 over active cells, with r = b - A x for the
+// finest level (x = p0, b = div); writes r into pcg_r when pcg != 0.
+__global__ __launch_bounds__(NT) void k_residual_norm(Batch b, int pcg_init) {
+  Site s = locate(b);
+  const DevInst* I = s.I;
+  double v[2] = {0.0, 0.0};
+  if (s.ok) {
+    const Level& L = I->lev[0];
+    size_t id = (size_t)s.y * L.w + s.x;
+    bool act = active_at(I, id);
+    float r = residual_value(I, L, s.x, s.y, 1.0f);
+    if (!act) r = 0.0f;
+    if (pcg_init) I->pcg_r[id] = r;
+    float bv = act ? L.b[id] : 0.0f;
+    v[0] = (double)r * (double)r;
+    v[1] = (double)bv * (double)bv;
+  }
+  finish_sums<2>(I, inst_blocks(b, s.slot), local_block(b, s.slot), v, 2, [](const DevInst*) {});
+}
+
+// z masked to active cells, rz = r . z, then (first = 1) p = z or
+// p = z + (rz / rz_old) p
+__global__ __launch_bounds__(NT) void k_pcg_rz(Batch b) {
+  Site s = locate(b);
+  const DevInst* I = s.I;
+  double v[1] = {0.0};
+  if (s.ok) {
+    size_t id = (size_t)s.y * I->W + s.x;
+    float z = I->pcg_z[id];
+    if (!active_at(I, id)) { z = 0.0f; I->pcg_z[id] = 0.0f; }
+    v[0] = (double)I->pcg_r[id] * (double)z;
+  }
+  finish_sums<1>(I, inst_blocks(b, s.slot), local_block(b, s.slot), v, 7, [](const DevInst* J) {
+    double rz_new = J->scal[7];
+    double rz_old = J->scal[0];
+    J->scal[5] = (J->scal[8] != 0.0) ? 0.0 : rz_new / rz_old;  // beta (0 on the first iteration)
+    J->scal[8] = 0.0;
+    J->scal[0] = rz_new;
+  });
+}
+__global__ __launch_bounds__(NT) void k_pcg_update_p(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  size_t id = (size_t)s.y * I->W + s.x;
+  float beta = (float)I->scal[5];
+  I->pcg_p[id] = I->pcg_z[id] + beta * I->pcg_p[id];
+}
+// q = A p, pq = p . q, alpha = rz / pq
+__global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
+  Site s = locate(b);
+  const DevInst* I = s.I;
+  double v[1] = {0.0};
+  if (s.ok) {
+    Level L = I->lev[0];
+    L.x = I->pcg_p;
+    L.b = nullptr;
+    size_t id = (size_t)s.y * L.w + s.x;
+    float q = 0.0f;
+    if (active_at(I, id)) {
+      // -(residual with b = 0): the operator of residual.wgsl
+      float xc = L.x[id];
+      float4 f = L.af[id];
+      float phi = L.ls[id];
+      float factor = I->dt / I->rho / I->dx / I->dx;
+      float res = 0.0f;
+      const float fv[4] = {f.x, f.y, f.z, f.w};
+      const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        int jx = s.x + ox[k], jy = s.y + oy[k];
+        if (inb(jx, jy, L.w, L.h)) {
+          size_t jd = (size_t)jy * L.w + jx;
+          float pn = L.ls[jd];
+          if (pn < 0.0f) res -= fv[k] * (xc - L.x[jd]) * factor;
+          else res -= fv[k] * (1.0f - pn / phi) * xc * factor;
+        }
+      }
+      q = -res;
+    }
+    I->pcg_q[id] = q;
+    v[0] = (double)L.x[id] * (double)q;
+  }
+  finish_sums<1>(I, inst_blocks(b, s.slot), local_block(b, s.slot), v, 1, [](const DevInst* J) {
+    double pq = J->scal[1];
+    J->scal[4] = pq > 0.0 ? J->scal[0] / pq : 0.0;  // alpha
+  });
+}
+// x += alpha p, r -= alpha q, rr = ||r||^2
+__global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
+  Site s = locate(b);
+  const DevInst* I = s.I;
+  double v[1] = {0.0};
+  if (s.ok) {
+    size_t id = (size_t)s.y * I->W + s.x;
+    float alpha = (float)I->scal[4];
+    I->p0[id] = I->p0[id] + alpha * I->pcg_p[id];
+    float r = I->pcg_r[id] - alpha * I->pcg_q[id];
+    I->pcg_r[id] = r;
+    v[0] = (double)r * (double)r;
+  }
+  finish_sums<1>(I, inst_blocks(b, s.slot), local_block(b, s.slot), v, 2, [](const DevInst*) {});
+}
+__global__ __launch_bounds__(NT) void k_zero(Batch b, int which) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  size_t id = (size_t)s.y * I->W + s.x;
+  if (which == 0) I->pcg_z[id] = 0.0f;
+  else if (!active_at(I, id)) I->p0[id] = 0.0f;  // which == 1: mask the initial guess
+}
+__global__ void k_pcg_first(Batch b) {
+  const DevInst* I = b.inst[blockIdx.x];
+  if (threadIdx.x == 0) I->scal[8] = 1.0;  // next k_pcg_rz sets beta = 0
+}
+
+// max over active cells of |D| (the K10 numerator) on u0/v0
+__global__ __launch_bounds__(NT) void k_max_divergence(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  const DevInst* I = s.I;
+  int W = I->W, H = I->H, x = s.x, y = s.y;
+  size_t id = (size_t)y * W + x;
+  float d = 0.0f;
+  if (I->ls0[id] < 0.0f) {
+    float4 f = fractions_at(I->lss, W, H, x, y);
+    if (!all_closed(f)) {
+      float du_f = f.y * ldz(I->u0, W + 1, H, x + 1, y) - f.x * ldz(I->u0, W + 1, H, x, y);
+      float dv_f = f.w * ldz(I->v0, W, H + 1, x, y + 1) - f.z * ldz(I->v0, W, H + 1, x, y);
+      float du_s = (1.0f - f.y) * ldz(I->us, W + 1, H, x + 1, y) - (1.0f - f.x) * ldz(I->us, W + 1, H, x, y);
+      float dv_s = (1.0f - f.w) * ldz(I->vs, W, H + 1, x, y + 1) - (1.0f - f.z) * ldz(I->vs, W, H + 1, x, y);
+      d = fabsf(((du_f + dv_f) + du_s) + dv_s);
+    }
+  }
+  // wave-level max, then one atomic per wave (|D| >= 0: float order == int order)
+  for (int off = 32; off > 0; off >>= 1) d = fmaxf(d, __shfl_xor(d, off, 64));
+  if ((threadIdx.x & 63) == 0 && d > 0.0f) atomicMax((int*)&I->diag[0], __float_as_int(d));
+}
+__global__ void k_clear_diag(Batch b) {
+  const DevInst* I = b.inst[blockIdx.x];
+  if (threadIdx.x < 4) I->diag[threadIdx.x] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+static inline int nblocks(const Batch& b) { return b.start[b.n]; }
+#define EFL_LAUNCH(kern, b, st, ...)                                                   \
+  do {                                                                                 \
+    int nb_ = nblocks(b);                                                              \
+    if (nb_ > 0) hipLaunchKernelGGL(kern, dim3(nb_), dim3(NT), 0, st, b, ##__VA_ARGS__); \
+  } while (0)
+#define EFL_LAUNCH_PER_INST(kern, b, st, ...)                                          \
+  do {                                                                                 \
+    if ((b).n > 0) hipLaunchKernelGGL(kern, dim3((b).n), dim3(256), 0, st, b, ##__VA_ARGS__); \
+  } while (0)
+
+void launch_init_center(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_init_center, b, st); }
+void launch_init_edge(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_init_edge, b, st); }
+void launch_fluid_source(hipStream_t st, const Batch& b, int oneshot) { EFL_LAUNCH(k_fluid_source, b, st, oneshot); }
+void launch_update_solid(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_update_solid, b, st); }
+void launch_area_fraction(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_area_fraction, b, st); }
+void launch_advect_u(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advect_u, b, st); }
+void launch_advect_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advect_v, b, st); }
+void launch_forces_u(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_forces_u, b, st); }
+void launch_forces_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_forces_v, b, st); }
+void launch_divergence(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_divergence, b, st); }
+void launch_gs_color(hipStream_t st, const Batch& b, int parity, float weight, int pcg) {
+  EFL_LAUNCH(k_gs_color, b, st, parity, weight, pcg);
+}
+void launch_residual(hipStream_t st, const Batch& b, int pcg) { EFL_LAUNCH(k_residual, b, st, pcg); }
+void launch_restrict(hipStream_t st, const Batch& b, int pcg) { EFL_LAUNCH(k_restrict, b, st, pcg); }
+void launch_prolong(hipStream_t st, const Batch& b, int pcg) { EFL_LAUNCH(k_prolong, b, st, pcg); }
+void launch_jacobi(hipStream_t st, const Batch& b, int reverse) { EFL_LAUNCH(k_jacobi, b, st, reverse); }
+void launch_solve_u(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_solve_u, b, st); }
+void launch_solve_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_solve_v, b, st); }
+void launch_valid_u(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_valid_u, b, st); }
+void launch_valid_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_valid_v, b, st); }
+void launch_extrapolate(hipStream_t st, const Batch& b, int comp, int dir) { EFL_LAUNCH(k_extrapolate, b, st, comp, dir); }
+void launch_advect_levelset(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advect_levelset, b, st); }
+void launch_jfa_init(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_jfa_init, b, st); }
+void launch_jfa_step(hipStream_t st, const Batch& b, int step) { EFL_LAUNCH(k_jfa_step, b, st, step); }
+void launch_jfa_sdf(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_jfa_sdf, b, st); }
+void launch_fim_init(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_init, b, st); }
+void launch_fim_active(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_active, b, st); }
+void launch_fim_phase_a(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_phase_a, b, st); }
+void launch_fim_phase_b(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_phase_b, b, st); }
+void launch_ls_gradient(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_ls_gradient, b, st); }
+void launch_clear_bins(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_clear_bins, b, st); }
+void launch_sample_forces(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_sample_forces, b, st); }
+void launch_accumulate_forces(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_accumulate_forces, b, st); }
+void launch_residual_norm(hipStream_t st, const Batch& b, int pcg_init) { EFL_LAUNCH(k_residual_norm, b, st, pcg_init); }
+void launch_pcg_rz(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_rz, b, st); }
+void launch_pcg_update_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_update_p, b, st); }
+void launch_pcg_apply(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_apply, b, st); }
+void launch_pcg_axpy(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_axpy, b, st); }
+void launch_zero_z(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_zero, b, st, 0); }
+void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_zero, b, st, 1); }
+void launch_pcg_first(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_pcg_first, b, st); }
+void launch_max_divergence(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_max_divergence, b, st); }
+void launch_clear_diag(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_clear_diag, b, st); }
+
+int tile_w() { return TX; }
+int tile_h() { return TY; }
+
+}  // namespace efl

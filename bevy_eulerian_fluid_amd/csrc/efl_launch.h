@@ -1,0 +1,56 @@
+// efl_launch.h - host-side launchers of the HIP kernels (efl_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "efl_internal.h"
+
+namespace efl {
+
+int tile_w();
+int tile_h();
+
+void launch_init_center(hipStream_t st, const Batch& b);
+void launch_init_edge(hipStream_t st, const Batch& b);
+void launch_fluid_source(hipStream_t st, const Batch& b, int oneshot);
+void launch_update_solid(hipStream_t st, const Batch& b);
+void launch_area_fraction(hipStream_t st, const Batch& b);
+void launch_advect_u(hipStream_t st, const Batch& b);
+void launch_advect_v(hipStream_t st, const Batch& b);
+void launch_forces_u(hipStream_t st, const Batch& b);
+void launch_forces_v(hipStream_t st, const Batch& b);
+void launch_divergence(hipStream_t st, const Batch& b);
+void launch_gs_color(hipStream_t st, const Batch& b, int parity, float weight, int pcg);
+void launch_residual(hipStream_t st, const Batch& b, int pcg);
+void launch_restrict(hipStream_t st, const Batch& b, int pcg);
+void launch_prolong(hipStream_t st, const Batch& b, int pcg);
+void launch_jacobi(hipStream_t st, const Batch& b, int reverse);
+void launch_solve_u(hipStream_t st, const Batch& b);
+void launch_solve_v(hipStream_t st, const Batch& b);
+void launch_valid_u(hipStream_t st, const Batch& b);
+void launch_valid_v(hipStream_t st, const Batch& b);
+void launch_extrapolate(hipStream_t st, const Batch& b, int comp, int dir);
+void launch_advect_levelset(hipStream_t st, const Batch& b);
+void launch_jfa_init(hipStream_t st, const Batch& b);
+void launch_jfa_step(hipStream_t st, const Batch& b, int step);  // Batch.level = source seed buffer
+void launch_jfa_sdf(hipStream_t st, const Batch& b);  // Batch.level = final seed buffer
+void launch_fim_init(hipStream_t st, const Batch& b);
+void launch_fim_active(hipStream_t st, const Batch& b);
+void launch_fim_phase_a(hipStream_t st, const Batch& b);
+void launch_fim_phase_b(hipStream_t st, const Batch& b);
+void launch_ls_gradient(hipStream_t st, const Batch& b);
+void launch_clear_bins(hipStream_t st, const Batch& b);
+void launch_sample_forces(hipStream_t st, const Batch& b);
+void launch_accumulate_forces(hipStream_t st, const Batch& b);
+void launch_residual_norm(hipStream_t st, const Batch& b, int pcg_init);
+void launch_pcg_rz(hipStream_t st, const Batch& b);
+void launch_pcg_update_p(hipStream_t st, const Batch& b);
+void launch_pcg_apply(hipStream_t st, const Batch& b);
+void launch_pcg_axpy(hipStream_t st, const Batch& b);
+void launch_zero_z(hipStream_t st, const Batch& b);
+void launch_mask_p(hipStream_t st, const Batch& b);
+void launch_pcg_first(hipStream_t st, const Batch& b);
+void launch_max_divergence(hipStream_t st, const Batch& b);
+void launch_clear_diag(hipStream_t st, const Batch& b);
+
+}  // namespace efl

@@ -1,0 +1,253 @@
+"""ctypes binding of libefl.so (the HIP library behind include/efl.h).
+
+This module is the only path from Python to the fluid pipeline: there is no
+CPU fallback.  If libefl.so is missing or cannot be loaded the import of a
+context fails with NativeLibraryError naming the build command.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libefl.so")
+_lib = None
+
+EXPORTS = [
+    "efl_ctx_create", "efl_ctx_destroy", "efl_last_error", "efl_sync", "efl_set_timing",
+    "efl_set_obstacles", "efl_instance_create", "efl_instance_destroy", "efl_set_projection",
+    "efl_set_reinit", "efl_set_uniforms", "efl_set_sources", "efl_set_forces", "efl_init_levelset",
+    "efl_reset", "efl_initialize", "efl_step", "efl_run_stage", "efl_field_info", "efl_get_field",
+    "efl_set_field", "efl_field_device_ptr", "efl_get_solid_forces", "efl_set_diagnostics",
+    "efl_measure", "efl_get_stats",
+]
+
+
+class NativeLibraryError(RuntimeError):
+    pass
+
+
+class EflError(RuntimeError):
+    def __init__(self, code, what, detail=""):
+        self.code = code
+        super().__init__(f"{what} failed: {abi.STATUS_NAMES.get(code, code)} {detail}".strip())
+
+
+def load():
+    """Load libefl.so and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeLibraryError(
+            f"{LIB_PATH} is missing; build it with `make -C bevy_eulerian_fluid_amd/csrc` "
+            "or __graft_entry__.build()")
+    try:
+        L = C.CDLL(LIB_PATH)
+    except OSError as e:
+        raise NativeLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+    P = C.c_void_p
+    PP = C.POINTER(C.c_void_p)
+    sig = {
+        "efl_ctx_create": [C.c_int, PP],
+        "efl_ctx_destroy": [P],
+        "efl_sync": [P],
+        "efl_set_timing": [P, C.c_int],
+        "efl_set_obstacles": [P, P, C.c_uint32],
+        "efl_instance_create": [P, C.POINTER(abi.efl_settings), PP],
+        "efl_instance_destroy": [P],
+        "efl_set_projection": [P, C.POINTER(abi.efl_projection)],
+        "efl_set_reinit": [P, C.POINTER(abi.efl_reinit)],
+        "efl_set_uniforms": [P, C.c_float, P, P],
+        "efl_set_sources": [P, P, C.c_uint32, C.c_int],
+        "efl_set_forces": [P, P, C.c_uint32],
+        "efl_init_levelset": [P, P],
+        "efl_reset": [P],
+        "efl_initialize": [P],
+        "efl_step": [P, C.POINTER(C.c_void_p), C.c_uint32, C.c_float],
+        "efl_run_stage": [P, C.c_int, C.c_float],
+        "efl_field_info": [P, C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_uint32),
+                           C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
+        "efl_get_field": [P, C.c_int, P, C.c_size_t],
+        "efl_set_field": [P, C.c_int, P, C.c_size_t],
+        "efl_field_device_ptr": [P, C.c_int, PP],
+        "efl_get_solid_forces": [P, P, C.c_uint32],
+        "efl_set_diagnostics": [P, C.c_int],
+        "efl_measure": [P, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "efl_get_stats": [P, C.POINTER(abi.efl_stats)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.efl_last_error.argtypes = [P]
+    L.efl_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+class Context:
+    """One efl_ctx: a device, a stream and the global obstacle list."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = C.c_void_p()
+        rc = L.efl_ctx_create(int(device), C.byref(h))
+        if rc != abi.OK:
+            raise EflError(rc, "efl_ctx_create", "(no HIP device?)")
+        self.h = h
+        self.L = L
+        self._instances = []
+
+    def _chk(self, rc, what):
+        if rc != abi.OK:
+            detail = self.L.efl_last_error(self.h)
+            raise EflError(rc, what, detail.decode() if detail else "")
+
+    def close(self):
+        if self.h:
+            self.L.efl_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        self._chk(self.L.efl_sync(self.h), "efl_sync")
+
+    def set_timing(self, on):
+        self._chk(self.L.efl_set_timing(self.h, int(bool(on))), "efl_set_timing")
+
+    def set_obstacles(self, obstacles):
+        arr, n = abi.struct_array(abi.efl_obstacle, list(obstacles))
+        self._chk(self.L.efl_set_obstacles(self.h, arr, n), "efl_set_obstacles")
+
+    def instance(self, settings):
+        return Instance(self, settings)
+
+    def step(self, instances, dt):
+        n = len(instances)
+        arr = (C.c_void_p * n)(*[i.h for i in instances])
+        self._chk(self.L.efl_step(self.h, arr, n, C.c_float(dt)), "efl_step")
+
+
+class Instance:
+    """One efl_instance: the grid buffer set of one fluid entity."""
+
+    def __init__(self, ctx, settings):
+        self.ctx = ctx
+        self.L = ctx.L
+        h = C.c_void_p()
+        ctx._chk(self.L.efl_instance_create(ctx.h, C.byref(settings), C.byref(h)), "efl_instance_create")
+        self.h = h
+        self.W = settings.width
+        self.H = settings.height
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.L.efl_instance_destroy(self.h)
+        self.h = None
+
+    def _chk(self, rc, what):
+        self.ctx._chk(rc, what)
+
+    def set_projection(self, p):
+        self._chk(self.L.efl_set_projection(self.h, C.byref(p)), "efl_set_projection")
+
+    def set_reinit(self, r):
+        self._chk(self.L.efl_set_reinit(self.h, C.byref(r)), "efl_set_reinit")
+
+    def set_uniforms(self, rho, gravity, transform=None):
+        g = (C.c_float * 2)(*gravity)
+        t = (C.c_float * 16)(*transform) if transform is not None else None
+        self._chk(self.L.efl_set_uniforms(self.h, C.c_float(rho), g, t), "efl_set_uniforms")
+
+    def set_sources(self, sources, oneshot):
+        arr, n = abi.struct_array(abi.efl_source, list(sources))
+        self._chk(self.L.efl_set_sources(self.h, arr, n, int(bool(oneshot))), "efl_set_sources")
+
+    def set_forces(self, forces):
+        arr, n = abi.struct_array(abi.efl_force, list(forces))
+        self._chk(self.L.efl_set_forces(self.h, arr, n), "efl_set_forces")
+
+    def init_levelset(self, ls):
+        if ls is None:
+            self._chk(self.L.efl_init_levelset(self.h, None), "efl_init_levelset")
+            return
+        a = np.ascontiguousarray(ls, dtype=np.float32).reshape(-1)
+        assert a.size == self.W * self.H
+        self._chk(self.L.efl_init_levelset(self.h, a.ctypes.data), "efl_init_levelset")
+
+    def reset(self):
+        self._chk(self.L.efl_reset(self.h), "efl_reset")
+
+    def initialize(self):
+        self._chk(self.L.efl_initialize(self.h), "efl_initialize")
+
+    def step(self, dt):
+        self.ctx.step([self], dt)
+
+    def run_stage(self, stage, dt):
+        if isinstance(stage, str):
+            stage = abi.STAGE[stage]
+        self._chk(self.L.efl_run_stage(self.h, int(stage), C.c_float(dt)), "efl_run_stage")
+
+    def set_diagnostics(self, on):
+        self._chk(self.L.efl_set_diagnostics(self.h, int(bool(on))), "efl_set_diagnostics")
+
+    def field_info(self, field):
+        if isinstance(field, str):
+            field = abi.FIELD[field]
+        n = C.c_size_t()
+        w, h, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._chk(self.L.efl_field_info(self.h, int(field), C.byref(n), C.byref(w), C.byref(h), C.byref(c)),
+                  "efl_field_info")
+        return n.value, w.value, h.value, c.value
+
+    def _dtype(self, name):
+        if name in abi.INT_FIELDS:
+            return np.int32
+        if name in abi.UINT_FIELDS:
+            return np.uint32
+        return np.float32
+
+    def get(self, field):
+        name = field if isinstance(field, str) else abi.FIELD_NAMES[field]
+        n, w, h, c = self.field_info(name)
+        out = np.empty(n, dtype=self._dtype(name))
+        self._chk(self.L.efl_get_field(self.h, abi.FIELD[name], out.ctypes.data, n), "efl_get_field")
+        return out.reshape((h, w, c)) if c > 1 else out.reshape((h, w))
+
+    def set(self, field, arr):
+        name = field if isinstance(field, str) else abi.FIELD_NAMES[field]
+        n, w, h, c = self.field_info(name)
+        a = np.ascontiguousarray(arr, dtype=self._dtype(name)).reshape(-1)
+        assert a.size == n, (name, a.size, n)
+        self._chk(self.L.efl_set_field(self.h, abi.FIELD[name], a.ctypes.data, n), "efl_set_field")
+
+    def device_ptr(self, field):
+        if isinstance(field, str):
+            field = abi.FIELD[field]
+        p = C.c_void_p()
+        self._chk(self.L.efl_field_device_ptr(self.h, int(field), C.byref(p)), "efl_field_device_ptr")
+        return p.value
+
+    def solid_forces(self, n=abi.MAX_SOLIDS):
+        out = (abi.efl_solid_force * n)()
+        self._chk(self.L.efl_get_solid_forces(self.h, out, n), "efl_get_solid_forces")
+        return np.array([[f.force[0], f.force[1], f.torque] for f in out], dtype=np.float32)
+
+    def measure(self, dt):
+        r, d = C.c_float(), C.c_float()
+        self._chk(self.L.efl_measure(self.h, C.c_float(dt), C.byref(r), C.byref(d)), "efl_measure")
+        return r.value, d.value
+
+    def stats(self):
+        s = abi.efl_stats()
+        self._chk(self.L.efl_get_stats(self.h, C.byref(s)), "efl_get_stats")
+        return s

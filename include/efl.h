@@ -1,0 +1,301 @@
+/*
+ * efl.h - C ABI of the MI355X-native per-step Eulerian fluid pipeline.
+ *
+ * This is the drop-in boundary for the render-world half of bevy_eulerian_fluid:
+ * the pass framework, the render-graph node and its 33 compute kernels
+ * (src/plugin.rs, src/pipeline.rs, src/euler_fluid/render_node.rs:84-476 and
+ * the WGSL shaders below src/euler_fluid).  A host (the Rust FFI crate, the C++
+ * mirror in bevy_eulerian_fluid_amd/host, or Python via ctypes) keeps the
+ * reference's main-world API (FluidPlugin, FluidSettings, FluidSource,
+ * LocalForces = ForcesToFluid, ProjectionMethod, ReinitializeMethod,
+ * FluidTextures) and calls these entry points instead of recording wgpu
+ * compute passes.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: EFL_OK (0) or a negative EFL_E_*.
+ *    Nothing unwinds or aborts across the ABI; efl_last_error() returns a
+ *    message for the last failure on a context.
+ *  - The library owns all device memory.  Callers own host arrays and pass
+ *    them by pointer and length for the duration of one call only.
+ *  - All GPU work of a context is issued in order on one HIP stream owned by
+ *    the context.  efl_step() is asynchronous; getters and efl_sync()
+ *    synchronise.
+ *  - Grids are row-major [y][x], x fastest, y = 0 is the TOP row (world y is
+ *    up).  u is (W+1) x H, v is W x (H+1), every other field is W x H.
+ *    Velocities are in cells per second.
+ *  - Structs are plain C, 4-byte scalars, no padding surprises (sizes are
+ *    asserted in the implementation).
+ */
+#ifndef EFL_H
+#define EFL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFL_ABI_VERSION 1
+
+/* Limits of the reference. */
+#define EFL_MAX_FLUID_SOURCES 16  /* fluid_source_uniform.rs:19 MAX_FLUID_SOURCE */
+#define EFL_MAX_SOLIDS 256        /* fluid_to_solid.rs:28 MAX_SOLIDS */
+#define EFL_MAX_BATCH 64          /* instances advanced by one efl_step call */
+
+/* Status codes. */
+#define EFL_OK 0
+#define EFL_E_INVALID (-1)        /* bad argument (null pointer, size, enum) */
+#define EFL_E_HIP (-2)            /* a HIP runtime call failed */
+#define EFL_E_OOM (-3)            /* device allocation failed */
+#define EFL_E_STATE (-4)          /* call not valid in the current state */
+#define EFL_E_RANGE (-5)          /* count beyond a documented maximum */
+#define EFL_E_NOT_CONVERGED (-6)  /* residual target missed (reported, not fatal) */
+
+typedef struct efl_ctx efl_ctx;
+typedef struct efl_instance efl_instance;
+
+/* ProjectionMethod (projection.rs:45-52). */
+typedef enum {
+  EFL_PROJECTION_JACOBI = 0,        /* 50 x (jacobi, jacobi_reverse), projection.rs:135-143 */
+  EFL_PROJECTION_GAUSS_SEIDEL = 1,  /* SOR red-black GS, weight 1.9, gauss_seidel.rs:60-148 */
+  EFL_PROJECTION_MULTIGRID = 2,     /* V-cycle(s) with RBGS smoother, multi_grid.rs:390-475 */
+  EFL_PROJECTION_MGPCG = 3          /* no counterpart: CG preconditioned by one symmetric V-cycle */
+} efl_projection_kind;
+
+/* ReinitializeMethod (reinitialize_levelset.rs:36-46). */
+typedef enum {
+  EFL_REINIT_JUMP_FLOODING = 0,
+  EFL_REINIT_FAST_ITERATIVE = 1
+} efl_reinit_kind;
+
+/* GaussSeidelConfig / MultiGridConfig (gauss_seidel.rs:33-42, multi_grid.rs:56-71)
+ * plus the residual-target loop that the reference does not have (it runs
+ * exactly one V-cycle per step).  max_cycles = 1 and rel_tol = 0 reproduce
+ * the reference. */
+typedef struct {
+  int32_t kind;               /* efl_projection_kind */
+  uint32_t gs_iterations;     /* GaussSeidelConfig.num_iterations (default 20) */
+  uint32_t mg_pre_iterations; /* MultiGridConfig.pre_smooth_config (default 2) */
+  uint32_t mg_post_iterations;/* MultiGridConfig.post_smooth_config (default 2) */
+  uint32_t mg_coarsest_iterations; /* MultiGridConfig.coarsest_config (default 3) */
+  uint32_t max_cycles;        /* V-cycles (MULTIGRID) or CG iterations (MGPCG) per step */
+  float rel_tol;              /* stop when ||r||2/||b||2 <= rel_tol; 0 = run max_cycles */
+  uint32_t check_every;       /* residual check period in cycles (0 = 1) */
+} efl_projection;
+
+/* FastIterativeMethodConfig (fast_iterative_method.rs:14-23). */
+typedef struct {
+  int32_t kind;               /* efl_reinit_kind */
+  uint32_t fim_iterations;    /* default 10 */
+} efl_reinit;
+
+/* FluidSettings (settings.rs:110-114) + FluidPlugin::new(length_unit)
+ * (euler_fluid.rs:43-52; dx = 1/length_unit, settings.rs:116-123) + the
+ * entity's Transform (fluid_uniform.rs:108-121) + its required
+ * ProjectionMethod / ReinitializeMethod components (settings.rs:100-109). */
+typedef struct {
+  uint32_t width;             /* FluidSettings.size.x */
+  uint32_t height;            /* FluidSettings.size.y */
+  float rho;
+  float gravity[2];
+  float length_unit;          /* > 0 */
+  float transform[16];        /* column-major Mat4; all zero = identity */
+  efl_projection projection;
+  efl_reinit reinit;
+} efl_settings;
+
+/* FluidSourceData (fluid_source_uniform.rs:48-55): shape 0 = circle
+ * (data[0] = radius), 1 = AABB (data = half_size); mode 0 = source,
+ * 1 = sink.  center is the source Transform translation, in pixels with the
+ * origin at the grid centre and y up. */
+typedef struct {
+  float center[2];
+  float data[2];
+  float velocity[2];
+  uint32_t shape;
+  uint32_t mode;
+} efl_source;
+
+/* ForceToFluid (apply_forces.rs:36-40): force in m/s^2, position in cells
+ * (x right, y down, like the grid index). */
+typedef struct {
+  float force[2];
+  float position[2];
+} efl_force;
+
+/* SolidObstacle (obstacle.rs:17-32, solid_obstacle.wgsl:29-41).  shape is
+ * parry's ShapeType id: 0 circle, 1 cuboid, 2 capsule, 4 triangle.  Matrices
+ * are column-major; inverse_transform must be the inverse of transform. */
+typedef struct {
+  uint32_t entity_id;
+  uint32_t shape;
+  float values[6];
+  float transform[16];
+  float inverse_transform[16];
+  float linear_velocity[2];
+  float angular_velocity;
+  float reserved;
+} efl_obstacle;
+
+/* FluidToSolidForce (fluid_to_solid.rs:60-64), before the readback's
+ * dt/dx scaling. */
+typedef struct {
+  float force[2];
+  float torque;
+  float reserved;
+} efl_solid_force;
+
+/* Per-instance results of the last step. */
+typedef struct {
+  uint64_t steps;             /* completed steps since the last initialisation */
+  uint32_t cycles;            /* V-cycles / CG iterations used by the last projection */
+  int32_t converged;          /* 1 if rel_tol was met (or rel_tol == 0) */
+  float rel_residual;         /* ||r||2 / ||b||2 after the last projection (if measured) */
+  float max_norm_divergence;  /* dt * max|D| over fluid, non-solid cells (if measured) */
+  float projection_ms;        /* device time of the projection stage (if timing on) */
+  float step_ms;              /* device time of the whole step (if timing on) */
+} efl_stats;
+
+/* Field identifiers for efl_get_field / efl_set_field / efl_field_info. */
+typedef enum {
+  EFL_FIELD_U0 = 0,           /* (W+1) x H f32; FluidTextures.u */
+  EFL_FIELD_U1,               /* (W+1) x H f32 */
+  EFL_FIELD_V0,               /* W x (H+1) f32; FluidTextures.v */
+  EFL_FIELD_V1,               /* W x (H+1) f32 */
+  EFL_FIELD_U_SOLID,          /* (W+1) x H f32; FluidTextures.u_solid */
+  EFL_FIELD_V_SOLID,          /* W x (H+1) f32; FluidTextures.v_solid */
+  EFL_FIELD_SOLID_ID,         /* W x H i32 */
+  EFL_FIELD_U_VALID0,         /* (W+1) x H i32 */
+  EFL_FIELD_U_VALID1,         /* (W+1) x H i32 */
+  EFL_FIELD_V_VALID0,         /* W x (H+1) i32 */
+  EFL_FIELD_V_VALID1,         /* W x (H+1) i32 */
+  EFL_FIELD_DIV,              /* W x H f32 */
+  EFL_FIELD_P0,               /* W x H f32 */
+  EFL_FIELD_P1,               /* W x H f32 */
+  EFL_FIELD_LS_AIR0,          /* W x H f32; FluidTextures.levelset_air */
+  EFL_FIELD_LS_AIR1,          /* W x H f32 */
+  EFL_FIELD_GRAD_LS,          /* W x H x 2 f32 */
+  EFL_FIELD_LS_SOLID,         /* W x H f32; FluidTextures.levelset_solid */
+  EFL_FIELD_AREA_FRACTION,    /* W x H x 4 f32 (i-1/2, i+1/2, j-1/2, j+1/2) */
+  EFL_FIELD_FIM_LABELS0,      /* W x H u32 */
+  EFL_FIELD_FIM_LABELS,       /* W x H u32 */
+  EFL_FIELD_JFA_SEEDS0,       /* W x H x 2 f32 */
+  EFL_FIELD_JFA_SEEDS1,       /* W x H x 2 f32 */
+  EFL_FIELD_SOLID_BINS,       /* 3 x 256 i32 (force x, force y, torque) */
+  EFL_FIELD_SOLID_FORCES,     /* 256 x 4 f32 (efl_solid_force) */
+  EFL_FIELD_MG_R0,            /* W x H f32: multigrid residual, finest level */
+  EFL_FIELD_COUNT
+} efl_field;
+
+/* Stages of EulerFluidNode::run (render_node.rs:193-476), each runnable on
+ * its own for parity tests and profiling. */
+typedef enum {
+  EFL_STAGE_INIT_GRID_CENTER = 0, /* K1 initialize_grid_center */
+  EFL_STAGE_INIT_GRID_EDGE,       /* K2 initialize_grid_edge */
+  EFL_STAGE_SOURCE_ONESHOT,       /* K3 with the oneshot (init) uniform */
+  EFL_STAGE_UPDATE_SOLID,         /* K4 */
+  EFL_STAGE_AREA_FRACTION,        /* K5 */
+  EFL_STAGE_ADVECT_VELOCITY,      /* K6, K7 */
+  EFL_STAGE_APPLY_FORCES,         /* K8, K9 */
+  EFL_STAGE_DIVERGENCE,           /* K10 */
+  EFL_STAGE_PROJECTION,           /* K11-K17 per efl_projection */
+  EFL_STAGE_SOLVE_VELOCITY,       /* K18, K19 */
+  EFL_STAGE_EXTRAPOLATE_VELOCITY, /* K20-K23 */
+  EFL_STAGE_ADVECT_LEVELSET,      /* K24 (cubic) */
+  EFL_STAGE_SOURCE,               /* K3 with the per-step uniform */
+  EFL_STAGE_REINITIALIZE,         /* K25-K27 or K28-K30 */
+  EFL_STAGE_LEVELSET_GRADIENT,    /* K31 */
+  EFL_STAGE_FLUID_TO_SOLID,       /* K32, K33 (bins are cleared first) */
+  EFL_STAGE_GS_SWEEP,             /* one red + black sweep at weight 1.9 */
+  EFL_STAGE_MG_VCYCLE,            /* one V-cycle (reference order) */
+  EFL_STAGE_MG_RESIDUAL,          /* finest-level residual into EFL_FIELD_MG_R0 */
+  EFL_STAGE_COUNT
+} efl_stage;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* EulerFluidNode + pass pipelines for one device.  device = HIP ordinal. */
+int efl_ctx_create(int device, efl_ctx** out);
+int efl_ctx_destroy(efl_ctx* ctx);
+const char* efl_last_error(const efl_ctx* ctx);
+int efl_sync(efl_ctx* ctx);
+/* Per-stage device timing (hipEvents) into efl_stats; off by default. */
+int efl_set_timing(efl_ctx* ctx, int enabled);
+
+/* construct_rigid_body_buffer_for_gpu (obstacle.rs:89-160): the obstacle
+ * list is global to the context, as SolidObstaclesBuffer is global to the
+ * app.  n <= EFL_MAX_SOLIDS. */
+int efl_set_obstacles(efl_ctx* ctx, const efl_obstacle* obstacles, uint32_t n);
+
+/* ---- instances --------------------------------------------------------- */
+
+/* watch_fluid_component (setup_components.rs:38-330): allocates the grid
+ * buffer set, the multigrid pyramid (multi_grid.rs:85-131) and the
+ * redistancing buffers (reinitialize_levelset.rs:107-160). */
+int efl_instance_create(efl_ctx* ctx, const efl_settings* settings, efl_instance** out);
+int efl_instance_destroy(efl_instance* inst);
+
+/* Replace the projection / reinitialisation config (ProjectionMethod and
+ * ReinitializeMethod are mutable components). */
+int efl_set_projection(efl_instance* inst, const efl_projection* projection);
+int efl_set_reinit(efl_instance* inst, const efl_reinit* reinit);
+/* FluidSettings.gravity, rho and the fluid Transform may change per frame
+ * (update_simulation_uniform, fluid_uniform.rs:108-121). */
+int efl_set_uniforms(efl_instance* inst, float rho, const float gravity[2], const float transform[16]);
+
+/* update_fluid_source_init_buffer / update_fluid_source_buffer
+ * (fluid_source_uniform.rs:77-158): oneshot = 1 sets the sources applied at
+ * initialisation (FluidSourceOneshot), oneshot = 0 the per-step sources.
+ * n <= EFL_MAX_FLUID_SOURCES, else EFL_E_RANGE. */
+int efl_set_sources(efl_instance* inst, const efl_source* sources, uint32_t n, int oneshot);
+
+/* apply_forces_and_clear (apply_forces.rs:157-167): forces used by the next
+ * step only, then cleared. */
+int efl_set_forces(efl_instance* inst, const efl_force* forces, uint32_t n);
+
+/* No counterpart (needed above 16 sources): a W x H level set that is
+ * min-combined into levelset_air1 after the oneshot sources at the next
+ * initialisation.  NULL clears it. */
+int efl_init_levelset(efl_instance* inst, const float* levelset);
+
+/* FluidStatus::Reset (fluid_status.rs:15-37): the next efl_step runs the
+ * initialisation dispatch instead of a step. */
+int efl_reset(efl_instance* inst);
+
+/* The Uninitialized branch of EulerFluidNode::run (render_node.rs:219-277),
+ * issued now. */
+int efl_initialize(efl_instance* inst);
+
+/* One fluid step for each of n instances (render_node.rs:205-217 loops over
+ * fluid entities; here they are batched).  Uninitialised instances are
+ * initialised instead, as the reference does.  dt = FluidTimeStep. */
+int efl_step(efl_ctx* ctx, efl_instance* const* insts, uint32_t n, float dt);
+
+/* Run one stage of the step on one instance (parity tests, profiling). */
+int efl_run_stage(efl_instance* inst, int stage, float dt);
+
+/* ---- results ------------------------------------------------------------ */
+
+/* Element count (floats or ints) and row width of a field. */
+int efl_field_info(const efl_instance* inst, int field, size_t* count, uint32_t* width, uint32_t* height, uint32_t* channels);
+/* Copy a field to / from host memory (count elements, 4 bytes each). */
+int efl_get_field(efl_instance* inst, int field, void* host, size_t count);
+int efl_set_field(efl_instance* inst, int field, const void* host, size_t count);
+/* Device pointer of a field (valid until the instance is destroyed). */
+int efl_field_device_ptr(efl_instance* inst, int field, void** dptr);
+/* forces_to_solid readback (setup_components.rs:293-294): out[EFL_MAX_SOLIDS]. */
+int efl_get_solid_forces(efl_instance* inst, efl_solid_force* out, uint32_t n);
+/* When enabled, each step records ||r||/||b|| after the projection and
+ * dt*max|D| after the velocity update (before extrapolation) into efl_stats;
+ * each costs one reduction pass.  Off by default. */
+int efl_set_diagnostics(efl_instance* inst, int enabled);
+/* Measure ||r||/||b|| and dt*max|D| of the current fields (forces a sync). */
+int efl_measure(efl_instance* inst, float dt, float* rel_residual, float* max_norm_divergence);
+int efl_get_stats(efl_instance* inst, efl_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EFL_H */

@@ -1,0 +1,197 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE).
+
+The oracle restates bevy_eulerian_fluid's per-step WGSL pipeline in C++
+(oracle/efl_oracle.cpp).  Only tests/, __graft_entry__.smoke() and the CPU
+baseline leg of bench.py may import this module; it is the checker, never the
+thing measured or shipped.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from bevy_eulerian_fluid_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libefl_oracle.so")
+_lib = None
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (g++, OpenMP)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
+                       check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [C.POINTER(abi.efl_settings)]
+        L.orc_destroy.argtypes = [P]
+        L.orc_destroy.restype = None
+        for name, args in [
+            ("orc_set_projection", [P, C.POINTER(abi.efl_projection)]),
+            ("orc_set_reinit", [P, C.POINTER(abi.efl_reinit)]),
+            ("orc_set_uniforms", [P, C.c_float, C.c_void_p, C.c_void_p]),
+            ("orc_set_sources", [P, C.c_void_p, C.c_uint32, C.c_int]),
+            ("orc_set_obstacles", [P, C.c_void_p, C.c_uint32]),
+            ("orc_set_forces", [P, C.c_void_p, C.c_uint32]),
+            ("orc_init_levelset", [P, C.c_void_p]),
+            ("orc_reset", [P]),
+            ("orc_initialize", [P]),
+            ("orc_step", [P, C.c_float]),
+            ("orc_run_stage", [P, C.c_int, C.c_float]),
+            ("orc_field_info", [P, C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+            ("orc_get_field", [P, C.c_int, C.c_void_p, C.c_size_t]),
+            ("orc_set_field", [P, C.c_int, C.c_void_p, C.c_size_t]),
+            ("orc_get_stats", [P, C.POINTER(abi.efl_stats)]),
+            ("orc_measure", [P, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+            ("orc_solve_fp64", [P, C.c_float, C.c_double, C.c_void_p, C.POINTER(C.c_double),
+                                C.POINTER(C.c_int)]),
+            ("orc_residual_of", [P, C.c_float, C.c_void_p, C.c_void_p]),
+            ("orc_set_threads", [C.c_int]),
+            ("orc_set_diagnostics", [P, C.c_int]),
+        ]:
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed: {abi.STATUS_NAMES.get(rc, rc)}")
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+class OracleInstance:
+    """One fluid instance run by the CPU oracle."""
+
+    def __init__(self, settings):
+        self._settings = settings
+        self.h = lib().orc_create(C.byref(settings))
+        if not self.h:
+            raise ValueError("oracle rejected the settings")
+        self.W = settings.width
+        self.H = settings.height
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_projection(self, p):
+        _check(lib().orc_set_projection(self.h, C.byref(p)), "set_projection")
+
+    def set_reinit(self, r):
+        _check(lib().orc_set_reinit(self.h, C.byref(r)), "set_reinit")
+
+    def set_uniforms(self, rho, gravity, transform=None):
+        g = (C.c_float * 2)(*gravity)
+        t = (C.c_float * 16)(*transform) if transform is not None else None
+        _check(lib().orc_set_uniforms(self.h, rho, g, t), "set_uniforms")
+
+    def set_sources(self, sources, oneshot):
+        arr, n = abi.struct_array(abi.efl_source, sources)
+        _check(lib().orc_set_sources(self.h, arr, n, int(oneshot)), "set_sources")
+
+    def set_obstacles(self, obstacles):
+        arr, n = abi.struct_array(abi.efl_obstacle, obstacles)
+        _check(lib().orc_set_obstacles(self.h, arr, n), "set_obstacles")
+
+    def set_forces(self, forces):
+        arr, n = abi.struct_array(abi.efl_force, forces)
+        _check(lib().orc_set_forces(self.h, arr, n), "set_forces")
+
+    def init_levelset(self, ls):
+        if ls is None:
+            _check(lib().orc_init_levelset(self.h, None), "init_levelset")
+            return
+        a = np.ascontiguousarray(ls, dtype=np.float32)
+        assert a.size == self.W * self.H
+        _check(lib().orc_init_levelset(self.h, a.ctypes.data), "init_levelset")
+
+    def reset(self):
+        _check(lib().orc_reset(self.h), "reset")
+
+    def initialize(self):
+        _check(lib().orc_initialize(self.h), "initialize")
+
+    def step(self, dt):
+        _check(lib().orc_step(self.h, dt), "step")
+
+    def run_stage(self, stage, dt):
+        if isinstance(stage, str):
+            stage = abi.STAGE[stage]
+        _check(lib().orc_run_stage(self.h, stage, dt), "run_stage")
+
+    def field_info(self, field):
+        if isinstance(field, str):
+            field = abi.FIELD[field]
+        n = C.c_size_t()
+        w, h, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(lib().orc_field_info(self.h, field, C.byref(n), C.byref(w), C.byref(h), C.byref(c)),
+               "field_info")
+        return n.value, w.value, h.value, c.value
+
+    def get(self, field):
+        name = field if isinstance(field, str) else abi.FIELD_NAMES[field]
+        n, w, h, c = self.field_info(name)
+        dt = np.int32 if name in abi.INT_FIELDS else (np.uint32 if name in abi.UINT_FIELDS else np.float32)
+        out = np.empty(n, dtype=dt)
+        _check(lib().orc_get_field(self.h, abi.FIELD[name], out.ctypes.data, n), "get_field")
+        return out.reshape((h, w, c)) if c > 1 else out.reshape((h, w))
+
+    def set(self, field, arr):
+        name = field if isinstance(field, str) else abi.FIELD_NAMES[field]
+        n, w, h, c = self.field_info(name)
+        dt = np.int32 if name in abi.INT_FIELDS else (np.uint32 if name in abi.UINT_FIELDS else np.float32)
+        a = np.ascontiguousarray(arr, dtype=dt).reshape(-1)
+        assert a.size == n, (name, a.size, n)
+        _check(lib().orc_set_field(self.h, abi.FIELD[name], a.ctypes.data, n), "set_field")
+
+    def stats(self):
+        s = abi.efl_stats()
+        _check(lib().orc_get_stats(self.h, C.byref(s)), "get_stats")
+        return s
+
+    def set_diagnostics(self, on):
+        _check(lib().orc_set_diagnostics(self.h, int(bool(on))), "set_diagnostics")
+
+    def measure(self, dt):
+        r, d = C.c_float(), C.c_float()
+        _check(lib().orc_measure(self.h, dt, C.byref(r), C.byref(d)), "measure")
+        return r.value, d.value
+
+    def solve_fp64(self, dt, rel_tol=1e-12):
+        out = np.zeros(self.W * self.H, dtype=np.float64)
+        rel = C.c_double()
+        it = C.c_int()
+        _check(lib().orc_solve_fp64(self.h, dt, rel_tol, out.ctypes.data, C.byref(rel), C.byref(it)),
+               "solve_fp64")
+        return out.reshape(self.H, self.W), rel.value, it.value
+
+    def residual_of(self, dt, x):
+        a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+        r = np.zeros_like(a)
+        _check(lib().orc_residual_of(self.h, dt, a.ctypes.data, r.ctypes.data), "residual_of")
+        return r.reshape(self.H, self.W)

@@ -1,0 +1,90 @@
+"""Helpers that drive the HIP library and the CPU oracle with the same scene
+and compare their fields."""
+import numpy as np
+
+from bevy_eulerian_fluid_amd import abi, scenes
+
+# every field both implementations expose, in efl_field order
+ALL_FIELDS = list(abi.FIELD_NAMES)
+
+
+def make_pair(ctx, sc, Oracle):
+    """(gpu instance, oracle instance) set up with the scene dict `sc`."""
+    s = sc["settings"]
+    g = ctx.instance(s)
+    o = Oracle(s)
+    for inst in (g, o):
+        inst.set_sources(sc.get("init_sources", []), True)
+        inst.set_sources(sc.get("sources", []), False)
+    o.set_obstacles(sc.get("obstacles", []))
+    ctx.set_obstacles(sc.get("obstacles", []))
+    return g, o
+
+
+def copy_state(src, dst):
+    for f in ALL_FIELDS:
+        dst.set(f, src.get(f))
+
+
+def compare(a, b, name, rtol=0.0, atol=0.0):
+    """Bit-exact (rtol = atol = 0) or tolerance comparison of two fields;
+    NaNs must sit at the same places.  Returns a message or None."""
+    if a.shape != b.shape:
+        return f"{name}: shape {a.shape} vs {b.shape}"
+    if a.dtype.kind in "iu":
+        bad = a != b
+        if bad.any():
+            idx = np.argwhere(bad)[:5].tolist()
+            return f"{name}: {int(bad.sum())} integer mismatches, first at {idx}"
+        return None
+    an, bn = np.isnan(a), np.isnan(b)
+    if (an != bn).any():
+        return f"{name}: NaN pattern differs ({int((an != bn).sum())} cells)"
+    a2 = np.where(an, 0, a).astype(np.float64)
+    b2 = np.where(bn, 0, b).astype(np.float64)
+    if rtol == 0.0 and atol == 0.0:
+        bad = a2 != b2
+    else:
+        bad = np.abs(a2 - b2) > atol + rtol * np.abs(b2)
+    if bad.any():
+        idx = np.argwhere(bad)[:3].tolist()
+        worst = np.max(np.abs(a2 - b2))
+        return f"{name}: {int(bad.sum())} mismatches (max abs diff {worst:.3e}), first at {idx}"
+    return None
+
+
+def compare_all(g, o, fields=ALL_FIELDS, tol=None):
+    tol = tol or {}
+    errs = []
+    for f in fields:
+        rtol, atol = tol.get(f, (0.0, 0.0))
+        m = compare(g.get(f), o.get(f), f, rtol, atol)
+        if m:
+            errs.append(m)
+    return errs
+
+
+def small_scene(W=64, H=64, bodies=True, seed=7, proj=None, rein=None, forces=True):
+    """A compact scene exercising every stage: a fluid box, one source and one
+    sink, and (optionally) one body of each shape and a few local forces."""
+    rng = scenes.SplitMix64(seed, 0)
+    s = scenes.settings(W, H, proj=proj or scenes.projection("gauss_seidel", gs_iterations=20),
+                        rein=rein or scenes.reinit("fast_iterative", 10))
+    init = [scenes.source((0.0, -0.2 * H), "aabb", half_size=(0.45 * W, 0.28 * H))]
+    srcs = [scenes.source((-0.3 * W, 0.3 * H), "circle", radius=4.0, velocity=(20.0, -5.0)),
+            scenes.source((0.35 * W, 0.1 * H), "aabb", half_size=(3.0, 3.0), sink=True)]
+    obs = []
+    if bodies:
+        k = min(W, H) / 64.0
+        obs = [
+            scenes.obstacle(0, 0, [5.0 * k], 0.0, (-0.2 * W, -0.1 * H), lin_vel=(1.0, -2.0), ang_vel=0.3),
+            scenes.obstacle(1, 1, [6.0 * k, 3.0 * k], 0.4, (0.15 * W, -0.25 * H), lin_vel=(0.0, 1.0)),
+            scenes.obstacle(2, 2, [0.0, -4.0 * k, 0.0, 4.0 * k, 2.5 * k], 1.1, (0.0, 0.2 * H), ang_vel=-0.5),
+            scenes.obstacle(3, 4, [-5.0 * k, -3.0 * k, 5.0 * k, -3.0 * k, 0.0, 6.0 * k], 0.7, (0.3 * W, 0.25 * H)),
+        ]
+    frc = []
+    if forces:
+        for _ in range(3):
+            frc.append(scenes.force((rng.uniform(-50, 50), rng.uniform(-50, 50)),
+                                    (rng.uniform(0, W), rng.uniform(0, H))))
+    return dict(settings=s, init_sources=init, sources=srcs, obstacles=obs, forces=frc, dt=1.0 / 60.0)
