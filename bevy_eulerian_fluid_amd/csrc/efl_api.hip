@@ -884,6 +884,25 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
   return EFL_OK;
 }
 
+int efl_time_stage(efl_instance* in, int stage, float dt, uint32_t reps, float* ms) {
+  if (!in || !ms || reps == 0) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = efl_run_stage(in, stage, dt);  // warm (and validate the stage id)
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  for (uint32_t k = 0; k < reps; ++k) {
+    rc = efl_run_stage(in, stage, dt);
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev[1]));
+  float t = 0.0f;
+  HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+  *ms = t / (float)reps;
+  return EFL_OK;
+}
+
 int efl_field_info(const efl_instance* in, int field, size_t* count, uint32_t* w, uint32_t* h, uint32_t* ch) {
   if (!in) return EFL_E_INVALID;
   FieldDesc d = field_desc(const_cast<efl_instance*>(in), field);

@@ -21,7 +21,7 @@ EXPORTS = [
     "efl_set_reinit", "efl_set_uniforms", "efl_set_sources", "efl_set_forces", "efl_init_levelset",
     "efl_reset", "efl_initialize", "efl_step", "efl_run_stage", "efl_field_info", "efl_get_field",
     "efl_set_field", "efl_field_device_ptr", "efl_get_solid_forces", "efl_set_diagnostics",
-    "efl_measure", "efl_get_stats",
+    "efl_measure", "efl_get_stats", "efl_time_stage",
 ]
 
 
@@ -77,6 +77,7 @@ def load():
         "efl_set_diagnostics": [P, C.c_int],
         "efl_measure": [P, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "efl_get_stats": [P, C.POINTER(abi.efl_stats)],
+        "efl_time_stage": [P, C.c_int, C.c_float, C.c_uint32, C.POINTER(C.c_float)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -199,6 +200,16 @@ class Instance:
 
     def set_diagnostics(self, on):
         self._chk(self.L.efl_set_diagnostics(self.h, int(bool(on))), "efl_set_diagnostics")
+
+    def time_stage(self, stage, dt, reps):
+        """Mean device ms per repetition of one stage (hipEvents on the
+        library's stream)."""
+        if isinstance(stage, str):
+            stage = abi.STAGE[stage]
+        ms = C.c_float()
+        self._chk(self.L.efl_time_stage(self.h, int(stage), C.c_float(dt), int(reps), C.byref(ms)),
+                  "efl_time_stage")
+        return ms.value
 
     def field_info(self, field):
         if isinstance(field, str):
