@@ -2,8 +2,9 @@
 
 Workload (the configuration the headline metric is quoted on): a 1024 x 1024
 FP32 MAC grid, fluid filling the lower 55 % of the domain, 16 rigid bodies
-(4 circles, 4 rectangles, 4 capsules, 4 triangles) at seeded positions with
-two-way area-fraction coupling, dt = 1/60, and a pressure solve iterated to
+(4 circles, 4 rectangles, 4 capsules, 4 triangles) at seeded positions,
+densities 0.5-2.0 rho, two-way area-fraction coupling (the bodies are
+integrated on the device from the fluid forces and gravity), dt = 1/60, and a pressure solve iterated to
 ||r||2/||b||2 <= 1e-5 (CG preconditioned by one symmetric RBGS V-cycle of the
 reference's multigrid; the reference's single fixed V-cycle cannot reach that
 target).  One step = one efl_step call: the whole render-node pipeline
@@ -43,12 +44,13 @@ def build_scene(size, rel_tol):
 
 
 def gpu_run(args):
-    from bevy_eulerian_fluid_amd import native
+    from bevy_eulerian_fluid_amd import native, scenes
     sc = build_scene(args.size, args.rel_tol)
     ctx = native.Context(0)
     inst = ctx.instance(sc["settings"])
     inst.set_sources(sc["init_sources"], True)
     ctx.set_obstacles(sc["obstacles"])
+    ctx.set_bodies(scenes.bodies_for(sc["obstacles"], sc["densities"]))
     log(f"[bench] {args.size}^2, {len(sc['obstacles'])} bodies: initialise + {args.warmup} warm-up steps")
     inst.step(DT)  # initialisation dispatch
     for _ in range(args.warmup):
@@ -103,6 +105,8 @@ def cpu_baseline(args):
     o = OracleInstance(sc["settings"])
     o.set_sources(sc["init_sources"], True)
     o.set_obstacles(sc["obstacles"])
+    from bevy_eulerian_fluid_amd import scenes
+    o.set_bodies(scenes.bodies_for(sc["obstacles"], sc["densities"]))
     o.step(DT)
     o.step(DT)  # one warm-up step
     t0 = time.perf_counter()
@@ -152,7 +156,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"coupled tank {args.size}x{args.size}, lower 55% fluid, 16 bodies, "
+        "config": {"workload": f"coupled tank {args.size}x{args.size}, lower 55% fluid, 16 two-way coupled bodies, "
                                f"dt=1/60, pressure to ||r||/||b||<={args.rel_tol:g} (MG-preconditioned CG)",
                    "grid": [args.size, args.size], "bodies": 16, "projection": "mgpcg",
                    "rel_tol": args.rel_tol, "reinit": "fast_iterative(10)"},

@@ -114,8 +114,12 @@ class efl_obstacle(C.Structure):
     ]
 
 
+class efl_body(C.Structure):
+    _fields_ = [("density", C.c_float), ("area", C.c_float), ("inertia", C.c_float), ("dynamic", C.c_uint32)]
+
+
 class efl_solid_force(C.Structure):
-    _fields_ = [("force", C.c_float * 2), ("torque", C.c_float), ("reserved", C.c_float)]
+    _fields_ = [("force", C.c_float * 2), ("torque", C.c_float), ("torque_world", C.c_float)]
 
 
 class efl_stats(C.Structure):
@@ -132,7 +136,7 @@ class efl_stats(C.Structure):
 
 STRUCT_SIZES = {
     "efl_projection": 32, "efl_reinit": 8, "efl_settings": 128, "efl_source": 32,
-    "efl_force": 16, "efl_obstacle": 176, "efl_solid_force": 16, "efl_stats": 32,
+    "efl_force": 16, "efl_obstacle": 176, "efl_body": 16, "efl_solid_force": 16, "efl_stats": 32,
 }
 
 

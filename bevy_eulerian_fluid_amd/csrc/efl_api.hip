@@ -42,6 +42,8 @@ struct efl_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double* h_scal = nullptr;  // pinned, 16 doubles per batch slot
   std::vector<efl_instance*> insts;
+  efl_body* d_bodies = nullptr;  // EFL_MAX_SOLIDS records
+  uint32_t nbodies = 0;          // 0 = no integration of the context's list
 };
 
 struct efl_instance {
@@ -61,6 +63,12 @@ struct efl_instance {
   float* d_custom_ls = nullptr;
   efl_stats stats{};
   int jfa_max_power = 0;
+  // own obstacle list (efl_set_instance_obstacles)
+  bool own_obs = false;
+  efl_obstacle* d_obs_own = nullptr;  // EFL_MAX_SOLIDS records
+  efl_body* d_bodies_own = nullptr;   // EFL_MAX_SOLIDS records
+  uint32_t nobs_own = 0;
+  bool own_bodies = false;
 };
 
 namespace {
@@ -127,7 +135,13 @@ Batch make_batch(const std::vector<efl_instance*>& v, Extent k, int lvl = 0, con
 // ---- uploads ----------------------------------------------------------------
 int upload(efl_instance* in) {
   efl_ctx* c = in->ctx;
-  if (in->obs_version != c->obs_version) {
+  if (in->own_obs) {
+    if (in->h.obs != in->d_obs_own || in->h.nobs != in->nobs_own) {
+      in->h.obs = in->d_obs_own;
+      in->h.nobs = in->nobs_own;
+      in->dirty = true;
+    }
+  } else if (in->obs_version != c->obs_version || in->h.obs != c->d_obs) {
     in->h.obs = c->d_obs;
     in->h.nobs = c->nobs;
     in->obs_version = c->obs_version;
@@ -393,7 +407,6 @@ int measure_divergence(efl_ctx* c, const IV& v) {
   IV d = filter(v, [](efl_instance* i) { return i->diag; });
   if (d.empty()) return EFL_OK;
   Batch b = make_batch(d, EX_FULL);
-  efl::launch_clear_diag(c->stream, b);
   efl::launch_max_divergence(c->stream, b);
   std::vector<float> host(4 * d.size());
   for (size_t j = 0; j < d.size(); ++j)
@@ -474,6 +487,53 @@ int run_step(efl_ctx* c, const IV& v) {  // render_node.rs:278-466
   return EFL_OK;
 }
 
+// Rigid-body step after the fluid step: one launch for the context's list
+// (forces of every stepped instance that uses it) and one per instance with
+// its own dynamic list.
+void integrate_bodies(efl_ctx* c, const IV& stepped, float dt) {
+  auto fill = [&](efl::BodyStep& a, const IV& from) {
+    a.ninst = 0;
+    for (auto* i : from) {
+      if (a.ninst >= efl::kMaxBatch) break;
+      a.fts[a.ninst] = i->h.fts;
+      a.dx[a.ninst] = i->h.dx;
+      a.inv_dx[a.ninst] = 1.0f / i->h.dx;
+      ++a.ninst;
+    }
+    efl_instance* f = from[0];
+    a.dt = dt;
+    a.gwx = f->h.gx;   // grid frame (y down) -> world frame (y up)
+    a.gwy = -f->h.gy;
+    a.half_w = 0.5f * (float)f->s.width;
+    a.half_h = 0.5f * (float)f->s.height;
+    a.rho_fluid = f->h.rho;
+  };
+  if (c->nbodies > 0 && c->nbodies == c->nobs) {
+    IV from = filter(stepped, [](efl_instance* i) { return !i->own_obs; });
+    if (!from.empty()) {
+      efl::BodyStep a;
+      std::memset(&a, 0, sizeof(a));
+      a.obs = c->d_obs;
+      a.bodies = c->d_bodies;
+      a.state = (float4*)(c->d_bodies + EFL_MAX_SOLIDS);
+      a.n = (int)c->nbodies;
+      fill(a, from);
+      efl::launch_integrate_bodies(c->stream, a);
+    }
+  }
+  for (auto* i : stepped) {
+    if (!i->own_obs || !i->own_bodies || i->nobs_own == 0) continue;
+    efl::BodyStep a;
+    std::memset(&a, 0, sizeof(a));
+    a.obs = i->d_obs_own;
+    a.bodies = i->d_bodies_own;
+    a.state = (float4*)(i->d_bodies_own + EFL_MAX_SOLIDS);
+    a.n = (int)i->nobs_own;
+    fill(a, IV{i});
+    efl::launch_integrate_bodies(c->stream, a);
+  }
+}
+
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct FieldDesc { void* p; size_t n; uint32_t w, h, c; };
@@ -505,7 +565,7 @@ FieldDesc field_desc(efl_instance* in, int f) {
     case EFL_FIELD_FIM_LABELS: return {h.lab, (size_t)W * H, W, H, 1};
     case EFL_FIELD_JFA_SEEDS0: return {h.seed0, (size_t)W * H * 2, W, H, 2};
     case EFL_FIELD_JFA_SEEDS1: return {h.seed1, (size_t)W * H * 2, W, H, 2};
-    case EFL_FIELD_SOLID_BINS: return {h.bins, (size_t)3 * EFL_MAX_SOLIDS, EFL_MAX_SOLIDS, 3, 1};
+    case EFL_FIELD_SOLID_BINS: return {h.bins, (size_t)4 * EFL_MAX_SOLIDS, EFL_MAX_SOLIDS, 4, 1};
     case EFL_FIELD_SOLID_FORCES: return {h.fts, (size_t)4 * EFL_MAX_SOLIDS, EFL_MAX_SOLIDS, 1, 4};
     case EFL_FIELD_MG_R0: return {h.lev[0].r, (size_t)W * H, W, H, 1};
     default: return {nullptr, 0, 0, 0, 0};
@@ -560,6 +620,7 @@ int efl_ctx_destroy(efl_ctx* c) {
   for (auto& e : c->ev)
     if (e) hipEventDestroy(e);
   if (c->d_obs) hipFree(c->d_obs);
+  if (c->d_bodies) hipFree(c->d_bodies);
   if (c->h_scal) hipHostFree(c->h_scal);
   hipStreamDestroy(c->stream);
   delete c;
@@ -588,6 +649,65 @@ int efl_set_obstacles(efl_ctx* c, const efl_obstacle* obs, uint32_t n) {
   if (n) HIPCHK(c, hipMemcpyAsync(c->d_obs, obs, sizeof(efl_obstacle) * n, hipMemcpyHostToDevice, c->stream));
   c->nobs = n;
   c->obs_version++;
+  return EFL_OK;
+}
+
+int efl_set_bodies(efl_ctx* c, const efl_body* bodies, uint32_t n) {
+  if (!c || (n && !bodies)) return EFL_E_INVALID;
+  if (n > EFL_MAX_SOLIDS) return fail(c, EFL_E_RANGE, "%u bodies > EFL_MAX_SOLIDS", n);
+  if (n && n != c->nobs) return fail(c, EFL_E_INVALID, "%u bodies for %u obstacles", n, c->nobs);
+  HIPCHK(c, hipSetDevice(c->device));
+  // records followed by the integrator state (previous accelerations)
+  if (n && !c->d_bodies) HIPCHK(c, hipMalloc(&c->d_bodies, (sizeof(efl_body) + 16) * EFL_MAX_SOLIDS));
+  if (n) HIPCHK(c, hipMemcpyAsync(c->d_bodies, bodies, sizeof(efl_body) * n, hipMemcpyHostToDevice, c->stream));
+  if (n) HIPCHK(c, hipMemsetAsync(c->d_bodies + EFL_MAX_SOLIDS, 0, 16 * EFL_MAX_SOLIDS, c->stream));
+  c->nbodies = n;
+  return EFL_OK;
+}
+
+int efl_get_obstacles(efl_ctx* c, efl_obstacle* out, uint32_t n) {
+  if (!c || (n && !out)) return EFL_E_INVALID;
+  if (n > c->nobs) return fail(c, EFL_E_RANGE, "%u > %u obstacles", n, c->nobs);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (n) HIPCHK(c, hipMemcpyAsync(out, c->d_obs, sizeof(efl_obstacle) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_set_instance_obstacles(efl_instance* in, const efl_obstacle* obs, const efl_body* bodies, uint32_t n) {
+  if (!in || (n && !obs)) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  if (n > EFL_MAX_SOLIDS) return fail(c, EFL_E_RANGE, "%u obstacles > EFL_MAX_SOLIDS", n);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!obs) {
+    in->own_obs = false;
+    in->own_bodies = false;
+    in->nobs_own = 0;
+    in->dirty = true;
+    return EFL_OK;
+  }
+  if (!in->d_obs_own) HIPCHK(c, hipMalloc(&in->d_obs_own, sizeof(efl_obstacle) * EFL_MAX_SOLIDS));
+  if (bodies && !in->d_bodies_own) HIPCHK(c, hipMalloc(&in->d_bodies_own, (sizeof(efl_body) + 16) * EFL_MAX_SOLIDS));
+  if (n) HIPCHK(c, hipMemcpyAsync(in->d_obs_own, obs, sizeof(efl_obstacle) * n, hipMemcpyHostToDevice, c->stream));
+  if (n && bodies) {
+    HIPCHK(c, hipMemcpyAsync(in->d_bodies_own, bodies, sizeof(efl_body) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(in->d_bodies_own + EFL_MAX_SOLIDS, 0, 16 * EFL_MAX_SOLIDS, c->stream));
+  }
+  in->own_obs = true;
+  in->own_bodies = bodies != nullptr;
+  in->nobs_own = n;
+  in->dirty = true;
+  return EFL_OK;
+}
+
+int efl_get_instance_obstacles(efl_instance* in, efl_obstacle* out, uint32_t n) {
+  if (!in || (n && !out)) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  if (!in->own_obs) return efl_get_obstacles(c, out, n);
+  if (n > in->nobs_own) return fail(c, EFL_E_RANGE, "%u > %u obstacles", n, in->nobs_own);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (n) HIPCHK(c, hipMemcpyAsync(out, in->d_obs_own, sizeof(efl_obstacle) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return EFL_OK;
 }
 
@@ -623,7 +743,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   size_t o_ls0 = take(n * 4), o_ls1 = take(n * 4), o_lss = take(n * 4), o_phit = take(n * 4);
   size_t o_lab0 = take(n * 4), o_lab = take(n * 4), o_labt = take(n * 4);
   size_t o_grad = take(n * 8), o_s0 = take(n * 8), o_s1 = take(n * 8), o_afs = take(n * 16);
-  size_t o_bins = take(3 * EFL_MAX_SOLIDS * 4), o_fts = take(EFL_MAX_SOLIDS * 16);
+  size_t o_bins = take(4 * EFL_MAX_SOLIDS * 4), o_fts = take(EFL_MAX_SOLIDS * 16);
   size_t o_r0 = take(n * 4);
   std::vector<size_t> o_lx(nl), o_lb(nl), o_lls(nl), o_laf(nl), o_lr(nl);
   for (int l = 1; l < nl; ++l) {
@@ -707,6 +827,8 @@ int efl_instance_destroy(efl_instance* in) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   if (in->d_forces) hipFree(in->d_forces);
+  if (in->d_obs_own) hipFree(in->d_obs_own);
+  if (in->d_bodies_own) hipFree(in->d_bodies_own);
   if (in->mem) hipFree(in->mem);
   c->insts.erase(std::remove(c->insts.begin(), c->insts.end(), in), c->insts.end());
   delete in;
@@ -829,6 +951,7 @@ int efl_step(efl_ctx* c, efl_instance* const* insts, uint32_t n, float dt) {
   if (!step.empty()) {
     int rc = run_step(c, step);
     if (rc) return rc;
+    integrate_bodies(c, step, dt);
     for (auto* i : step) {  // forces were cleared after use
       rc = upload(i);
       if (rc) return rc;

@@ -70,6 +70,19 @@ struct DevInst {
   float* diag;         // [0] max |D| (as float bits via atomic max), ...
 };
 
+// Arguments of the rigid-body integrator (one launch per obstacle list).
+struct BodyStep {
+  efl_obstacle* obs;
+  const efl_body* bodies;
+  int n;
+  int ninst;
+  const float4* fts[kMaxBatch];
+  float inv_dx[kMaxBatch];
+  float dx[kMaxBatch];
+  float dt, gwx, gwy, half_w, half_h, rho_fluid;
+  float4* state;  // per body: previous (ax, ay, angular acceleration)
+};
+
 // Launch map of a batched kernel: the flat grid of blocks is split into
 // per-instance ranges; inside a range, blocks tile the instance's extent.
 struct Batch {

@@ -878,51 +878,82 @@ __global__ __launch_bounds__(NT) void k_ls_gradient(Batch b) {
 // ---------------------------------------------------------------------------
 // K32 / K33 fluid -> solid forces (fluid_to_solid/*.wgsl)
 // ---------------------------------------------------------------------------
+// Bins are planar i32 fixed point: force x, force y, torque (the reference's
+// three), plus a fourth, the torque about the centre of mass in the world
+// frame (-rx * fy - ry * fx with f in grid axes), which has no counterpart and
+// feeds the rigid-body integrator.
 __global__ void k_clear_bins(Batch b) {
   const DevInst* I = b.inst[blockIdx.x];
-  for (int i = threadIdx.x; i < 3 * EFL_MAX_SOLIDS; i += blockDim.x) I->bins[i] = 0;
+  for (int i = threadIdx.x; i < 4 * EFL_MAX_SOLIDS; i += blockDim.x) I->bins[i] = 0;
 }
 __device__ __forceinline__ int fixed_point(float v) { return to_i32(v * (1.0f / 0.1f)); }
 __global__ __launch_bounds__(NT) void k_sample_forces(Batch b) {
   Site s = locate(b);
-  if (!s.ok) return;
   const DevInst* I = s.I;
-  int W = I->W, H = I->H, x = s.x, y = s.y;
-  float4 f = fractions_at(I->lss, W, H, x, y);
-  if (all_closed(f)) return;
-  size_t id = (size_t)y * W + x;
-  int sid = I->sid[id];
-  if (sid == -1) return;
-  float p = I->p0[id];
-  float fx = ((-(f.y - f.x)) * p) * I->dx;
-  float fy = ((-(f.w - f.z)) * p) * I->dx;
-  float cx = 0.0f, cy = 0.0f;
-  if (sid >= 0 && (uint32_t)sid < I->nobs) {
-    const efl_obstacle& o = I->obs[sid];
-    if (o.shape == 4u) {
-      float lx = ((o.values[0] + o.values[2]) + o.values[4]) / 3.0f;
-      float ly = ((o.values[1] + o.values[3]) + o.values[5]) / 3.0f;
-      float2 c = xform(o.transform, lx, ly);
-      cx = c.x; cy = c.y;
-    } else if (o.shape <= 2u) {
-      cx = o.transform[12]; cy = o.transform[13];
+  int sid = -1;
+  int q[4] = {0, 0, 0, 0};
+  if (s.ok) {
+    int W = I->W, H = I->H, x = s.x, y = s.y;
+    float4 f = fractions_at(I->lss, W, H, x, y);
+    size_t id = (size_t)y * W + x;
+    int cand = all_closed(f) ? -1 : I->sid[id];
+    if (cand >= 0 && cand < EFL_MAX_SOLIDS) {
+      float p = I->p0[id];
+      float fx = ((-(f.y - f.x)) * p) * I->dx;
+      float fy = ((-(f.w - f.z)) * p) * I->dx;
+      float cx = 0.0f, cy = 0.0f;
+      if ((uint32_t)cand < I->nobs) {
+        const efl_obstacle& o = I->obs[cand];
+        if (o.shape == 4u) {
+          float lx = ((o.values[0] + o.values[2]) + o.values[4]) / 3.0f;
+          float ly = ((o.values[1] + o.values[3]) + o.values[5]) / 3.0f;
+          float2 c = xform(o.transform, lx, ly);
+          cx = c.x; cy = c.y;
+        } else if (o.shape <= 2u) {
+          cx = o.transform[12]; cy = o.transform[13];
+        }
+      }
+      float ux = (float)x / I->sizex, uy = (float)y / I->sizey;
+      float2 w = xform(I->T, (ux - 0.5f) * I->sizex, (-uy + 0.5f) * I->sizey);
+      float rx = w.x - cx, ry = w.y - cy;
+      float tq = rx * fy - ry * fx;
+      float tw = -rx * fy - ry * fx;
+      sid = cand;
+      q[0] = fixed_point(fx);
+      q[1] = fixed_point(fy);
+      q[2] = fixed_point(tq);
+      q[3] = fixed_point(tw);
     }
   }
-  float ux = (float)x / I->sizex, uy = (float)y / I->sizey;
-  float2 w = xform(I->T, (ux - 0.5f) * I->sizex, (-uy + 0.5f) * I->sizey);
-  float rx = w.x - cx, ry = w.y - cy;
-  float tq = rx * fy - ry * fx;
-  if (sid >= 0 && sid < EFL_MAX_SOLIDS) {
-    atomicAdd(&I->bins[sid], fixed_point(fx));
-    atomicAdd(&I->bins[EFL_MAX_SOLIDS + sid], fixed_point(fy));
-    atomicAdd(&I->bins[2 * EFL_MAX_SOLIDS + sid], fixed_point(tq));
+  // Integer sums are exact in any order: when every contributing lane of the
+  // wavefront hits the same body, fold the wave first and add once.
+  int lane = threadIdx.x & 63;
+  unsigned long long act = __ballot(sid >= 0);
+  if (act == 0ull) return;
+  int first = __ffsll((long long)act) - 1;
+  int sid0 = __shfl(sid, first, 64);
+  bool uniform = __all(sid < 0 || sid == sid0);
+  if (uniform) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      unsigned v = (unsigned)q[k];  // wrapping i32 sums, as atomicAdd
+      for (int off = 32; off > 0; off >>= 1) v += (unsigned)__shfl_xor((int)v, off, 64);
+      q[k] = (int)v;
+    }
+    if (lane == first)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) atomicAdd(&I->bins[k * EFL_MAX_SOLIDS + sid0], q[k]);
+  } else if (sid >= 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) atomicAdd(&I->bins[k * EFL_MAX_SOLIDS + sid], q[k]);
   }
 }
 __global__ void k_accumulate_forces(Batch b) {
   const DevInst* I = b.inst[blockIdx.x];
   for (int i = threadIdx.x; i < EFL_MAX_SOLIDS; i += blockDim.x)
     I->fts[i] = make_float4((float)I->bins[i] * 0.1f, (float)I->bins[EFL_MAX_SOLIDS + i] * 0.1f,
-                            (float)I->bins[2 * EFL_MAX_SOLIDS + i] * 0.1f, 0.0f);
+                            (float)I->bins[2 * EFL_MAX_SOLIDS + i] * 0.1f,
+                            (float)I->bins[3 * EFL_MAX_SOLIDS + i] * 0.1f);
 }
 
 // ---------------------------------------------------------------------------
@@ -933,53 +964,28 @@ __device__ __forceinline__ bool active_at(const DevInst* I, size_t id) {
   return I->ls0[id] < 0.0f && !all_closed(I->afs[id]);
 }
 
+// Block partial sums: wavefront shuffles, then the 4 wave results in a
+// fixed order; thread 0 stores them at red[k * red_stride + local block].
 template <int N>
-__device__ __forceinline__ void block_sum(double (&v)[N], double (*sh)[NT]) {
+__device__ __forceinline__ void block_partials(const DevInst* I, int lb, double (&v)[N], bool is_max = false) {
+  __shared__ double sh[N][NT / 64];
 #pragma unroll
-  for (int k = 0; k < N; ++k) sh[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int st = NT / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-#pragma unroll
-      for (int k = 0; k < N; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+  for (int k = 0; k < N; ++k) {
+    double x = v[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      double o = __shfl_xor(x, off, 64);
+      x = is_max ? fmax(x, o) : x + o;
     }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int k = 0; k < N; ++k) v[k] = sh[k][0];
-}
-
-// Sum N partials of this instance's blocks; the last block to finish folds
-// them (in block order) into scal[dst..dst+N) and runs `fin`.
-template <int N, class Fin>
-__device__ void finish_sums(const DevInst* I, int nblk, int lb, double (&v)[N], int dst, Fin fin) {
-  __shared__ double sh[N][NT];
-  __shared__ bool last;
-  block_sum<N>(v, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) I->red[(size_t)k * I->red_stride + lb] = v[k];
-    __threadfence();
-    uint32_t prev = atomicAdd(I->counter, 1u);
-    last = (prev == (uint32_t)nblk - 1);
+    if ((threadIdx.x & 63) == 0) sh[k][threadIdx.x >> 6] = x;
   }
   __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double acc[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) acc[k] = 0.0;
-  // threads sum strided chunks, then a fixed tree: deterministic for fixed nblk
-  for (int i = threadIdx.x; i < nblk; i += NT) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) acc[k] += ((volatile double*)I->red)[(size_t)k * I->red_stride + i];
-  }
-  block_sum<N>(acc, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) I->scal[dst + k] = acc[k];
-    fin(I);
-    *I->counter = 0u;
+    for (int k = 0; k < N; ++k) {
+      double acc = sh[k][0];
+      for (int w = 1; w < NT / 64; ++w) acc = is_max ? fmax(acc, sh[k][w]) : acc + sh[k][w];
+      I->red[(size_t)k * I->red_stride + lb] = acc;
+    }
   }
 }
 
@@ -1003,11 +1009,10 @@ __global__ __launch_bounds__(NT) void k_residual_norm(Batch b, int pcg_init) {
     v[0] = (double)r * (double)r;
     v[1] = (double)bv * (double)bv;
   }
-  finish_sums<2>(I, inst_blocks(b, s.slot), local_block(b, s.slot), v, 2, [](const DevInst*) {});
+  block_partials<2>(I, local_block(b, s.slot), v);
 }
 
-// z masked to active cells, rz = r . z, then (first = 1) p = z or
-// p = z + (rz / rz_old) p
+// z masked to active cells, partial r . z
 __global__ __launch_bounds__(NT) void k_pcg_rz(Batch b) {
   Site s = locate(b);
   const DevInst* I = s.I;
@@ -1018,13 +1023,7 @@ __global__ __launch_bounds__(NT) void k_pcg_rz(Batch b) {
     if (!active_at(I, id)) { z = 0.0f; I->pcg_z[id] = 0.0f; }
     v[0] = (double)I->pcg_r[id] * (double)z;
   }
-  finish_sums<1>(I, inst_blocks(b, s.slot), local_block(b, s.slot), v, 7, [](const DevInst* J) {
-    double rz_new = J->scal[7];
-    double rz_old = J->scal[0];
-    J->scal[5] = (J->scal[8] != 0.0) ? 0.0 : rz_new / rz_old;  // beta (0 on the first iteration)
-    J->scal[8] = 0.0;
-    J->scal[0] = rz_new;
-  });
+  block_partials<1>(I, local_block(b, s.slot), v);
 }
 __global__ __launch_bounds__(NT) void k_pcg_update_p(Batch b) {
   Site s = locate(b);
@@ -1034,7 +1033,7 @@ __global__ __launch_bounds__(NT) void k_pcg_update_p(Batch b) {
   float beta = (float)I->scal[5];
   I->pcg_p[id] = I->pcg_z[id] + beta * I->pcg_p[id];
 }
-// q = A p, pq = p . q, alpha = rz / pq
+// q = A p (the operator of residual.wgsl with b = 0, negated), partial p . q
 __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
   Site s = locate(b);
   const DevInst* I = s.I;
@@ -1042,11 +1041,9 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
   if (s.ok) {
     Level L = I->lev[0];
     L.x = I->pcg_p;
-    L.b = nullptr;
     size_t id = (size_t)s.y * L.w + s.x;
     float q = 0.0f;
     if (active_at(I, id)) {
-      // -(residual with b = 0): the operator of residual.wgsl
       float xc = L.x[id];
       float4 f = L.af[id];
       float phi = L.ls[id];
@@ -1069,12 +1066,9 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
     I->pcg_q[id] = q;
     v[0] = (double)L.x[id] * (double)q;
   }
-  finish_sums<1>(I, inst_blocks(b, s.slot), local_block(b, s.slot), v, 1, [](const DevInst* J) {
-    double pq = J->scal[1];
-    J->scal[4] = pq > 0.0 ? J->scal[0] / pq : 0.0;  // alpha
-  });
+  block_partials<1>(I, local_block(b, s.slot), v);
 }
-// x += alpha p, r -= alpha q, rr = ||r||^2
+// x += alpha p, r -= alpha q, partial ||r||^2
 __global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
   Site s = locate(b);
   const DevInst* I = s.I;
@@ -1087,7 +1081,7 @@ __global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
     I->pcg_r[id] = r;
     v[0] = (double)r * (double)r;
   }
-  finish_sums<1>(I, inst_blocks(b, s.slot), local_block(b, s.slot), v, 2, [](const DevInst*) {});
+  block_partials<1>(I, local_block(b, s.slot), v);
 }
 __global__ __launch_bounds__(NT) void k_zero(Batch b, int which) {
   Site s = locate(b);
@@ -1099,34 +1093,145 @@ __global__ __launch_bounds__(NT) void k_zero(Batch b, int which) {
 }
 __global__ void k_pcg_first(Batch b) {
   const DevInst* I = b.inst[blockIdx.x];
-  if (threadIdx.x == 0) I->scal[8] = 1.0;  // next k_pcg_rz sets beta = 0
+  if (threadIdx.x == 0) I->scal[8] = 1.0;  // the next RZ finalize sets beta = 0
 }
 
-// max over active cells of |D| (the K10 numerator) on u0/v0
+// partial max over active cells of |D| (the K10 numerator) on u0/v0
 __global__ __launch_bounds__(NT) void k_max_divergence(Batch b) {
   Site s = locate(b);
-  if (!s.ok) return;
   const DevInst* I = s.I;
-  int W = I->W, H = I->H, x = s.x, y = s.y;
-  size_t id = (size_t)y * W + x;
-  float d = 0.0f;
-  if (I->ls0[id] < 0.0f) {
-    float4 f = fractions_at(I->lss, W, H, x, y);
-    if (!all_closed(f)) {
-      float du_f = f.y * ldz(I->u0, W + 1, H, x + 1, y) - f.x * ldz(I->u0, W + 1, H, x, y);
-      float dv_f = f.w * ldz(I->v0, W, H + 1, x, y + 1) - f.z * ldz(I->v0, W, H + 1, x, y);
-      float du_s = (1.0f - f.y) * ldz(I->us, W + 1, H, x + 1, y) - (1.0f - f.x) * ldz(I->us, W + 1, H, x, y);
-      float dv_s = (1.0f - f.w) * ldz(I->vs, W, H + 1, x, y + 1) - (1.0f - f.z) * ldz(I->vs, W, H + 1, x, y);
-      d = fabsf(((du_f + dv_f) + du_s) + dv_s);
+  double v[1] = {0.0};
+  if (s.ok) {
+    int W = I->W, H = I->H, x = s.x, y = s.y;
+    size_t id = (size_t)y * W + x;
+    if (I->ls0[id] < 0.0f) {
+      float4 f = fractions_at(I->lss, W, H, x, y);
+      if (!all_closed(f)) {
+        float du_f = f.y * ldz(I->u0, W + 1, H, x + 1, y) - f.x * ldz(I->u0, W + 1, H, x, y);
+        float dv_f = f.w * ldz(I->v0, W, H + 1, x, y + 1) - f.z * ldz(I->v0, W, H + 1, x, y);
+        float du_s = (1.0f - f.y) * ldz(I->us, W + 1, H, x + 1, y) - (1.0f - f.x) * ldz(I->us, W + 1, H, x, y);
+        float dv_s = (1.0f - f.w) * ldz(I->vs, W, H + 1, x, y + 1) - (1.0f - f.z) * ldz(I->vs, W, H + 1, x, y);
+        v[0] = fabsf(((du_f + dv_f) + du_s) + dv_s);
+      }
     }
   }
-  // wave-level max, then one atomic per wave (|D| >= 0: float order == int order)
-  for (int off = 32; off > 0; off >>= 1) d = fmaxf(d, __shfl_xor(d, off, 64));
-  if ((threadIdx.x & 63) == 0 && d > 0.0f) atomicMax((int*)&I->diag[0], __float_as_int(d));
+  block_partials<1>(I, local_block(b, s.slot), v, true);
 }
-__global__ void k_clear_diag(Batch b) {
-  const DevInst* I = b.inst[blockIdx.x];
-  if (threadIdx.x < 4) I->diag[threadIdx.x] = 0.0f;
+
+// One block per instance folds the block partials of the preceding
+// reduction kernel (launched with the same Batch) in block order:
+//   kind 0: rr, bb -> scal[2], scal[3]
+//   kind 1: pq -> scal[1]; alpha = rz / pq -> scal[4]
+//   kind 2: rr -> scal[2]
+//   kind 3: rz -> scal[0]; beta = rz / rz_old (0 on the first iteration) -> scal[5]
+//   kind 4: max |D| -> diag[0]
+__global__ __launch_bounds__(NT) void k_finalize(Batch b, int kind) {
+  const int slot = blockIdx.x;
+  const DevInst* I = b.inst[slot];
+  const int nblk = b.start[slot + 1] - b.start[slot];
+  const int N = kind == 0 ? 2 : 1;
+  const bool is_max = kind == 4;
+  __shared__ double sh[2][NT];
+  for (int k = 0; k < N; ++k) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += NT) {
+      double v = I->red[(size_t)k * I->red_stride + i];
+      acc = is_max ? fmax(acc, v) : acc + v;
+    }
+    sh[k][threadIdx.x] = acc;
+  }
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < N; ++k)
+        sh[k][threadIdx.x] = is_max ? fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + st])
+                                    : sh[k][threadIdx.x] + sh[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double* sc = I->scal;
+  switch (kind) {
+    case 0: sc[2] = sh[0][0]; sc[3] = sh[1][0]; break;
+    case 1: sc[1] = sh[0][0]; sc[4] = sh[0][0] > 0.0 ? sc[0] / sh[0][0] : 0.0; break;
+    case 2: sc[2] = sh[0][0]; break;
+    case 3: {
+      double rz_new = sh[0][0];
+      sc[5] = (sc[8] != 0.0) ? 0.0 : rz_new / sc[0];
+      sc[8] = 0.0;
+      sc[0] = rz_new;
+      break;
+    }
+    case 4: I->diag[0] = (float)sh[0][0]; break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rigid bodies: semi-implicit Euler of the dynamic obstacles, one thread per
+// body (the role avian2d plays for the reference).  The fluid forces of the
+// contributing instances are summed; gravity, walls, fluid density and dx
+// come from the first contributing instance.
+// ---------------------------------------------------------------------------
+__device__ float body_radius(const efl_obstacle& o) {
+  switch (o.shape) {
+    case 0u: return o.values[0];
+    case 1u: return hyp(o.values[0], o.values[1]);
+    case 2u: return fmaxf(hyp(o.values[0], o.values[1]), hyp(o.values[2], o.values[3])) + o.values[4];
+    case 4u:
+      return fmaxf(fmaxf(hyp(o.values[0], o.values[1]), hyp(o.values[2], o.values[3])), hyp(o.values[4], o.values[5]));
+    default: return 0.0f;
+  }
+}
+
+__global__ void k_integrate_bodies(BodyStep a) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  efl_obstacle o = a.obs[k];
+  const efl_body bd = a.bodies[k];
+  if (!bd.dynamic) return;
+  // F = m a with the fluid force in cells (F_true = F dx, m_true = m dx^2)
+  // and an added-mass estimate (m_a = rho_fluid * area, previous
+  // acceleration) that keeps explicit coupling of light bodies stable.
+  float m = bd.density * bd.area;
+  float J = bd.density * bd.inertia;
+  float ma = 4.0f * a.rho_fluid * bd.area, Ja = 2.0f * a.rho_fluid * bd.inertia;
+  float4 prev = a.state[k];
+  float Fx = 0.0f, Fy = 0.0f, Tq = 0.0f;
+  uint32_t id = o.entity_id;
+  if (id < (uint32_t)EFL_MAX_SOLIDS) {
+    for (int i = 0; i < a.ninst; ++i) {
+      float4 f = a.fts[i][id];
+      float s = 1.0f / (a.dx[i] * a.dx[i]);
+      Fx += f.x * s;
+      Fy += -f.y * s;  // grid y points down
+      Tq += f.w * s;   // world-frame torque bin
+    }
+  }
+  float ax = (Fx + ma * prev.x + m * a.gwx / a.dx[0]) / (m + ma);
+  float ay = (Fy + ma * prev.y + m * a.gwy / a.dx[0]) / (m + ma);
+  float al = (J + Ja) > 0.0f ? (Tq + Ja * prev.z) / (J + Ja) : 0.0f;
+  a.state[k] = make_float4(ax, ay, al, 0.0f);
+  float vx = o.linear_velocity[0] + ax * a.dt;
+  float vy = o.linear_velocity[1] + ay * a.dt;
+  float w = o.angular_velocity + al * a.dt;
+  float tx = o.transform[12] + vx * a.dt;
+  float ty = o.transform[13] + vy * a.dt;
+  float ang = atan2f(o.transform[1], o.transform[0]) + w * a.dt;
+  float R = body_radius(o);
+  float lx = fmaxf(a.half_w - 1.0f - R, 0.0f), ly = fmaxf(a.half_h - 1.0f - R, 0.0f);
+  if (tx > lx) { tx = lx; vx = fminf(vx, 0.0f); }
+  if (tx < -lx) { tx = -lx; vx = fmaxf(vx, 0.0f); }
+  if (ty > ly) { ty = ly; vy = fminf(vy, 0.0f); }
+  if (ty < -ly) { ty = -ly; vy = fmaxf(vy, 0.0f); }
+  float c = cosf(ang), sn = sinf(ang);
+  float T[16] = {c, sn, 0.f, 0.f, -sn, c, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, tx, ty, 0.f, 1.f};
+  float Ti[16] = {c, -sn, 0.f, 0.f, sn, c, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, -(c * tx + sn * ty), -(-sn * tx + c * ty), 0.f, 1.f};
+  for (int q = 0; q < 16; ++q) {
+    a.obs[k].transform[q] = T[q];
+    a.obs[k].inverse_transform[q] = Ti[q];
+  }
+  a.obs[k].linear_velocity[0] = vx;
+  a.obs[k].linear_velocity[1] = vy;
+  a.obs[k].angular_velocity = w;
 }
 
 // ---------------------------------------------------------------------------
@@ -1177,16 +1282,33 @@ void launch_ls_gradient(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_ls_gradie
 void launch_clear_bins(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_clear_bins, b, st); }
 void launch_sample_forces(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_sample_forces, b, st); }
 void launch_accumulate_forces(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_accumulate_forces, b, st); }
-void launch_residual_norm(hipStream_t st, const Batch& b, int pcg_init) { EFL_LAUNCH(k_residual_norm, b, st, pcg_init); }
-void launch_pcg_rz(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_rz, b, st); }
+void launch_residual_norm(hipStream_t st, const Batch& b, int pcg_init) {
+  EFL_LAUNCH(k_residual_norm, b, st, pcg_init);
+  EFL_LAUNCH_PER_INST(k_finalize, b, st, 0);
+}
+void launch_pcg_rz(hipStream_t st, const Batch& b) {
+  EFL_LAUNCH(k_pcg_rz, b, st);
+  EFL_LAUNCH_PER_INST(k_finalize, b, st, 3);
+}
 void launch_pcg_update_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_update_p, b, st); }
-void launch_pcg_apply(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_apply, b, st); }
-void launch_pcg_axpy(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_axpy, b, st); }
+void launch_pcg_apply(hipStream_t st, const Batch& b) {
+  EFL_LAUNCH(k_pcg_apply, b, st);
+  EFL_LAUNCH_PER_INST(k_finalize, b, st, 1);
+}
+void launch_pcg_axpy(hipStream_t st, const Batch& b) {
+  EFL_LAUNCH(k_pcg_axpy, b, st);
+  EFL_LAUNCH_PER_INST(k_finalize, b, st, 2);
+}
 void launch_zero_z(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_zero, b, st, 0); }
 void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_zero, b, st, 1); }
 void launch_pcg_first(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_pcg_first, b, st); }
-void launch_max_divergence(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_max_divergence, b, st); }
-void launch_clear_diag(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_clear_diag, b, st); }
+void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
+  if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);
+}
+void launch_max_divergence(hipStream_t st, const Batch& b) {
+  EFL_LAUNCH(k_max_divergence, b, st);
+  EFL_LAUNCH_PER_INST(k_finalize, b, st, 4);
+}
 
 int tile_w() { return TX; }
 int tile_h() { return TY; }

@@ -51,6 +51,6 @@ void launch_zero_z(hipStream_t st, const Batch& b);
 void launch_mask_p(hipStream_t st, const Batch& b);
 void launch_pcg_first(hipStream_t st, const Batch& b);
 void launch_max_divergence(hipStream_t st, const Batch& b);
-void launch_clear_diag(hipStream_t st, const Batch& b);
+void launch_integrate_bodies(hipStream_t st, const BodyStep& a);
 
 }  // namespace efl

@@ -21,7 +21,8 @@ EXPORTS = [
     "efl_set_reinit", "efl_set_uniforms", "efl_set_sources", "efl_set_forces", "efl_init_levelset",
     "efl_reset", "efl_initialize", "efl_step", "efl_run_stage", "efl_field_info", "efl_get_field",
     "efl_set_field", "efl_field_device_ptr", "efl_get_solid_forces", "efl_set_diagnostics",
-    "efl_measure", "efl_get_stats", "efl_time_stage",
+    "efl_measure", "efl_get_stats", "efl_time_stage", "efl_set_bodies", "efl_get_obstacles",
+    "efl_set_instance_obstacles", "efl_get_instance_obstacles",
 ]
 
 
@@ -78,6 +79,10 @@ def load():
         "efl_measure": [P, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "efl_get_stats": [P, C.POINTER(abi.efl_stats)],
         "efl_time_stage": [P, C.c_int, C.c_float, C.c_uint32, C.POINTER(C.c_float)],
+        "efl_set_bodies": [P, P, C.c_uint32],
+        "efl_get_obstacles": [P, P, C.c_uint32],
+        "efl_set_instance_obstacles": [P, P, P, C.c_uint32],
+        "efl_get_instance_obstacles": [P, P, C.c_uint32],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -128,6 +133,15 @@ class Context:
         arr, n = abi.struct_array(abi.efl_obstacle, list(obstacles))
         self._chk(self.L.efl_set_obstacles(self.h, arr, n), "efl_set_obstacles")
 
+    def set_bodies(self, bodies):
+        arr, n = abi.struct_array(abi.efl_body, list(bodies))
+        self._chk(self.L.efl_set_bodies(self.h, arr, n), "efl_set_bodies")
+
+    def get_obstacles(self, n):
+        out = (abi.efl_obstacle * n)()
+        self._chk(self.L.efl_get_obstacles(self.h, out, n), "efl_get_obstacles")
+        return list(out)
+
     def instance(self, settings):
         return Instance(self, settings)
 
@@ -175,6 +189,27 @@ class Instance:
     def set_forces(self, forces):
         arr, n = abi.struct_array(abi.efl_force, list(forces))
         self._chk(self.L.efl_set_forces(self.h, arr, n), "efl_set_forces")
+
+    def set_obstacles(self, obstacles, bodies=None):
+        """This instance's own obstacle list (None = use the context's)."""
+        if obstacles is None:
+            self._chk(self.L.efl_set_instance_obstacles(self.h, None, None, 0), "efl_set_instance_obstacles")
+            return
+        obs = list(obstacles)
+        arr, n = abi.struct_array(abi.efl_obstacle, obs)
+        if arr is None:
+            arr = (abi.efl_obstacle * 1)()
+        barr = None
+        if bodies is not None:
+            barr, _ = abi.struct_array(abi.efl_body, list(bodies))
+            if barr is None:
+                barr = (abi.efl_body * 1)()
+        self._chk(self.L.efl_set_instance_obstacles(self.h, arr, barr, n), "efl_set_instance_obstacles")
+
+    def get_obstacles(self, n):
+        out = (abi.efl_obstacle * n)()
+        self._chk(self.L.efl_get_instance_obstacles(self.h, out, n), "efl_get_instance_obstacles")
+        return list(out)
 
     def init_levelset(self, ls):
         if ls is None:

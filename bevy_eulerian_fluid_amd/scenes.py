@@ -151,6 +151,53 @@ def random_body(rng, kind):
     return 4, [pts[0][0], pts[0][1], pts[1][0], pts[1][1], pts[2][0], pts[2][1]], rad, area
 
 
+def shape_moments(shape, values, h=0.25):
+    """(area, second moment of area about the local origin) of a body shape,
+    by midpoint sampling of its inside on an h-spaced lattice (cells^2,
+    cells^4).  Shapes are centred on their local origin."""
+    import numpy as np
+    v = list(values) + [0.0] * (6 - len(values))
+    if shape == 0:
+        R = v[0]
+    elif shape == 1:
+        R = math.hypot(v[0], v[1])
+    elif shape == 2:
+        R = max(math.hypot(v[0], v[1]), math.hypot(v[2], v[3])) + v[4]
+    else:
+        R = max(math.hypot(v[0], v[1]), math.hypot(v[2], v[3]), math.hypot(v[4], v[5]))
+    t = np.arange(-R - h, R + h, h) + 0.5 * h
+    X, Y = np.meshgrid(t, t)
+    if shape == 0:
+        inside = X * X + Y * Y < v[0] ** 2
+    elif shape == 1:
+        inside = (np.abs(X) < v[0]) & (np.abs(Y) < v[1])
+    elif shape == 2:
+        ax, ay, bx, by, r = v[:5]
+        abx, aby = bx - ax, by - ay
+        tt = np.clip(((X - ax) * abx + (Y - ay) * aby) / (abx * abx + aby * aby), 0, 1)
+        inside = np.hypot(X - (ax + tt * abx), Y - (ay + tt * aby)) < r
+    else:
+        p = [(v[0], v[1]), (v[2], v[3]), (v[4], v[5])]
+        s = [np.sign((Y - p[i][1]) * (p[(i + 1) % 3][0] - p[i][0]) - (X - p[i][0]) * (p[(i + 1) % 3][1] - p[i][1]))
+             for i in range(3)]
+        inside = ((s[0] > 0) & (s[1] > 0) & (s[2] > 0)) | ((s[0] < 0) & (s[1] < 0) & (s[2] < 0))
+    area = inside.sum() * h * h
+    inertia = ((X * X + Y * Y) * inside).sum() * h * h
+    return float(area), float(inertia)
+
+
+def bodies_for(obstacles, densities, dynamic=True):
+    """efl_body records (density relative units, area, inertia) for obstacles."""
+    out = []
+    for o, d in zip(obstacles, densities):
+        b = abi.efl_body()
+        b.density = d
+        b.area, b.inertia = shape_moments(o.shape, list(o.values))
+        b.dynamic = 1 if dynamic else 0
+        out.append(b)
+    return out
+
+
 def random_bodies(rng, width, height, counts, rho, clearance=4.0, max_tries=2000):
     """Non-overlapping bodies (bounding circles) inside the walls.
 

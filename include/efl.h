@@ -139,12 +139,25 @@ typedef struct {
 } efl_obstacle;
 
 /* FluidToSolidForce (fluid_to_solid.rs:60-64), before the readback's
- * dt/dx scaling. */
+ * dt/dx scaling, in the std430 16-byte slot whose padding here carries the
+ * torque about the centre of mass in the world frame (no counterpart; the
+ * reference's torque mixes a world-frame arm with a grid-frame force). */
 typedef struct {
   float force[2];
   float torque;
-  float reserved;
+  float torque_world;
 } efl_solid_force;
+
+/* No counterpart (avian2d's rigid-body response): per-obstacle dynamics for
+ * the library's own integrator.  density is in units of FluidSettings.rho,
+ * area in cells^2, inertia is the second moment of area about the centre of
+ * mass in cells^4.  dynamic = 0 keeps the obstacle prescribed. */
+typedef struct {
+  float density;
+  float area;
+  float inertia;
+  uint32_t dynamic;
+} efl_body;
 
 /* Per-instance results of the last step. */
 typedef struct {
@@ -182,7 +195,7 @@ typedef enum {
   EFL_FIELD_FIM_LABELS,       /* W x H u32 */
   EFL_FIELD_JFA_SEEDS0,       /* W x H x 2 f32 */
   EFL_FIELD_JFA_SEEDS1,       /* W x H x 2 f32 */
-  EFL_FIELD_SOLID_BINS,       /* 3 x 256 i32 (force x, force y, torque) */
+  EFL_FIELD_SOLID_BINS,       /* 4 x 256 i32 (force x, force y, torque, world torque) */
   EFL_FIELD_SOLID_FORCES,     /* 256 x 4 f32 (efl_solid_force) */
   EFL_FIELD_MG_R0,            /* W x H f32: multigrid residual, finest level */
   EFL_FIELD_COUNT
@@ -228,6 +241,21 @@ int efl_set_timing(efl_ctx* ctx, int enabled);
  * app.  n <= EFL_MAX_SOLIDS. */
 int efl_set_obstacles(efl_ctx* ctx, const efl_obstacle* obstacles, uint32_t n);
 
+/* No counterpart (replaces avian2d for the fluid's bodies): when bodies are
+ * set, each efl_step ends with a semi-implicit Euler step of the dynamic
+ * obstacles, on the device, driven by gravity and the fluid->solid forces of
+ * the instances stepped in that call.  With m = density * area and
+ * J = density * inertia (cell units), the acceleration in cells/s^2 is
+ *   a = (F / dx^2 + m_a a_prev + m g / dx) / (m + m_a),  m_a = 4 rho area,
+ *   alpha = (T_world / dx^2 + J_a alpha_prev) / (J + J_a),  J_a = 2 rho J,
+ * with F the force bins (y flipped to the world frame) and T_world the
+ * world-frame torque; the added-mass terms keep the explicit coupling of
+ * light bodies stable.  Bodies are kept inside the wall ring (inelastic).
+ * n must equal the obstacle count; n = 0 turns integration off. */
+int efl_set_bodies(efl_ctx* ctx, const efl_body* bodies, uint32_t n);
+/* Current obstacle records of the context's list (after integration). */
+int efl_get_obstacles(efl_ctx* ctx, efl_obstacle* out, uint32_t n);
+
 /* ---- instances --------------------------------------------------------- */
 
 /* watch_fluid_component (setup_components.rs:38-330): allocates the grid
@@ -235,6 +263,14 @@ int efl_set_obstacles(efl_ctx* ctx, const efl_obstacle* obstacles, uint32_t n);
  * redistancing buffers (reinitialize_levelset.rs:107-160). */
 int efl_instance_create(efl_ctx* ctx, const efl_settings* settings, efl_instance** out);
 int efl_instance_destroy(efl_instance* inst);
+
+/* No counterpart: give this instance its own obstacle list (and, with
+ * bodies != NULL, dynamics), used instead of the context's.  This is what lets
+ * instances of one batched efl_step carry different bodies.  n = 0 and
+ * obstacles = NULL return the instance to the context's list. */
+int efl_set_instance_obstacles(efl_instance* inst, const efl_obstacle* obstacles, const efl_body* bodies, uint32_t n);
+/* Current obstacle records this instance sees (its own list or the context's). */
+int efl_get_instance_obstacles(efl_instance* inst, efl_obstacle* out, uint32_t n);
 
 /* Replace the projection / reinitialisation config (ProjectionMethod and
  * ReinitializeMethod are mutable components). */

@@ -29,6 +29,7 @@
 #include "efl_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -267,6 +268,8 @@ struct orc_inst {
   std::vector<efl_source> src_init, src_step;
   std::vector<efl_force> forces;
   std::vector<efl_obstacle> obstacles;
+  std::vector<efl_body> bodies;
+  std::vector<std::array<float, 3>> body_state;  // previous accelerations
   std::vector<float> custom_ls;
   bool initialized = false;
   bool diag = false;
@@ -1248,21 +1251,85 @@ void k_fluid_to_solid(orc_inst* o) {
       mat_xy(U.T, xx, yy, 0.0f, 1.0f, wx, wy);
       float rx = wx - cx, ry = wy - cy;
       float tq = rx * fy - ry * fx;
+      // no counterpart: torque about the centre in the world frame (grid y
+      // points down, so the world force is (fx, -fy))
+      float tw = -rx * fy - ry * fx;
       if (id >= 0 && id < EFL_MAX_SOLIDS) {  // atomicAdd into bins (wrapping i32 add)
         o->bins.at(id, 0) = (int32_t)((uint32_t)o->bins.at(id, 0) + (uint32_t)f32_to_i32(fx));
         o->bins.at(id, 1) = (int32_t)((uint32_t)o->bins.at(id, 1) + (uint32_t)f32_to_i32(fy));
         o->bins.at(id, 2) = (int32_t)((uint32_t)o->bins.at(id, 2) + (uint32_t)f32_to_i32(tq));
+        o->bins.at(id, 3) = (int32_t)((uint32_t)o->bins.at(id, 3) + (uint32_t)f32_to_i32(tw));
       }
     }
   for (int i = 0; i < EFL_MAX_SOLIDS; ++i) {  // accumulate_forces.wgsl:16-32
     o->fts.at(i, 0, 0) = (float)o->bins.at(i, 0) * 0.1f;
     o->fts.at(i, 0, 1) = (float)o->bins.at(i, 1) * 0.1f;
     o->fts.at(i, 0, 2) = (float)o->bins.at(i, 2) * 0.1f;
-    o->fts.at(i, 0, 3) = 0.0f;
+    o->fts.at(i, 0, 3) = (float)o->bins.at(i, 3) * 0.1f;
   }
 }
 
 void measure_state(orc_inst* o, float dt, float* rel, float* mdiv);
+
+// Rigid bodies after the step (the role avian2d plays for the reference):
+// a = (F / dx^2 + m_a a_prev + m g / dx) / (m + m_a) in cells/s^2 with
+// m = density * area, m_a = 4 rho area (an added-mass bound, with the previous
+// acceleration, which keeps explicit coupling of light bodies stable) and the
+// same form for the world-frame torque with J = density * inertia,
+// J_a = 2 rho inertia; bodies held inside the wall ring.
+void integrate_bodies(orc_inst* o, float dt) {
+  size_t n = o->obstacles.size();
+  if (o->bodies.size() != n || n == 0) return;
+  if (o->body_state.size() != n) o->body_state.assign(n, {0.0f, 0.0f, 0.0f});
+  float dx = o->uni.dx;
+  for (size_t k = 0; k < n; ++k) {
+    efl_obstacle& ob = o->obstacles[k];
+    const efl_body& bd = o->bodies[k];
+    if (!bd.dynamic) continue;
+    float m = bd.density * bd.area, J = bd.density * bd.inertia;
+    float ma = 4.0f * o->uni.rho * bd.area, Ja = 2.0f * o->uni.rho * bd.inertia;
+    float Fx = 0.0f, Fy = 0.0f, Tq = 0.0f;
+    uint32_t id = ob.entity_id;
+    if (id < (uint32_t)EFL_MAX_SOLIDS) {
+      float s = 1.0f / (dx * dx);
+      Fx = o->fts.at(id, 0, 0) * s;
+      Fy = -o->fts.at(id, 0, 1) * s;
+      Tq = o->fts.at(id, 0, 3) * s;
+    }
+    std::array<float, 3>& prev = o->body_state[k];
+    float ax = (Fx + ma * prev[0] + m * o->uni.g[0] / dx) / (m + ma);
+    float ay = (Fy + ma * prev[1] + m * (-o->uni.g[1]) / dx) / (m + ma);
+    float al = (J + Ja) > 0.0f ? (Tq + Ja * prev[2]) / (J + Ja) : 0.0f;
+    prev = {ax, ay, al};
+    float vx = ob.linear_velocity[0] + ax * dt;
+    float vy = ob.linear_velocity[1] + ay * dt;
+    float w = ob.angular_velocity + al * dt;
+    float tx = ob.transform[12] + vx * dt, ty = ob.transform[13] + vy * dt;
+    float ang = std::atan2(ob.transform[1], ob.transform[0]) + w * dt;
+    const float* v = ob.values;
+    float R = 0.0f;
+    switch (ob.shape) {
+      case 0: R = v[0]; break;
+      case 1: R = wlen(v[0], v[1]); break;
+      case 2: R = std::fmax(wlen(v[0], v[1]), wlen(v[2], v[3])) + v[4]; break;
+      case 4: R = std::fmax(std::fmax(wlen(v[0], v[1]), wlen(v[2], v[3])), wlen(v[4], v[5])); break;
+      default: break;
+    }
+    float lx = std::fmax(0.5f * (float)o->W - 1.0f - R, 0.0f), ly = std::fmax(0.5f * (float)o->H - 1.0f - R, 0.0f);
+    if (tx > lx) { tx = lx; vx = std::fmin(vx, 0.0f); }
+    if (tx < -lx) { tx = -lx; vx = std::fmax(vx, 0.0f); }
+    if (ty > ly) { ty = ly; vy = std::fmin(vy, 0.0f); }
+    if (ty < -ly) { ty = -ly; vy = std::fmax(vy, 0.0f); }
+    float c = std::cos(ang), sn = std::sin(ang);
+    float T[16] = {c, sn, 0.f, 0.f, -sn, c, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, tx, ty, 0.f, 1.f};
+    float Ti[16] = {c, -sn, 0.f, 0.f, sn, c, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, -(c * tx + sn * ty), -(-sn * tx + c * ty), 0.f, 1.f};
+    std::memcpy(ob.transform, T, sizeof(T));
+    std::memcpy(ob.inverse_transform, Ti, sizeof(Ti));
+    ob.linear_velocity[0] = vx;
+    ob.linear_velocity[1] = vy;
+    ob.angular_velocity = w;
+  }
+}
 
 // EulerFluidNode::run Uninitialized branch, render_node.rs:219-277
 void initialize_all(orc_inst* o) {
@@ -1402,7 +1469,7 @@ orc_inst* orc_create(const efl_settings* s) {
   o->grad.alloc(W, H, 2); o->afs.alloc(W, H, 4);
   o->seeds[0].alloc(W, H, 2); o->seeds[1].alloc(W, H, 2);
   o->lab0.alloc(W, H); o->lab.alloc(W, H);
-  o->bins.alloc(EFL_MAX_SOLIDS, 3);  // planar: force x, force y, torque
+  o->bins.alloc(EFL_MAX_SOLIDS, 4);  // planar: force x, force y, torque, world torque
   o->fts.alloc(EFL_MAX_SOLIDS, 1, 4);
   o->pcg_r.alloc(W, H); o->pcg_z.alloc(W, H); o->pcg_p.alloc(W, H); o->pcg_q.alloc(W, H);
   // multi_grid.rs:95-118
@@ -1454,8 +1521,24 @@ int orc_init_levelset(orc_inst* o, const float* ls) {
 int orc_reset(orc_inst* o) { o->initialized = false; return 0; }
 int orc_initialize(orc_inst* o) { initialize_all(o); return 0; }
 int orc_step(orc_inst* o, float dt) {
-  if (!o->initialized) initialize_all(o);
-  else step_all(o, dt);
+  if (!o->initialized) {
+    initialize_all(o);
+  } else {
+    step_all(o, dt);
+    integrate_bodies(o, dt);
+  }
+  return 0;
+}
+
+int orc_set_bodies(orc_inst* o, const efl_body* b, uint32_t n) {
+  if (n && n != o->obstacles.size()) return EFL_E_INVALID;
+  o->bodies.assign(b, b + n);
+  return 0;
+}
+
+int orc_get_obstacles(orc_inst* o, efl_obstacle* out, uint32_t n) {
+  if (n > o->obstacles.size()) return EFL_E_RANGE;
+  std::memcpy(out, o->obstacles.data(), sizeof(efl_obstacle) * n);
   return 0;
 }
 

@@ -29,6 +29,8 @@ int orc_set_uniforms(orc_inst* o, float rho, const float gravity[2], const float
 int orc_set_sources(orc_inst* o, const efl_source* s, uint32_t n, int oneshot);
 int orc_set_obstacles(orc_inst* o, const efl_obstacle* ob, uint32_t n);
 int orc_set_forces(orc_inst* o, const efl_force* f, uint32_t n);
+int orc_set_bodies(orc_inst* o, const efl_body* b, uint32_t n);
+int orc_get_obstacles(orc_inst* o, efl_obstacle* out, uint32_t n);
 int orc_init_levelset(orc_inst* o, const float* ls);
 int orc_reset(orc_inst* o);
 int orc_initialize(orc_inst* o);

@@ -60,6 +60,8 @@ def lib():
             ("orc_residual_of", [P, C.c_float, C.c_void_p, C.c_void_p]),
             ("orc_set_threads", [C.c_int]),
             ("orc_set_diagnostics", [P, C.c_int]),
+            ("orc_set_bodies", [P, C.c_void_p, C.c_uint32]),
+            ("orc_get_obstacles", [P, C.c_void_p, C.c_uint32]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -121,6 +123,15 @@ class OracleInstance:
     def set_forces(self, forces):
         arr, n = abi.struct_array(abi.efl_force, forces)
         _check(lib().orc_set_forces(self.h, arr, n), "set_forces")
+
+    def set_bodies(self, bodies):
+        arr, n = abi.struct_array(abi.efl_body, list(bodies))
+        _check(lib().orc_set_bodies(self.h, arr, n), "set_bodies")
+
+    def get_obstacles(self, n):
+        out = (abi.efl_obstacle * n)()
+        _check(lib().orc_get_obstacles(self.h, out, n), "get_obstacles")
+        return list(out)
 
     def init_levelset(self, ls):
         if ls is None:

@@ -159,3 +159,31 @@ def test_multistep_bitexact(gpu_ctx, size, proj):
     finally:
         g.close()
         o.close()
+
+
+def test_bodies_two_way_parity(gpu_ctx):
+    """Dynamic bodies integrated on the device follow the oracle's
+    trajectories (transcendentals differ by ulps, so within tolerance)."""
+    _say("bodies two-way 128x128")
+    sc = scenes.coupled_tank(128, proj=scenes.projection("mgpcg", max_cycles=100, rel_tol=1e-6), n_each=1)
+    g, o = make_pair(gpu_ctx, sc, OracleInstance)
+    bodies = scenes.bodies_for(sc["obstacles"], sc["densities"])
+    gpu_ctx.set_bodies(bodies)
+    o.set_bodies(bodies)
+    try:
+        for _ in range(16):
+            g.step(DT)
+            o.step(DT)
+        n = len(sc["obstacles"])
+        og, oo = gpu_ctx.get_obstacles(n), o.get_obstacles(n)
+        for a, b in zip(og, oo):
+            assert abs(a.transform[12] - b.transform[12]) < 1e-2
+            assert abs(a.transform[13] - b.transform[13]) < 1e-2
+            assert abs(a.linear_velocity[1] - b.linear_velocity[1]) < 1e-2 * (1 + abs(b.linear_velocity[1]))
+        moved = max(abs(b.linear_velocity[1]) for b in oo)
+        assert moved > 0.1
+    finally:
+        gpu_ctx.set_bodies([])
+        gpu_ctx.set_obstacles([])
+        g.close()
+        o.close()
