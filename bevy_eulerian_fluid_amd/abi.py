@@ -51,7 +51,7 @@ STAGE_NAMES = [
     "INIT_GRID_CENTER", "INIT_GRID_EDGE", "SOURCE_ONESHOT", "UPDATE_SOLID", "AREA_FRACTION",
     "ADVECT_VELOCITY", "APPLY_FORCES", "DIVERGENCE", "PROJECTION", "SOLVE_VELOCITY",
     "EXTRAPOLATE_VELOCITY", "ADVECT_LEVELSET", "SOURCE", "REINITIALIZE", "LEVELSET_GRADIENT",
-    "FLUID_TO_SOLID", "GS_SWEEP", "MG_VCYCLE", "MG_RESIDUAL",
+    "FLUID_TO_SOLID", "GS_SWEEP", "MG_VCYCLE", "MG_RESIDUAL", "MG_SMOOTH", "GS_SWEEP_SPLIT",
 ]
 STAGE = {n: i for i, n in enumerate(STAGE_NAMES)}
 STAGE_COUNT = len(STAGE_NAMES)

@@ -107,10 +107,12 @@ void extent_of(const efl_instance* in, Extent k, int lvl, int& ex, int& ey) {
 }
 
 // levels: per-instance Batch.level values (or -1 = use `lvl`)
-Batch make_batch(const std::vector<efl_instance*>& v, Extent k, int lvl = 0, const std::vector<int>* levels = nullptr) {
+Batch make_batch(const std::vector<efl_instance*>& v, Extent k, int lvl = 0, const std::vector<int>* levels = nullptr,
+                 int tw = 0, int th = 0) {
   Batch b;
   std::memset(&b, 0, sizeof(b));
-  int tw = efl::tile_w(), th = efl::tile_h();
+  if (tw <= 0) tw = efl::tile_w();
+  if (th <= 0) th = efl::tile_h();
   int n = 0, start = 0;
   for (size_t i = 0; i < v.size(); ++i) {
     int l = levels ? (*levels)[i] : lvl;
@@ -205,15 +207,26 @@ void stage_reinit(hipStream_t st, const IV& v) {  // reinitialize_levelset.rs:16
   }
 }
 
+// count(i) red-black sweeps (black-red with reverse_colour) of one level,
+// up to 3 sweeps per tiled launch; with `prolong`, the first launch applies
+// the prolongation from level lvl + 1 while staging (and runs even for 0
+// sweeps).
 void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_instance*), float weight, int pcg,
-               bool reverse_colour, Extent ext = EX_LEVEL) {
+               bool reverse_colour, Extent ext = EX_LEVEL, bool prolong = false) {
   uint32_t M = 0;
   for (auto* i : v) M = std::max(M, count(i));
-  for (uint32_t k = 0; k < M; ++k) {
-    IV part = filter(v, [&](efl_instance* i) { return count(i) > k; });
-    Batch b = make_batch(part, ext, lvl);
-    efl::launch_gs_color(st, b, reverse_colour ? 0 : 1, weight, pcg);
-    efl::launch_gs_color(st, b, reverse_colour ? 1 : 0, weight, pcg);
+  const int tw = efl::mg_tile_w(), th = efl::mg_tile_h();
+  for (uint32_t k = 0; k < M || (prolong && k == 0); k += 3) {
+    for (uint32_t n = 0; n <= 3; ++n) {
+      IV part = filter(v, [&](efl_instance* i) {
+        uint32_t c = count(i);
+        uint32_t left = c > k ? std::min<uint32_t>(3, c - k) : 0;
+        return left == n && (n > 0 || (prolong && k == 0));
+      });
+      if (part.empty()) continue;
+      Batch b = make_batch(part, ext, lvl, nullptr, tw, th);
+      efl::launch_smooth_tile(st, b, pcg, (int)n, reverse_colour ? 0 : 1, 0, weight, (prolong && k == 0) ? 1 : 0);
+    }
   }
 }
 
@@ -222,22 +235,50 @@ uint32_t cnt_post(efl_instance* i) { return i->s.projection.mg_post_iterations; 
 uint32_t cnt_coarse(efl_instance* i) { return i->s.projection.mg_coarsest_iterations; }
 uint32_t cnt_gs(efl_instance* i) { return i->s.projection.gs_iterations; }
 
-// v_cycle, multi_grid.rs:390-475 (symmetric: see the oracle's v_cycle)
+// First level whose grid holds at most mg_bottom_cells() cells: from there
+// the rest of the cycle runs in one workgroup.  -1 if none.
+int bottom_level(const efl_instance* i) {
+  for (int l = 0; l < i->h.nlev; ++l)
+    if (i->h.lev[l].w * i->h.lev[l].h <= efl::mg_bottom_cells()) return l;
+  return -1;
+}
+
+// v_cycle, multi_grid.rs:390-475 (symmetric: post-smoothing black-red and
+// the coarsest level also run back in reverse colour order, see the oracle)
 void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
   if (v.empty()) return;
-  IV coarse = filter(v, [l](efl_instance* i) { return l == i->h.nlev - 1; });
-  IV rest = filter(v, [l](efl_instance* i) { return l < i->h.nlev - 1; });
+  IV bot = filter(v, [l](efl_instance* i) { return bottom_level(i) == l; });
+  IV coarse = filter(v, [l](efl_instance* i) { return bottom_level(i) < 0 && l == i->h.nlev - 1; });
+  IV rest = filter(v, [l](efl_instance* i) {
+    int b = bottom_level(i);
+    return (b < 0 || l < b) && l < i->h.nlev - 1;
+  });
+  if (!bot.empty()) {
+    Batch b;
+    std::memset(&b, 0, sizeof(b));
+    int cells = 0;
+    for (auto* i : bot) {
+      int n = b.n++;
+      b.inst[n] = i->d;
+      b.level[n] = l;
+      b.ex[n] = (int)i->s.projection.mg_pre_iterations;
+      b.ey[n] = (int)(i->s.projection.mg_post_iterations & 0xffff) |
+                (int)(i->s.projection.mg_coarsest_iterations << 16);
+      int c = 0;
+      for (int q = l; q < i->h.nlev; ++q) c += i->h.lev[q].w * i->h.lev[q].h;
+      cells = std::max(cells, c);
+    }
+    efl::launch_bottom(st, b, pcg, symmetric ? 1 : 0, cells);
+  }
   if (!coarse.empty()) {
     gs_sweeps(st, coarse, l, cnt_coarse, 1.0f, pcg, false);
     if (symmetric) gs_sweeps(st, coarse, l, cnt_coarse, 1.0f, pcg, true);
   }
   if (rest.empty()) return;
   gs_sweeps(st, rest, l, cnt_pre, 1.0f, pcg, false);
-  efl::launch_residual(st, make_batch(rest, EX_LEVEL, l), pcg);
-  efl::launch_restrict(st, make_batch(rest, EX_COARSE, l), pcg);
+  efl::launch_resid_restrict(st, make_batch(rest, EX_COARSE, l), pcg);
   v_cycle(st, rest, l + 1, pcg, symmetric);
-  efl::launch_prolong(st, make_batch(rest, EX_COARSE, l), pcg);
-  gs_sweeps(st, rest, l, cnt_post, 1.0f, pcg, symmetric);
+  gs_sweeps(st, rest, l, cnt_post, 1.0f, pcg, symmetric, EX_LEVEL, true);
 }
 
 int read_scalars(efl_ctx* c, const IV& v) {
@@ -992,7 +1033,17 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
     case EFL_STAGE_REINITIALIZE: stage_reinit(st, v); break;
     case EFL_STAGE_LEVELSET_GRADIENT: efl::launch_ls_gradient(st, center); break;
     case EFL_STAGE_FLUID_TO_SOLID: stage_fluid_to_solid(st, v); break;
-    case EFL_STAGE_GS_SWEEP: {
+    case EFL_STAGE_GS_SWEEP: {  // one red + black sweep, weight 1.9, gauss_seidel.rs:142-147
+      Batch t = make_batch(v, EX_CENTER, 0, nullptr, efl::mg_tile_w(), efl::mg_tile_h());
+      efl::launch_smooth_tile(st, t, 0, 1, 1, 0, 1.9f, 0);
+      break;
+    }
+    case EFL_STAGE_MG_SMOOTH: {  // two red-black sweeps at weight 1 on level 0 in one tiled launch
+      Batch t = make_batch(v, EX_LEVEL, 0, nullptr, efl::mg_tile_w(), efl::mg_tile_h());
+      efl::launch_smooth_tile(st, t, 0, 2, 1, 0, 1.0f, 0);
+      break;
+    }
+    case EFL_STAGE_GS_SWEEP_SPLIT: {  // one sweep as two kernels (one per colour), as the reference dispatches it
       efl::launch_gs_color(st, center, 1, 1.9f, 0);
       efl::launch_gs_color(st, center, 0, 1.9f, 0);
       break;

@@ -54,3 +54,15 @@ void launch_max_divergence(hipStream_t st, const Batch& b);
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a);
 
 }  // namespace efl
+
+namespace efl {
+// fused multigrid (efl_mg.hip)
+int mg_tile_w();
+int mg_tile_h();
+int mg_bottom_cells();
+size_t bottom_lds_bytes(int cells);
+void launch_smooth_tile(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, int alternate, float weight,
+                        int prolong);
+void launch_resid_restrict(hipStream_t st, const Batch& b, int pcg);
+void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells);
+}  // namespace efl

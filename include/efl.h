@@ -223,6 +223,8 @@ typedef enum {
   EFL_STAGE_GS_SWEEP,             /* one red + black sweep at weight 1.9 */
   EFL_STAGE_MG_VCYCLE,            /* one V-cycle (reference order) */
   EFL_STAGE_MG_RESIDUAL,          /* finest-level residual into EFL_FIELD_MG_R0 */
+  EFL_STAGE_MG_SMOOTH,            /* two red-black sweeps at weight 1 on level 0 (pre-smoothing) */
+  EFL_STAGE_GS_SWEEP_SPLIT,       /* GS_SWEEP dispatched one colour per kernel */
   EFL_STAGE_COUNT
 } efl_stage;
 

@@ -1561,11 +1561,20 @@ int orc_run_stage(orc_inst* o, int stage, float dt) {
     case EFL_STAGE_REINITIALIZE: k_reinit(o); break;
     case EFL_STAGE_LEVELSET_GRADIENT: k_levelset_gradient(o); break;
     case EFL_STAGE_FLUID_TO_SOLID: k_fluid_to_solid(o); break;
-    case EFL_STAGE_GS_SWEEP: {
+    case EFL_STAGE_GS_SWEEP:
+    case EFL_STAGE_GS_SWEEP_SPLIT: {
       Lvl<float> L{o->W, o->H, o->p0.d.data(), o->div.d.data(), o->ls0.d.data(), o->afs.d.data(), o->mr[0].d.data(), 1.0f};
       Cov c = cov_center(o);
       gs_color(L, 1, 1.9f, o->uni, c.x1, c.y1);
       gs_color(L, 0, 1.9f, o->uni, c.x1, c.y1);
+      break;
+    }
+    case EFL_STAGE_MG_SMOOTH: {  // two red-black sweeps at weight 1 on the whole level 0
+      Lvl<float> L{o->W, o->H, o->p0.d.data(), o->div.d.data(), o->ls0.d.data(), o->afs.d.data(), o->mr[0].d.data(), 1.0f};
+      for (int k = 0; k < 2; ++k) {
+        gs_color(L, 1, 1.0f, o->uni, o->W, o->H);
+        gs_color(L, 0, 1.0f, o->uni, o->W, o->H);
+      }
       break;
     }
     case EFL_STAGE_MG_VCYCLE: {
