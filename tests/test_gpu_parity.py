@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 STEP_ORDER = ["UPDATE_SOLID", "AREA_FRACTION", "ADVECT_VELOCITY", "APPLY_FORCES", "DIVERGENCE",
               "PROJECTION", "SOLVE_VELOCITY", "EXTRAPOLATE_VELOCITY", "ADVECT_LEVELSET", "SOURCE",
               "REINITIALIZE", "LEVELSET_GRADIENT", "FLUID_TO_SOLID"]
-SIZES = [(64, 64), (96, 80), (100, 70)]
+SIZES = [(64, 64), (96, 80), (100, 70), (200, 136)]
 DT = 1.0 / 60.0
 
 
@@ -82,7 +82,7 @@ def test_init_stage_parity(gpu_ctx, size, stage):
 
 
 @pytest.mark.parametrize("method", ["gauss_seidel", "jacobi", "multigrid", "gs_sweep", "mg_vcycle",
-                                    "mg_residual", "mgpcg"])
+                                    "mg_residual", "mgpcg", "mg_smooth", "gs_sweep_split", "multigrid_3cycles"])
 @pytest.mark.parametrize("size", SIZES, ids=lambda s: f"{s[0]}x{s[1]}")
 def test_projection_parity(gpu_ctx, size, method):
     _say(f"projection parity {method} {size}")
@@ -98,6 +98,12 @@ def test_projection_parity(gpu_ctx, size, method):
         stage = "MG_VCYCLE"
     elif method == "mg_residual":
         stage = "MG_RESIDUAL"
+    elif method == "mg_smooth":
+        stage = "MG_SMOOTH"
+    elif method == "gs_sweep_split":
+        stage = "GS_SWEEP_SPLIT"
+    elif method == "multigrid_3cycles":
+        proj = scenes.projection("multigrid", pre=3, post=1, coarsest=4, max_cycles=3)
     sc = small_scene(*size, proj=proj)
     g, o = make_pair(gpu_ctx, sc, OracleInstance)
     try:
