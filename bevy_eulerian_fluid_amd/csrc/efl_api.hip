@@ -211,21 +211,63 @@ void stage_reinit(hipStream_t st, const IV& v) {  // reinitialize_levelset.rs:16
 // up to 3 sweeps per tiled launch; with `prolong`, the first launch applies
 // the prolongation from level lvl + 1 while staging (and runs even for 0
 // sweeps).
+// Tiled smoothing of one level, up to 3 red-black sweeps (black-red with
+// reverse_colour) per launch.
+//   DOWN: the last launch also forms the residual and restricts it to level
+//         lvl + 1 (runs even for 0 sweeps).
+//   UP:   the first launch applies the prolongation from level lvl + 1
+//         (runs even for 0 sweeps); with rz_out, instances whose last launch
+//         is that first one get the fused z mask + r . z partials, and are
+//         returned in *rz_done with the batch used (for the finalize).
+//   PLAIN: sweeps only.
+enum SweepMode { PLAIN, DOWN, UP };
 void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_instance*), float weight, int pcg,
-               bool reverse_colour, Extent ext = EX_LEVEL, bool prolong = false) {
+               bool reverse_colour, Extent ext = EX_LEVEL, SweepMode mode = PLAIN, bool rz = false,
+               IV* rz_done = nullptr, Batch* rz_batch = nullptr) {
   uint32_t M = 0;
   for (auto* i : v) M = std::max(M, count(i));
   const int tw = efl::mg_tile_w(), th = efl::mg_tile_h();
-  for (uint32_t k = 0; k < M || (prolong && k == 0); k += 3) {
+  const int first = reverse_colour ? 0 : 1;
+  auto chunks = [&](efl_instance* i) {  // number of launches for this instance
+    uint32_t c = count(i);
+    uint32_t n = (c + 2) / 3;
+    return mode == PLAIN ? n : std::max<uint32_t>(n, 1);
+  };
+  uint32_t K = 0;
+  for (auto* i : v) K = std::max(K, chunks(i));
+  for (uint32_t k = 0; k < K; ++k) {
     for (uint32_t n = 0; n <= 3; ++n) {
       IV part = filter(v, [&](efl_instance* i) {
+        if (k >= chunks(i)) return false;
         uint32_t c = count(i);
-        uint32_t left = c > k ? std::min<uint32_t>(3, c - k) : 0;
-        return left == n && (n > 0 || (prolong && k == 0));
+        uint32_t left = c > 3 * k ? std::min<uint32_t>(3, c - 3 * k) : 0;
+        return left == n;
       });
       if (part.empty()) continue;
       Batch b = make_batch(part, ext, lvl, nullptr, tw, th);
-      efl::launch_smooth_tile(st, b, pcg, (int)n, reverse_colour ? 0 : 1, 0, weight, (prolong && k == 0) ? 1 : 0);
+      if (mode == UP && k == 0) {
+        IV fuse = filter(part, [&](efl_instance* i) { return rz && chunks(i) == 1; });
+        IV rest = filter(part, [&](efl_instance* i) { return !(rz && chunks(i) == 1); });
+        if (!fuse.empty()) {
+          Batch bf = make_batch(fuse, ext, lvl, nullptr, tw, th);
+          efl::launch_mg_up(st, bf, pcg, (int)n, first, weight, 1);
+          if (rz_done) *rz_done = fuse;
+          if (rz_batch) *rz_batch = bf;
+        }
+        if (!rest.empty()) efl::launch_mg_up(st, make_batch(rest, ext, lvl, nullptr, tw, th), pcg, (int)n, first, weight, 0);
+      } else {
+        bool last = false;
+        for (auto* i : part) last = last || (k + 1 == chunks(i));
+        if (mode == DOWN && last) {
+          IV fin = filter(part, [&](efl_instance* i) { return k + 1 == chunks(i); });
+          IV more = filter(part, [&](efl_instance* i) { return k + 1 != chunks(i); });
+          efl::launch_mg_down(st, make_batch(fin, ext, lvl, nullptr, tw, th), pcg, (int)n, first, weight, 1);
+          if (!more.empty())
+            efl::launch_mg_down(st, make_batch(more, ext, lvl, nullptr, tw, th), pcg, (int)n, first, weight, 0);
+        } else {
+          efl::launch_mg_down(st, b, pcg, (int)n, first, weight, 0);
+        }
+      }
     }
   }
 }
@@ -275,10 +317,40 @@ void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
     if (symmetric) gs_sweeps(st, coarse, l, cnt_coarse, 1.0f, pcg, true);
   }
   if (rest.empty()) return;
-  gs_sweeps(st, rest, l, cnt_pre, 1.0f, pcg, false);
-  efl::launch_resid_restrict(st, make_batch(rest, EX_COARSE, l), pcg);
+  gs_sweeps(st, rest, l, cnt_pre, 1.0f, pcg, false, EX_LEVEL, DOWN);
   v_cycle(st, rest, l + 1, pcg, symmetric);
-  gs_sweeps(st, rest, l, cnt_post, 1.0f, pcg, symmetric, EX_LEVEL, true);
+  gs_sweeps(st, rest, l, cnt_post, 1.0f, pcg, symmetric, EX_LEVEL, UP);
+}
+
+// z = M r (one symmetric V-cycle from z = 0), z masked to the pressure
+// unknowns, rz = r . z and beta (finalize kind 3).  Instances whose level-0
+// post-smoothing fits one launch get the mask and the partial sums fused
+// into it; the others take the separate k_pcg_rz pass.
+void precondition(hipStream_t st, const IV& v) {
+  efl::launch_zero_z(st, make_batch(v, EX_FULL));
+  IV bot = filter(v, [](efl_instance* i) { return bottom_level(i) == 0; });
+  IV tiled = filter(v, [](efl_instance* i) { return bottom_level(i) != 0; });
+  if (!bot.empty()) v_cycle(st, bot, 0, 1, true);
+  IV fused;
+  Batch fb;
+  std::memset(&fb, 0, sizeof(fb));
+  if (!tiled.empty()) {
+    // level 0 by hand so that its post-smoothing can carry r . z
+    IV coarse = filter(tiled, [](efl_instance* i) { return bottom_level(i) < 0 && i->h.nlev == 1; });
+    IV rest = filter(tiled, [](efl_instance* i) { return !(bottom_level(i) < 0 && i->h.nlev == 1); });
+    if (!coarse.empty()) {
+      gs_sweeps(st, coarse, 0, cnt_coarse, 1.0f, 1, false);
+      gs_sweeps(st, coarse, 0, cnt_coarse, 1.0f, 1, true);
+    }
+    if (!rest.empty()) {
+      gs_sweeps(st, rest, 0, cnt_pre, 1.0f, 1, false, EX_LEVEL, DOWN);
+      v_cycle(st, rest, 1, 1, true);
+      gs_sweeps(st, rest, 0, cnt_post, 1.0f, 1, true, EX_LEVEL, UP, true, &fused, &fb);
+    }
+  }
+  if (!fused.empty()) efl::launch_finalize(st, fb, 3);
+  IV sep = filter(v, [&](efl_instance* i) { return std::find(fused.begin(), fused.end(), i) == fused.end(); });
+  if (!sep.empty()) efl::launch_pcg_rz(st, make_batch(sep, EX_FULL));
 }
 
 int read_scalars(efl_ctx* c, const IV& v) {
@@ -367,9 +439,7 @@ int stage_projection(efl_ctx* c, const IV& v) {  // projection.rs:89-164
     if (!act.empty()) {
       Batch b = make_batch(act, EX_FULL);
       efl::launch_pcg_first(st, b);
-      efl::launch_zero_z(st, b);
-      v_cycle(st, act, 0, 1, true);
-      efl::launch_pcg_rz(st, b);
+      precondition(st, act);
       efl::launch_pcg_update_p(st, b);
     }
     uint32_t k = 0;
@@ -403,9 +473,7 @@ int stage_projection(efl_ctx* c, const IV& v) {  // projection.rs:89-164
       act.swap(keep);
       if (!act.empty()) {
         Batch bb = make_batch(act, EX_FULL);
-        efl::launch_zero_z(st, bb);
-        v_cycle(st, act, 0, 1, true);
-        efl::launch_pcg_rz(st, bb);
+        precondition(st, act);
         efl::launch_pcg_update_p(st, bb);
       }
     }
@@ -1035,12 +1103,12 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
     case EFL_STAGE_FLUID_TO_SOLID: stage_fluid_to_solid(st, v); break;
     case EFL_STAGE_GS_SWEEP: {  // one red + black sweep, weight 1.9, gauss_seidel.rs:142-147
       Batch t = make_batch(v, EX_CENTER, 0, nullptr, efl::mg_tile_w(), efl::mg_tile_h());
-      efl::launch_smooth_tile(st, t, 0, 1, 1, 0, 1.9f, 0);
+      efl::launch_mg_down(st, t, 0, 1, 1, 1.9f, 0);
       break;
     }
     case EFL_STAGE_MG_SMOOTH: {  // two red-black sweeps at weight 1 on level 0 in one tiled launch
       Batch t = make_batch(v, EX_LEVEL, 0, nullptr, efl::mg_tile_w(), efl::mg_tile_h());
-      efl::launch_smooth_tile(st, t, 0, 2, 1, 0, 1.0f, 0);
+      efl::launch_mg_down(st, t, 0, 2, 1, 1.0f, 0);
       break;
     }
     case EFL_STAGE_GS_SWEEP_SPLIT: {  // one sweep as two kernels (one per colour), as the reference dispatches it

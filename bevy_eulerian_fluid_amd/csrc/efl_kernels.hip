@@ -1286,6 +1286,7 @@ void launch_residual_norm(hipStream_t st, const Batch& b, int pcg_init) {
   EFL_LAUNCH(k_residual_norm, b, st, pcg_init);
   EFL_LAUNCH_PER_INST(k_finalize, b, st, 0);
 }
+void launch_finalize(hipStream_t st, const Batch& b, int kind) { EFL_LAUNCH_PER_INST(k_finalize, b, st, kind); }
 void launch_pcg_rz(hipStream_t st, const Batch& b) {
   EFL_LAUNCH(k_pcg_rz, b, st);
   EFL_LAUNCH_PER_INST(k_finalize, b, st, 3);

@@ -61,8 +61,8 @@ int mg_tile_w();
 int mg_tile_h();
 int mg_bottom_cells();
 size_t bottom_lds_bytes(int cells);
-void launch_smooth_tile(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, int alternate, float weight,
-                        int prolong);
-void launch_resid_restrict(hipStream_t st, const Batch& b, int pcg);
+void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_);
+void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz);
+void launch_finalize(hipStream_t st, const Batch& b, int kind);
 void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells);
 }  // namespace efl
