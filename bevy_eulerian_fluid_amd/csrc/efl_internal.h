@@ -25,6 +25,7 @@ struct Level {  // one multigrid level (multi_grid.rs:99-118)
   float* ls;
   float4* af;
   float* r;
+  float* xs;  // second buffer of x: the tiled smoothers read one and write the other
 };
 
 // Per-instance descriptor, copied to device memory; kernels read it through
@@ -91,6 +92,7 @@ struct Batch {
   int tiles_x[kMaxBatch];
   int ex[kMaxBatch], ey[kMaxBatch];
   int level[kMaxBatch];
+  int sel[kMaxBatch];  // fused multigrid: bit 0 x is in Level::xs, bit 1 the coarse x is
   const DevInst* inst[kMaxBatch];
 };
 

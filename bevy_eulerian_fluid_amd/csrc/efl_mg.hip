@@ -19,6 +19,12 @@
 // Red-black Gauss-Seidel inside a colour is order independent, so these
 // kernels produce exactly the values of the one-kernel-per-half-sweep
 // reference dataflow (same operation order per cell; -ffp-contract=off).
+//
+// A block's halo belongs to other blocks' tiles, so a tiled launch never
+// writes the x it reads: x and Level::xs form a pair, each launch reads one
+// (Batch::sel) and writes every in-level cell of its tiles to the other.
+// Writing in place would let a block that starts late stage neighbour
+// values that were already smoothed.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -38,6 +44,25 @@ __device__ __forceinline__ bool mg_inb(int x, int y, int w, int h) {
 }
 __device__ __forceinline__ bool closed4(float4 f) { return f.x == 0.0f && f.y == 0.0f && f.z == 0.0f && f.w == 0.0f; }
 
+// Level arrays are reached through pointers held in device memory, which the
+// compiler can only treat as generic (flat) addresses; these name the global
+// address space so the accesses become global_load / global_store.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float ldg(const float* p, size_t i) {
+  return ((const __attribute__((address_space(1))) float*)p)[i];
+}
+__device__ __forceinline__ float4 ldg(const float4* p, size_t i) {
+  const f4v v = ((const __attribute__((address_space(1))) f4v*)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void stg(float* p, size_t i, float v) {
+  ((__attribute__((address_space(1))) float*)p)[i] = v;
+}
+__device__ __forceinline__ void stg(float4* p, size_t i, float4 v) {
+  const f4v w = {v.x, v.y, v.z, v.w};
+  ((__attribute__((address_space(1))) f4v*)p)[i] = w;
+}
+
 __device__ __forceinline__ const Level& mg_level(const DevInst* I, int l, int pcg) {
   return (pcg && l == 0) ? I->lev_pcg0 : I->lev[l];
 }
@@ -48,6 +73,7 @@ struct MSite {
   int lvl;
   int ex, ey;    // update extent
   int slot;
+  int sel;       // buffer selectors (Batch::sel)
 };
 
 __device__ __forceinline__ MSite mg_locate(const Batch& b) {
@@ -63,62 +89,86 @@ __device__ __forceinline__ MSite mg_locate(const Batch& b) {
   s.ex = b.ex[i];
   s.ey = b.ey[i];
   s.slot = i;
+  s.sel = b.sel[i];
   return s;
 }
 
-// The GS update of gauss_seidel.wgsl:44-88 on LDS-staged data; neighbour k
-// sits at LDS index q + ox[k] + oy[k] * rowlen, in-bounds tests use global
-// coordinates.
-__device__ __forceinline__ float gs_lds(const float* sx, const float* sb, const float* sl, const float4* sa, int q,
-                                        int gx, int gy, int w, int h, int rowlen, float weight, float c0) {
-  float lc = sl[q];
-  if (lc >= 0.0f) return 0.0f;
-  float4 f = sa[q];
-  float p_old = sx[q];
-  if (closed4(f)) return 0.0f;
-  float denom = 0.0f;
-  float nume = c0 * sb[q];
-  const float fv[4] = {f.x, f.y, f.z, f.w};
-  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (mg_inb(gx + ox[k], gy + oy[k], w, h)) {
-      int j = q + ox[k] + oy[k] * rowlen;
-      float lv = sl[j];
-      if (lv < 0.0f) {
-        denom += fv[k];
-        nume += fv[k] * sx[j];
-      } else {
-        denom += fv[k] * (1.0f - lv / lc);
-      }
-    }
-  }
-  if (fabsf(denom) < 1.0e-6f) return 0.0f;
-  float pn = nume / denom;
-  return weight * pn + (1.0f - weight) * p_old;
+// LDS indices of the four neighbours (-x, +x, -y, +y) of LDS cell q whose
+// level cell is (gx, gy); a neighbour outside the level maps to q itself so
+// every read stays inside the staged region.
+struct Nbr {
+  bool in[4];
+  int j[4];
+};
+__device__ __forceinline__ Nbr mg_nbr(int q, int gx, int gy, int w, int h, int rowlen) {
+  Nbr n;
+  n.in[0] = gx > 0;
+  n.in[1] = gx + 1 < w;
+  n.in[2] = gy > 0;
+  n.in[3] = gy + 1 < h;
+  n.j[0] = n.in[0] ? q - 1 : q;
+  n.j[1] = n.in[1] ? q + 1 : q;
+  n.j[2] = n.in[2] ? q - rowlen : q;
+  n.j[3] = n.in[3] ? q + rowlen : q;
+  return n;
 }
 
-// residual.wgsl:12-57 on LDS-staged data
-__device__ __forceinline__ float resid_lds(const float* sx, const float* sb, const float* sl, const float4* sa, int q,
-                                           int gx, int gy, int w, int h, int rowlen, float factor) {
-  float phi = sl[q];
-  if (phi >= 0.0f) return 0.0f;
-  float4 f = sa[q];
-  if (closed4(f)) return 0.0f;
-  float xc = sx[q];
-  float res = sb[q];
-  const float fv[4] = {f.x, f.y, f.z, f.w};
-  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+// The GS update of gauss_seidel.wgsl:44-88 on LDS-staged data, written
+// without branches: every LDS read of the cell is issued up front and the
+// early exits and the fluid/air/outside cases of each neighbour become
+// selects.  A skipped term leaves its sum untouched (no "+ 0"), so the
+// arithmetic is exactly that of the branching form.
+__device__ __forceinline__ float gs_lds(const float* sx, const float* sb, const float* sl, const float4* sa, int q,
+                                        int gx, int gy, int w, int h, int rowlen, float weight, float c0) {
+  const Nbr n = mg_nbr(q, gx, gy, w, h, rowlen);
+  const float lc = sl[q], p_old = sx[q], bq = sb[q];
+  const float4 f = sa[q];
+  float lv[4], xv[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    if (mg_inb(gx + ox[k], gy + oy[k], w, h)) {
-      int j = q + ox[k] + oy[k] * rowlen;
-      float pn = sl[j];
-      if (pn < 0.0f) res -= fv[k] * (xc - sx[j]) * factor;
-      else res -= fv[k] * (1.0f - pn / phi) * xc * factor;
-    }
+    lv[k] = sl[n.j[k]];
+    xv[k] = sx[n.j[k]];
   }
-  return res;
+  const float fv[4] = {f.x, f.y, f.z, f.w};
+  float denom = 0.0f;
+  float nume = c0 * bq;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const bool fl = lv[k] < 0.0f;
+    const float d_fl = denom + fv[k];
+    const float n_fl = nume + fv[k] * xv[k];
+    const float d_air = denom + fv[k] * (1.0f - lv[k] / lc);
+    denom = n.in[k] ? (fl ? d_fl : d_air) : denom;
+    nume = (n.in[k] && fl) ? n_fl : nume;
+  }
+  const float pn = nume / denom;
+  const float v = weight * pn + (1.0f - weight) * p_old;
+  const bool zero = lc >= 0.0f || closed4(f) || fabsf(denom) < 1.0e-6f;
+  return zero ? 0.0f : v;
+}
+
+// residual.wgsl:12-57 on LDS-staged data, branch-free like gs_lds
+__device__ __forceinline__ float resid_lds(const float* sx, const float* sb, const float* sl, const float4* sa, int q,
+                                           int gx, int gy, int w, int h, int rowlen, float factor) {
+  const Nbr n = mg_nbr(q, gx, gy, w, h, rowlen);
+  const float phi = sl[q], xc = sx[q];
+  const float4 f = sa[q];
+  float res = sb[q];
+  float lv[4], xv[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    lv[k] = sl[n.j[k]];
+    xv[k] = sx[n.j[k]];
+  }
+  const float fv[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float r_fl = res - fv[k] * (xc - xv[k]) * factor;
+    const float r_air = res - fv[k] * (1.0f - lv[k] / phi) * xc * factor;
+    res = n.in[k] ? (lv[k] < 0.0f ? r_fl : r_air) : res;
+  }
+  const bool zero = phi >= 0.0f || closed4(f);
+  return zero ? 0.0f : res;
 }
 
 // dx * dx * rho / dt and dt / rho / dx / dx, evaluated left to right as the
@@ -134,37 +184,42 @@ __device__ __forceinline__ float resid_factor(const DevInst* I, float scale) {
 
 // Stage [tile + halo] of a level into LDS.  All global loads of a thread are
 // issued before its LDS stores (latency overlap).  Cells outside the level
-// read 0.  With C, prolongation.wgsl is applied to x on the fly.
+// read 0.  x comes from xin; with C, prolongation.wgsl adds the coarse x in
+// cxin on the fly.
 template <int HALO>
-__device__ __forceinline__ void stage_tile(const Level& L, const Level* C, int x0, int y0, float* sx, float* sb,
-                                           float* sl, float4* sa) {
+__device__ __forceinline__ void stage_tile(const Level& L, const float* xin, const Level* C, const float* cxin, int x0,
+                                           int y0, float* sx, float* sb, float* sl, float4* sa) {
   constexpr int RW = MT_W + 2 * HALO, RH = MT_H + 2 * HALO, RN = RW * RH;
   constexpr int NIT = (RN + MNT - 1) / MNT;
-  float rx[NIT], rb[NIT], rl[NIT];
+  float rx[NIT], rb[NIT], rl[NIT], rc[NIT];
   float4 ra[NIT];
+  bool par[NIT];
 #pragma unroll
   for (int k = 0; k < NIT; ++k) {
     int q = threadIdx.x + k * MNT;
     int gx = x0 - HALO + q % RW, gy = y0 - HALO + q / RW;
-    rx[k] = rb[k] = rl[k] = 0.0f;
+    rx[k] = rb[k] = rl[k] = rc[k] = 0.0f;
     ra[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    par[k] = false;
     if (q < RN && mg_inb(gx, gy, L.w, L.h)) {
       size_t id = (size_t)gy * L.w + gx;
-      rx[k] = L.x[id];
-      rb[k] = L.b[id];
-      rl[k] = L.ls[id];
-      ra[k] = L.af[id];
+      rx[k] = ldg(xin, id);
+      rb[k] = ldg(L.b, id);
+      rl[k] = ldg(L.ls, id);
+      ra[k] = ldg(L.af, id);
+      // the coarse value is fetched with the fine ones and used below only
+      // on fluid cells
+      int cx = gx >> 1, cy = gy >> 1;
+      if (C && cx < C->w && cy < C->h) {
+        par[k] = true;
+        rc[k] = ldg(cxin, (size_t)cy * C->w + cx);
+      }
     }
   }
   if (C) {  // prolongation: x_f += x_c on fluid fine cells with a parent
 #pragma unroll
-    for (int k = 0; k < NIT; ++k) {
-      int q = threadIdx.x + k * MNT;
-      int gx = x0 - HALO + q % RW, gy = y0 - HALO + q / RW;
-      int cx = gx >> 1, cy = gy >> 1;
-      if (q < RN && mg_inb(gx, gy, L.w, L.h) && cx < C->w && cy < C->h && rl[k] < 0.0f)
-        rx[k] = C->x[(size_t)cy * C->w + cx] + rx[k];
-    }
+    for (int k = 0; k < NIT; ++k)
+      if (par[k] && rl[k] < 0.0f) rx[k] = rc[k] + rx[k];
   }
 #pragma unroll
   for (int k = 0; k < NIT; ++k) {
@@ -178,43 +233,63 @@ __device__ __forceinline__ void stage_tile(const Level& L, const Level* C, int x
   }
 }
 
-// 2 * nsweeps half-sweeps on the staged region (first colour of each sweep:
-// 1 red, 0 black).  Threads walk the cells of one colour directly.  Cells
-// outside [0, ex) x [0, ey) are not updated (dispatch coverage).
-template <int HALO>
-__device__ __forceinline__ void sweep_tile(const Level& L, int x0, int y0, int ex, int ey, int nsweeps, int first,
-                                           float weight, float c0, float* sx, const float* sb, const float* sl,
-                                           const float4* sa) {
+// 2 * NS half-sweeps on the staged region (first colour of each sweep:
+// 1 red, 0 black).  Threads walk the cells of one colour directly; a thread
+// computes all of its cells of a half-sweep before it stores any (cells of
+// one colour read only the other colour), so their LDS reads overlap.  Cells
+// outside [0, ex) x [0, ey) are not updated (dispatch coverage); half-sweep
+// hs updates only cells at least hs + 1 from the region border, whose
+// neighbours are still exact.
+template <int HALO, int NS>
+__device__ __forceinline__ void sweep_tile(int w, int h, int x0, int y0, int ex, int ey, int first, float weight,
+                                           float c0, float* sx, const float* sb, const float* sl, const float4* sa) {
   constexpr int RW = MT_W + 2 * HALO, RH = MT_H + 2 * HALO, HW = RW / 2;
-  const int base = (x0 - HALO + y0 - HALO) & 1;
-  int hs = 0;
-  for (int s = 0; s < nsweeps; ++s) {
-    for (int half = 0; half < 2; ++half, ++hs) {
-      const int parity = half == 0 ? first : 1 - first;
-      const int m = hs + 1;
-      for (int t = threadIdx.x; t < HW * RH; t += MNT) {
-        int ly = t / HW;
-        int lx = 2 * (t % HW) + ((parity + ly + base) & 1);
-        if (lx < m || lx >= RW - m || ly < m || ly >= RH - m) continue;
-        int gx = x0 - HALO + lx, gy = y0 - HALO + ly;
-        if (gx < 0 || gy < 0 || gx >= ex || gy >= ey || gx >= L.w || gy >= L.h) continue;
-        int q = ly * RW + lx;
-        sx[q] = gs_lds(sx, sb, sl, sa, q, gx, gy, L.w, L.h, RW, weight, c0);
-      }
-      __syncthreads();
+  constexpr int NIT = (HW * RH + MNT - 1) / MNT;
+  const int ox = x0 - HALO, oy = y0 - HALO;  // level coordinates of LDS (0, 0)
+  const int base = (ox + oy) & 1;
+  // local bounds from the level and the dispatch extent
+  const int bx0 = -ox, by0 = -oy, bx1 = min(ex, w) - ox, by1 = min(ey, h) - oy;
+  int ly[NIT], lx2[NIT];
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int t = threadIdx.x + k * MNT;
+    ly[k] = t / HW;  // >= RH for t past the colour's cells: never active
+    lx2[k] = 2 * (t - ly[k] * HW);
+  }
+#pragma unroll
+  for (int hs = 0; hs < 2 * NS; ++hs) {
+    const int parity = (hs & 1) == 0 ? first : 1 - first;
+    const int m = hs + 1;
+    const int lo_x = max(m, bx0), hi_x = min(RW - m, bx1), lo_y = max(m, by0), hi_y = min(RH - m, by1);
+    float nv[NIT];
+    int qq[NIT];
+    bool act[NIT];
+#pragma unroll
+    for (int k = 0; k < NIT; ++k) {
+      const int lx = lx2[k] + ((parity + ly[k] + base) & 1);
+      act[k] = lx >= lo_x && lx < hi_x && ly[k] >= lo_y && ly[k] < hi_y;
+      // an inactive slot evaluates an interior dummy cell and discards it
+      qq[k] = act[k] ? ly[k] * RW + lx : RW + 1;
+      nv[k] = gs_lds(sx, sb, sl, sa, qq[k], ox + lx, oy + ly[k], w, h, RW, weight, c0);
     }
+#pragma unroll
+    for (int k = 0; k < NIT; ++k)
+      if (act[k]) sx[qq[k]] = nv[k];
+    __syncthreads();
   }
 }
 
+// Every in-level cell of the tile goes to xout, updated or not, so xout is a
+// complete copy of x after the launch.
 template <int HALO>
-__device__ __forceinline__ void write_tile(const Level& L, int x0, int y0, int ex, int ey, const float* sx) {
+__device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, int y0, const float* sx) {
   constexpr int RW = MT_W + 2 * HALO;
 #pragma unroll
   for (int k = 0; k < (MT_W * MT_H) / MNT; ++k) {
     int t = threadIdx.x + k * MNT;
     int lx = t % MT_W, ly = t / MT_W;
     int gx = x0 + lx, gy = y0 + ly;
-    if (gx < ex && gy < ey && gx < L.w && gy < L.h) L.x[(size_t)gy * L.w + gx] = sx[(ly + HALO) * RW + lx + HALO];
+    if (gx < L.w && gy < L.h) stg(xout, (size_t)gy * L.w + gx, sx[(ly + HALO) * RW + lx + HALO]);
   }
 }
 
@@ -230,15 +305,17 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   __shared__ float4 sa[RN];
   MSite s = mg_locate(b);
   const DevInst* I = s.I;
-  const Level& L = mg_level(I, s.lvl, pcg);
-  stage_tile<HALO>(L, nullptr, s.tx0, s.ty0, sx, sb, sl, sa);
+  const Level L = mg_level(I, s.lvl, pcg);
+  float* xin = (s.sel & 1) ? L.xs : L.x;
+  float* xout = (s.sel & 1) ? L.x : L.xs;
+  stage_tile<HALO>(L, xin, nullptr, nullptr, s.tx0, s.ty0, sx, sb, sl, sa);
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
-  sweep_tile<HALO>(L, s.tx0, s.ty0, s.ex, s.ey, NS, first, weight, gs_c0(I, scale), sx, sb, sl, sa);
-  write_tile<HALO>(L, s.tx0, s.ty0, s.ex, s.ey, sx);
+  sweep_tile<HALO, NS>(L.w, L.h, s.tx0, s.ty0, s.ex, s.ey, first, weight, gs_c0(I, scale), sx, sb, sl, sa);
+  write_tile<HALO>(L, xout, s.tx0, s.ty0, sx);
   if (!restrict_) return;
   // one coarse cell per thread: 32 x 8 coarse cells under the 64 x 16 tile
-  const Level& Cc = I->lev[s.lvl + 1];
+  const Level Cc = I->lev[s.lvl + 1];
   const int cx = s.tx0 / 2 + (int)(threadIdx.x % 32), cy = s.ty0 / 2 + (int)(threadIdx.x / 32);
   if (cx >= Cc.w || cy >= Cc.h) return;
   const float factor = resid_factor(I, scale);
@@ -260,10 +337,11 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   if (fabsf(p3) < fabsf(pr)) pr = p3;
   float4 a00 = sa[q00], a10 = sa[q10], a01 = sa[q01], a11 = sa[q11];
   size_t id = (size_t)cy * Cc.w + cx;
-  Cc.af[id] = make_float4(0.5f * (a00.x + a01.x), 0.5f * (a10.y + a11.y), 0.5f * (a00.z + a10.z), 0.5f * (a01.w + a11.w));
-  Cc.b[id] = br;
-  Cc.ls[id] = pr;
-  Cc.x[id] = 0.0f;
+  stg(Cc.af, id,
+      make_float4(0.5f * (a00.x + a01.x), 0.5f * (a10.y + a11.y), 0.5f * (a00.z + a10.z), 0.5f * (a01.w + a11.w)));
+  stg(Cc.b, id, br);
+  stg(Cc.ls, id, pr);
+  stg(Cc.x, id, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -279,14 +357,16 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   __shared__ double wsum[MNT / 64];
   MSite s = mg_locate(b);
   const DevInst* I = s.I;
-  const Level& L = mg_level(I, s.lvl, pcg);
-  const Level* C = &I->lev[s.lvl + 1];
-  stage_tile<HALO>(L, C, s.tx0, s.ty0, sx, sb, sl, sa);
+  const Level L = mg_level(I, s.lvl, pcg);
+  const Level C = I->lev[s.lvl + 1];
+  float* xin = (s.sel & 1) ? L.xs : L.x;
+  float* xout = (s.sel & 1) ? L.x : L.xs;
+  stage_tile<HALO>(L, xin, &C, (s.sel & 2) ? C.xs : C.x, s.tx0, s.ty0, sx, sb, sl, sa);
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
-  sweep_tile<HALO>(L, s.tx0, s.ty0, s.ex, s.ey, NS, first, weight, gs_c0(I, scale), sx, sb, sl, sa);
+  sweep_tile<HALO, NS>(L.w, L.h, s.tx0, s.ty0, s.ex, s.ey, first, weight, gs_c0(I, scale), sx, sb, sl, sa);
   if (!rz) {
-    write_tile<HALO>(L, s.tx0, s.ty0, s.ex, s.ey, sx);
+    write_tile<HALO>(L, xout, s.tx0, s.ty0, sx);
     return;
   }
   double acc = 0.0;
@@ -299,7 +379,7 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
       int q = (ly + HALO) * RW + lx + HALO;
       float z = sx[q];
       if (!(sl[q] < 0.0f) || closed4(sa[q])) z = 0.0f;
-      L.x[(size_t)gy * L.w + gx] = z;
+      stg(xout, (size_t)gy * L.w + gx, z);
       acc += (double)sb[q] * (double)z;
     }
   }
@@ -315,7 +395,9 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
 
 // ---------------------------------------------------------------------------
 // Bottom of the V-cycle in one workgroup of 1024 threads (32 x 32): levels
-// l0 .. nlev-1 staged in LDS; the level table lives in LDS too.
+// l0 .. nlev-1 staged in LDS; the level table lives in LDS too.  One
+// workgroup owns the whole level, so x is read from and written back to the
+// buffer Batch::sel names.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void b_sweep(int w, int h, int off, float* x, const float* bb, const float* ls,
                                         const float4* af, int parity, float c0) {
@@ -355,12 +437,13 @@ __global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric
   float* bb = x + total;
   float* ls = bb + total;
   {
-    const Level& L = mg_level(I, l0, pcg);
+    const Level L = mg_level(I, l0, pcg);
+    const float* xb = (b.sel[slot] & 1) ? L.xs : L.x;
     for (int t = threadIdx.x; t < L.w * L.h; t += 1024) {
-      x[t] = L.x[t];
-      bb[t] = L.b[t];
-      ls[t] = L.ls[t];
-      af[t] = L.af[t];
+      x[t] = ldg(xb, t);
+      bb[t] = ldg(L.b, t);
+      ls[t] = ldg(L.ls, t);
+      af[t] = ldg(L.af, t);
     }
   }
   __syncthreads();
@@ -439,8 +522,9 @@ __global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric
       }
     }
   }
-  const Level& L = mg_level(I, l0, pcg);
-  for (int t = threadIdx.x; t < L.w * L.h; t += 1024) L.x[t] = x[t];
+  const Level L = mg_level(I, l0, pcg);
+  float* xb = (b.sel[slot] & 1) ? L.xs : L.x;
+  for (int t = threadIdx.x; t < L.w * L.h; t += 1024) stg(xb, t, x[t]);
 }
 
 // ---------------------------------------------------------------------------
