@@ -46,6 +46,40 @@ def build_scene(size, rel_tol):
     return scenes.coupled_tank(size, proj=proj)
 
 
+def roofline_only(args):
+    """Initialise the workload, run two steps, then only the timed smoother
+    launches: a profile of this run holds the roofline kernel alone at level 0."""
+    from bevy_eulerian_fluid_amd import native, scenes
+    sc = build_scene(args.size, args.rel_tol)
+    ctx = native.Context(0)
+    inst = ctx.instance(sc["settings"])
+    inst.set_sources(sc["init_sources"], True)
+    ctx.set_obstacles(sc["obstacles"])
+    inst.step(DT)
+    inst.step(DT)
+    ctx.sync()
+    ms = inst.time_stage("MG_SMOOTH", DT, args.kernel_reps)
+    n = args.size * args.size
+    print(json.dumps({"kernel": "k_mg_down<2,5> level 0 (MG_SMOOTH)", "grid": [args.size, args.size],
+                      "reps": args.kernel_reps, "ms": ms, "algorithmic_bytes": 32 * n,
+                      "achieved_gbs": 32 * n / (ms * 1e-3) / 1e9}), flush=True)
+    inst.close()
+    ctx.close()
+
+
+def copy_probe(args):
+    """A plain device-to-device copy of known size (256 MiB, 10 times): the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 4 * n, "reps": 10}), flush=True)
+
+
 def gpu_run(args):
     from bevy_eulerian_fluid_amd import native, scenes
     sc = build_scene(args.size, args.rel_tol)
@@ -139,11 +173,21 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=25.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="time only the roofline kernel (for profiling it alone)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is "
               "a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args)
+        return
+    if args.roofline_only:
+        roofline_only(args)
+        return
     g = gpu_run(args)
     cpu = None if args.no_cpu else cpu_baseline(args)
     out = {

@@ -39,8 +39,11 @@ struct efl_ctx {
   uint32_t nobs = 0;
   uint64_t obs_version = 0;
   bool timing = false;
-  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  // efl_time_stage: ev[4] / ev[5] bracket the principal kernel of a stage
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // efl_time_stage: kev[2k] / kev[2k + 1] bracket the principal kernel of
+  // repetition k of a stage
+  std::vector<hipEvent_t> kev;
+  uint32_t krep = 0;
   bool kmark = false, kmarked = false;
   double* h_scal = nullptr;  // pinned, 16 doubles per batch slot
   std::vector<efl_instance*> insts;
@@ -74,6 +77,8 @@ struct efl_instance {
   // per multigrid level: 1 while the current x is in Level::xs (level 0
   // shares one pair state between the pressure and the CG z)
   uint8_t xsel[efl::kMaxLevels + 1] = {};
+  // per multigrid level: 1 while x is known to be zero (not read)
+  uint8_t xzero[efl::kMaxLevels + 1] = {};
 };
 
 namespace {
@@ -227,8 +232,8 @@ enum SweepMode { PLAIN, DOWN, UP };
 // Event before (at = 0) / after (at = 1) the principal kernel of a stage
 // while efl_time_stage asks for it.
 void kernel_mark(efl_ctx* c, int at) {
-  if (!c->kmark) return;
-  (void)hipEventRecord(c->ev[4 + at], c->stream);
+  if (!c->kmark || 2 * c->krep + 1 >= c->kev.size()) return;
+  (void)hipEventRecord(c->kev[2 * c->krep + at], c->stream);
   if (at) c->kmarked = true;
 }
 
@@ -239,12 +244,29 @@ Batch mg_batch(const IV& v, Extent ext, int lvl, bool with_coarse) {
   Batch b = make_batch(v, EX_LEVEL, lvl, nullptr, efl::mg_tile_w(), efl::mg_tile_h());
   for (int n = 0; n < b.n; ++n) {
     extent_of(v[n], ext, lvl, b.ex[n], b.ey[n]);
-    b.sel[n] = v[n]->xsel[lvl] | (with_coarse ? v[n]->xsel[lvl + 1] << 1 : 0);
+    b.sel[n] = v[n]->xsel[lvl] | (with_coarse ? v[n]->xsel[lvl + 1] << 1 : 0) | (v[n]->xzero[lvl] << 2);
   }
   return b;
 }
 void mg_flip(const IV& v, int lvl) {
-  for (auto* i : v) i->xsel[lvl] ^= 1;
+  for (auto* i : v) {
+    i->xsel[lvl] ^= 1;
+    i->xzero[lvl] = 0;
+  }
+}
+
+// Stencils of level 0 (all: of every level, with the level sets and area
+// fractions of the coarse levels restricted on the way), one launch per
+// level; the inputs are fixed for the rest of the projection.
+void mg_setup(hipStream_t st, const IV& v, bool all) {
+  int top = 0;
+  for (auto* i : v) top = std::max(top, all ? i->h.nlev : 1);
+  for (int l = 0; l < top; ++l) {
+    IV part = filter(v, [&](efl_instance* i) { return l < (all ? i->h.nlev : 1); });
+    Batch b = make_batch(part, EX_LEVEL, l, nullptr, efl::mg_setup_tile_w(), efl::mg_setup_tile_h());
+    for (int n = 0; n < b.n; ++n) b.sel[n] = (all && l + 1 < part[n]->h.nlev) ? 1 : 0;
+    efl::launch_mg_setup(st, b);
+  }
 }
 
 // Bring level 0 back into its primary buffer (the pressure, or z for the CG
@@ -255,6 +277,7 @@ void mg_settle(hipStream_t st, const IV& v, int pcg) {
     const efl::Level& L = pcg ? i->h.lev_pcg0 : i->h.lev[0];
     (void)hipMemcpyAsync(L.x, L.xs, (size_t)L.w * L.h * sizeof(float), hipMemcpyDeviceToDevice, st);
     i->xsel[0] = 0;
+    i->xzero[0] = 0;
   }
 }
 
@@ -300,7 +323,10 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
           IV more = filter(part, [&](efl_instance* i) { return k + 1 != chunks(i); });
           efl::launch_mg_down(st, mg_batch(fin, ext, lvl, false), pcg, (int)n, first, weight, 1);
           mg_flip(fin, lvl);
-          for (auto* i : fin) i->xsel[lvl + 1] = 0;  // the restriction zeroed the coarse x
+          for (auto* i : fin) {  // the coarse x starts at zero (not stored)
+            i->xsel[lvl + 1] = 0;
+            i->xzero[lvl + 1] = 1;
+          }
           if (!more.empty()) {
             efl::launch_mg_down(st, mg_batch(more, ext, lvl, false), pcg, (int)n, first, weight, 0);
             mg_flip(more, lvl);
@@ -345,7 +371,8 @@ void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
       int n = b.n++;
       b.inst[n] = i->d;
       b.level[n] = l;
-      b.sel[n] = i->xsel[l];
+      b.sel[n] = i->xsel[l] | (i->xzero[l] << 2);
+      i->xzero[l] = 0;
       b.ex[n] = (int)i->s.projection.mg_pre_iterations;
       b.ey[n] = (int)(i->s.projection.mg_post_iterations & 0xffff) |
                 (int)(i->s.projection.mg_coarsest_iterations << 16);
@@ -373,6 +400,10 @@ void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
 // into it; the others take the separate k_pcg_rz pass.
 void precondition(hipStream_t st, const IV& v) {
   efl::launch_zero_z(st, make_batch(v, EX_FULL));
+  for (auto* i : v) {
+    i->xsel[0] = 0;
+    i->xzero[0] = 1;
+  }
   IV bot = filter(v, [](efl_instance* i) { return bottom_level(i) == 0; });
   IV tiled = filter(v, [](efl_instance* i) { return bottom_level(i) != 0; });
   if (!bot.empty()) v_cycle(st, bot, 0, 1, true);
@@ -423,6 +454,7 @@ int stage_projection(efl_ctx* c, const IV& v) {  // projection.rs:89-164
   IV mg = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_MULTIGRID; });
   IV cg = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_MGPCG; });
   if (!gs.empty()) {  // gauss_seidel.rs:120-148 with weight 1.9
+    mg_setup(st, gs, false);
     gs_sweeps(st, gs, 0, cnt_gs, 1.9f, 0, false, EX_CENTER);
     mg_settle(st, gs, 0);
     for (auto* i : gs) i->stats.cycles = i->s.projection.gs_iterations;
@@ -436,6 +468,7 @@ int stage_projection(efl_ctx* c, const IV& v) {  // projection.rs:89-164
     for (auto* i : jac) i->stats.cycles = 100;
   }
   if (!mg.empty()) {  // one V-cycle per cycle; repeat to the residual target
+    mg_setup(st, mg, true);
     IV act = mg;
     uint32_t k = 0;
     while (!act.empty()) {
@@ -469,6 +502,7 @@ int stage_projection(efl_ctx* c, const IV& v) {  // projection.rs:89-164
     }
   }
   if (!cg.empty()) {  // flexible-free PCG with one symmetric V-cycle as M
+    mg_setup(st, cg, true);
     Batch full = make_batch(cg, EX_FULL);
     efl::launch_mask_p(st, full);
     efl::launch_residual_norm(st, full, 1);
@@ -775,6 +809,7 @@ int efl_ctx_destroy(efl_ctx* c) {
   for (auto* i : left) efl_instance_destroy(i);
   for (auto& e : c->ev)
     if (e) hipEventDestroy(e);
+  for (auto& e : c->kev) hipEventDestroy(e);
   if (c->d_obs) hipFree(c->d_obs);
   if (c->d_bodies) hipFree(c->d_bodies);
   if (c->h_scal) hipHostFree(c->h_scal);
@@ -901,11 +936,14 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   size_t o_grad = take(n * 8), o_s0 = take(n * 8), o_s1 = take(n * 8), o_afs = take(n * 16);
   size_t o_bins = take(4 * EFL_MAX_SOLIDS * 4), o_fts = take(EFL_MAX_SOLIDS * 16);
   size_t o_r0 = take(n * 4), o_xs0 = take(n * 4);
-  std::vector<size_t> o_lx(nl), o_lb(nl), o_lls(nl), o_laf(nl), o_lr(nl), o_lxs(nl);
-  for (int l = 1; l < nl; ++l) {
+  std::vector<size_t> o_lx(nl), o_lb(nl), o_lls(nl), o_laf(nl), o_lr(nl), o_lxs(nl), o_lcf(nl), o_ldn(nl), o_lfl(nl);
+  for (int l = 0; l < nl; ++l) {
     size_t m = (size_t)lv[l].first * lv[l].second;
-    o_lx[l] = take(m * 4); o_lb[l] = take(m * 4); o_lls[l] = take(m * 4); o_laf[l] = take(m * 16); o_lr[l] = take(m * 4);
-    o_lxs[l] = take(m * 4);
+    if (l > 0) {
+      o_lx[l] = take(m * 4); o_lb[l] = take(m * 4); o_lls[l] = take(m * 4); o_laf[l] = take(m * 16);
+      o_lr[l] = take(m * 4); o_lxs[l] = take(m * 4);
+    }
+    o_lcf[l] = take(m * 16); o_ldn[l] = take(m * 4); o_lfl[l] = take(m * 4);
   }
   size_t o_pr = take(n * 4), o_pz = take(n * 4), o_pp = take(n * 4), o_pq = take(n * 4);
   size_t o_red = take((size_t)3 * nblk * 8), o_scal = take(16 * 8), o_cnt = take(16), o_diag = take(16);
@@ -948,12 +986,16 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   h.afs = (float4*)(base + o_afs);
   h.bins = (int32_t*)(base + o_bins); h.fts = (float4*)(base + o_fts);
   h.nlev = nl;
-  h.lev[0] = {(int)W, (int)H, h.p0, h.div, h.ls0, h.afs, (float*)(base + o_r0), (float*)(base + o_xs0)};
+  h.lev[0] = {(int)W, (int)H, h.p0, h.div, h.ls0, h.afs, (float*)(base + o_r0), (float*)(base + o_xs0),
+              (float4*)(base + o_lcf[0]), (float*)(base + o_ldn[0]), (int32_t*)(base + o_lfl[0])};
   for (int l = 1; l < nl; ++l)
     h.lev[l] = {lv[l].first, lv[l].second, (float*)(base + o_lx[l]), (float*)(base + o_lb[l]), (float*)(base + o_lls[l]),
-                (float4*)(base + o_laf[l]), (float*)(base + o_lr[l]), (float*)(base + o_lxs[l])};
+                (float4*)(base + o_laf[l]), (float*)(base + o_lr[l]), (float*)(base + o_lxs[l]),
+                (float4*)(base + o_lcf[l]), (float*)(base + o_ldn[l]), (int32_t*)(base + o_lfl[l])};
   h.pcg_r = (float*)(base + o_pr); h.pcg_z = (float*)(base + o_pz); h.pcg_p = (float*)(base + o_pp); h.pcg_q = (float*)(base + o_pq);
-  h.lev_pcg0 = {(int)W, (int)H, h.pcg_z, h.pcg_r, h.ls0, h.afs, (float*)(base + o_r0), (float*)(base + o_xs0)};
+  h.lev_pcg0 = h.lev[0];  // z over r, same level set, area fractions and stencil
+  h.lev_pcg0.x = h.pcg_z;
+  h.lev_pcg0.b = h.pcg_r;
   h.red = (double*)(base + o_red);
   h.red_stride = nblk;
   h.scal = (double*)(base + o_scal);
@@ -1150,6 +1192,7 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
     case EFL_STAGE_LEVELSET_GRADIENT: efl::launch_ls_gradient(st, center); break;
     case EFL_STAGE_FLUID_TO_SOLID: stage_fluid_to_solid(st, v); break;
     case EFL_STAGE_GS_SWEEP: {  // one red + black sweep, weight 1.9, gauss_seidel.rs:142-147
+      mg_setup(st, v, false);
       kernel_mark(c, 0);
       efl::launch_mg_down(st, mg_batch(v, EX_CENTER, 0, false), 0, 1, 1, 1.9f, 0);
       kernel_mark(c, 1);
@@ -1158,6 +1201,7 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
       break;
     }
     case EFL_STAGE_MG_SMOOTH: {  // two red-black sweeps at weight 1 on level 0 in one tiled launch
+      mg_setup(st, v, false);
       kernel_mark(c, 0);
       efl::launch_mg_down(st, mg_batch(v, EX_LEVEL, 0, false), 0, 2, 1, 1.0f, 0);
       kernel_mark(c, 1);
@@ -1170,7 +1214,10 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
       efl::launch_gs_color(st, center, 0, 1.9f, 0);
       break;
     }
-    case EFL_STAGE_MG_VCYCLE: v_cycle(st, v, 0, 0, false); break;
+    case EFL_STAGE_MG_VCYCLE:
+      mg_setup(st, v, true);
+      v_cycle(st, v, 0, 0, false);
+      break;
     case EFL_STAGE_MG_RESIDUAL: efl::launch_residual(st, make_batch(v, EX_LEVEL, 0), 0); break;
     default: return fail(c, EFL_E_INVALID, "unknown stage %d", stage);
   }
@@ -1188,21 +1235,31 @@ int efl_time_stage(efl_instance* in, int stage, float dt, uint32_t reps, float* 
   if (rc) return rc;
   // stages built around one tiled smoother launch report that launch alone
   // (their x copy-back is not part of the kernel)
+  while (c->kev.size() < 2 * (size_t)reps) {
+    hipEvent_t e = nullptr;
+    HIPCHK(c, hipEventCreate(&e));
+    c->kev.push_back(e);
+  }
   c->kmarked = false;
+  c->krep = 0;
   c->kmark = true;
   rc = efl_run_stage(in, stage, dt);
   c->kmark = false;
   if (rc) return rc;
-  if (c->kmarked) {
+  if (c->kmarked) {  // back to back, one event pair per repetition
+    c->kmark = true;
+    for (uint32_t k = 0; k < reps && !rc; ++k) {
+      c->krep = k;
+      rc = efl_run_stage(in, stage, dt);
+    }
+    c->kmark = false;
+    c->krep = 0;
+    if (rc) return rc;
+    HIPCHK(c, hipEventSynchronize(c->kev[2 * reps - 1]));
     double sum = 0.0;
     for (uint32_t k = 0; k < reps; ++k) {
-      c->kmark = true;
-      rc = efl_run_stage(in, stage, dt);
-      c->kmark = false;
-      if (rc) return rc;
-      HIPCHK(c, hipEventSynchronize(c->ev[5]));
       float t = 0.0f;
-      HIPCHK(c, hipEventElapsedTime(&t, c->ev[4], c->ev[5]));
+      HIPCHK(c, hipEventElapsedTime(&t, c->kev[2 * k], c->kev[2 * k + 1]));
       sum += t;
     }
     *ms = (float)(sum / reps);

@@ -26,6 +26,11 @@ struct Level {  // one multigrid level (multi_grid.rs:99-118)
   float4* af;
   float* r;
   float* xs;  // second buffer of x: the tiled smoothers read one and write the other
+  // per-cell stencil, fixed during a solve (k_mg_setup): neighbour
+  // coefficients, Gauss-Seidel denominator, flags
+  float4* cf;
+  float* dn;
+  int32_t* fl;
 };
 
 // Per-instance descriptor, copied to device memory; kernels read it through

@@ -60,6 +60,9 @@ namespace efl {
 int mg_tile_w();
 int mg_tile_h();
 int mg_bottom_cells();
+int mg_setup_tile_w();
+int mg_setup_tile_h();
+void launch_mg_setup(hipStream_t st, const Batch& b);
 size_t bottom_lds_bytes(int cells);
 void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_);
 void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz);

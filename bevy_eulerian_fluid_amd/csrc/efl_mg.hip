@@ -2,13 +2,19 @@
 // projection/multi_grid.rs:390-475 (and its symmetric variant used as the
 // CG preconditioner) in two launches per level plus one for the bottom.
 //
-//   mg_down    a block stages a 64 x 16 tile of one level plus a halo of
-//              x, b, the air level set and the area fractions in LDS, runs
-//              the pre-smoothing half-sweeps there (each one shrinks the
-//              region whose values are exact by one cell), writes the tile
-//              and, with `restrict`, forms the residual (residual.wgsl) of
-//              its cells and restricts it (restriction.wgsl) to the 32 x 8
-//              coarse cells under the tile; r is never stored.
+//   mg_setup   once per projection and level: the stencil of every cell
+//              (the 4 neighbour coefficients and the denominator that
+//              gauss_seidel.wgsl:44-88 and residual.wgsl:12-57 derive from
+//              the air level set and the area fractions, which do not change
+//              during a solve) and, for the next level, restriction.wgsl's
+//              level set and area fractions.
+//   mg_down    a block owns a 64 x 16 tile of one level; it stages x of the
+//              tile plus a halo in LDS and keeps the stencils of its cells in
+//              registers, runs the pre-smoothing half-sweeps (each one
+//              shrinks the region whose values are exact by one cell),
+//              writes the tile and, with `restrict`, forms the residual of
+//              its cells and restricts it to the 32 x 8 coarse cells under
+//              the tile; r is never stored.
 //   mg_up      prolongation.wgsl applied while staging, then the
 //              post-smoothing half-sweeps; at level 0 of the CG
 //              preconditioner it also masks z and forms the block partial
@@ -16,15 +22,19 @@
 //   bottom     one workgroup runs the V-cycle below the first level of at
 //              most 1024 cells in LDS (reference order, no launches).
 //
-// Red-black Gauss-Seidel inside a colour is order independent, so these
-// kernels produce exactly the values of the one-kernel-per-half-sweep
-// reference dataflow (same operation order per cell; -ffp-contract=off).
+// Red-black Gauss-Seidel inside a colour is order independent, and the
+// precomputed stencil holds exactly the values the WGSL recomputes at every
+// sweep (same expressions, same summation order, -ffp-contract=off), so
+// these kernels produce the values of the one-kernel-per-half-sweep
+// reference dataflow bit for bit.
 //
 // A block's halo belongs to other blocks' tiles, so a tiled launch never
 // writes the x it reads: x and Level::xs form a pair, each launch reads one
 // (Batch::sel) and writes every in-level cell of its tiles to the other.
 // Writing in place would let a block that starts late stage neighbour
-// values that were already smoothed.
+// values that were already smoothed.  Batch::sel bit 2 says that x is zero
+// (after a restriction, or z at the start of the preconditioner): then it is
+// not read at all.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -39,6 +49,14 @@ constexpr int MT_H = 16;   // tile height
 constexpr int MNT = 256;   // threads per block
 constexpr int BOTTOM_CELLS = 1024;
 
+// Level::fl bits
+constexpr int F_IN0 = 0;           // bits 0-3: neighbour k (-x, +x, -y, +y) inside the level
+constexpr int F_FL0 = 4;           // bits 4-7: neighbour k inside the level and fluid (ls < 0)
+constexpr int F_GSZ = 1 << 8;      // the GS update gives 0 (ls >= 0, closed, |denominator| < 1e-6)
+constexpr int F_RZ = 1 << 9;       // the residual is 0 (ls >= 0 or closed)
+constexpr int F_LSNEG = 1 << 10;   // ls < 0 (prolongation target, z mask)
+constexpr int F_CLOSED = 1 << 11;  // all four area fractions are 0
+
 __device__ __forceinline__ bool mg_inb(int x, int y, int w, int h) {
   return (unsigned)x < (unsigned)w && (unsigned)y < (unsigned)h;
 }
@@ -51,12 +69,18 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float ldg(const float* p, size_t i) {
   return ((const __attribute__((address_space(1))) float*)p)[i];
 }
+__device__ __forceinline__ int ldg(const int32_t* p, size_t i) {
+  return ((const __attribute__((address_space(1))) int32_t*)p)[i];
+}
 __device__ __forceinline__ float4 ldg(const float4* p, size_t i) {
   const f4v v = ((const __attribute__((address_space(1))) f4v*)p)[i];
   return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ void stg(float* p, size_t i, float v) {
   ((__attribute__((address_space(1))) float*)p)[i] = v;
+}
+__device__ __forceinline__ void stg(int32_t* p, size_t i, int v) {
+  ((__attribute__((address_space(1))) int32_t*)p)[i] = v;
 }
 __device__ __forceinline__ void stg(float4* p, size_t i, float4 v) {
   const f4v w = {v.x, v.y, v.z, v.w};
@@ -67,6 +91,127 @@ __device__ __forceinline__ const Level& mg_level(const DevInst* I, int l, int pc
   return (pcg && l == 0) ? I->lev_pcg0 : I->lev[l];
 }
 
+// dx * dx * rho / dt and dt / rho / dx / dx, evaluated left to right as the
+// WGSL does
+__device__ __forceinline__ float gs_c0(const DevInst* I, float scale) {
+  float dx = scale * I->dx;
+  return dx * dx * I->rho / I->dt;
+}
+__device__ __forceinline__ float resid_factor(const DevInst* I, float scale) {
+  float dx = I->dx * scale;
+  return I->dt / I->rho / dx / dx;
+}
+
+// gauss_seidel.wgsl:44-88 with the stencil precomputed: nume = c0 b plus the
+// fluid neighbours' c_k x_k in neighbour order, then nume / denominator and
+// the relaxation.  A skipped term leaves the sum untouched.
+__device__ __forceinline__ float gs_upd(float b, float4 c, float dn, int fl, float xo, float x0, float x1, float x2,
+                                        float x3, float weight, float c0) {
+  const float cv[4] = {c.x, c.y, c.z, c.w};
+  const float xv[4] = {x0, x1, x2, x3};
+  float nume = c0 * b;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float t = nume + cv[k] * xv[k];
+    nume = ((fl >> (F_FL0 + k)) & 1) ? t : nume;
+  }
+  const float pn = nume / dn;
+  const float v = weight * pn + (1.0f - weight) * xo;
+  return (fl & F_GSZ) ? 0.0f : v;
+}
+
+// residual.wgsl:12-57 with the stencil precomputed: fluid neighbours
+// subtract c_k (x - x_k) factor, air neighbours c_k x factor.
+__device__ __forceinline__ float res_upd(float b, float4 c, int fl, float xc, float x0, float x1, float x2, float x3,
+                                         float factor) {
+  const float cv[4] = {c.x, c.y, c.z, c.w};
+  const float xv[4] = {x0, x1, x2, x3};
+  float res = b;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float r_fl = res - cv[k] * (xc - xv[k]) * factor;
+    const float r_air = res - cv[k] * xc * factor;
+    const bool in = (fl >> (F_IN0 + k)) & 1, fluid = (fl >> (F_FL0 + k)) & 1;
+    res = in ? (fluid ? r_fl : r_air) : res;
+  }
+  return (fl & F_RZ) ? 0.0f : res;
+}
+
+// ---------------------------------------------------------------------------
+// setup: stencils of one level (+ level set and area fractions of the next)
+// ---------------------------------------------------------------------------
+constexpr int SU_W = 64, SU_H = 4;
+
+__global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
+  int blk = blockIdx.x;
+  int i = 0;
+  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  const int lb = blk - b.start[i];
+  const DevInst* I = b.inst[i];
+  const int l = b.level[i];
+  const Level L = I->lev[l];
+  const int gx = (lb % b.tiles_x[i]) * SU_W + (int)(threadIdx.x % SU_W);
+  const int gy = (lb / b.tiles_x[i]) * SU_H + (int)(threadIdx.x / SU_W);
+  if (gx >= L.w || gy >= L.h) return;
+  const size_t id = (size_t)gy * L.w + gx;
+  const float lc = ldg(L.ls, id);
+  const float4 f = ldg(L.af, id);
+  const bool in[4] = {gx > 0, gx + 1 < L.w, gy > 0, gy + 1 < L.h};
+  const size_t nb[4] = {id - 1, id + 1, id - L.w, id + L.w};
+  const float fv[4] = {f.x, f.y, f.z, f.w};
+  float cv[4];
+  float denom = 0.0f;
+  int fl = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    cv[k] = 0.0f;
+    if (in[k]) {
+      const float lv = ldg(L.ls, nb[k]);
+      const bool fluid = lv < 0.0f;
+      cv[k] = fluid ? fv[k] : fv[k] * (1.0f - lv / lc);
+      denom += cv[k];
+      fl |= (1 << (F_IN0 + k)) | (fluid ? 1 << (F_FL0 + k) : 0);
+    }
+  }
+  const bool closed = closed4(f);
+  if (lc >= 0.0f || closed || fabsf(denom) < 1.0e-6f) fl |= F_GSZ;
+  if (lc >= 0.0f || closed) fl |= F_RZ;
+  if (lc < 0.0f) fl |= F_LSNEG;
+  if (closed) fl |= F_CLOSED;
+  stg(L.cf, id, make_float4(cv[0], cv[1], cv[2], cv[3]));
+  stg(L.dn, id, denom);
+  stg(L.fl, id, fl);
+  // restriction.wgsl:10-60 of the level set and the area fractions, by the
+  // thread of the coarse cell's (0, 0) child
+  if (!(b.sel[i] & 1) || (gx & 1) || (gy & 1)) return;
+  const Level C = I->lev[l + 1];
+  const int cx = gx >> 1, cy = gy >> 1;
+  if (cx >= C.w || cy >= C.h) return;
+  const int dxs[4] = {0, 0, 1, 1}, dys[4] = {0, 1, 0, 1};  // (0,0) (0,1) (1,0) (1,1)
+  float l4[4];
+  float4 a4[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int x = gx + dxs[k], y = gy + dys[k];
+    const bool inl = mg_inb(x, y, L.w, L.h);
+    const size_t j = (size_t)y * L.w + x;
+    l4[k] = inl ? ldg(L.ls, j) : 0.0f;
+    a4[k] = inl ? ldg(L.af, j) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  float pr = l4[0];
+  if (fabsf(l4[1]) < fabsf(pr)) pr = l4[1];
+  if (fabsf(l4[2]) < fabsf(pr)) pr = l4[2];
+  if (fabsf(l4[3]) < fabsf(pr)) pr = l4[3];
+  const size_t cid = (size_t)cy * C.w + cx;
+  stg(C.ls, cid, pr);
+  stg(C.af, cid,
+      make_float4(0.5f * (a4[0].x + a4[1].x), 0.5f * (a4[2].y + a4[3].y), 0.5f * (a4[0].z + a4[2].z),
+                  0.5f * (a4[1].w + a4[3].w)));
+}
+
+// ---------------------------------------------------------------------------
+// tiled smoothers
+// ---------------------------------------------------------------------------
 struct MSite {
   const DevInst* I;
   int tx0, ty0;  // tile origin (cells)
@@ -93,194 +238,114 @@ __device__ __forceinline__ MSite mg_locate(const Batch& b) {
   return s;
 }
 
-// LDS indices of the four neighbours (-x, +x, -y, +y) of LDS cell q whose
-// level cell is (gx, gy); a neighbour outside the level maps to q itself so
-// every read stays inside the staged region.
-struct Nbr {
-  bool in[4];
-  int j[4];
-};
-__device__ __forceinline__ Nbr mg_nbr(int q, int gx, int gy, int w, int h, int rowlen) {
-  Nbr n;
-  n.in[0] = gx > 0;
-  n.in[1] = gx + 1 < w;
-  n.in[2] = gy > 0;
-  n.in[3] = gy + 1 < h;
-  n.j[0] = n.in[0] ? q - 1 : q;
-  n.j[1] = n.in[1] ? q + 1 : q;
-  n.j[2] = n.in[2] ? q - rowlen : q;
-  n.j[3] = n.in[3] ? q + rowlen : q;
-  return n;
-}
-
-// The GS update of gauss_seidel.wgsl:44-88 on LDS-staged data, written
-// without branches: every LDS read of the cell is issued up front and the
-// early exits and the fluid/air/outside cases of each neighbour become
-// selects.  A skipped term leaves its sum untouched (no "+ 0"), so the
-// arithmetic is exactly that of the branching form.
-__device__ __forceinline__ float gs_lds(const float* sx, const float* sb, const float* sl, const float4* sa, int q,
-                                        int gx, int gy, int w, int h, int rowlen, float weight, float c0) {
-  const Nbr n = mg_nbr(q, gx, gy, w, h, rowlen);
-  const float lc = sl[q], p_old = sx[q], bq = sb[q];
-  const float4 f = sa[q];
-  float lv[4], xv[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    lv[k] = sl[n.j[k]];
-    xv[k] = sx[n.j[k]];
-  }
-  const float fv[4] = {f.x, f.y, f.z, f.w};
-  float denom = 0.0f;
-  float nume = c0 * bq;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const bool fl = lv[k] < 0.0f;
-    const float d_fl = denom + fv[k];
-    const float n_fl = nume + fv[k] * xv[k];
-    const float d_air = denom + fv[k] * (1.0f - lv[k] / lc);
-    denom = n.in[k] ? (fl ? d_fl : d_air) : denom;
-    nume = (n.in[k] && fl) ? n_fl : nume;
-  }
-  const float pn = nume / denom;
-  const float v = weight * pn + (1.0f - weight) * p_old;
-  const bool zero = lc >= 0.0f || closed4(f) || fabsf(denom) < 1.0e-6f;
-  return zero ? 0.0f : v;
-}
-
-// residual.wgsl:12-57 on LDS-staged data, branch-free like gs_lds
-__device__ __forceinline__ float resid_lds(const float* sx, const float* sb, const float* sl, const float4* sa, int q,
-                                           int gx, int gy, int w, int h, int rowlen, float factor) {
-  const Nbr n = mg_nbr(q, gx, gy, w, h, rowlen);
-  const float phi = sl[q], xc = sx[q];
-  const float4 f = sa[q];
-  float res = sb[q];
-  float lv[4], xv[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    lv[k] = sl[n.j[k]];
-    xv[k] = sx[n.j[k]];
-  }
-  const float fv[4] = {f.x, f.y, f.z, f.w};
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const float r_fl = res - fv[k] * (xc - xv[k]) * factor;
-    const float r_air = res - fv[k] * (1.0f - lv[k] / phi) * xc * factor;
-    res = n.in[k] ? (lv[k] < 0.0f ? r_fl : r_air) : res;
-  }
-  const bool zero = phi >= 0.0f || closed4(f);
-  return zero ? 0.0f : res;
-}
-
-// dx * dx * rho / dt and dt / rho / dx / dx, evaluated left to right as the
-// WGSL does
-__device__ __forceinline__ float gs_c0(const DevInst* I, float scale) {
-  float dx = scale * I->dx;
-  return dx * dx * I->rho / I->dt;
-}
-__device__ __forceinline__ float resid_factor(const DevInst* I, float scale) {
-  float dx = I->dx * scale;
-  return I->dt / I->rho / dx / dx;
-}
-
-// Stage [tile + halo] of a level into LDS.  All global loads of a thread are
-// issued before its LDS stores (latency overlap).  Cells outside the level
-// read 0.  x comes from xin; with C, prolongation.wgsl adds the coarse x in
-// cxin on the fly.
+// The cells of a region [tile + halo] a thread owns: slots 0..NIT-1 hold
+// cells of the colour swept first, NIT..2NIT-1 the other colour; slot k of a
+// colour is the thread's k-th cell in a row-major walk over that colour's
+// cells.  Stencil, b and x of its cells live in registers; x is mirrored in
+// LDS for the neighbours.
 template <int HALO>
-__device__ __forceinline__ void stage_tile(const Level& L, const float* xin, const Level* C, const float* cxin, int x0,
-                                           int y0, float* sx, float* sb, float* sl, float4* sa) {
-  constexpr int RW = MT_W + 2 * HALO, RH = MT_H + 2 * HALO, RN = RW * RH;
-  constexpr int NIT = (RN + MNT - 1) / MNT;
-  float rx[NIT], rb[NIT], rl[NIT], rc[NIT];
-  float4 ra[NIT];
-  bool par[NIT];
+struct Tile {
+  static constexpr int RW = MT_W + 2 * HALO, RH = MT_H + 2 * HALO, HW = RW / 2;
+  static constexpr int NC = HW * RH;                 // cells of one colour
+  static constexpr int NIT = (NC + MNT - 1) / MNT;   // slots per colour
+  static constexpr int NSL = 2 * NIT;
+  float b[NSL], x[NSL], dn[NSL];
+  float4 c[NSL];
+  int fl[NSL];
+  int lx[NSL], ly[NSL];
+  bool valid[NSL], inl[NSL];
+
+  // ox, oy: level coordinates of LDS cell (0, 0)
+  __device__ __forceinline__ void place(int ox, int oy, int first, int w, int h) {
+    const int base = (ox + oy) & 1;
 #pragma unroll
-  for (int k = 0; k < NIT; ++k) {
-    int q = threadIdx.x + k * MNT;
-    int gx = x0 - HALO + q % RW, gy = y0 - HALO + q / RW;
-    rx[k] = rb[k] = rl[k] = rc[k] = 0.0f;
-    ra[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    par[k] = false;
-    if (q < RN && mg_inb(gx, gy, L.w, L.h)) {
-      size_t id = (size_t)gy * L.w + gx;
-      rx[k] = ldg(xin, id);
-      rb[k] = ldg(L.b, id);
-      rl[k] = ldg(L.ls, id);
-      ra[k] = ldg(L.af, id);
-      // the coarse value is fetched with the fine ones and used below only
-      // on fluid cells
-      int cx = gx >> 1, cy = gy >> 1;
-      if (C && cx < C->w && cy < C->h) {
-        par[k] = true;
-        rc[k] = ldg(cxin, (size_t)cy * C->w + cx);
+    for (int s = 0; s < NSL; ++s) {
+      const int colour = s < NIT ? first : 1 - first;
+      const int u = (int)threadIdx.x + (s % NIT) * MNT;
+      valid[s] = u < NC;
+      ly[s] = u / HW;
+      lx[s] = 2 * (u - ly[s] * HW) + ((colour + ly[s] + base) & 1);
+      inl[s] = valid[s] && mg_inb(ox + lx[s], oy + ly[s], w, h);
+    }
+  }
+
+  // stencils, b and x from the level (x = 0 when xzero); with C, the
+  // prolongation adds the coarse x (cxin) on fluid cells that have a parent
+  __device__ __forceinline__ void load(const Level& L, const float* xin, bool xzero, const Level* C,
+                                       const float* cxin, int ox, int oy, float* sx) {
+    float xc[NSL];
+    bool par[NSL];
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      b[s] = x[s] = dn[s] = xc[s] = 0.0f;
+      c[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      fl[s] = 0;
+      par[s] = false;
+      if (inl[s]) {
+        const int gx = ox + lx[s], gy = oy + ly[s];
+        const size_t id = (size_t)gy * L.w + gx;
+        if (!xzero) x[s] = ldg(xin, id);
+        b[s] = ldg(L.b, id);
+        c[s] = ldg(L.cf, id);
+        dn[s] = ldg(L.dn, id);
+        fl[s] = ldg(L.fl, id);
+        if (C) {
+          const int cx = gx >> 1, cy = gy >> 1;
+          if (cx < C->w && cy < C->h) {
+            par[s] = true;
+            xc[s] = ldg(cxin, (size_t)cy * C->w + cx);
+          }
+        }
       }
     }
-  }
-  if (C) {  // prolongation: x_f += x_c on fluid fine cells with a parent
 #pragma unroll
-    for (int k = 0; k < NIT; ++k)
-      if (par[k] && rl[k] < 0.0f) rx[k] = rc[k] + rx[k];
-  }
-#pragma unroll
-  for (int k = 0; k < NIT; ++k) {
-    int q = threadIdx.x + k * MNT;
-    if (q < RN) {
-      sx[q] = rx[k];
-      sb[q] = rb[k];
-      sl[q] = rl[k];
-      sa[q] = ra[k];
+    for (int s = 0; s < NSL; ++s) {
+      if (C && par[s] && (fl[s] & F_LSNEG)) x[s] = xc[s] + x[s];
+      if (valid[s]) sx[ly[s] * RW + lx[s]] = x[s];
     }
   }
-}
 
-// 2 * NS half-sweeps on the staged region (first colour of each sweep:
-// 1 red, 0 black).  Threads walk the cells of one colour directly; a thread
-// computes all of its cells of a half-sweep before it stores any (cells of
-// one colour read only the other colour), so their LDS reads overlap.  Cells
-// outside [0, ex) x [0, ey) are not updated (dispatch coverage); half-sweep
-// hs updates only cells at least hs + 1 from the region border, whose
-// neighbours are still exact.
-template <int HALO, int NS>
-__device__ __forceinline__ void sweep_tile(int w, int h, int x0, int y0, int ex, int ey, int first, float weight,
-                                           float c0, float* sx, const float* sb, const float* sl, const float4* sa) {
-  constexpr int RW = MT_W + 2 * HALO, RH = MT_H + 2 * HALO, HW = RW / 2;
-  constexpr int NIT = (HW * RH + MNT - 1) / MNT;
-  const int ox = x0 - HALO, oy = y0 - HALO;  // level coordinates of LDS (0, 0)
-  const int base = (ox + oy) & 1;
-  // local bounds from the level and the dispatch extent
-  const int bx0 = -ox, by0 = -oy, bx1 = min(ex, w) - ox, by1 = min(ey, h) - oy;
-  int ly[NIT], lx2[NIT];
+  // 2 * NS half-sweeps (first colour = the one placed in slots 0..NIT-1).
+  // Half-sweep hs updates cells at least hs + 1 from the region border (their
+  // neighbours are still exact) inside the level and [0, ex) x [0, ey).
+  template <int NS>
+  __device__ __forceinline__ void sweep(int ox, int oy, int w, int h, int ex, int ey, float weight, float c0,
+                                        float* sx) {
+    const int bx0 = -ox, by0 = -oy, bx1 = min(ex, w) - ox, by1 = min(ey, h) - oy;
 #pragma unroll
-  for (int k = 0; k < NIT; ++k) {
-    const int t = threadIdx.x + k * MNT;
-    ly[k] = t / HW;  // >= RH for t past the colour's cells: never active
-    lx2[k] = 2 * (t - ly[k] * HW);
-  }
+    for (int hs = 0; hs < 2 * NS; ++hs) {
+      const int m = hs + 1;
+      const int lo_x = max(m, bx0), hi_x = min(RW - m, bx1), lo_y = max(m, by0), hi_y = min(RH - m, by1);
+      const int s0 = (hs & 1) ? NIT : 0;
+      float nv[NIT];
+      int qq[NIT];
+      bool act[NIT];
 #pragma unroll
-  for (int hs = 0; hs < 2 * NS; ++hs) {
-    const int parity = (hs & 1) == 0 ? first : 1 - first;
-    const int m = hs + 1;
-    const int lo_x = max(m, bx0), hi_x = min(RW - m, bx1), lo_y = max(m, by0), hi_y = min(RH - m, by1);
-    float nv[NIT];
-    int qq[NIT];
-    bool act[NIT];
+      for (int k = 0; k < NIT; ++k) {
+        const int s = s0 + k;
+        act[k] = valid[s] && lx[s] >= lo_x && lx[s] < hi_x && ly[s] >= lo_y && ly[s] < hi_y;
+        // an inactive slot evaluates an interior dummy cell and discards it
+        const int q = act[k] ? ly[s] * RW + lx[s] : RW + 1;
+        qq[k] = q;
+        nv[k] = gs_upd(b[s], c[s], dn[s], fl[s], x[s], sx[q - 1], sx[q + 1], sx[q - RW], sx[q + RW], weight, c0);
+      }
 #pragma unroll
-    for (int k = 0; k < NIT; ++k) {
-      const int lx = lx2[k] + ((parity + ly[k] + base) & 1);
-      act[k] = lx >= lo_x && lx < hi_x && ly[k] >= lo_y && ly[k] < hi_y;
-      // an inactive slot evaluates an interior dummy cell and discards it
-      qq[k] = act[k] ? ly[k] * RW + lx : RW + 1;
-      nv[k] = gs_lds(sx, sb, sl, sa, qq[k], ox + lx, oy + ly[k], w, h, RW, weight, c0);
+      for (int k = 0; k < NIT; ++k)
+        if (act[k]) {
+          x[s0 + k] = nv[k];
+          sx[qq[k]] = nv[k];
+        }
+      __syncthreads();
     }
-#pragma unroll
-    for (int k = 0; k < NIT; ++k)
-      if (act[k]) sx[qq[k]] = nv[k];
-    __syncthreads();
   }
-}
+
+  __device__ __forceinline__ bool in_tile(int s) const {
+    return valid[s] && lx[s] >= HALO && lx[s] < HALO + MT_W && ly[s] >= HALO && ly[s] < HALO + MT_H;
+  }
+};
 
 // Every in-level cell of the tile goes to xout, updated or not, so xout is a
-// complete copy of x after the launch.
+// complete copy of x after the launch (row-major from LDS: coalesced).
 template <int HALO>
 __device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, int y0, const float* sx) {
   constexpr int RW = MT_W + 2 * HALO;
@@ -294,54 +359,49 @@ __device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, 
 }
 
 // ---------------------------------------------------------------------------
-// down: NS pre-smoothing sweeps (+ residual + restriction).  The halo is one
-// cell wider than the sweeps need so that the residual of every tile cell
-// sees exact neighbours.
+// down: NS pre-smoothing sweeps (+ residual + restriction of b).  The halo is
+// one cell wider than the sweeps need so that the residual of every tile
+// cell sees exact neighbours.
 // ---------------------------------------------------------------------------
 template <int NS, int HALO>
 __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, float weight, int restrict_) {
-  constexpr int RW = MT_W + 2 * HALO, RH = MT_H + 2 * HALO, RN = RW * RH;
-  __shared__ float sx[RN], sb[RN], sl[RN];
-  __shared__ float4 sa[RN];
-  MSite s = mg_locate(b);
+  using T = Tile<HALO>;
+  __shared__ float sx[T::RW * T::RH];
+  __shared__ float sr[MT_W * MT_H];
+  const MSite s = mg_locate(b);
   const DevInst* I = s.I;
   const Level L = mg_level(I, s.lvl, pcg);
-  float* xin = (s.sel & 1) ? L.xs : L.x;
+  const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
-  stage_tile<HALO>(L, xin, nullptr, nullptr, s.tx0, s.ty0, sx, sb, sl, sa);
+  const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
+  T t;
+  t.place(ox, oy, first, L.w, L.h);
+  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, sx);
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
-  sweep_tile<HALO, NS>(L.w, L.h, s.tx0, s.ty0, s.ex, s.ey, first, weight, gs_c0(I, scale), sx, sb, sl, sa);
+  t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(I, scale), sx);
   write_tile<HALO>(L, xout, s.tx0, s.ty0, sx);
   if (!restrict_) return;
+  const float factor = resid_factor(I, scale);
+#pragma unroll
+  for (int k = 0; k < T::NSL; ++k) {
+    if (!t.in_tile(k)) continue;
+    const int q = t.ly[k] * T::RW + t.lx[k];
+    const float r = t.inl[k] ? res_upd(t.b[k], t.c[k], t.fl[k], t.x[k], sx[q - 1], sx[q + 1], sx[q - T::RW],
+                                       sx[q + T::RW], factor)
+                             : 0.0f;
+    sr[(t.ly[k] - HALO) * MT_W + t.lx[k] - HALO] = r;
+  }
+  __syncthreads();
   // one coarse cell per thread: 32 x 8 coarse cells under the 64 x 16 tile
   const Level Cc = I->lev[s.lvl + 1];
-  const int cx = s.tx0 / 2 + (int)(threadIdx.x % 32), cy = s.ty0 / 2 + (int)(threadIdx.x / 32);
+  const int ux = (int)threadIdx.x % 32, uy = (int)threadIdx.x / 32;
+  const int cx = s.tx0 / 2 + ux, cy = s.ty0 / 2 + uy;
   if (cx >= Cc.w || cy >= Cc.h) return;
-  const float factor = resid_factor(I, scale);
-  const int lx = 2 * (cx - s.tx0 / 2) + HALO, ly = 2 * (cy - s.ty0 / 2) + HALO;
-  float r4[4];  // (0,0), (0,1), (1,0), (1,1) as restriction.wgsl sums them
-  const int dxs[4] = {0, 0, 1, 1}, dys[4] = {0, 1, 0, 1};
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int gx = 2 * cx + dxs[k], gy = 2 * cy + dys[k];
-    int q = (ly + dys[k]) * RW + lx + dxs[k];
-    r4[k] = mg_inb(gx, gy, L.w, L.h) ? resid_lds(sx, sb, sl, sa, q, gx, gy, L.w, L.h, RW, factor) : 0.0f;
-  }
-  float br = 0.25f * (((r4[0] + r4[1]) + r4[2]) + r4[3]);
-  const int q00 = ly * RW + lx, q01 = q00 + RW, q10 = q00 + 1, q11 = q00 + RW + 1;
-  float p0 = sl[q00], p1 = sl[q01], p2 = sl[q10], p3 = sl[q11];
-  float pr = p0;
-  if (fabsf(p1) < fabsf(pr)) pr = p1;
-  if (fabsf(p2) < fabsf(pr)) pr = p2;
-  if (fabsf(p3) < fabsf(pr)) pr = p3;
-  float4 a00 = sa[q00], a10 = sa[q10], a01 = sa[q01], a11 = sa[q11];
-  size_t id = (size_t)cy * Cc.w + cx;
-  stg(Cc.af, id,
-      make_float4(0.5f * (a00.x + a01.x), 0.5f * (a10.y + a11.y), 0.5f * (a00.z + a10.z), 0.5f * (a01.w + a11.w)));
-  stg(Cc.b, id, br);
-  stg(Cc.ls, id, pr);
-  stg(Cc.x, id, 0.0f);
+  const int q00 = 2 * uy * MT_W + 2 * ux;
+  // (0,0), (0,1), (1,0), (1,1) as restriction.wgsl sums them
+  const float br = 0.25f * (((sr[q00] + sr[q00 + MT_W]) + sr[q00 + 1]) + sr[q00 + MT_W + 1]);
+  stg(Cc.b, (size_t)cy * Cc.w + cx, br);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,62 +411,66 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
 // ---------------------------------------------------------------------------
 template <int NS, int HALO>
 __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, float weight, int rz) {
-  constexpr int RW = MT_W + 2 * HALO, RH = MT_H + 2 * HALO, RN = RW * RH;
-  __shared__ float sx[RN], sb[RN], sl[RN];
-  __shared__ float4 sa[RN];
+  using T = Tile<HALO>;
+  __shared__ float sx[T::RW * T::RH];
   __shared__ double wsum[MNT / 64];
-  MSite s = mg_locate(b);
+  const MSite s = mg_locate(b);
   const DevInst* I = s.I;
   const Level L = mg_level(I, s.lvl, pcg);
   const Level C = I->lev[s.lvl + 1];
-  float* xin = (s.sel & 1) ? L.xs : L.x;
+  const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
-  stage_tile<HALO>(L, xin, &C, (s.sel & 2) ? C.xs : C.x, s.tx0, s.ty0, sx, sb, sl, sa);
+  const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
+  T t;
+  t.place(ox, oy, first, L.w, L.h);
+  t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, sx);
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
-  sweep_tile<HALO, NS>(L.w, L.h, s.tx0, s.ty0, s.ex, s.ey, first, weight, gs_c0(I, scale), sx, sb, sl, sa);
+  t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(I, scale), sx);
   if (!rz) {
     write_tile<HALO>(L, xout, s.tx0, s.ty0, sx);
     return;
   }
   double acc = 0.0;
 #pragma unroll
-  for (int k = 0; k < (MT_W * MT_H) / MNT; ++k) {
-    int t = threadIdx.x + k * MNT;
-    int lx = t % MT_W, ly = t / MT_W;
-    int gx = s.tx0 + lx, gy = s.ty0 + ly;
-    if (gx < L.w && gy < L.h) {
-      int q = (ly + HALO) * RW + lx + HALO;
-      float z = sx[q];
-      if (!(sl[q] < 0.0f) || closed4(sa[q])) z = 0.0f;
-      stg(xout, (size_t)gy * L.w + gx, z);
-      acc += (double)sb[q] * (double)z;
-    }
+  for (int k = 0; k < T::NSL; ++k) {
+    if (!t.in_tile(k) || !t.inl[k]) continue;
+    float z = t.x[k];
+    if (!(t.fl[k] & F_LSNEG) || (t.fl[k] & F_CLOSED)) z = 0.0f;
+    stg(xout, (size_t)(oy + t.ly[k]) * L.w + ox + t.lx[k], z);
+    acc += (double)t.b[k] * (double)z;
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t = wsum[0];
-    for (int w = 1; w < MNT / 64; ++w) t += wsum[w];
-    I->red[(size_t)blockIdx.x - b.start[s.slot]] = t;
+    double tot = wsum[0];
+    for (int w = 1; w < MNT / 64; ++w) tot += wsum[w];
+    I->red[(size_t)blockIdx.x - b.start[s.slot]] = tot;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Bottom of the V-cycle in one workgroup of 1024 threads (32 x 32): levels
-// l0 .. nlev-1 staged in LDS; the level table lives in LDS too.  One
-// workgroup owns the whole level, so x is read from and written back to the
-// buffer Batch::sel names.
+// l0 .. nlev-1 staged in LDS (stencil, x, b); the level table lives in LDS
+// too.  One workgroup owns the whole level, so x is read from and written
+// back to the buffer Batch::sel names.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void b_sweep(int w, int h, int off, float* x, const float* bb, const float* ls,
-                                        const float4* af, int parity, float c0) {
+struct BotLds {
+  float4* c;
+  float *x, *b, *dn;
+  int* fl;
+};
+
+__device__ __forceinline__ void b_sweep(int w, int h, int off, const BotLds& m, int parity, float c0) {
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   for (int gy = ty; gy < h; gy += 32)
     for (int gx = tx; gx < w; gx += 32) {
       if (((gx + gy) & 1) != parity) continue;
-      int q = off + gy * w + gx;
-      x[q] = gs_lds(x, bb, ls, af, q, gx, gy, w, h, w, 1.0f, c0);
+      const int q = off + gy * w + gx;
+      const int j0 = gx > 0 ? q - 1 : q, j1 = gx + 1 < w ? q + 1 : q;
+      const int j2 = gy > 0 ? q - w : q, j3 = gy + 1 < h ? q + w : q;
+      m.x[q] = gs_upd(m.b[q], m.c[q], m.dn[q], m.fl[q], m.x[q], m.x[j0], m.x[j1], m.x[j2], m.x[j3], 1.0f, c0);
     }
   __syncthreads();
 }
@@ -432,18 +496,24 @@ __global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric
   }
   __syncthreads();
   const int total = loff[nl];
-  float4* af = smem_f4;
-  float* x = (float*)(af + total);
-  float* bb = x + total;
-  float* ls = bb + total;
-  {
-    const Level L = mg_level(I, l0, pcg);
-    const float* xb = (b.sel[slot] & 1) ? L.xs : L.x;
+  BotLds m;
+  m.c = smem_f4;
+  m.x = (float*)(m.c + total);
+  m.b = m.x + total;
+  m.dn = m.b + total;
+  m.fl = (int*)(m.dn + total);
+  const bool xzero = (b.sel[slot] & 4) != 0;
+  for (int l = l0; l < nl; ++l) {
+    const Level L = mg_level(I, l, pcg);
+    const int off = loff[l];
     for (int t = threadIdx.x; t < L.w * L.h; t += 1024) {
-      x[t] = ldg(xb, t);
-      bb[t] = ldg(L.b, t);
-      ls[t] = ldg(L.ls, t);
-      af[t] = ldg(L.af, t);
+      m.c[off + t] = ldg(L.cf, t);
+      m.dn[off + t] = ldg(L.dn, t);
+      m.fl[off + t] = ldg(L.fl, t);
+      if (l == l0) {
+        m.x[t] = xzero ? 0.0f : ldg((b.sel[slot] & 1) ? L.xs : L.x, t);
+        m.b[t] = ldg(L.b, t);
+      }
     }
   }
   __syncthreads();
@@ -455,49 +525,41 @@ __global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric
     const float c0 = gs_c0(I, scale);
     if (l == nl - 1) {
       for (int k = 0; k < coarse; ++k) {
-        b_sweep(w, h, off, x, bb, ls, af, 1, c0);
-        b_sweep(w, h, off, x, bb, ls, af, 0, c0);
+        b_sweep(w, h, off, m, 1, c0);
+        b_sweep(w, h, off, m, 0, c0);
       }
       if (symmetric)
         for (int k = 0; k < coarse; ++k) {
-          b_sweep(w, h, off, x, bb, ls, af, 0, c0);
-          b_sweep(w, h, off, x, bb, ls, af, 1, c0);
+          b_sweep(w, h, off, m, 0, c0);
+          b_sweep(w, h, off, m, 1, c0);
         }
       break;
     }
     for (int k = 0; k < pre; ++k) {
-      b_sweep(w, h, off, x, bb, ls, af, 1, c0);
-      b_sweep(w, h, off, x, bb, ls, af, 0, c0);
+      b_sweep(w, h, off, m, 1, c0);
+      b_sweep(w, h, off, m, 0, c0);
     }
-    // residual + restriction into level l + 1 (each coarse cell forms the
-    // residuals of its four children)
+    // residual + restriction of b into level l + 1 (each coarse cell forms
+    // the residuals of its four children); the coarse x starts at 0
     const int cw = lw[l + 1], ch = lh[l + 1], coff = loff[l + 1];
     const float factor = resid_factor(I, scale);
     for (int cy = ty; cy < ch; cy += 32)
       for (int cx = tx; cx < cw; cx += 32) {
-        const int dxs[4] = {0, 0, 1, 1}, dys[4] = {0, 1, 0, 1};
-        float r4[4], l4[4];
-        float4 a4[4];
+        const int dxs[4] = {0, 0, 1, 1}, dys[4] = {0, 1, 0, 1};  // (0,0) (0,1) (1,0) (1,1)
+        float r4[4];
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
-          int gx = 2 * cx + dxs[k], gy = 2 * cy + dys[k];
-          bool in = mg_inb(gx, gy, w, h);
-          int q = off + gy * w + gx;
-          r4[k] = in ? resid_lds(x, bb, ls, af, q, gx, gy, w, h, w, factor) : 0.0f;
-          l4[k] = in ? ls[q] : 0.0f;
-          a4[k] = in ? af[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+          const int gx = 2 * cx + dxs[k], gy = 2 * cy + dys[k];
+          const bool in = mg_inb(gx, gy, w, h);
+          const int q = in ? off + gy * w + gx : off;
+          const int j0 = gx > 0 ? q - 1 : q, j1 = gx + 1 < w ? q + 1 : q;
+          const int j2 = gy > 0 ? q - w : q, j3 = gy + 1 < h ? q + w : q;
+          const float r = res_upd(m.b[q], m.c[q], m.fl[q], m.x[q], m.x[j0], m.x[j1], m.x[j2], m.x[j3], factor);
+          r4[k] = in ? r : 0.0f;
         }
-        float br = 0.25f * (((r4[0] + r4[1]) + r4[2]) + r4[3]);
-        float pr = l4[0];
-        if (fabsf(l4[1]) < fabsf(pr)) pr = l4[1];
-        if (fabsf(l4[2]) < fabsf(pr)) pr = l4[2];
-        if (fabsf(l4[3]) < fabsf(pr)) pr = l4[3];
-        // a4 order: (0,0) (0,1) (1,0) (1,1)
-        int q = coff + cy * cw + cx;
-        af[q] = make_float4(0.5f * (a4[0].x + a4[1].x), 0.5f * (a4[2].y + a4[3].y), 0.5f * (a4[0].z + a4[2].z),
-                            0.5f * (a4[1].w + a4[3].w));
-        bb[q] = br;
-        ls[q] = pr;
-        x[q] = 0.0f;
+        const int q = coff + cy * cw + cx;
+        m.b[q] = 0.25f * (((r4[0] + r4[1]) + r4[2]) + r4[3]);
+        m.x[q] = 0.0f;
       }
     __syncthreads();
   }
@@ -506,31 +568,37 @@ __global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric
     const int cw = lw[l + 1], ch = lh[l + 1], coff = loff[l + 1];
     for (int gy = ty; gy < h; gy += 32)
       for (int gx = tx; gx < w; gx += 32) {
-        int cx = gx >> 1, cy = gy >> 1;
-        int q = off + gy * w + gx;
-        if (cx < cw && cy < ch && ls[q] < 0.0f) x[q] = x[coff + cy * cw + cx] + x[q];
+        const int cx = gx >> 1, cy = gy >> 1;
+        const int q = off + gy * w + gx;
+        if (cx < cw && cy < ch && (m.fl[q] & F_LSNEG)) m.x[q] = m.x[coff + cy * cw + cx] + m.x[q];
       }
     __syncthreads();
     const float c0 = gs_c0(I, (float)(1 << l));
     for (int k = 0; k < post; ++k) {
       if (symmetric) {
-        b_sweep(w, h, off, x, bb, ls, af, 0, c0);
-        b_sweep(w, h, off, x, bb, ls, af, 1, c0);
+        b_sweep(w, h, off, m, 0, c0);
+        b_sweep(w, h, off, m, 1, c0);
       } else {
-        b_sweep(w, h, off, x, bb, ls, af, 1, c0);
-        b_sweep(w, h, off, x, bb, ls, af, 0, c0);
+        b_sweep(w, h, off, m, 1, c0);
+        b_sweep(w, h, off, m, 0, c0);
       }
     }
   }
   const Level L = mg_level(I, l0, pcg);
   float* xb = (b.sel[slot] & 1) ? L.xs : L.x;
-  for (int t = threadIdx.x; t < L.w * L.h; t += 1024) stg(xb, t, x[t]);
+  for (int t = threadIdx.x; t < L.w * L.h; t += 1024) stg(xb, t, m.x[t]);
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 #define MG_LAUNCH(kern, nb, st, ...) hipLaunchKernelGGL(kern, dim3(nb), dim3(MNT), 0, st, __VA_ARGS__)
+
+void launch_mg_setup(hipStream_t st, const Batch& b) {
+  int nb = b.start[b.n];
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_mg_setup, dim3(nb), dim3(SU_W * SU_H), 0, st, b);
+}
 
 // nsweeps 0..3 pre-smoothing sweeps (+ residual/restriction)
 void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_) {
@@ -556,7 +624,8 @@ void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int firs
   }
 }
 
-size_t bottom_lds_bytes(int cells) { return (size_t)cells * (16 + 3 * 4); }
+// per cell: stencil (16) + x, b, denominator (12) + flags (4)
+size_t bottom_lds_bytes(int cells) { return (size_t)cells * 32; }
 
 void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells) {
   if (b.n <= 0) return;
@@ -565,6 +634,8 @@ void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int m
 
 int mg_tile_w() { return MT_W; }
 int mg_tile_h() { return MT_H; }
+int mg_setup_tile_w() { return SU_W; }
+int mg_setup_tile_h() { return SU_H; }
 int mg_bottom_cells() { return BOTTOM_CELLS; }
 
 }  // namespace efl
