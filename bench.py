@@ -123,6 +123,11 @@ def gpu_run(args):
     n = args.size * args.size
     sweep_ms = inst.time_stage("MG_SMOOTH", DT, args.kernel_reps)
     alg_bytes = 8 * 4 * n
+    traffic = None  # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE profile
+    prof = os.path.join(ROOT, "profiles", "k_mg_down_smooth_level0_1024.json")
+    if args.size == 1024 and os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("traffic_bytes")
     achieved = alg_bytes / (sweep_ms * 1e-3) / 1e9
     inst.close()
     ctx.close()
@@ -130,7 +135,7 @@ def gpu_run(args):
                 cycles_mean=sum(cycles) / len(cycles), cycles_max=max(cycles),
                 pressure_solve_ms=sum(proj_ms) / len(proj_ms), device_step_ms=sum(step_ms) / len(step_ms),
                 max_norm_divergence=max(divs), rel_residual=max(rels),
-                sweep_ms=sweep_ms, sweep_bytes=alg_bytes, achieved_gbs=achieved)
+                sweep_ms=sweep_ms, sweep_bytes=alg_bytes, achieved_gbs=achieved, traffic=traffic)
 
 
 def cpu_baseline(args):
@@ -214,7 +219,7 @@ def main():
         "rel_residual": g["rel_residual"],
         "roofline": {"bound": "hbm", "kernel": "k_mg_down<2,5>: 2 fused red-black GS sweeps, level 0 (MG_SMOOTH)",
                      "achieved": round(g["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(g["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(g["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": g["traffic"],
                      "algorithmic_bytes": g["sweep_bytes"], "ms": round(g["sweep_ms"], 5)},
         "cpu_baseline": cpu,
     }

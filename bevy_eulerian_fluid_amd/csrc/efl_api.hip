@@ -40,11 +40,6 @@ struct efl_ctx {
   uint64_t obs_version = 0;
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  // efl_time_stage: kev[2k] / kev[2k + 1] bracket the principal kernel of
-  // repetition k of a stage
-  std::vector<hipEvent_t> kev;
-  uint32_t krep = 0;
-  bool kmark = false, kmarked = false;
   double* h_scal = nullptr;  // pinned, 16 doubles per batch slot
   std::vector<efl_instance*> insts;
   efl_body* d_bodies = nullptr;  // EFL_MAX_SOLIDS records
@@ -229,13 +224,6 @@ void stage_reinit(hipStream_t st, const IV& v) {  // reinitialize_levelset.rs:16
 //   PLAIN: sweeps only.
 enum SweepMode { PLAIN, DOWN, UP };
 
-// Event before (at = 0) / after (at = 1) the principal kernel of a stage
-// while efl_time_stage asks for it.
-void kernel_mark(efl_ctx* c, int at) {
-  if (!c->kmark || 2 * c->krep + 1 >= c->kev.size()) return;
-  (void)hipEventRecord(c->kev[2 * c->krep + at], c->stream);
-  if (at) c->kmarked = true;
-}
 
 // Batch of a tiled multigrid launch: the tiles cover the whole level (each
 // launch writes a complete x into the other buffer of the pair), `ext` only
@@ -809,7 +797,6 @@ int efl_ctx_destroy(efl_ctx* c) {
   for (auto* i : left) efl_instance_destroy(i);
   for (auto& e : c->ev)
     if (e) hipEventDestroy(e);
-  for (auto& e : c->kev) hipEventDestroy(e);
   if (c->d_obs) hipFree(c->d_obs);
   if (c->d_bodies) hipFree(c->d_bodies);
   if (c->h_scal) hipHostFree(c->h_scal);
@@ -1193,18 +1180,14 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
     case EFL_STAGE_FLUID_TO_SOLID: stage_fluid_to_solid(st, v); break;
     case EFL_STAGE_GS_SWEEP: {  // one red + black sweep, weight 1.9, gauss_seidel.rs:142-147
       mg_setup(st, v, false);
-      kernel_mark(c, 0);
       efl::launch_mg_down(st, mg_batch(v, EX_CENTER, 0, false), 0, 1, 1, 1.9f, 0);
-      kernel_mark(c, 1);
       mg_flip(v, 0);
       mg_settle(st, v, 0);
       break;
     }
     case EFL_STAGE_MG_SMOOTH: {  // two red-black sweeps at weight 1 on level 0 in one tiled launch
       mg_setup(st, v, false);
-      kernel_mark(c, 0);
       efl::launch_mg_down(st, mg_batch(v, EX_LEVEL, 0, false), 0, 2, 1, 1.0f, 0);
-      kernel_mark(c, 1);
       mg_flip(v, 0);
       mg_settle(st, v, 0);
       break;
@@ -1233,36 +1216,26 @@ int efl_time_stage(efl_instance* in, int stage, float dt, uint32_t reps, float* 
   HIPCHK(c, hipSetDevice(c->device));
   int rc = efl_run_stage(in, stage, dt);  // warm (and validate the stage id)
   if (rc) return rc;
-  // stages built around one tiled smoother launch report that launch alone
-  // (their x copy-back is not part of the kernel)
-  while (c->kev.size() < 2 * (size_t)reps) {
-    hipEvent_t e = nullptr;
-    HIPCHK(c, hipEventCreate(&e));
-    c->kev.push_back(e);
-  }
-  c->kmarked = false;
-  c->krep = 0;
-  c->kmark = true;
-  rc = efl_run_stage(in, stage, dt);
-  c->kmark = false;
-  if (rc) return rc;
-  if (c->kmarked) {  // back to back, one event pair per repetition
-    c->kmark = true;
-    for (uint32_t k = 0; k < reps && !rc; ++k) {
-      c->krep = k;
-      rc = efl_run_stage(in, stage, dt);
-    }
-    c->kmark = false;
-    c->krep = 0;
-    if (rc) return rc;
-    HIPCHK(c, hipEventSynchronize(c->kev[2 * reps - 1]));
-    double sum = 0.0;
+  if (stage == EFL_STAGE_GS_SWEEP || stage == EFL_STAGE_MG_SMOOTH) {
+    // the tiled smoother alone: reps launches back to back (ping-ponging the
+    // x pair), without the stencil setup and the final copy-back
+    hipStream_t st = c->stream;
+    IV v{in};
+    const bool gs = stage == EFL_STAGE_GS_SWEEP;
+    Batch b = mg_batch(v, gs ? EX_CENTER : EX_LEVEL, 0, false);
+    HIPCHK(c, hipEventRecord(c->ev[0], st));
     for (uint32_t k = 0; k < reps; ++k) {
-      float t = 0.0f;
-      HIPCHK(c, hipEventElapsedTime(&t, c->kev[2 * k], c->kev[2 * k + 1]));
-      sum += t;
+      b.sel[0] = in->xsel[0];
+      efl::launch_mg_down(st, b, 0, gs ? 1 : 2, 1, gs ? 1.9f : 1.0f, 0);
+      mg_flip(v, 0);
     }
-    *ms = (float)(sum / reps);
+    HIPCHK(c, hipEventRecord(c->ev[1], st));
+    mg_settle(st, v, 0);
+    HIPCHK(c, hipEventSynchronize(c->ev[1]));
+    float t = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+    *ms = t / (float)reps;
+    HIPCHK(c, hipStreamSynchronize(st));
     return EFL_OK;
   }
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));

@@ -313,9 +313,10 @@ int efl_step(efl_ctx* ctx, efl_instance* const* insts, uint32_t n, float dt);
 /* Run one stage of the step on one instance (parity tests, profiling). */
 int efl_run_stage(efl_instance* inst, int stage, float dt);
 /* Run a stage reps times back to back on the context stream, bracketed by
- * hipEvents; *ms = mean device time per repetition.  For the stages built
- * around one tiled smoother launch (GS_SWEEP, MG_SMOOTH) the events bracket
- * that launch alone, without the copy that returns x to its buffer. */
+ * hipEvents; *ms = mean device time per repetition.  GS_SWEEP and MG_SMOOTH
+ * time their tiled smoother launch alone: reps launches back to back after
+ * one full run of the stage (stencil setup and the copy of x back to its
+ * buffer stay outside the events). */
 int efl_time_stage(efl_instance* inst, int stage, float dt, uint32_t reps, float* ms);
 
 /* ---- results ------------------------------------------------------------ */
