@@ -212,8 +212,17 @@ __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
 // ---------------------------------------------------------------------------
 // tiled smoothers
 // ---------------------------------------------------------------------------
+// Workgroups go to the 8 XCDs round robin (block i to XCD i mod 8), and
+// each XCD has its own L2.  Renumbering gives XCD k a contiguous band of
+// tiles, so the halos that neighbouring tiles share are read through one L2.
+__device__ __forceinline__ int xcd_tile(int blk, int nb) {
+  const int q = nb >> 3;
+  return blk < 8 * q ? (blk & 7) * q + (blk >> 3) : blk;
+}
+
 struct MSite {
   const DevInst* I;
+  int blk;       // tile index in the launch
   int tx0, ty0;  // tile origin (cells)
   int lvl;
   int ex, ey;    // update extent
@@ -222,11 +231,12 @@ struct MSite {
 };
 
 __device__ __forceinline__ MSite mg_locate(const Batch& b) {
-  int blk = blockIdx.x;
+  const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   int i = 0;
   while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
   int lb = blk - b.start[i];
   MSite s;
+  s.blk = blk;
   s.I = b.inst[i];
   s.tx0 = (lb % b.tiles_x[i]) * MT_W;
   s.ty0 = (lb / b.tiles_x[i]) * MT_H;
@@ -446,7 +456,7 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   if (threadIdx.x == 0) {
     double tot = wsum[0];
     for (int w = 1; w < MNT / 64; ++w) tot += wsum[w];
-    I->red[(size_t)blockIdx.x - b.start[s.slot]] = tot;
+    I->red[(size_t)s.blk - b.start[s.slot]] = tot;
   }
 }
 
