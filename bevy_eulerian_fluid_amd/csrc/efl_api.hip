@@ -18,6 +18,7 @@
 #include "../../include/efl.h"
 #include "efl_internal.h"
 #include "efl_launch.h"
+#include "efl_reduce.h"
 
 using efl::Batch;
 using efl::DevInst;
@@ -74,6 +75,7 @@ struct efl_instance {
   uint8_t xsel[efl::kMaxLevels + 1] = {};
   // per multigrid level: 1 while x is known to be zero (not read)
   uint8_t xzero[efl::kMaxLevels + 1] = {};
+  uint32_t cg_last = 0;  // CG iterations of the previous solve
 };
 
 namespace {
@@ -112,6 +114,13 @@ void extent_of(const efl_instance* in, Extent k, int lvl, int& ex, int& ey) {
 }
 
 // levels: per-instance Batch.level values (or -1 = use `lvl`)
+Batch make_batch(const std::vector<efl_instance*>& v, Extent k, int lvl, const std::vector<int>* levels, int tw,
+                 int th);
+// Full-grid batch of the CG and residual kernels (64 x 16 tiles, one
+// reduction partial per tile).
+Batch cg_batch(const std::vector<efl_instance*>& v) {
+  return make_batch(v, EX_FULL, 0, nullptr, efl::tile_w(), efl::cg_tile_h());
+}
 Batch make_batch(const std::vector<efl_instance*>& v, Extent k, int lvl = 0, const std::vector<int>* levels = nullptr,
                  int tw = 0, int th = 0) {
   Batch b;
@@ -152,6 +161,14 @@ int upload(efl_instance* in) {
     in->h.obs = c->d_obs;
     in->h.nobs = c->nobs;
     in->obs_version = c->obs_version;
+    in->dirty = true;
+  }
+  const efl_projection& pj = in->s.projection;
+  const uint32_t maxc = pj.max_cycles ? pj.max_cycles : 1, every = pj.check_every ? pj.check_every : 1;
+  if (in->h.cg_tol != pj.rel_tol || in->h.cg_maxc != maxc || in->h.cg_every != every) {
+    in->h.cg_tol = pj.rel_tol;
+    in->h.cg_maxc = maxc;
+    in->h.cg_every = every;
     in->dirty = true;
   }
   if (!in->dirty) return EFL_OK;
@@ -383,12 +400,11 @@ void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
 }
 
 // z = M r (one symmetric V-cycle from z = 0), z masked to the pressure
-// unknowns, rz = r . z and beta (finalize kind 3).  Instances whose level-0
-// post-smoothing fits one launch get the mask and the partial sums fused
-// into it; the others take the separate k_pcg_rz pass.
+// unknowns, rz = r . z and beta.  Instances whose level-0 post-smoothing
+// fits one launch get the mask and the r . z reduction fused into it; the
+// others take the separate k_pcg_rz pass.
 void precondition(hipStream_t st, const IV& v) {
-  efl::launch_zero_z(st, make_batch(v, EX_FULL));
-  for (auto* i : v) {
+  for (auto* i : v) {  // z starts at zero: the first launch reads nothing
     i->xsel[0] = 0;
     i->xzero[0] = 1;
   }
@@ -396,8 +412,6 @@ void precondition(hipStream_t st, const IV& v) {
   IV tiled = filter(v, [](efl_instance* i) { return bottom_level(i) != 0; });
   if (!bot.empty()) v_cycle(st, bot, 0, 1, true);
   IV fused;
-  Batch fb;
-  std::memset(&fb, 0, sizeof(fb));
   if (!tiled.empty()) {
     // level 0 by hand so that its post-smoothing can carry r . z
     IV coarse = filter(tiled, [](efl_instance* i) { return bottom_level(i) < 0 && i->h.nlev == 1; });
@@ -409,13 +423,12 @@ void precondition(hipStream_t st, const IV& v) {
     if (!rest.empty()) {
       gs_sweeps(st, rest, 0, cnt_pre, 1.0f, 1, false, EX_LEVEL, DOWN);
       v_cycle(st, rest, 1, 1, true);
-      gs_sweeps(st, rest, 0, cnt_post, 1.0f, 1, true, EX_LEVEL, UP, true, &fused, &fb);
+      gs_sweeps(st, rest, 0, cnt_post, 1.0f, 1, true, EX_LEVEL, UP, true, &fused);
     }
   }
-  if (!fused.empty()) efl::launch_finalize(st, fb, 3);
   mg_settle(st, v, 1);
   IV sep = filter(v, [&](efl_instance* i) { return std::find(fused.begin(), fused.end(), i) == fused.end(); });
-  if (!sep.empty()) efl::launch_pcg_rz(st, make_batch(sep, EX_FULL));
+  if (!sep.empty()) efl::launch_pcg_rz(st, cg_batch(sep));
 }
 
 int read_scalars(efl_ctx* c, const IV& v) {
@@ -473,7 +486,7 @@ int stage_projection(efl_ctx* c, const IV& v) {  // projection.rs:89-164
         else if (k < maxc) keep.push_back(i);
       }
       if (!chk.empty()) {
-        efl::launch_residual_norm(st, make_batch(chk, EX_FULL), 0);
+        efl::launch_residual_norm(st, cg_batch(chk));
         int rc = read_scalars(c, chk);
         if (rc) return rc;
         for (size_t j = 0; j < chk.size(); ++j) {
@@ -489,62 +502,49 @@ int stage_projection(efl_ctx* c, const IV& v) {  // projection.rs:89-164
       act.swap(keep);
     }
   }
-  if (!cg.empty()) {  // flexible-free PCG with one symmetric V-cycle as M
+  if (!cg.empty()) {
+    // CG preconditioned by one symmetric V-cycle.  The stopping test runs on
+    // the device (k_pcg_axpy) and every kernel of a stopped instance returns
+    // at once, so iterations are queued ahead - as many as the previous
+    // solve took - and the host only reads the flags between batches.
     mg_setup(st, cg, true);
-    Batch full = make_batch(cg, EX_FULL);
+    Batch full = cg_batch(cg);
     efl::launch_mask_p(st, full);
-    efl::launch_residual_norm(st, full, 1);
+    efl::launch_pcg_init(st, full);  // r = b - A p0, the initial test
+    precondition(st, cg);
+    IV act = cg;
+    bool first = true;
+    int iter = 0;
+    while (!act.empty()) {
+      uint32_t n = 1;
+      if (first) {
+        n = 0;
+        for (auto* i : act) n = std::max(n, std::min(i->cg_last ? i->cg_last : 4u, i->h.cg_maxc));
+        first = false;
+      }
+      Batch b = cg_batch(act);
+      for (uint32_t k = 0; k < n; ++k) {
+        ++iter;
+        efl::launch_pcg_apply(st, b, iter);  // p = z + beta p, q = A p
+        efl::launch_pcg_axpy(st, b, iter);   // x, r; the stopping test
+        precondition(st, act);               // z = M r, r . z
+      }
+      int rc = read_scalars(c, act);
+      if (rc) return rc;
+      IV keep;
+      for (size_t j = 0; j < act.size(); ++j)
+        if (c->h_scal[16 * j + efl::SC_DONE] == 0.0) keep.push_back(act[j]);
+      act.swap(keep);
+    }
     int rc = read_scalars(c, cg);
     if (rc) return rc;
-    IV act;
     for (size_t j = 0; j < cg.size(); ++j) {
+      const double* sc = c->h_scal + 16 * j;
       efl_instance* i = cg[j];
-      double rel = rel_of(c->h_scal + 16 * j);
-      i->stats.rel_residual = (float)rel;
-      bool tol = i->s.projection.rel_tol > 0.0f;
-      if (tol && rel <= i->s.projection.rel_tol) i->stats.converged = 1;
-      else act.push_back(i);
-    }
-    if (!act.empty()) {
-      Batch b = make_batch(act, EX_FULL);
-      efl::launch_pcg_first(st, b);
-      precondition(st, act);
-      efl::launch_pcg_update_p(st, b);
-    }
-    uint32_t k = 0;
-    while (!act.empty()) {
-      Batch b = make_batch(act, EX_FULL);
-      efl::launch_pcg_apply(st, b);
-      efl::launch_pcg_axpy(st, b);
-      ++k;
-      for (auto* i : act) i->stats.cycles = k;
-      IV keep, chk;
-      for (auto* i : act) {
-        uint32_t maxc = i->s.projection.max_cycles ? i->s.projection.max_cycles : 1;
-        uint32_t every = i->s.projection.check_every ? i->s.projection.check_every : 1;
-        bool tol = i->s.projection.rel_tol > 0.0f;
-        if (tol && (k % every == 0 || k >= maxc)) chk.push_back(i);
-        else if (k < maxc) keep.push_back(i);
-      }
-      if (!chk.empty()) {
-        int rc2 = read_scalars(c, chk);
-        if (rc2) return rc2;
-        for (size_t j = 0; j < chk.size(); ++j) {
-          efl_instance* i = chk[j];
-          double rel = rel_of(c->h_scal + 16 * j);
-          i->stats.rel_residual = (float)rel;
-          uint32_t maxc = i->s.projection.max_cycles ? i->s.projection.max_cycles : 1;
-          bool ok = rel <= i->s.projection.rel_tol;
-          i->stats.converged = ok;
-          if (!ok && k < maxc) keep.push_back(i);
-        }
-      }
-      act.swap(keep);
-      if (!act.empty()) {
-        Batch bb = make_batch(act, EX_FULL);
-        precondition(st, act);
-        efl::launch_pcg_update_p(st, bb);
-      }
+      i->stats.cycles = (uint32_t)sc[efl::SC_ITER];
+      i->stats.converged = sc[efl::SC_CONV] != 0.0;
+      i->stats.rel_residual = (float)sc[efl::SC_REL];
+      i->cg_last = i->stats.cycles;
     }
   }
   return EFL_OK;
@@ -574,7 +574,7 @@ void stage_fluid_to_solid(hipStream_t st, const IV& v) {  // render_node.rs:610-
 int measure_after_projection(efl_ctx* c, const IV& v) {
   IV d = filter(v, [](efl_instance* i) { return i->diag && i->stats.rel_residual < 0.0f; });
   if (d.empty()) return EFL_OK;
-  efl::launch_residual_norm(c->stream, make_batch(d, EX_FULL), 0);
+  efl::launch_residual_norm(c->stream, cg_batch(d));
   int rc = read_scalars(c, d);
   if (rc) return rc;
   for (size_t j = 0; j < d.size(); ++j) d[j]->stats.rel_residual = (float)rel_of(c->h_scal + 16 * j);
@@ -932,7 +932,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
     }
     o_lcf[l] = take(m * 16); o_ldn[l] = take(m * 4); o_lfl[l] = take(m * 4);
   }
-  size_t o_pr = take(n * 4), o_pz = take(n * 4), o_pp = take(n * 4), o_pq = take(n * 4);
+  size_t o_pr = take(n * 4), o_pz = take(n * 4), o_pp = take(n * 4), o_pp1 = take(n * 4), o_pq = take(n * 4);
   size_t o_red = take((size_t)3 * nblk * 8), o_scal = take(16 * 8), o_cnt = take(16), o_diag = take(16);
   size_t o_src0 = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES), o_src = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES);
   size_t o_desc = take(sizeof(DevInst));
@@ -979,7 +979,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
     h.lev[l] = {lv[l].first, lv[l].second, (float*)(base + o_lx[l]), (float*)(base + o_lb[l]), (float*)(base + o_lls[l]),
                 (float4*)(base + o_laf[l]), (float*)(base + o_lr[l]), (float*)(base + o_lxs[l]),
                 (float4*)(base + o_lcf[l]), (float*)(base + o_ldn[l]), (int32_t*)(base + o_lfl[l])};
-  h.pcg_r = (float*)(base + o_pr); h.pcg_z = (float*)(base + o_pz); h.pcg_p = (float*)(base + o_pp); h.pcg_q = (float*)(base + o_pq);
+  h.pcg_r = (float*)(base + o_pr); h.pcg_z = (float*)(base + o_pz); h.pcg_p[0] = (float*)(base + o_pp); h.pcg_p[1] = (float*)(base + o_pp1); h.pcg_q = (float*)(base + o_pq);
   h.lev_pcg0 = h.lev[0];  // z over r, same level set, area fractions and stencil
   h.lev_pcg0.x = h.pcg_z;
   h.lev_pcg0.b = h.pcg_r;
@@ -1315,7 +1315,7 @@ int efl_measure(efl_instance* in, float dt, float* rel, float* mdiv) {
   int rc = upload(in);
   if (rc) return rc;
   IV v{in};
-  efl::launch_residual_norm(c->stream, make_batch(v, EX_FULL), 0);
+  efl::launch_residual_norm(c->stream, cg_batch(v));
   rc = read_scalars(c, v);
   if (rc) return rc;
   if (rel) *rel = (float)rel_of(c->h_scal);

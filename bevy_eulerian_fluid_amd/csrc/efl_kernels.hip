@@ -22,12 +22,14 @@
 
 #include "efl_internal.h"
 #include "efl_launch.h"
+#include "efl_reduce.h"
 
 namespace efl {
 
 constexpr int TX = 64;
 constexpr int TY = 4;
 constexpr int NT = TX * TY;
+static_assert(NT == RED_NT, "reduction kernels use RED_NT threads per block");
 
 // ---------------------------------------------------------------------------
 // indexing helpers
@@ -957,17 +959,17 @@ __global__ void k_accumulate_forces(Batch b) {
 }
 
 // ---------------------------------------------------------------------------
-// Reductions (deterministic: fixed per-block tree, then the last block sums
-// the block partials in index order).
+// Reductions (deterministic: a fixed tree per block, partials folded in
+// index order by fold_partials, efl_reduce.h).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool active_at(const DevInst* I, size_t id) {
   return I->ls0[id] < 0.0f && !all_closed(I->afs[id]);
 }
 
 // Block partial sums: wavefront shuffles, then the 4 wave results in a
-// fixed order; thread 0 stores them at red[k * red_stride + local block].
+// fixed order; thread 0 stores component k at red[(c0 + k) * red_stride + lb].
 template <int N>
-__device__ __forceinline__ void block_partials(const DevInst* I, int lb, double (&v)[N], bool is_max = false) {
+__device__ __forceinline__ void block_partials(const DevInst* I, int c0, int lb, double (&v)[N], bool is_max = false) {
   __shared__ double sh[N][NT / 64];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
@@ -984,7 +986,7 @@ __device__ __forceinline__ void block_partials(const DevInst* I, int lb, double 
     for (int k = 0; k < N; ++k) {
       double acc = sh[k][0];
       for (int w = 1; w < NT / 64; ++w) acc = is_max ? fmax(acc, sh[k][w]) : acc + sh[k][w];
-      I->red[(size_t)k * I->red_stride + lb] = acc;
+      I->red[(size_t)(c0 + k) * I->red_stride + lb] = acc;
     }
   }
 }
@@ -992,108 +994,223 @@ __device__ __forceinline__ void block_partials(const DevInst* I, int lb, double 
 __device__ __forceinline__ int inst_blocks(const Batch& b, int slot) { return b.start[slot + 1] - b.start[slot]; }
 __device__ __forceinline__ int local_block(const Batch& b, int slot) { return (int)blockIdx.x - b.start[slot]; }
 
-// rr = ||r||^2 and bb = ||b||^2 over active cells, with r = b - A x for the
-// finest level (x = p0, b = div); writes r into pcg_r when pcg != 0.
-__global__ __launch_bounds__(NT) void k_residual_norm(Batch b, int pcg_init) {
-  Site s = locate(b);
-  const DevInst* I = s.I;
-  double v[2] = {0.0, 0.0};
-  if (s.ok) {
-    const Level& L = I->lev[0];
-    size_t id = (size_t)s.y * L.w + s.x;
-    bool act = active_at(I, id);
-    float r = residual_value(I, L, s.x, s.y, 1.0f);
-    if (!act) r = 0.0f;
-    if (pcg_init) I->pcg_r[id] = r;
-    float bv = act ? L.b[id] : 0.0f;
-    v[0] = (double)r * (double)r;
-    v[1] = (double)bv * (double)bv;
-  }
-  block_partials<2>(I, local_block(b, s.slot), v);
+// The CG kernels work on 64 x 16 tiles (one partial per tile): thread t
+// takes column t % 64 of rows t / 64 + 4 r, r = 0..3.
+constexpr int CG_TH = 16, CG_ROWS = CG_TH / TY;
+struct CSite {
+  const DevInst* I;
+  int x, y0, ex, ey, slot, lb;
+};
+__device__ __forceinline__ CSite locate_cg(const Batch& b) {
+  int blk = blockIdx.x;
+  int i = 0;
+  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  CSite s;
+  s.lb = blk - b.start[i];
+  s.I = b.inst[i];
+  s.x = (s.lb % b.tiles_x[i]) * TX + (int)(threadIdx.x % TX);
+  s.y0 = (s.lb / b.tiles_x[i]) * CG_TH + (int)(threadIdx.x / TX);
+  s.ex = b.ex[i];
+  s.ey = b.ey[i];
+  s.slot = i;
+  return s;
 }
 
-// z masked to active cells, partial r . z
-__global__ __launch_bounds__(NT) void k_pcg_rz(Batch b) {
-  Site s = locate(b);
+// rr = ||r||^2 and bb = ||b||^2 over active cells (partials RED_RR, RED_BB),
+// with r = b - A x for the finest level (x = p0, b = div); with pcg_init, r
+// also goes to pcg_r.
+__global__ __launch_bounds__(NT) void k_residual_norm(Batch b, int pcg_init) {
+  const CSite s = locate_cg(b);
   const DevInst* I = s.I;
+  double v[2] = {0.0, 0.0};
+  const Level& L = I->lev[0];
+  for (int r = 0; r < CG_ROWS; ++r) {
+    const int y = s.y0 + TY * r;
+    if (s.x >= s.ex || y >= s.ey) continue;
+    size_t id = (size_t)y * L.w + s.x;
+    bool act = active_at(I, id);
+    float rv = residual_value(I, L, s.x, y, 1.0f);
+    if (!act) rv = 0.0f;
+    if (pcg_init) I->pcg_r[id] = rv;
+    float bv = act ? L.b[id] : 0.0f;
+    v[0] += (double)rv * (double)rv;
+    v[1] += (double)bv * (double)bv;
+  }
+  block_partials<2>(I, RED_RR, s.lb, v);
+}
+
+// One block per instance folds the partials of the preceding launch (same
+// Batch):  kind 0: rr, bb -> scal[SC_RR], scal[SC_BB];  kind 4: max |D| ->
+// diag[0].
+__global__ __launch_bounds__(NT) void k_finalize(Batch b, int kind) {
+  const int slot = blockIdx.x;
+  const DevInst* I = b.inst[slot];
+  const int nblk = inst_blocks(b, slot);
+  if (kind == 0) {
+    double t[2];
+    fold_partials<2>(I, RED_RR, nblk, false, t);
+    if (threadIdx.x == 0) {
+      I->scal[SC_RR] = t[0];
+      I->scal[SC_BB] = t[1];
+    }
+  } else {
+    double t[1];
+    fold_partials<1>(I, RED_PQ, nblk, true, t);
+    if (threadIdx.x == 0) I->diag[0] = (float)t[0];
+  }
+}
+
+// The CG stopping test after iteration `iter` (0: the initial residual, whose
+// launch also left ||b||^2): folds ||r||^2, sets scal[SC_DONE] when
+// ||r|| / ||b|| <= rel_tol at a checked iteration (every check_every, and the
+// last) or when max_cycles iterations have run.  One block per instance.
+__global__ __launch_bounds__(NT) void k_cg_check(Batch b, int iter) {
+  const int slot = blockIdx.x;
+  const DevInst* I = b.inst[slot];
+  if (iter > 0 && cg_done(I)) return;
+  const int nblk = inst_blocks(b, slot);
+  double* sc = I->scal;
+  const uint32_t k = (uint32_t)iter;
+  if (k == 0) {
+    double t[2];
+    fold_partials<2>(I, RED_RR, nblk, false, t);
+    if (threadIdx.x != 0) return;
+    sc[SC_RR] = t[0];
+    sc[SC_BB] = t[1];
+    sc[SC_ITER] = 0.0;
+    const double rel = t[1] > 0.0 ? sqrt(t[0] / t[1]) : sqrt(t[0]);
+    sc[SC_REL] = rel;
+    sc[SC_CONV] = 1.0;
+    sc[SC_DONE] = (I->cg_tol > 0.0f && rel <= (double)I->cg_tol) ? 1.0 : 0.0;
+    return;
+  }
+  double t[1];
+  fold_partials<1>(I, RED_RR, nblk, false, t);
+  if (threadIdx.x != 0) return;
+  sc[SC_RR] = t[0];
+  sc[SC_ITER] = (double)k;
+  bool stop = k >= I->cg_maxc;
+  if (I->cg_tol > 0.0f && (k % I->cg_every == 0u || k >= I->cg_maxc)) {
+    const double bb = sc[SC_BB];
+    const double rel = bb > 0.0 ? sqrt(t[0] / bb) : sqrt(t[0]);
+    const bool ok = rel <= (double)I->cg_tol;
+    sc[SC_REL] = rel;
+    sc[SC_CONV] = ok ? 1.0 : 0.0;
+    stop = stop || ok;
+  }
+  if (stop) sc[SC_DONE] = 1.0;
+}
+
+// z masked to active cells, partials of r . z (RED_RZ)
+__global__ __launch_bounds__(NT) void k_pcg_rz(Batch b) {
+  const CSite s = locate_cg(b);
+  const DevInst* I = s.I;
+  if (cg_done(I)) return;
   double v[1] = {0.0};
-  if (s.ok) {
-    size_t id = (size_t)s.y * I->W + s.x;
+  for (int r = 0; r < CG_ROWS; ++r) {
+    const int y = s.y0 + TY * r;
+    if (s.x >= s.ex || y >= s.ey) continue;
+    size_t id = (size_t)y * I->W + s.x;
     float z = I->pcg_z[id];
     if (!active_at(I, id)) { z = 0.0f; I->pcg_z[id] = 0.0f; }
-    v[0] = (double)I->pcg_r[id] * (double)z;
+    v[0] += (double)I->pcg_r[id] * (double)z;
   }
-  block_partials<1>(I, local_block(b, s.slot), v);
+  block_partials<1>(I, RED_RZ, s.lb, v);
+  if (s.lb == 0 && threadIdx.x == 0) I->scal[SC_NRZ] = (double)inst_blocks(b, s.slot);
 }
-__global__ __launch_bounds__(NT) void k_pcg_update_p(Batch b) {
-  Site s = locate(b);
-  if (!s.ok) return;
+
+// Iteration `iter` (1, 2, ...), first half: every block folds the r . z of
+// the preconditioner that just ran, beta = rz / rz_prev (0 for iter 1), and
+// forms the search direction p = z + beta p_prev on the fly (written to
+// pcg_p[iter & 1], recomputed at the neighbours it needs); q = A p (the
+// operator of residual.wgsl with b = 0, negated) and partials of p . q.
+__global__ __launch_bounds__(NT) void k_pcg_apply(Batch b, int iter) {
+  const CSite s = locate_cg(b);
   const DevInst* I = s.I;
-  size_t id = (size_t)s.y * I->W + s.x;
-  float beta = (float)I->scal[5];
-  I->pcg_p[id] = I->pcg_z[id] + beta * I->pcg_p[id];
-}
-// q = A p (the operator of residual.wgsl with b = 0, negated), partial p . q
-__global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
-  Site s = locate(b);
-  const DevInst* I = s.I;
+  if (cg_done(I)) return;
+  double rz[1];
+  fold_partials<1>(I, RED_RZ, (int)I->scal[SC_NRZ], false, rz);
+  const double beta = iter == 1 ? 0.0 : rz[0] / I->scal[SC_RZ0 + ((iter - 2) & 1)];
+  if (s.lb == 0 && threadIdx.x == 0) {
+    I->scal[SC_RZ0 + ((iter - 1) & 1)] = rz[0];
+    I->scal[SC_BETA] = beta;
+  }
+  const float fb = (float)beta;
+  const float* pold = I->pcg_p[(iter - 1) & 1];
+  float* pnew = I->pcg_p[iter & 1];
+  const float* z = I->pcg_z;
+  const Level& L = I->lev[0];
+  const float factor = I->dt / I->rho / I->dx / I->dx;
   double v[1] = {0.0};
-  if (s.ok) {
-    Level L = I->lev[0];
-    L.x = I->pcg_p;
-    size_t id = (size_t)s.y * L.w + s.x;
+  for (int r = 0; r < CG_ROWS; ++r) {
+    const int y = s.y0 + TY * r;
+    if (s.x >= s.ex || y >= s.ey) continue;
+    const size_t id = (size_t)y * L.w + s.x;
+    const float pc = z[id] + fb * pold[id];
+    pnew[id] = pc;
     float q = 0.0f;
     if (active_at(I, id)) {
-      float xc = L.x[id];
-      float4 f = L.af[id];
-      float phi = L.ls[id];
-      float factor = I->dt / I->rho / I->dx / I->dx;
+      const float4 f = L.af[id];
+      const float phi = L.ls[id];
       float res = 0.0f;
       const float fv[4] = {f.x, f.y, f.z, f.w};
       const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        int jx = s.x + ox[k], jy = s.y + oy[k];
+        const int jx = s.x + ox[k], jy = y + oy[k];
         if (inb(jx, jy, L.w, L.h)) {
-          size_t jd = (size_t)jy * L.w + jx;
-          float pn = L.ls[jd];
-          if (pn < 0.0f) res -= fv[k] * (xc - L.x[jd]) * factor;
-          else res -= fv[k] * (1.0f - pn / phi) * xc * factor;
+          const size_t jd = (size_t)jy * L.w + jx;
+          const float pn = L.ls[jd];
+          if (pn < 0.0f) res -= fv[k] * (pc - (z[jd] + fb * pold[jd])) * factor;
+          else res -= fv[k] * (1.0f - pn / phi) * pc * factor;
         }
       }
       q = -res;
     }
     I->pcg_q[id] = q;
-    v[0] = (double)L.x[id] * (double)q;
+    v[0] += (double)pc * (double)q;
   }
-  block_partials<1>(I, local_block(b, s.slot), v);
+  block_partials<1>(I, RED_PQ, s.lb, v);
 }
-// x += alpha p, r -= alpha q, partial ||r||^2
-__global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
-  Site s = locate(b);
+
+// Iteration `iter`, second half: every block folds p . q itself, alpha =
+// rz / pq; x += alpha p, r -= alpha q, partials of ||r||^2 (RED_RR).
+__global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b, int iter) {
+  const CSite s = locate_cg(b);
   const DevInst* I = s.I;
+  if (cg_done(I)) return;
+  double t[1];
+  fold_partials<1>(I, RED_PQ, inst_blocks(b, s.slot), false, t);
+  const double alpha = t[0] > 0.0 ? I->scal[SC_RZ0 + ((iter - 1) & 1)] / t[0] : 0.0;
+  if (s.lb == 0 && threadIdx.x == 0) {
+    I->scal[SC_PQ] = t[0];
+    I->scal[SC_ALPHA] = alpha;
+  }
+  const float fa = (float)alpha;
+  const float* p = I->pcg_p[iter & 1];
   double v[1] = {0.0};
-  if (s.ok) {
-    size_t id = (size_t)s.y * I->W + s.x;
-    float alpha = (float)I->scal[4];
-    I->p0[id] = I->p0[id] + alpha * I->pcg_p[id];
-    float r = I->pcg_r[id] - alpha * I->pcg_q[id];
-    I->pcg_r[id] = r;
-    v[0] = (double)r * (double)r;
+  for (int r = 0; r < CG_ROWS; ++r) {
+    const int y = s.y0 + TY * r;
+    if (s.x >= s.ex || y >= s.ey) continue;
+    size_t id = (size_t)y * I->W + s.x;
+    I->p0[id] = I->p0[id] + fa * p[id];
+    float rv = I->pcg_r[id] - fa * I->pcg_q[id];
+    I->pcg_r[id] = rv;
+    v[0] += (double)rv * (double)rv;
   }
-  block_partials<1>(I, local_block(b, s.slot), v);
+  block_partials<1>(I, RED_RR, s.lb, v);
 }
-__global__ __launch_bounds__(NT) void k_zero(Batch b, int which) {
-  Site s = locate(b);
-  if (!s.ok) return;
+
+// the CG initial guess (the previous pressure) restricted to the unknowns
+__global__ __launch_bounds__(NT) void k_mask_p(Batch b) {
+  const CSite s = locate_cg(b);
   const DevInst* I = s.I;
-  size_t id = (size_t)s.y * I->W + s.x;
-  if (which == 0) I->pcg_z[id] = 0.0f;
-  else if (!active_at(I, id)) I->p0[id] = 0.0f;  // which == 1: mask the initial guess
-}
-__global__ void k_pcg_first(Batch b) {
-  const DevInst* I = b.inst[blockIdx.x];
-  if (threadIdx.x == 0) I->scal[8] = 1.0;  // the next RZ finalize sets beta = 0
+  for (int r = 0; r < CG_ROWS; ++r) {
+    const int y = s.y0 + TY * r;
+    if (s.x >= s.ex || y >= s.ey) continue;
+    size_t id = (size_t)y * I->W + s.x;
+    if (!active_at(I, id)) I->p0[id] = 0.0f;
+  }
 }
 
 // partial max over active cells of |D| (the K10 numerator) on u0/v0
@@ -1115,54 +1232,7 @@ __global__ __launch_bounds__(NT) void k_max_divergence(Batch b) {
       }
     }
   }
-  block_partials<1>(I, local_block(b, s.slot), v, true);
-}
-
-// One block per instance folds the block partials of the preceding
-// reduction kernel (launched with the same Batch) in block order:
-//   kind 0: rr, bb -> scal[2], scal[3]
-//   kind 1: pq -> scal[1]; alpha = rz / pq -> scal[4]
-//   kind 2: rr -> scal[2]
-//   kind 3: rz -> scal[0]; beta = rz / rz_old (0 on the first iteration) -> scal[5]
-//   kind 4: max |D| -> diag[0]
-__global__ __launch_bounds__(NT) void k_finalize(Batch b, int kind) {
-  const int slot = blockIdx.x;
-  const DevInst* I = b.inst[slot];
-  const int nblk = b.start[slot + 1] - b.start[slot];
-  const int N = kind == 0 ? 2 : 1;
-  const bool is_max = kind == 4;
-  __shared__ double sh[2][NT];
-  for (int k = 0; k < N; ++k) {
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += NT) {
-      double v = I->red[(size_t)k * I->red_stride + i];
-      acc = is_max ? fmax(acc, v) : acc + v;
-    }
-    sh[k][threadIdx.x] = acc;
-  }
-  __syncthreads();
-  for (int st = NT / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st)
-      for (int k = 0; k < N; ++k)
-        sh[k][threadIdx.x] = is_max ? fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + st])
-                                    : sh[k][threadIdx.x] + sh[k][threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x != 0) return;
-  double* sc = I->scal;
-  switch (kind) {
-    case 0: sc[2] = sh[0][0]; sc[3] = sh[1][0]; break;
-    case 1: sc[1] = sh[0][0]; sc[4] = sh[0][0] > 0.0 ? sc[0] / sh[0][0] : 0.0; break;
-    case 2: sc[2] = sh[0][0]; break;
-    case 3: {
-      double rz_new = sh[0][0];
-      sc[5] = (sc[8] != 0.0) ? 0.0 : rz_new / sc[0];
-      sc[8] = 0.0;
-      sc[0] = rz_new;
-      break;
-    }
-    case 4: I->diag[0] = (float)sh[0][0]; break;
-  }
+  block_partials<1>(I, RED_PQ, local_block(b, s.slot), v, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1282,27 +1352,21 @@ void launch_ls_gradient(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_ls_gradie
 void launch_clear_bins(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_clear_bins, b, st); }
 void launch_sample_forces(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_sample_forces, b, st); }
 void launch_accumulate_forces(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_accumulate_forces, b, st); }
-void launch_residual_norm(hipStream_t st, const Batch& b, int pcg_init) {
-  EFL_LAUNCH(k_residual_norm, b, st, pcg_init);
+void launch_residual_norm(hipStream_t st, const Batch& b) {
+  EFL_LAUNCH(k_residual_norm, b, st, 0);
   EFL_LAUNCH_PER_INST(k_finalize, b, st, 0);
 }
-void launch_finalize(hipStream_t st, const Batch& b, int kind) { EFL_LAUNCH_PER_INST(k_finalize, b, st, kind); }
-void launch_pcg_rz(hipStream_t st, const Batch& b) {
-  EFL_LAUNCH(k_pcg_rz, b, st);
-  EFL_LAUNCH_PER_INST(k_finalize, b, st, 3);
+void launch_pcg_init(hipStream_t st, const Batch& b) {
+  EFL_LAUNCH(k_residual_norm, b, st, 1);
+  EFL_LAUNCH_PER_INST(k_cg_check, b, st, 0);
 }
-void launch_pcg_update_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_update_p, b, st); }
-void launch_pcg_apply(hipStream_t st, const Batch& b) {
-  EFL_LAUNCH(k_pcg_apply, b, st);
-  EFL_LAUNCH_PER_INST(k_finalize, b, st, 1);
+void launch_pcg_rz(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_rz, b, st); }
+void launch_pcg_apply(hipStream_t st, const Batch& b, int iter) { EFL_LAUNCH(k_pcg_apply, b, st, iter); }
+void launch_pcg_axpy(hipStream_t st, const Batch& b, int iter) {
+  EFL_LAUNCH(k_pcg_axpy, b, st, iter);
+  EFL_LAUNCH_PER_INST(k_cg_check, b, st, iter);
 }
-void launch_pcg_axpy(hipStream_t st, const Batch& b) {
-  EFL_LAUNCH(k_pcg_axpy, b, st);
-  EFL_LAUNCH_PER_INST(k_finalize, b, st, 2);
-}
-void launch_zero_z(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_zero, b, st, 0); }
-void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_zero, b, st, 1); }
-void launch_pcg_first(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_pcg_first, b, st); }
+void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_mask_p, b, st); }
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
   if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);
 }
@@ -1313,5 +1377,6 @@ void launch_max_divergence(hipStream_t st, const Batch& b) {
 
 int tile_w() { return TX; }
 int tile_h() { return TY; }
+int cg_tile_h() { return CG_TH; }
 
 }  // namespace efl

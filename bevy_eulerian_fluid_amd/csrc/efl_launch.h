@@ -9,6 +9,7 @@ namespace efl {
 
 int tile_w();
 int tile_h();
+int cg_tile_h();
 
 void launch_init_center(hipStream_t st, const Batch& b);
 void launch_init_edge(hipStream_t st, const Batch& b);
@@ -42,14 +43,12 @@ void launch_ls_gradient(hipStream_t st, const Batch& b);
 void launch_clear_bins(hipStream_t st, const Batch& b);
 void launch_sample_forces(hipStream_t st, const Batch& b);
 void launch_accumulate_forces(hipStream_t st, const Batch& b);
-void launch_residual_norm(hipStream_t st, const Batch& b, int pcg_init);
+void launch_residual_norm(hipStream_t st, const Batch& b);
+void launch_pcg_init(hipStream_t st, const Batch& b);
 void launch_pcg_rz(hipStream_t st, const Batch& b);
-void launch_pcg_update_p(hipStream_t st, const Batch& b);
-void launch_pcg_apply(hipStream_t st, const Batch& b);
-void launch_pcg_axpy(hipStream_t st, const Batch& b);
-void launch_zero_z(hipStream_t st, const Batch& b);
+void launch_pcg_apply(hipStream_t st, const Batch& b, int iter);
+void launch_pcg_axpy(hipStream_t st, const Batch& b, int iter);
 void launch_mask_p(hipStream_t st, const Batch& b);
-void launch_pcg_first(hipStream_t st, const Batch& b);
 void launch_max_divergence(hipStream_t st, const Batch& b);
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a);
 
@@ -66,6 +65,5 @@ void launch_mg_setup(hipStream_t st, const Batch& b);
 size_t bottom_lds_bytes(int cells);
 void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_);
 void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz);
-void launch_finalize(hipStream_t st, const Batch& b, int kind);
 void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells);
 }  // namespace efl

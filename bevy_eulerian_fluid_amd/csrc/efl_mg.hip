@@ -41,6 +41,7 @@
 
 #include "efl_internal.h"
 #include "efl_launch.h"
+#include "efl_reduce.h"
 
 namespace efl {
 
@@ -48,6 +49,7 @@ constexpr int MT_W = 64;   // tile width (one wavefront per tile row)
 constexpr int MT_H = 16;   // tile height
 constexpr int MNT = 256;   // threads per block
 constexpr int BOTTOM_CELLS = 1024;
+static_assert(MNT == RED_NT, "the r . z reduction uses RED_NT threads per block");
 
 // Level::fl bits
 constexpr int F_IN0 = 0;           // bits 0-3: neighbour k (-x, +x, -y, +y) inside the level
@@ -380,6 +382,7 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   __shared__ float sr[MT_W * MT_H];
   const MSite s = mg_locate(b);
   const DevInst* I = s.I;
+  if (pcg && cg_done(I)) return;
   const Level L = mg_level(I, s.lvl, pcg);
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
@@ -416,8 +419,8 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
 
 // ---------------------------------------------------------------------------
 // up: prolongation while staging, NS post-smoothing sweeps.  With rz (level 0
-// of the CG preconditioner) z is masked to the pressure unknowns and the
-// block partial of r . z goes to red[local block].
+// of the CG preconditioner) z is masked to the pressure unknowns and the block
+// partial of r . z goes to red[RED_RZ][tile].
 // ---------------------------------------------------------------------------
 template <int NS, int HALO>
 __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, float weight, int rz) {
@@ -426,6 +429,7 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   __shared__ double wsum[MNT / 64];
   const MSite s = mg_locate(b);
   const DevInst* I = s.I;
+  if (pcg && cg_done(I)) return;
   const Level L = mg_level(I, s.lvl, pcg);
   const Level C = I->lev[s.lvl + 1];
   const float* xin = (s.sel & 1) ? L.xs : L.x;
@@ -456,7 +460,9 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   if (threadIdx.x == 0) {
     double tot = wsum[0];
     for (int w = 1; w < MNT / 64; ++w) tot += wsum[w];
-    I->red[(size_t)s.blk - b.start[s.slot]] = tot;
+    const int tile = s.blk - b.start[s.slot];
+    I->red[(size_t)RED_RZ * I->red_stride + tile] = tot;
+    if (tile == 0) I->scal[SC_NRZ] = (double)(b.start[s.slot + 1] - b.start[s.slot]);
   }
 }
 
@@ -490,6 +496,7 @@ __global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric
   __shared__ int lw[kMaxLevels + 1], lh[kMaxLevels + 1], loff[kMaxLevels + 1];
   const int slot = blockIdx.x;
   const DevInst* I = b.inst[slot];
+  if (pcg && cg_done(I)) return;
   const int l0 = b.level[slot];
   const int nl = I->nlev;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
