@@ -84,6 +84,8 @@ def gpu_run(args):
     from bevy_eulerian_fluid_amd import native, scenes
     sc = build_scene(args.size, args.rel_tol)
     ctx = native.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     inst = ctx.instance(sc["settings"])
     inst.set_sources(sc["init_sources"], True)
     ctx.set_obstacles(sc["obstacles"])
@@ -178,6 +180,7 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=25.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel instead of replaying graphs")
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the roofline kernel (for profiling it alone)")
     ap.add_argument("--copy-probe", action="store_true",

@@ -32,6 +32,14 @@ static_assert(sizeof(efl_obstacle) == 176, "efl_obstacle layout (std430 SolidObs
 static_assert(sizeof(efl_solid_force) == 16, "efl_solid_force layout");
 static_assert(sizeof(efl_stats) == 32, "efl_stats layout");
 
+struct efl_instance;
+struct StepGraph {
+  std::vector<efl_instance*> insts;
+  uint32_t dt_bits = 0;
+  uint64_t version = 0;
+  hipGraphExec_t pre = nullptr, iter = nullptr, post = nullptr;
+};
+
 struct efl_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -41,6 +49,11 @@ struct efl_ctx {
   uint64_t obs_version = 0;
   bool timing = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // replayed steps (HIP graphs), keyed by instance list, dt and gversion;
+  // gversion changes whenever launch arguments would
+  std::vector<StepGraph> graphs;
+  uint64_t gversion = 1;
+  bool graphs_on = true;
   double* h_scal = nullptr;  // pinned, 16 doubles per batch slot
   std::vector<efl_instance*> insts;
   efl_body* d_bodies = nullptr;  // EFL_MAX_SOLIDS records
@@ -288,7 +301,7 @@ void mg_settle(hipStream_t st, const IV& v, int pcg) {
 
 void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_instance*), float weight, int pcg,
                bool reverse_colour, Extent ext = EX_LEVEL, SweepMode mode = PLAIN, bool rz = false,
-               IV* rz_done = nullptr, Batch* rz_batch = nullptr) {
+               IV* rz_done = nullptr, Batch* rz_batch = nullptr, bool cg_check = false) {
   const int first = reverse_colour ? 0 : 1;
   auto chunks = [&](efl_instance* i) {  // number of launches for this instance
     uint32_t c = count(i);
@@ -326,18 +339,19 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
         if (mode == DOWN && last) {
           IV fin = filter(part, [&](efl_instance* i) { return k + 1 == chunks(i); });
           IV more = filter(part, [&](efl_instance* i) { return k + 1 != chunks(i); });
-          efl::launch_mg_down(st, mg_batch(fin, ext, lvl, false), pcg, (int)n, first, weight, 1);
+          efl::launch_mg_down(st, mg_batch(fin, ext, lvl, false), pcg, (int)n, first, weight, 1, cg_check && k == 0);
           mg_flip(fin, lvl);
           for (auto* i : fin) {  // the coarse x starts at zero (not stored)
             i->xsel[lvl + 1] = 0;
             i->xzero[lvl + 1] = 1;
           }
           if (!more.empty()) {
-            efl::launch_mg_down(st, mg_batch(more, ext, lvl, false), pcg, (int)n, first, weight, 0);
+            efl::launch_mg_down(st, mg_batch(more, ext, lvl, false), pcg, (int)n, first, weight, 0,
+                                cg_check && k == 0);
             mg_flip(more, lvl);
           }
         } else {
-          efl::launch_mg_down(st, mg_batch(part, ext, lvl, false), pcg, (int)n, first, weight, 0);
+          efl::launch_mg_down(st, mg_batch(part, ext, lvl, false), pcg, (int)n, first, weight, 0, cg_check && k == 0);
           mg_flip(part, lvl);
         }
       }
@@ -403,7 +417,11 @@ void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
 // unknowns, rz = r . z and beta.  Instances whose level-0 post-smoothing
 // fits one launch get the mask and the r . z reduction fused into it; the
 // others take the separate k_pcg_rz pass.
-void precondition(hipStream_t st, const IV& v) {
+bool check_in_smoother(const efl_instance* i) { return bottom_level(i) != 0 && i->h.nlev > 1; }
+
+// with cg_check, the first tiled launch of level 0 runs the CG stopping test
+// (instances that start in the bottom solver have run k_cg_check instead)
+void precondition(hipStream_t st, const IV& v, bool cg_check = false) {
   for (auto* i : v) {  // z starts at zero: the first launch reads nothing
     i->xsel[0] = 0;
     i->xzero[0] = 1;
@@ -421,7 +439,7 @@ void precondition(hipStream_t st, const IV& v) {
       gs_sweeps(st, coarse, 0, cnt_coarse, 1.0f, 1, true);
     }
     if (!rest.empty()) {
-      gs_sweeps(st, rest, 0, cnt_pre, 1.0f, 1, false, EX_LEVEL, DOWN);
+      gs_sweeps(st, rest, 0, cnt_pre, 1.0f, 1, false, EX_LEVEL, DOWN, false, nullptr, nullptr, cg_check);
       v_cycle(st, rest, 1, 1, true);
       gs_sweeps(st, rest, 0, cnt_post, 1.0f, 1, true, EX_LEVEL, UP, true, &fused);
     }
@@ -443,123 +461,172 @@ double rel_of(const double* sc) {
   return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 
-int stage_projection(efl_ctx* c, const IV& v) {  // projection.rs:89-164
-  hipStream_t st = c->stream;
-  for (auto* i : v) {
-    i->stats.cycles = 0;
-    i->stats.converged = 1;
-    i->stats.rel_residual = -1.0f;
-  }
+// ---- projection (projection.rs:89-164) ----------------------------------------
+bool mg_needs_host(const efl_instance* i) {  // multigrid repeated to a residual target
+  return i->s.projection.kind == EFL_PROJECTION_MULTIGRID && i->s.projection.rel_tol > 0.0f;
+}
+bool is_cg(const efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_MGPCG; }
+uint32_t max_cycles(const efl_instance* i) { return i->s.projection.max_cycles ? i->s.projection.max_cycles : 1; }
+
+// The solves that need no host decision: Gauss-Seidel (gauss_seidel.rs:
+// 120-148, weight 1.9), Jacobi (projection.rs:135-143) and multigrid with a
+// fixed cycle count.  Enqueue only.
+void proj_fixed(hipStream_t st, const IV& v) {
   IV gs = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_GAUSS_SEIDEL; });
   IV jac = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_JACOBI; });
-  IV mg = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_MULTIGRID; });
-  IV cg = filter(v, [](efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_MGPCG; });
-  if (!gs.empty()) {  // gauss_seidel.rs:120-148 with weight 1.9
+  IV mg = filter(v, [](efl_instance* i) {
+    return i->s.projection.kind == EFL_PROJECTION_MULTIGRID && !mg_needs_host(i);
+  });
+  if (!gs.empty()) {
     mg_setup(st, gs, false);
     gs_sweeps(st, gs, 0, cnt_gs, 1.9f, 0, false, EX_CENTER);
     mg_settle(st, gs, 0);
-    for (auto* i : gs) i->stats.cycles = i->s.projection.gs_iterations;
   }
-  if (!jac.empty()) {  // projection.rs:135-143
+  if (!jac.empty()) {
     Batch b = make_batch(jac, EX_CENTER);
     for (int k = 0; k < 50; ++k) {
       efl::launch_jacobi(st, b, 0);
       efl::launch_jacobi(st, b, 1);
     }
-    for (auto* i : jac) i->stats.cycles = 100;
   }
-  if (!mg.empty()) {  // one V-cycle per cycle; repeat to the residual target
+  if (!mg.empty()) {
     mg_setup(st, mg, true);
-    IV act = mg;
-    uint32_t k = 0;
-    while (!act.empty()) {
+    for (uint32_t k = 0;; ++k) {
+      IV act = filter(mg, [k](efl_instance* i) { return k < max_cycles(i); });
+      if (act.empty()) break;
       v_cycle(st, act, 0, 0, false);
-      ++k;
-      for (auto* i : act) i->stats.cycles = k;
-      IV keep;
-      IV chk;
-      for (auto* i : act) {
-        uint32_t maxc = i->s.projection.max_cycles ? i->s.projection.max_cycles : 1;
-        uint32_t every = i->s.projection.check_every ? i->s.projection.check_every : 1;
-        bool tol = i->s.projection.rel_tol > 0.0f;
-        if (tol && (k % every == 0 || k >= maxc)) chk.push_back(i);
-        else if (k < maxc) keep.push_back(i);
-      }
-      if (!chk.empty()) {
-        efl::launch_residual_norm(st, cg_batch(chk));
-        int rc = read_scalars(c, chk);
-        if (rc) return rc;
-        for (size_t j = 0; j < chk.size(); ++j) {
-          efl_instance* i = chk[j];
-          double rel = rel_of(c->h_scal + 16 * j);
-          i->stats.rel_residual = (float)rel;
-          uint32_t maxc = i->s.projection.max_cycles ? i->s.projection.max_cycles : 1;
-          bool ok = rel <= i->s.projection.rel_tol;
-          i->stats.converged = ok;
-          if (!ok && k < maxc) keep.push_back(i);
-        }
-      }
-      act.swap(keep);
     }
   }
-  if (!cg.empty()) {
-    // CG preconditioned by one symmetric V-cycle.  The stopping test runs on
-    // the device (k_pcg_axpy) and every kernel of a stopped instance returns
-    // at once, so iterations are queued ahead - as many as the previous
-    // solve took - and the host only reads the flags between batches.
-    mg_setup(st, cg, true);
-    Batch full = cg_batch(cg);
-    efl::launch_mask_p(st, full);
-    efl::launch_pcg_init(st, full);  // r = b - A p0, the initial test
-    precondition(st, cg);
-    IV act = cg;
-    bool first = true;
-    int iter = 0;
-    while (!act.empty()) {
-      uint32_t n = 1;
-      if (first) {
-        n = 0;
-        for (auto* i : act) n = std::max(n, std::min(i->cg_last ? i->cg_last : 4u, i->h.cg_maxc));
-        first = false;
-      }
-      Batch b = cg_batch(act);
-      for (uint32_t k = 0; k < n; ++k) {
-        ++iter;
-        efl::launch_pcg_apply(st, b, iter);  // p = z + beta p, q = A p
-        efl::launch_pcg_axpy(st, b, iter);   // x, r; the stopping test
-        precondition(st, act);               // z = M r, r . z
-      }
-      int rc = read_scalars(c, act);
+}
+void proj_fixed_stats(const IV& v) {
+  for (auto* i : v) {
+    i->stats.cycles = 0;
+    i->stats.converged = 1;
+    i->stats.rel_residual = -1.0f;
+    switch (i->s.projection.kind) {
+      case EFL_PROJECTION_GAUSS_SEIDEL: i->stats.cycles = i->s.projection.gs_iterations; break;
+      case EFL_PROJECTION_JACOBI: i->stats.cycles = 100; break;
+      case EFL_PROJECTION_MULTIGRID: i->stats.cycles = max_cycles(i); break;
+      default: break;
+    }
+  }
+}
+
+// Multigrid repeated until ||r|| / ||b|| <= rel_tol (checked every
+// check_every cycles on the host).
+int proj_mg_tol(efl_ctx* c, const IV& mg) {
+  hipStream_t st = c->stream;
+  mg_setup(st, mg, true);
+  IV act = mg;
+  uint32_t k = 0;
+  while (!act.empty()) {
+    v_cycle(st, act, 0, 0, false);
+    ++k;
+    for (auto* i : act) i->stats.cycles = k;
+    IV keep, chk;
+    for (auto* i : act) {
+      uint32_t every = i->s.projection.check_every ? i->s.projection.check_every : 1;
+      if (k % every == 0 || k >= max_cycles(i)) chk.push_back(i);
+      else if (k < max_cycles(i)) keep.push_back(i);
+    }
+    if (!chk.empty()) {
+      efl::launch_residual_norm(st, cg_batch(chk));
+      int rc = read_scalars(c, chk);
       if (rc) return rc;
-      IV keep;
-      for (size_t j = 0; j < act.size(); ++j)
-        if (c->h_scal[16 * j + efl::SC_DONE] == 0.0) keep.push_back(act[j]);
-      act.swap(keep);
+      for (size_t j = 0; j < chk.size(); ++j) {
+        efl_instance* i = chk[j];
+        double rel = rel_of(c->h_scal + 16 * j);
+        i->stats.rel_residual = (float)rel;
+        bool ok = rel <= i->s.projection.rel_tol;
+        i->stats.converged = ok;
+        if (!ok && k < max_cycles(i)) keep.push_back(i);
+      }
     }
-    int rc = read_scalars(c, cg);
-    if (rc) return rc;
-    for (size_t j = 0; j < cg.size(); ++j) {
-      const double* sc = c->h_scal + 16 * j;
-      efl_instance* i = cg[j];
-      i->stats.cycles = (uint32_t)sc[efl::SC_ITER];
-      i->stats.converged = sc[efl::SC_CONV] != 0.0;
-      i->stats.rel_residual = (float)sc[efl::SC_REL];
-      i->cg_last = i->stats.cycles;
-    }
+    act.swap(keep);
   }
   return EFL_OK;
 }
 
-void stage_extrapolate(hipStream_t st, const IV& v) {  // render_node.rs:540-608
-  Batch bu = make_batch(v, EX_XMAJOR), bv = make_batch(v, EX_YMAJOR);
-  efl::launch_valid_u(st, bu);
-  efl::launch_valid_v(st, bv);
-  for (int k = 0; k < 10 / 2; ++k) {
-    efl::launch_extrapolate(st, bu, 0, 0);
-    efl::launch_extrapolate(st, bu, 0, 1);
-    efl::launch_extrapolate(st, bv, 1, 0);
-    efl::launch_extrapolate(st, bv, 1, 1);
+// CG preconditioned by one symmetric V-cycle.  The stopping test runs on the
+// device (k_cg_check after k_pcg_axpy) and every kernel of a stopped
+// instance returns at once, so iterations are queued ahead and the host only
+// reads the flags between batches.
+void cg_begin(hipStream_t st, const IV& cg) {
+  mg_setup(st, cg, true);
+  Batch full = cg_batch(cg);
+  efl::launch_mask_p(st, full);
+  efl::launch_pcg_init(st, full);  // r = b - A p0, the initial test
+  precondition(st, cg);
+}
+void cg_iteration(hipStream_t st, const IV& cg) {
+  Batch b = cg_batch(cg);
+  efl::launch_pcg_apply(st, b);  // p = z + beta p, q = A p
+  efl::launch_pcg_axpy(st, b);   // x, r, ||r||^2
+  IV sep = filter(cg, [](efl_instance* i) { return !check_in_smoother(i); });
+  if (!sep.empty()) efl::launch_cg_check(st, cg_batch(sep));
+  precondition(st, cg, true);  // the stopping test, z = M r, r . z
+}
+// Queue as many iterations as the previous solve took, then one at a time
+// until every instance has stopped; `iter` replays one iteration (a graph)
+// or is null (eager launches).
+int cg_run(efl_ctx* c, const IV& cg, hipGraphExec_t iter) {
+  hipStream_t st = c->stream;
+  uint32_t n = 0;
+  for (auto* i : cg) n = std::max(n, std::min(i->cg_last ? i->cg_last : 4u, i->h.cg_maxc));
+  IV act = cg;
+  while (!act.empty()) {
+    for (uint32_t k = 0; k < n; ++k) {
+      if (iter) HIPCHK(c, hipGraphLaunch(iter, st));
+      else cg_iteration(st, act);
+    }
+    n = 1;
+    int rc = read_scalars(c, act);
+    if (rc) return rc;
+    IV keep;
+    for (size_t j = 0; j < act.size(); ++j)
+      if (c->h_scal[16 * j + efl::SC_DONE] == 0.0) keep.push_back(act[j]);
+    if (iter) keep = keep.empty() ? keep : act;  // the graph advances the whole set
+    act.swap(keep);
   }
+  int rc = read_scalars(c, cg);
+  if (rc) return rc;
+  for (size_t j = 0; j < cg.size(); ++j) {
+    const double* sc = c->h_scal + 16 * j;
+    efl_instance* i = cg[j];
+    i->stats.cycles = (uint32_t)sc[efl::SC_ITER];
+    i->stats.converged = sc[efl::SC_CONV] != 0.0;
+    i->stats.rel_residual = (float)sc[efl::SC_REL];
+    i->cg_last = i->stats.cycles;
+  }
+  return EFL_OK;
+}
+
+int stage_projection(efl_ctx* c, const IV& v) {
+  hipStream_t st = c->stream;
+  IV fixed = filter(v, [](efl_instance* i) { return !is_cg(i) && !mg_needs_host(i); });
+  IV mgt = filter(v, [](efl_instance* i) { return mg_needs_host(i); });
+  IV cg = filter(v, [](efl_instance* i) { return is_cg(i); });
+  proj_fixed_stats(v);
+  proj_fixed(st, fixed);
+  if (!mgt.empty()) {
+    int rc = proj_mg_tol(c, mgt);
+    if (rc) return rc;
+  }
+  if (!cg.empty()) {
+    cg_begin(st, cg);
+    int rc = cg_run(c, cg, nullptr);
+    if (rc) return rc;
+  }
+  return EFL_OK;
+}
+
+// render_node.rs:540-608: the validity masks, then 10 passes per component
+// (u and v are independent, so all u passes run in one launch and all v
+// passes in another)
+void stage_extrapolate(hipStream_t st, const IV& v) {
+  const int tw = efl::extrapolate_tile_w(), th = efl::extrapolate_tile_h();
+  efl::launch_extrapolate_fused(st, make_batch(v, EX_XMAJOR, 0, nullptr, tw, th), 0);
+  efl::launch_extrapolate_fused(st, make_batch(v, EX_YMAJOR, 0, nullptr, tw, th), 1);
 }
 
 void stage_fluid_to_solid(hipStream_t st, const IV& v) {  // render_node.rs:610-641
@@ -618,10 +685,9 @@ int run_init(efl_ctx* c, const IV& v) {
   return EFL_OK;
 }
 
-int run_step(efl_ctx* c, const IV& v) {  // render_node.rs:278-466
-  hipStream_t st = c->stream;
-  bool timed = c->timing && v.size() == 1;
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[0], st));
+// ---- the step (render_node.rs:278-466) ---------------------------------------
+// solids .. divergence (render_node.rs:278-330)
+void phase_pre(hipStream_t st, const IV& v) {
   Batch center = make_batch(v, EX_CENTER), bx = make_batch(v, EX_XMAJOR), by = make_batch(v, EX_YMAJOR);
   efl::launch_update_solid(st, center);
   efl::launch_area_fraction(st, center);
@@ -630,23 +696,23 @@ int run_step(efl_ctx* c, const IV& v) {  // render_node.rs:278-466
   efl::launch_forces_u(st, bx);
   efl::launch_forces_v(st, by);
   efl::launch_divergence(st, center);
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[1], st));
-  int rc = stage_projection(c, v);
-  if (rc) return rc;
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[2], st));
-  rc = measure_after_projection(c, v);
-  if (rc) return rc;
-  efl::launch_solve_u(st, bx);
-  efl::launch_solve_v(st, by);
-  rc = measure_divergence(c, v);
-  if (rc) return rc;
+}
+// velocity update (render_node.rs:331-350)
+void phase_velocity(hipStream_t st, const IV& v) {
+  efl::launch_solve_u(st, make_batch(v, EX_XMAJOR));
+  efl::launch_solve_v(st, make_batch(v, EX_YMAJOR));
+}
+// extrapolation .. fluid -> solid (render_node.rs:351-466)
+void phase_post(hipStream_t st, const IV& v) {
+  Batch center = make_batch(v, EX_CENTER);
   stage_extrapolate(st, v);
   efl::launch_advect_levelset(st, center);
   efl::launch_fluid_source(st, center, 0);
   stage_reinit(st, v);
   efl::launch_ls_gradient(st, center);
   stage_fluid_to_solid(st, v);
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[3], st));
+}
+void step_done(const IV& v) {
   for (auto* i : v) {
     i->stats.steps += 1;
     if (i->h.nforces) {  // apply_forces_and_clear (apply_forces.rs:157-167)
@@ -654,6 +720,25 @@ int run_step(efl_ctx* c, const IV& v) {  // render_node.rs:278-466
       i->dirty = true;
     }
   }
+}
+
+int run_step(efl_ctx* c, const IV& v) {
+  hipStream_t st = c->stream;
+  bool timed = c->timing && v.size() == 1;
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[0], st));
+  phase_pre(st, v);
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[1], st));
+  int rc = stage_projection(c, v);
+  if (rc) return rc;
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[2], st));
+  rc = measure_after_projection(c, v);
+  if (rc) return rc;
+  phase_velocity(st, v);
+  rc = measure_divergence(c, v);
+  if (rc) return rc;
+  phase_post(st, v);
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[3], st));
+  step_done(v);
   if (timed) {
     HIPCHK(c, hipEventSynchronize(c->ev[3]));
     float a = 0, b = 0;
@@ -710,6 +795,106 @@ void integrate_bodies(efl_ctx* c, const IV& stepped, float dt) {
     fill(a, IV{i});
     efl::launch_integrate_bodies(c->stream, a);
   }
+}
+
+// ---- replayed steps -------------------------------------------------------------
+void drop_graphs(efl_ctx* c) {
+  for (auto& g : c->graphs)
+    for (hipGraphExec_t e : {g.pre, g.iter, g.post})
+      if (e) hipGraphExecDestroy(e);
+  c->graphs.clear();
+}
+
+template <class F>
+hipGraphExec_t capture(efl_ctx* c, F&& enqueue) {
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) return nullptr;
+  enqueue();
+  if (hipStreamEndCapture(c->stream, &g) != hipSuccess || !g) return nullptr;
+  hipGraphExec_t e = nullptr;
+  if (hipGraphInstantiate(&e, g, nullptr, nullptr, 0) != hipSuccess) e = nullptr;
+  hipGraphDestroy(g);
+  return e;
+}
+
+// The step as up to three replayed graphs: everything before the CG
+// iterations (solids .. divergence, the fixed solves, the CG start), one CG
+// iteration (replayed as often as the host loop asks), everything after the
+// solve including the rigid bodies.  The launch arguments of a step depend
+// only on the instance list, dt and what bumps gversion; the multigrid
+// buffer-pair state the launches were recorded with is the state every step
+// starts and ends in (checked at capture).  *used = false when the step must
+// run eagerly (timing, diagnostics, multigrid to a residual target).
+int run_step_graph(efl_ctx* c, const IV& v, float dt, bool* used) {
+  *used = false;
+  if (!c->graphs_on || c->timing) return EFL_OK;
+  for (auto* i : v)
+    if (i->diag || mg_needs_host(i)) return EFL_OK;
+  uint32_t db = 0;
+  std::memcpy(&db, &dt, 4);
+  StepGraph* g = nullptr;
+  for (auto& e : c->graphs)
+    if (e.insts == v && e.dt_bits == db && e.version == c->gversion) g = &e;
+  IV cg = filter(v, [](efl_instance* i) { return is_cg(i); });
+  IV fixed = filter(v, [](efl_instance* i) { return !is_cg(i); });
+  hipStream_t st = c->stream;
+  if (!g) {
+    if (!c->graphs.empty() && c->graphs[0].version != c->gversion) drop_graphs(c);
+    std::vector<std::vector<uint8_t>> snap;
+    for (auto* i : v) {
+      snap.emplace_back(i->xsel, i->xsel + efl::kMaxLevels + 1);
+      snap.back().insert(snap.back().end(), i->xzero, i->xzero + efl::kMaxLevels + 1);
+    }
+    StepGraph n;
+    n.insts = v;
+    n.dt_bits = db;
+    n.version = c->gversion;
+    n.pre = capture(c, [&] {
+      phase_pre(st, v);
+      proj_fixed(st, fixed);
+      if (!cg.empty()) cg_begin(st, cg);
+    });
+    if (!cg.empty()) n.iter = capture(c, [&] { cg_iteration(st, cg); });
+    n.post = capture(c, [&] {
+      phase_velocity(st, v);
+      phase_post(st, v);
+      integrate_bodies(c, v, dt);
+    });
+    bool ok = n.pre && n.post && (cg.empty() || n.iter);
+    for (size_t k = 0; k < v.size() && ok; ++k) {
+      std::vector<uint8_t> now(v[k]->xsel, v[k]->xsel + efl::kMaxLevels + 1);
+      now.insert(now.end(), v[k]->xzero, v[k]->xzero + efl::kMaxLevels + 1);
+      ok = now == snap[k];
+    }
+    if (!ok) {  // run this configuration eagerly from now on
+      for (hipGraphExec_t e : {n.pre, n.iter, n.post})
+        if (e) hipGraphExecDestroy(e);
+      (void)hipGetLastError();
+      c->graphs_on = false;
+      for (size_t k = 0; k < v.size(); ++k) {  // back to the recorded start state
+        std::copy(snap[k].begin(), snap[k].begin() + efl::kMaxLevels + 1, v[k]->xsel);
+        std::copy(snap[k].begin() + efl::kMaxLevels + 1, snap[k].end(), v[k]->xzero);
+      }
+      return EFL_OK;
+    }
+    if (c->graphs.size() >= 8) {
+      for (hipGraphExec_t e : {c->graphs[0].pre, c->graphs[0].iter, c->graphs[0].post})
+        if (e) hipGraphExecDestroy(e);
+      c->graphs.erase(c->graphs.begin());
+    }
+    c->graphs.push_back(n);
+    g = &c->graphs.back();
+  }
+  *used = true;
+  proj_fixed_stats(v);
+  HIPCHK(c, hipGraphLaunch(g->pre, st));
+  if (!cg.empty()) {
+    int rc = cg_run(c, cg, g->iter);
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipGraphLaunch(g->post, st));
+  step_done(v);
+  return EFL_OK;
 }
 
 size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -795,6 +980,7 @@ int efl_ctx_destroy(efl_ctx* c) {
   hipStreamSynchronize(c->stream);
   std::vector<efl_instance*> left = c->insts;
   for (auto* i : left) efl_instance_destroy(i);
+  drop_graphs(c);
   for (auto& e : c->ev)
     if (e) hipEventDestroy(e);
   if (c->d_obs) hipFree(c->d_obs);
@@ -820,11 +1006,21 @@ int efl_set_timing(efl_ctx* c, int enabled) {
   return EFL_OK;
 }
 
+int efl_set_graphs(efl_ctx* c, int enabled) {
+  if (!c) return EFL_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  c->graphs_on = enabled != 0;
+  return EFL_OK;
+}
+
 int efl_set_obstacles(efl_ctx* c, const efl_obstacle* obs, uint32_t n) {
   if (!c || (n && !obs)) return EFL_E_INVALID;
   if (n > EFL_MAX_SOLIDS) return fail(c, EFL_E_RANGE, "%u obstacles > EFL_MAX_SOLIDS", n);
   HIPCHK(c, hipSetDevice(c->device));
   if (n) HIPCHK(c, hipMemcpyAsync(c->d_obs, obs, sizeof(efl_obstacle) * n, hipMemcpyHostToDevice, c->stream));
+  if (n != c->nobs) c->gversion++;  // the body integrator's launch depends on the count
   c->nobs = n;
   c->obs_version++;
   return EFL_OK;
@@ -840,6 +1036,7 @@ int efl_set_bodies(efl_ctx* c, const efl_body* bodies, uint32_t n) {
   if (n) HIPCHK(c, hipMemcpyAsync(c->d_bodies, bodies, sizeof(efl_body) * n, hipMemcpyHostToDevice, c->stream));
   if (n) HIPCHK(c, hipMemsetAsync(c->d_bodies + EFL_MAX_SOLIDS, 0, 16 * EFL_MAX_SOLIDS, c->stream));
   c->nbodies = n;
+  c->gversion++;
   return EFL_OK;
 }
 
@@ -858,6 +1055,7 @@ int efl_set_instance_obstacles(efl_instance* in, const efl_obstacle* obs, const 
   if (n > EFL_MAX_SOLIDS) return fail(c, EFL_E_RANGE, "%u obstacles > EFL_MAX_SOLIDS", n);
   HIPCHK(c, hipSetDevice(c->device));
   if (!obs) {
+    if (in->own_obs) c->gversion++;
     in->own_obs = false;
     in->own_bodies = false;
     in->nobs_own = 0;
@@ -871,6 +1069,7 @@ int efl_set_instance_obstacles(efl_instance* in, const efl_obstacle* obs, const 
     HIPCHK(c, hipMemcpyAsync(in->d_bodies_own, bodies, sizeof(efl_body) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(in->d_bodies_own + EFL_MAX_SOLIDS, 0, 16 * EFL_MAX_SOLIDS, c->stream));
   }
+  if (!in->own_obs || in->own_bodies != (bodies != nullptr) || in->nobs_own != n) c->gversion++;
   in->own_obs = true;
   in->own_bodies = bodies != nullptr;
   in->nobs_own = n;
@@ -1017,6 +1216,8 @@ int efl_instance_destroy(efl_instance* in) {
   if (in->d_bodies_own) hipFree(in->d_bodies_own);
   if (in->mem) hipFree(in->mem);
   c->insts.erase(std::remove(c->insts.begin(), c->insts.end(), in), c->insts.end());
+  drop_graphs(c);  // their launches name this instance
+  c->gversion++;
   delete in;
   return EFL_OK;
 }
@@ -1025,6 +1226,7 @@ int efl_set_projection(efl_instance* in, const efl_projection* p) {
   if (!in || !p) return EFL_E_INVALID;
   if (!valid_projection(*p)) return fail(in->ctx, EFL_E_INVALID, "invalid projection");
   in->s.projection = *p;
+  in->ctx->gversion++;
   return EFL_OK;
 }
 
@@ -1032,6 +1234,7 @@ int efl_set_reinit(efl_instance* in, const efl_reinit* r) {
   if (!in || !r) return EFL_E_INVALID;
   if (!valid_reinit(*r)) return fail(in->ctx, EFL_E_INVALID, "invalid reinitialisation method");
   in->s.reinit = *r;
+  in->ctx->gversion++;
   return EFL_OK;
 }
 
@@ -1042,6 +1245,7 @@ int efl_set_uniforms(efl_instance* in, float rho, const float g[2], const float 
   if (g) { in->h.gx = g[0]; in->h.gy = g[1]; in->s.gravity[0] = g[0]; in->s.gravity[1] = g[1]; }
   if (T) for (int k = 0; k < 16; ++k) in->h.T[k] = T[k];
   in->dirty = true;
+  in->ctx->gversion++;  // the body integrator takes gravity and rho as arguments
   return EFL_OK;
 }
 
@@ -1135,9 +1339,14 @@ int efl_step(efl_ctx* c, efl_instance* const* insts, uint32_t n, float dt) {
     if (rc) return rc;
   }
   if (!step.empty()) {
-    int rc = run_step(c, step);
+    bool graphed = false;
+    int rc = run_step_graph(c, step, dt, &graphed);
     if (rc) return rc;
-    integrate_bodies(c, step, dt);
+    if (!graphed) {
+      rc = run_step(c, step);
+      if (rc) return rc;
+      integrate_bodies(c, step, dt);
+    }
     for (auto* i : step) {  // forces were cleared after use
       rc = upload(i);
       if (rc) return rc;

@@ -93,6 +93,22 @@ struct BodyStep {
   float4* state;  // per body: previous (ax, ay, angular acceleration)
 };
 
+// Level::fl bits, written by k_mg_setup
+constexpr int MG_IN0 = 0;           // bits 0-3: neighbour k (-x, +x, -y, +y) inside the level
+constexpr int MG_FL0 = 4;           // bits 4-7: neighbour k inside the level and fluid (ls < 0)
+constexpr int MG_GSZ = 1 << 8;      // the GS update gives 0 (ls >= 0, closed, |denominator| < 1e-6)
+constexpr int MG_RZ = 1 << 9;       // the residual is 0 (ls >= 0 or closed)
+constexpr int MG_LSNEG = 1 << 10;   // ls < 0 (prolongation target, z mask)
+constexpr int MG_CLOSED = 1 << 11;  // all four area fractions are 0
+
+// Workgroups go to the 8 XCDs round robin (block i to XCD i mod 8), and
+// each XCD has its own L2.  Renumbering gives XCD k a contiguous band of
+// tiles, so halos that neighbouring tiles share are read through one L2.
+__device__ __forceinline__ int xcd_tile(int blk, int nb) {
+  const int q = nb >> 3;
+  return blk < 8 * q ? (blk & 7) * q + (blk >> 3) : blk;
+}
+
 // Launch map of a batched kernel: the flat grid of blocks is split into
 // per-instance ranges; inside a range, blocks tile the instance's extent.
 struct Batch {

@@ -291,10 +291,58 @@ __device__ float obstacle_level(const efl_obstacle& o, float x, float y) {
   }
 }
 
+// A disk (cx, cy, R) in world units that holds the whole shape: the signed
+// distance of every point p is then >= |p - c| - R (outside: the distance to
+// the shape is at least that to the disk; inside: the boundary is no farther
+// than the disk's).  R = -1 marks a shape whose level is the 1e6 sentinel.
+__device__ float4 obstacle_bound(const efl_obstacle& o) {
+  switch (o.shape) {
+    case 0u: return make_float4(o.transform[12], o.transform[13], o.values[0], 0.0f);
+    case 1u: {
+      if (det_mat4(o.transform) == 0.0f) return make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+      // local q = A p + t (inverse transform): centre c = -A^-1 t, and
+      // |q| >= s_min(A) |p - c|, so the radius in world units is
+      // |half extents| / s_min(A)
+      const float* m = o.inverse_transform;
+      const double a = m[0], bb = m[4], c = m[1], d = m[5], tx = m[12], ty = m[13];
+      const double det = a * d - bb * c;
+      if (det == 0.0) return make_float4(0.0f, 0.0f, 3.0e38f, 0.0f);
+      const double cx = -(d * tx - bb * ty) / det, cy = -(-c * tx + a * ty) / det;
+      const double S = a * a + bb * bb + c * c + d * d;
+      const double smin = sqrt(fmax(0.0, 0.5 * (S - sqrt(fmax(0.0, S * S - 4.0 * det * det)))));
+      const double hr = sqrt((double)o.values[0] * o.values[0] + (double)o.values[1] * o.values[1]);
+      return make_float4((float)cx, (float)cy, smin > 0.0 ? (float)(hr / smin) : 3.0e38f, 0.0f);
+    }
+    case 2u: {
+      float2 a = xform(o.transform, o.values[0], o.values[1]);
+      float2 c = xform(o.transform, o.values[2], o.values[3]);
+      return make_float4(0.5f * (a.x + c.x), 0.5f * (a.y + c.y), 0.5f * hyp(c.x - a.x, c.y - a.y) + o.values[4], 0.0f);
+    }
+    case 4u: {
+      float2 p0 = xform(o.transform, o.values[0], o.values[1]);
+      float2 p1 = xform(o.transform, o.values[2], o.values[3]);
+      float2 p2 = xform(o.transform, o.values[4], o.values[5]);
+      float cx = (p0.x + p1.x + p2.x) / 3.0f, cy = (p0.y + p1.y + p2.y) / 3.0f;
+      float r = fmaxf(fmaxf(hyp(p0.x - cx, p0.y - cy), hyp(p1.x - cx, p1.y - cy)), hyp(p2.x - cx, p2.y - cy));
+      return make_float4(cx, cy, r, 0.0f);
+    }
+    default: return make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+  }
+}
+
+// An obstacle can change a cell's results only where one of its three
+// levels (cell centre, u face, v face) is below the current level or below
+// 0.5; obstacles whose bounding disk proves otherwise (with a one-cell margin
+// for rounding) are skipped, the others are evaluated in index order as the
+// reference loops (the minimum, and the last obstacle below 0.5 winning the
+// id and the face velocities, are unchanged).
 __global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
+  __shared__ float4 bnd[EFL_MAX_SOLIDS];
   Site s = locate(b);
-  if (!s.ok) return;
   const DevInst* I = s.I;
+  for (uint32_t k = threadIdx.x; k < I->nobs; k += NT) bnd[k] = obstacle_bound(I->obs[k]);
+  __syncthreads();
+  if (!s.ok) return;
   int W = I->W, H = I->H, x = s.x, y = s.y;
   size_t id = (size_t)y * W + x;
   if (x == 0 || y == 0 || x == W - 1 || y == H - 1) {
@@ -315,7 +363,13 @@ __global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
   float level = (float)min(tx, ty);
   float uu = 0.0f, vv = 0.0f;
   int32_t id_s = -1;
+  // the faces' sample points lie within fd of the centre
+  const float fd = fmaxf(hyp(pu.x - pc.x, pu.y - pc.y), hyp(pv.x - pc.x, pv.y - pc.y));
   for (uint32_t k = 0; k < I->nobs; ++k) {
+    const float4 bd = bnd[k];
+    if (bd.z < 0.0f) continue;  // level 1e6 everywhere: changes nothing
+    const float lb = hyp(pc.x - bd.x, pc.y - bd.y) - bd.z - 1.0f;
+    if (lb >= level && lb >= 0.5f && lb - fd >= 0.5f) continue;
     const efl_obstacle& o = I->obs[k];
     float lc = obstacle_level(o, pc.x, pc.y);
     if (lc < level) level = lc;
@@ -611,56 +665,118 @@ __global__ __launch_bounds__(NT) void k_solve_v(Batch b) {
 }
 
 // ---------------------------------------------------------------------------
-// K20 - K23 velocity extrapolation (extrapolate/*.wgsl)
+// K20-K23 velocity extrapolation (extrapolate/*.wgsl), fused: the validity
+// mask of one velocity component (initialize_u_valid.wgsl /
+// initialize_v_valid.wgsl: a face is valid (1) where the open fraction of the
+// air level set averaged to its two end corners is 0) and its 10
+// extrapolation passes (extrapolate_u.wgsl / extrapolate_v.wgsl, loop
+// render_node.rs:540-608) in one launch.  A 64 x 16 tile of faces stages its faces plus a 10-face
+// halo (one per pass) in LDS, with both validity planes and the level set
+// cells the mask needs; pass p is exact on faces at least p + 1 from the
+// region border, so the tile's faces are exact after 10.  Each pass updates
+// only faces inside the dispatch coverage [0, ex) x [0, ey), so faces in the
+// reference's coverage holes keep their values, as there.  Within a pass the
+// face values change in place: a pass writes only faces that were invalid
+// and reads only faces that were valid.
+//
+// The launch writes its fields in place while later tiles stage halos, and
+// what a late tile may stage is never consumed: the velocity of a face that
+// some tile fills was invalid at the start, so no tile reads its staged
+// value before computing it itself; valid faces are never written; the
+// first validity plane is recomputed from the level set on covered faces and
+// written unchanged elsewhere; the second is overwritten on covered faces by
+// the first pass before any pass reads it.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_valid_u(Batch b) {
-  Site s = locate(b);
-  if (!s.ok) return;
-  const DevInst* I = s.I;
-  int W = I->W, H = I->H, x = s.x, y = s.y;
-  const float* L = I->ls0;
-  float c0 = ldz(L, W, H, x - 1, y - 1), c1 = ldz(L, W, H, x, y - 1), c2 = ldz(L, W, H, x - 1, y);
-  float c3 = ldz(L, W, H, x, y), c4 = ldz(L, W, H, x - 1, y + 1), c5 = ldz(L, W, H, x, y + 1);
-  float vm = 0.25f * (((c0 + c1) + c2) + c3), vp = 0.25f * (((c2 + c3) + c4) + c5);
-  I->uval0[(size_t)y * (W + 1) + x] = face_open(vm, vp) > 0.0f ? 0 : 1;
-}
-__global__ __launch_bounds__(NT) void k_valid_v(Batch b) {
-  Site s = locate(b);
-  if (!s.ok) return;
-  const DevInst* I = s.I;
-  int W = I->W, H = I->H, x = s.x, y = s.y;
-  const float* L = I->ls0;
-  float c0 = ldz(L, W, H, x - 1, y - 1), c1 = ldz(L, W, H, x - 1, y), c2 = ldz(L, W, H, x, y - 1);
-  float c3 = ldz(L, W, H, x, y), c4 = ldz(L, W, H, x + 1, y - 1), c5 = ldz(L, W, H, x + 1, y);
-  float vm = 0.25f * (((c0 + c1) + c2) + c3), vp = 0.25f * (((c2 + c3) + c4) + c5);
-  I->vval0[(size_t)y * W + x] = face_open(vm, vp) > 0.0f ? 0 : 1;
-}
-// extrapolate_u / extrapolate_v (identical bodies): comp 0 = u, 1 = v;
-// dir 0 = valid0 -> valid1, dir 1 = valid1 -> valid0 (reverse bind group)
-__global__ __launch_bounds__(NT) void k_extrapolate(Batch b, int comp, int dir) {
-  Site s = locate(b);
-  if (!s.ok) return;
-  const DevInst* I = s.I;
-  int w = comp == 0 ? I->W + 1 : I->W;
-  int h = comp == 0 ? I->H : I->H + 1;
-  float* f = comp == 0 ? I->u0 : I->v0;
-  const int32_t* vin = comp == 0 ? (dir ? I->uval1 : I->uval0) : (dir ? I->vval1 : I->vval0);
-  int32_t* vout = comp == 0 ? (dir ? I->uval0 : I->uval1) : (dir ? I->vval0 : I->vval1);
-  int x = s.x, y = s.y;
-  size_t id = (size_t)y * w + x;
-  if (vin[id] == 1) { vout[id] = 1; return; }
-  int count = 0;
-  float nu = 0.0f;
-  if (x + 1 < w && y < h && vin[id + 1] == 1) { nu += f[id + 1]; count += 1; }
-  if (x - 1 > 0 && y > 0 && vin[id - 1] == 1) { nu += f[id - 1]; count += 1; }
-  if (x < w && y + 1 < h && vin[id + w] == 1) { nu += f[id + w]; count += 1; }
-  if (x > 0 && y - 1 > 0 && vin[id - w] == 1) { nu += f[id - w]; count += 1; }
-  if (count > 0) {
-    nu /= (float)count;
-    f[id] = nu;
-    vout[id] = 1;
-  } else {
-    vout[id] = 0;
+constexpr int XP = 10;  // passes = halo
+constexpr int XT_W = 64, XT_H = 16;
+constexpr int XR_W = XT_W + 2 * XP, XR_H = XT_H + 2 * XP, XR_N = XR_W * XR_H;
+constexpr int XL_W = XR_W + 2, XL_N = XL_W * (XR_H + 2);
+
+template <int COMP>
+__global__ __launch_bounds__(NT) void k_extrapolate_fused(Batch b) {
+  __shared__ float sf[XR_N];
+  __shared__ uint8_t sv[2][XR_N];
+  __shared__ float sl[XL_N];
+  const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  int i = 0;
+  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  const int lb = blk - b.start[i];
+  const DevInst* I = b.inst[i];
+  const int W = I->W, H = I->H;
+  const int w = COMP == 0 ? W + 1 : W, h = COMP == 0 ? H : H + 1;
+  const int ex = b.ex[i], ey = b.ey[i];
+  const int ox = (lb % b.tiles_x[i]) * XT_W - XP, oy = (lb / b.tiles_x[i]) * XT_H - XP;
+  float* F = COMP == 0 ? I->u0 : I->v0;
+  int32_t* V0 = COMP == 0 ? I->uval0 : I->vval0;
+  int32_t* V1 = COMP == 0 ? I->uval1 : I->vval1;
+  // level-set cells [ox - 1, ox + XR_W + 1) x [oy - 1, oy + XR_H + 1)
+  for (int q = threadIdx.x; q < XL_N; q += NT) sl[q] = ldz(I->ls0, W, H, ox - 1 + q % XL_W, oy - 1 + q / XL_W);
+  __syncthreads();
+  for (int q = threadIdx.x; q < XR_N; q += NT) {
+    const int lx = q % XR_W, ly = q / XR_W, x = ox + lx, y = oy + ly;
+    float f = 0.0f;
+    uint8_t v0 = 0, v1 = 0;
+    if (inb(x, y, w, h)) {
+      const size_t id = (size_t)y * w + x;
+      f = F[id];
+      v1 = (uint8_t)V1[id];
+      if (x < ex && y < ey) {  // valid_u / valid_v cover the same faces
+        const int c = (ly + 1) * XL_W + lx + 1;  // cell (x, y)
+        float c0, c1, c2, c3, c4, c5;
+        if (COMP == 0) {
+          c0 = sl[c - XL_W - 1]; c1 = sl[c - XL_W]; c2 = sl[c - 1]; c3 = sl[c]; c4 = sl[c + XL_W - 1]; c5 = sl[c + XL_W];
+        } else {
+          c0 = sl[c - XL_W - 1]; c1 = sl[c - 1]; c2 = sl[c - XL_W]; c3 = sl[c]; c4 = sl[c - XL_W + 1]; c5 = sl[c + 1];
+        }
+        const float vm = 0.25f * (((c0 + c1) + c2) + c3), vp = 0.25f * (((c2 + c3) + c4) + c5);
+        v0 = face_open(vm, vp) > 0.0f ? 0 : 1;
+      } else {
+        v0 = (uint8_t)V0[id];
+      }
+    }
+    sf[q] = f;
+    sv[0][q] = v0;
+    sv[1][q] = v1;
+  }
+  __syncthreads();
+  const int cx1 = min(ex, w) - ox, cy1 = min(ey, h) - oy;  // coverage in local coordinates
+  for (int p = 0; p < XP; ++p) {
+    const uint8_t* vin = sv[p & 1];  // pass 0: valid0 -> valid1, then back
+    uint8_t* vout = sv[1 - (p & 1)];
+    const int m = p + 1;
+    const int lo_x = max(m, -ox), hi_x = min(XR_W - m, cx1), lo_y = max(m, -oy), hi_y = min(XR_H - m, cy1);
+    for (int q = threadIdx.x; q < XR_N; q += NT) {
+      const int lx = q % XR_W, ly = q / XR_W;
+      if (lx < lo_x || lx >= hi_x || ly < lo_y || ly >= hi_y) continue;
+      if (vin[q] == 1) {
+        vout[q] = 1;
+        continue;
+      }
+      const int x = ox + lx, y = oy + ly;
+      int count = 0;
+      float nu = 0.0f;
+      if (x + 1 < w && y < h && vin[q + 1] == 1) { nu += sf[q + 1]; count += 1; }
+      if (x - 1 > 0 && y > 0 && vin[q - 1] == 1) { nu += sf[q - 1]; count += 1; }
+      if (x < w && y + 1 < h && vin[q + XR_W] == 1) { nu += sf[q + XR_W]; count += 1; }
+      if (x > 0 && y - 1 > 0 && vin[q - XR_W] == 1) { nu += sf[q - XR_W]; count += 1; }
+      if (count > 0) {
+        nu /= (float)count;
+        sf[q] = nu;
+        vout[q] = 1;
+      } else {
+        vout[q] = 0;
+      }
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < XT_W * XT_H; t += NT) {
+    const int lx = XP + t % XT_W, ly = XP + t / XT_W, x = ox + lx, y = oy + ly;
+    if (!inb(x, y, w, h)) continue;
+    const size_t id = (size_t)y * w + x;
+    const int q = ly * XR_W + lx;
+    F[id] = sf[q];
+    V0[id] = sv[0][q];
+    V1[id] = sv[1][q];
   }
 }
 
@@ -1060,18 +1176,17 @@ __global__ __launch_bounds__(NT) void k_finalize(Batch b, int kind) {
   }
 }
 
-// The CG stopping test after iteration `iter` (0: the initial residual, whose
-// launch also left ||b||^2): folds ||r||^2, sets scal[SC_DONE] when
-// ||r|| / ||b|| <= rel_tol at a checked iteration (every check_every, and the
-// last) or when max_cycles iterations have run.  One block per instance.
-__global__ __launch_bounds__(NT) void k_cg_check(Batch b, int iter) {
+// The CG stopping test (init: of the initial residual, whose launch also
+// left ||b||^2; else cg_stop() of the iteration k_pcg_axpy just ran): folds
+// ||r||^2 and records the outcome.  One block per instance; the tiled
+// preconditioner runs the same test in its first launch instead.
+__global__ __launch_bounds__(NT) void k_cg_check(Batch b, int init) {
   const int slot = blockIdx.x;
   const DevInst* I = b.inst[slot];
-  if (iter > 0 && cg_done(I)) return;
+  if (!init && cg_done(I)) return;
   const int nblk = inst_blocks(b, slot);
   double* sc = I->scal;
-  const uint32_t k = (uint32_t)iter;
-  if (k == 0) {
+  if (init) {
     double t[2];
     fold_partials<2>(I, RED_RR, nblk, false, t);
     if (threadIdx.x != 0) return;
@@ -1086,19 +1201,7 @@ __global__ __launch_bounds__(NT) void k_cg_check(Batch b, int iter) {
   }
   double t[1];
   fold_partials<1>(I, RED_RR, nblk, false, t);
-  if (threadIdx.x != 0) return;
-  sc[SC_RR] = t[0];
-  sc[SC_ITER] = (double)k;
-  bool stop = k >= I->cg_maxc;
-  if (I->cg_tol > 0.0f && (k % I->cg_every == 0u || k >= I->cg_maxc)) {
-    const double bb = sc[SC_BB];
-    const double rel = bb > 0.0 ? sqrt(t[0] / bb) : sqrt(t[0]);
-    const bool ok = rel <= (double)I->cg_tol;
-    sc[SC_REL] = rel;
-    sc[SC_CONV] = ok ? 1.0 : 0.0;
-    stop = stop || ok;
-  }
-  if (stop) sc[SC_DONE] = 1.0;
+  if (threadIdx.x == 0) cg_stop(I, t[0], true);
 }
 
 // z masked to active cells, partials of r . z (RED_RZ)
@@ -1119,15 +1222,17 @@ __global__ __launch_bounds__(NT) void k_pcg_rz(Batch b) {
   if (s.lb == 0 && threadIdx.x == 0) I->scal[SC_NRZ] = (double)inst_blocks(b, s.slot);
 }
 
-// Iteration `iter` (1, 2, ...), first half: every block folds the r . z of
-// the preconditioner that just ran, beta = rz / rz_prev (0 for iter 1), and
-// forms the search direction p = z + beta p_prev on the fly (written to
-// pcg_p[iter & 1], recomputed at the neighbours it needs); q = A p (the
-// operator of residual.wgsl with b = 0, negated) and partials of p . q.
-__global__ __launch_bounds__(NT) void k_pcg_apply(Batch b, int iter) {
+// Iteration iter = scal[SC_ITER] + 1, first half: every block folds the
+// r . z of the preconditioner that just ran, beta = rz / rz_prev (0 for
+// iter 1), and forms the search direction p = z + beta p_prev on the fly
+// (written to pcg_p[iter & 1], recomputed at the neighbours it needs);
+// q = A p (the operator of residual.wgsl with b = 0, negated) and partials
+// of p . q.
+__global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
   const CSite s = locate_cg(b);
   const DevInst* I = s.I;
   if (cg_done(I)) return;
+  const int iter = (int)I->scal[SC_ITER] + 1;
   double rz[1];
   fold_partials<1>(I, RED_RZ, (int)I->scal[SC_NRZ], false, rz);
   const double beta = iter == 1 ? 0.0 : rz[0] / I->scal[SC_RZ0 + ((iter - 2) & 1)];
@@ -1148,21 +1253,23 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b, int iter) {
     const size_t id = (size_t)y * L.w + s.x;
     const float pc = z[id] + fb * pold[id];
     pnew[id] = pc;
+    // residual.wgsl's operator from the precomputed stencil (k_mg_setup):
+    // fluid neighbours c_k (p - p_k), air neighbours c_k p, times factor
+    const int fl = L.fl[id];
     float q = 0.0f;
-    if (active_at(I, id)) {
-      const float4 f = L.af[id];
-      const float phi = L.ls[id];
+    if ((fl & MG_LSNEG) && !(fl & MG_CLOSED)) {  // active_at()
+      const float4 c = L.cf[id];
+      const float cv[4] = {c.x, c.y, c.z, c.w};
+      const long off[4] = {-1, 1, -(long)L.w, (long)L.w};
       float res = 0.0f;
-      const float fv[4] = {f.x, f.y, f.z, f.w};
-      const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const int jx = s.x + ox[k], jy = y + oy[k];
-        if (inb(jx, jy, L.w, L.h)) {
-          const size_t jd = (size_t)jy * L.w + jx;
-          const float pn = L.ls[jd];
-          if (pn < 0.0f) res -= fv[k] * (pc - (z[jd] + fb * pold[jd])) * factor;
-          else res -= fv[k] * (1.0f - pn / phi) * pc * factor;
+        if (!((fl >> k) & 1)) continue;
+        if ((fl >> (4 + k)) & 1) {
+          const size_t jd = (size_t)((long)id + off[k]);
+          res = res - cv[k] * (pc - (z[jd] + fb * pold[jd])) * factor;
+        } else {
+          res = res - cv[k] * pc * factor;
         }
       }
       q = -res;
@@ -1173,18 +1280,23 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b, int iter) {
   block_partials<1>(I, RED_PQ, s.lb, v);
 }
 
-// Iteration `iter`, second half: every block folds p . q itself, alpha =
-// rz / pq; x += alpha p, r -= alpha q, partials of ||r||^2 (RED_RR).
-__global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b, int iter) {
+// Second half of the same iteration: every block folds p . q itself,
+// alpha = rz / pq; x += alpha p, r -= alpha q, partials of ||r||^2 (RED_RR).
+__global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
   const CSite s = locate_cg(b);
   const DevInst* I = s.I;
   if (cg_done(I)) return;
+  const int iter = (int)I->scal[SC_ITER] + 1;
   double t[1];
   fold_partials<1>(I, RED_PQ, inst_blocks(b, s.slot), false, t);
   const double alpha = t[0] > 0.0 ? I->scal[SC_RZ0 + ((iter - 1) & 1)] / t[0] : 0.0;
   if (s.lb == 0 && threadIdx.x == 0) {
     I->scal[SC_PQ] = t[0];
     I->scal[SC_ALPHA] = alpha;
+  }
+  if (s.lb == 0 && threadIdx.x == 0) {
+    I->scal[SC_KCUR] = (double)iter;
+    I->scal[SC_NRR] = (double)inst_blocks(b, s.slot);
   }
   const float fa = (float)alpha;
   const float* p = I->pcg_p[iter & 1];
@@ -1337,9 +1449,16 @@ void launch_prolong(hipStream_t st, const Batch& b, int pcg) { EFL_LAUNCH(k_prol
 void launch_jacobi(hipStream_t st, const Batch& b, int reverse) { EFL_LAUNCH(k_jacobi, b, st, reverse); }
 void launch_solve_u(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_solve_u, b, st); }
 void launch_solve_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_solve_v, b, st); }
-void launch_valid_u(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_valid_u, b, st); }
-void launch_valid_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_valid_v, b, st); }
-void launch_extrapolate(hipStream_t st, const Batch& b, int comp, int dir) { EFL_LAUNCH(k_extrapolate, b, st, comp, dir); }
+// valid mask + 10 passes of one component; b tiles the coverage in 64 x 16 tiles
+void launch_extrapolate_fused(hipStream_t st, const Batch& b, int comp) {
+  const int nb = b.start[b.n];
+  if (nb <= 0) return;
+  if (comp == 0) hipLaunchKernelGGL(k_extrapolate_fused<0>, dim3(nb), dim3(NT), 0, st, b);
+  else hipLaunchKernelGGL(k_extrapolate_fused<1>, dim3(nb), dim3(NT), 0, st, b);
+}
+int extrapolate_tile_w() { return XT_W; }
+int extrapolate_tile_h() { return XT_H; }
+int extrapolate_passes() { return XP; }
 void launch_advect_levelset(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advect_levelset, b, st); }
 void launch_jfa_init(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_jfa_init, b, st); }
 void launch_jfa_step(hipStream_t st, const Batch& b, int step) { EFL_LAUNCH(k_jfa_step, b, st, step); }
@@ -1358,14 +1477,12 @@ void launch_residual_norm(hipStream_t st, const Batch& b) {
 }
 void launch_pcg_init(hipStream_t st, const Batch& b) {
   EFL_LAUNCH(k_residual_norm, b, st, 1);
-  EFL_LAUNCH_PER_INST(k_cg_check, b, st, 0);
+  EFL_LAUNCH_PER_INST(k_cg_check, b, st, 1);
 }
 void launch_pcg_rz(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_rz, b, st); }
-void launch_pcg_apply(hipStream_t st, const Batch& b, int iter) { EFL_LAUNCH(k_pcg_apply, b, st, iter); }
-void launch_pcg_axpy(hipStream_t st, const Batch& b, int iter) {
-  EFL_LAUNCH(k_pcg_axpy, b, st, iter);
-  EFL_LAUNCH_PER_INST(k_cg_check, b, st, iter);
-}
+void launch_pcg_apply(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_apply, b, st); }
+void launch_pcg_axpy(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_axpy, b, st); }
+void launch_cg_check(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_cg_check, b, st, 0); }
 void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_mask_p, b, st); }
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
   if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);

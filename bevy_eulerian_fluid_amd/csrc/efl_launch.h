@@ -28,9 +28,10 @@ void launch_prolong(hipStream_t st, const Batch& b, int pcg);
 void launch_jacobi(hipStream_t st, const Batch& b, int reverse);
 void launch_solve_u(hipStream_t st, const Batch& b);
 void launch_solve_v(hipStream_t st, const Batch& b);
-void launch_valid_u(hipStream_t st, const Batch& b);
-void launch_valid_v(hipStream_t st, const Batch& b);
-void launch_extrapolate(hipStream_t st, const Batch& b, int comp, int dir);
+void launch_extrapolate_fused(hipStream_t st, const Batch& b, int comp);
+int extrapolate_tile_w();
+int extrapolate_tile_h();
+int extrapolate_passes();
 void launch_advect_levelset(hipStream_t st, const Batch& b);
 void launch_jfa_init(hipStream_t st, const Batch& b);
 void launch_jfa_step(hipStream_t st, const Batch& b, int step);  // Batch.level = source seed buffer
@@ -46,8 +47,9 @@ void launch_accumulate_forces(hipStream_t st, const Batch& b);
 void launch_residual_norm(hipStream_t st, const Batch& b);
 void launch_pcg_init(hipStream_t st, const Batch& b);
 void launch_pcg_rz(hipStream_t st, const Batch& b);
-void launch_pcg_apply(hipStream_t st, const Batch& b, int iter);
-void launch_pcg_axpy(hipStream_t st, const Batch& b, int iter);
+void launch_pcg_apply(hipStream_t st, const Batch& b);
+void launch_pcg_axpy(hipStream_t st, const Batch& b);
+void launch_cg_check(hipStream_t st, const Batch& b);
 void launch_mask_p(hipStream_t st, const Batch& b);
 void launch_max_divergence(hipStream_t st, const Batch& b);
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a);
@@ -63,7 +65,8 @@ int mg_setup_tile_w();
 int mg_setup_tile_h();
 void launch_mg_setup(hipStream_t st, const Batch& b);
 size_t bottom_lds_bytes(int cells);
-void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_);
+void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_,
+                    int cg_check = 0);
 void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz);
 void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells);
 }  // namespace efl

@@ -51,13 +51,8 @@ constexpr int MNT = 256;   // threads per block
 constexpr int BOTTOM_CELLS = 1024;
 static_assert(MNT == RED_NT, "the r . z reduction uses RED_NT threads per block");
 
-// Level::fl bits
-constexpr int F_IN0 = 0;           // bits 0-3: neighbour k (-x, +x, -y, +y) inside the level
-constexpr int F_FL0 = 4;           // bits 4-7: neighbour k inside the level and fluid (ls < 0)
-constexpr int F_GSZ = 1 << 8;      // the GS update gives 0 (ls >= 0, closed, |denominator| < 1e-6)
-constexpr int F_RZ = 1 << 9;       // the residual is 0 (ls >= 0 or closed)
-constexpr int F_LSNEG = 1 << 10;   // ls < 0 (prolongation target, z mask)
-constexpr int F_CLOSED = 1 << 11;  // all four area fractions are 0
+// Level::fl bits (efl_internal.h)
+constexpr int F_IN0 = MG_IN0, F_FL0 = MG_FL0, F_GSZ = MG_GSZ, F_RZ = MG_RZ, F_LSNEG = MG_LSNEG, F_CLOSED = MG_CLOSED;
 
 __device__ __forceinline__ bool mg_inb(int x, int y, int w, int h) {
   return (unsigned)x < (unsigned)w && (unsigned)y < (unsigned)h;
@@ -214,14 +209,6 @@ __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
 // ---------------------------------------------------------------------------
 // tiled smoothers
 // ---------------------------------------------------------------------------
-// Workgroups go to the 8 XCDs round robin (block i to XCD i mod 8), and
-// each XCD has its own L2.  Renumbering gives XCD k a contiguous band of
-// tiles, so the halos that neighbouring tiles share are read through one L2.
-__device__ __forceinline__ int xcd_tile(int blk, int nb) {
-  const int q = nb >> 3;
-  return blk < 8 * q ? (blk & 7) * q + (blk >> 3) : blk;
-}
-
 struct MSite {
   const DevInst* I;
   int blk;       // tile index in the launch
@@ -376,13 +363,19 @@ __device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, 
 // cell sees exact neighbours.
 // ---------------------------------------------------------------------------
 template <int NS, int HALO>
-__global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, float weight, int restrict_) {
+__global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, float weight, int restrict_,
+                                                 int cg_check) {
   using T = Tile<HALO>;
   __shared__ float sx[T::RW * T::RH];
   __shared__ float sr[MT_W * MT_H];
   const MSite s = mg_locate(b);
   const DevInst* I = s.I;
   if (pcg && cg_done(I)) return;
+  if (cg_check) {  // the CG stopping test of the iteration that just ran
+    double rr[1];
+    fold_partials<1>(I, RED_RR, (int)I->scal[SC_NRR], false, rr);
+    if (cg_stop(I, rr[0], s.blk == b.start[s.slot] && threadIdx.x == 0)) return;
+  }
   const Level L = mg_level(I, s.lvl, pcg);
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
@@ -618,14 +611,16 @@ void launch_mg_setup(hipStream_t st, const Batch& b) {
 }
 
 // nsweeps 0..3 pre-smoothing sweeps (+ residual/restriction)
-void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_) {
+// with cg_check, the launch first runs the CG stopping test (every block)
+void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_,
+                    int cg_check) {
   int nb = b.start[b.n];
   if (nb <= 0) return;
   switch (nsweeps) {
-    case 0: MG_LAUNCH((k_mg_down<0, 1>), nb, st, b, pcg, first, weight, restrict_); break;
-    case 1: MG_LAUNCH((k_mg_down<1, 3>), nb, st, b, pcg, first, weight, restrict_); break;
-    case 2: MG_LAUNCH((k_mg_down<2, 5>), nb, st, b, pcg, first, weight, restrict_); break;
-    default: MG_LAUNCH((k_mg_down<3, 7>), nb, st, b, pcg, first, weight, restrict_); break;
+    case 0: MG_LAUNCH((k_mg_down<0, 1>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+    case 1: MG_LAUNCH((k_mg_down<1, 3>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+    case 2: MG_LAUNCH((k_mg_down<2, 5>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+    default: MG_LAUNCH((k_mg_down<3, 7>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
   }
 }
 

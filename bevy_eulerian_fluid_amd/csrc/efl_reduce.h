@@ -25,14 +25,18 @@ constexpr int RED_PQ = 0, RED_RR = 1, RED_BB = 2, RED_RZ = 2;  // bb is folded b
 // scal[] slots
 constexpr int SC_PQ = 1, SC_RR = 2, SC_BB = 3, SC_ALPHA = 4, SC_BETA = 5, SC_DONE = 6;
 constexpr int SC_NRZ = 7;            // number of r . z partials of the last preconditioner
+constexpr int SC_NRR = 8;            // number of ||r||^2 partials of the last k_pcg_axpy
 constexpr int SC_ITER = 9, SC_REL = 10, SC_CONV = 11;
 constexpr int SC_RZ0 = 12;           // r . z of iteration k at SC_RZ0 + (k & 1)
+constexpr int SC_KCUR = 14;          // the iteration k_pcg_axpy just ran
 
 // Fold the nblk partials of components c0 .. c0 + N - 1 (all threads of a
-// block of RED_NT call it; every thread gets the sums).
+// block of RED_NT call it; every thread gets the sums): thread t sums
+// partials t, t + 256, ... in order, a butterfly over the wavefront's lanes
+// (the same tree in every lane), then the 4 wavefront sums in order.
 template <int N>
 __device__ __forceinline__ void fold_partials(const DevInst* I, int c0, int nblk, bool is_max, double (&out)[N]) {
-  __shared__ double sh[N][RED_NT];
+  __shared__ double sh[N][RED_NT / 64];
   const int t = threadIdx.x;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
@@ -41,21 +45,53 @@ __device__ __forceinline__ void fold_partials(const DevInst* I, int c0, int nblk
       const double v = I->red[(size_t)(c0 + k) * I->red_stride + i];
       acc = is_max ? fmax(acc, v) : acc + v;
     }
-    sh[k][t] = acc;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(acc, off, 64);
+      acc = is_max ? fmax(acc, o) : acc + o;
+    }
+    if ((t & 63) == 0) sh[k][t >> 6] = acc;
   }
   __syncthreads();
-  for (int st = RED_NT / 2; st > 0; st >>= 1) {
-    if (t < st)
 #pragma unroll
-      for (int k = 0; k < N; ++k) sh[k][t] = is_max ? fmax(sh[k][t], sh[k][t + st]) : sh[k][t] + sh[k][t + st];
-    __syncthreads();
+  for (int k = 0; k < N; ++k) {
+    double acc = sh[k][0];
+    for (int w = 1; w < RED_NT / 64; ++w) acc = is_max ? fmax(acc, sh[k][w]) : acc + sh[k][w];
+    out[k] = acc;
   }
-#pragma unroll
-  for (int k = 0; k < N; ++k) out[k] = sh[k][0];
 }
 
 // True once the CG solve of this instance has stopped: every kernel of an
 // iteration returns at once (the host may queue iterations ahead).
 __device__ __forceinline__ bool cg_done(const DevInst* I) { return I->scal[SC_DONE] != 0.0; }
+
+// The stopping test of iteration k = scal[SC_KCUR] with ||r||^2 = rr: stop
+// when ||r|| / ||b|| <= rel_tol at a checked iteration (every check_every,
+// and the last) or when max_cycles iterations have run.  Every block that
+// evaluates it gets the same answer (it reads only values of earlier
+// launches); the one with `record` stores the outcome.
+__device__ __forceinline__ bool cg_stop(const DevInst* I, double rr, bool record) {
+  double* sc = I->scal;
+  const uint32_t k = (uint32_t)sc[SC_KCUR];
+  bool stop = k >= I->cg_maxc;
+  bool checked = false, ok = false;
+  double rel = 0.0;
+  if (I->cg_tol > 0.0f && (k % I->cg_every == 0u || k >= I->cg_maxc)) {
+    const double bb = sc[SC_BB];
+    rel = bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+    ok = rel <= (double)I->cg_tol;
+    checked = true;
+    stop = stop || ok;
+  }
+  if (record) {
+    sc[SC_RR] = rr;
+    sc[SC_ITER] = (double)k;
+    if (checked) {
+      sc[SC_REL] = rel;
+      sc[SC_CONV] = ok ? 1.0 : 0.0;
+    }
+    if (stop) sc[SC_DONE] = 1.0;
+  }
+  return stop;
+}
 
 }  // namespace efl

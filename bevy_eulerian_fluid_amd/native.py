@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libefl.so")
 _lib = None
 
 EXPORTS = [
-    "efl_ctx_create", "efl_ctx_destroy", "efl_last_error", "efl_sync", "efl_set_timing",
+    "efl_ctx_create", "efl_ctx_destroy", "efl_last_error", "efl_sync", "efl_set_timing", "efl_set_graphs",
     "efl_set_obstacles", "efl_instance_create", "efl_instance_destroy", "efl_set_projection",
     "efl_set_reinit", "efl_set_uniforms", "efl_set_sources", "efl_set_forces", "efl_init_levelset",
     "efl_reset", "efl_initialize", "efl_step", "efl_run_stage", "efl_field_info", "efl_get_field",
@@ -56,6 +56,7 @@ def load():
         "efl_ctx_destroy": [P],
         "efl_sync": [P],
         "efl_set_timing": [P, C.c_int],
+        "efl_set_graphs": [P, C.c_int],
         "efl_set_obstacles": [P, P, C.c_uint32],
         "efl_instance_create": [P, C.POINTER(abi.efl_settings), PP],
         "efl_instance_destroy": [P],
@@ -128,6 +129,10 @@ class Context:
 
     def set_timing(self, on):
         self._chk(self.L.efl_set_timing(self.h, int(bool(on))), "efl_set_timing")
+
+    def set_graphs(self, on):
+        """Replay steps as HIP graphs (default) or launch every kernel eagerly."""
+        self._chk(self.L.efl_set_graphs(self.h, int(bool(on))), "efl_set_graphs")
 
     def set_obstacles(self, obstacles):
         arr, n = abi.struct_array(abi.efl_obstacle, list(obstacles))

@@ -237,6 +237,12 @@ const char* efl_last_error(const efl_ctx* ctx);
 int efl_sync(efl_ctx* ctx);
 /* Per-stage device timing (hipEvents) into efl_stats; off by default. */
 int efl_set_timing(efl_ctx* ctx, int enabled);
+/* No counterpart in the reference (it records wgpu command buffers each
+ * frame).  On by default: efl_step records a step's launches once as HIP
+ * graphs (per instance list, dt and launch-relevant settings) and replays
+ * them; results are identical to eager launches.  Steps with timing,
+ * diagnostics or a multigrid residual target always run eagerly. */
+int efl_set_graphs(efl_ctx* ctx, int enabled);
 
 /* construct_rigid_body_buffer_for_gpu (obstacle.rs:89-160): the obstacle
  * list is global to the context, as SolidObstaclesBuffer is global to the
