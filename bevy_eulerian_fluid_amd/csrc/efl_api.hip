@@ -54,6 +54,7 @@ struct efl_ctx {
   std::vector<StepGraph> graphs;
   uint64_t gversion = 1;
   bool graphs_on = true;
+  int bottom_cells = 1024;  // the bottom solve takes the levels of at most this many cells
   double* h_scal = nullptr;  // pinned, 16 doubles per batch slot
   std::vector<efl_instance*> insts;
   efl_body* d_bodies = nullptr;  // EFL_MAX_SOLIDS records
@@ -364,11 +365,11 @@ uint32_t cnt_post(efl_instance* i) { return i->s.projection.mg_post_iterations; 
 uint32_t cnt_coarse(efl_instance* i) { return i->s.projection.mg_coarsest_iterations; }
 uint32_t cnt_gs(efl_instance* i) { return i->s.projection.gs_iterations; }
 
-// First level whose grid holds at most mg_bottom_cells() cells: from there
+// First level whose grid holds at most ctx->bottom_cells cells: from there
 // the rest of the cycle runs in one workgroup.  -1 if none.
 int bottom_level(const efl_instance* i) {
   for (int l = 0; l < i->h.nlev; ++l)
-    if (i->h.lev[l].w * i->h.lev[l].h <= efl::mg_bottom_cells()) return l;
+    if (i->h.lev[l].w * i->h.lev[l].h <= i->ctx->bottom_cells) return l;
   return -1;
 }
 
@@ -620,13 +621,33 @@ int stage_projection(efl_ctx* c, const IV& v) {
   return EFL_OK;
 }
 
-// render_node.rs:540-608: the validity masks, then 10 passes per component
-// (u and v are independent, so all u passes run in one launch and all v
-// passes in another)
+// render_node.rs:540-608: the validity masks, then 10 passes per component.
+// u and v are independent: one launch covers both (Batch::level = the
+// component), two when the instance list fills more than half a batch.
 void stage_extrapolate(hipStream_t st, const IV& v) {
   const int tw = efl::extrapolate_tile_w(), th = efl::extrapolate_tile_h();
-  efl::launch_extrapolate_fused(st, make_batch(v, EX_XMAJOR, 0, nullptr, tw, th), 0);
-  efl::launch_extrapolate_fused(st, make_batch(v, EX_YMAJOR, 0, nullptr, tw, th), 1);
+  Batch bu = make_batch(v, EX_XMAJOR, 0, nullptr, tw, th), bv = make_batch(v, EX_YMAJOR, 0, nullptr, tw, th);
+  for (int n = 0; n < bv.n; ++n) bv.level[n] = 1;
+  if (bu.n + bv.n > efl::kMaxBatch) {
+    efl::launch_extrapolate_fused(st, bu);
+    efl::launch_extrapolate_fused(st, bv);
+    return;
+  }
+  Batch b = bu;  // append the v slots
+  const int base = bu.start[bu.n];
+  for (int n = 0; n < bv.n; ++n) {
+    const int k = bu.n + n;
+    b.start[k] = base + bv.start[n];
+    b.tiles_x[k] = bv.tiles_x[n];
+    b.ex[k] = bv.ex[n];
+    b.ey[k] = bv.ey[n];
+    b.level[k] = 1;
+    b.sel[k] = 0;
+    b.inst[k] = bv.inst[n];
+  }
+  b.n = bu.n + bv.n;
+  b.start[b.n] = base + bv.start[bv.n];
+  efl::launch_extrapolate_fused(st, b);
 }
 
 void stage_fluid_to_solid(hipStream_t st, const IV& v) {  // render_node.rs:610-641
@@ -970,6 +991,7 @@ int efl_ctx_create(int device, efl_ctx** out) {
     return EFL_E_OOM;
   }
   for (auto& e : c->ev) hipEventCreate(&e);
+  c->bottom_cells = efl::bottom_cells_for_device(device);
   *out = c;
   return EFL_OK;
 }

@@ -693,14 +693,8 @@ constexpr int XR_W = XT_W + 2 * XP, XR_H = XT_H + 2 * XP, XR_N = XR_W * XR_H;
 constexpr int XL_W = XR_W + 2, XL_N = XL_W * (XR_H + 2);
 
 template <int COMP>
-__global__ __launch_bounds__(NT) void k_extrapolate_fused(Batch b) {
-  __shared__ float sf[XR_N];
-  __shared__ uint8_t sv[2][XR_N];
-  __shared__ float sl[XL_N];
-  const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  int i = 0;
-  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
-  const int lb = blk - b.start[i];
+__device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, float* sf, uint8_t* sv0, uint8_t* sv1,
+                                                 float* sl) {
   const DevInst* I = b.inst[i];
   const int W = I->W, H = I->H;
   const int w = COMP == 0 ? W + 1 : W, h = COMP == 0 ? H : H + 1;
@@ -720,7 +714,7 @@ __global__ __launch_bounds__(NT) void k_extrapolate_fused(Batch b) {
       const size_t id = (size_t)y * w + x;
       f = F[id];
       v1 = (uint8_t)V1[id];
-      if (x < ex && y < ey) {  // valid_u / valid_v cover the same faces
+      if (x < ex && y < ey) {  // the validity kernels cover the same faces
         const int c = (ly + 1) * XL_W + lx + 1;  // cell (x, y)
         float c0, c1, c2, c3, c4, c5;
         if (COMP == 0) {
@@ -735,38 +729,39 @@ __global__ __launch_bounds__(NT) void k_extrapolate_fused(Batch b) {
       }
     }
     sf[q] = f;
-    sv[0][q] = v0;
-    sv[1][q] = v1;
+    sv0[q] = v0;
+    sv1[q] = v1;
   }
   __syncthreads();
   const int cx1 = min(ex, w) - ox, cy1 = min(ey, h) - oy;  // coverage in local coordinates
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads walk the window
   for (int p = 0; p < XP; ++p) {
-    const uint8_t* vin = sv[p & 1];  // pass 0: valid0 -> valid1, then back
-    uint8_t* vout = sv[1 - (p & 1)];
+    const uint8_t* vin = (p & 1) ? sv1 : sv0;  // pass 0: valid0 -> valid1, then back
+    uint8_t* vout = (p & 1) ? sv0 : sv1;
     const int m = p + 1;
     const int lo_x = max(m, -ox), hi_x = min(XR_W - m, cx1), lo_y = max(m, -oy), hi_y = min(XR_H - m, cy1);
-    for (int q = threadIdx.x; q < XR_N; q += NT) {
-      const int lx = q % XR_W, ly = q / XR_W;
-      if (lx < lo_x || lx >= hi_x || ly < lo_y || ly >= hi_y) continue;
-      if (vin[q] == 1) {
-        vout[q] = 1;
-        continue;
+    for (int ly = lo_y + ty; ly < hi_y; ly += 8)
+      for (int lx = lo_x + tx; lx < hi_x; lx += 32) {
+        const int q = ly * XR_W + lx;
+        if (vin[q] == 1) {
+          vout[q] = 1;
+          continue;
+        }
+        const int x = ox + lx, y = oy + ly;
+        int count = 0;
+        float nu = 0.0f;
+        if (x + 1 < w && y < h && vin[q + 1] == 1) { nu += sf[q + 1]; count += 1; }
+        if (x - 1 > 0 && y > 0 && vin[q - 1] == 1) { nu += sf[q - 1]; count += 1; }
+        if (x < w && y + 1 < h && vin[q + XR_W] == 1) { nu += sf[q + XR_W]; count += 1; }
+        if (x > 0 && y - 1 > 0 && vin[q - XR_W] == 1) { nu += sf[q - XR_W]; count += 1; }
+        if (count > 0) {
+          nu /= (float)count;
+          sf[q] = nu;
+          vout[q] = 1;
+        } else {
+          vout[q] = 0;
+        }
       }
-      const int x = ox + lx, y = oy + ly;
-      int count = 0;
-      float nu = 0.0f;
-      if (x + 1 < w && y < h && vin[q + 1] == 1) { nu += sf[q + 1]; count += 1; }
-      if (x - 1 > 0 && y > 0 && vin[q - 1] == 1) { nu += sf[q - 1]; count += 1; }
-      if (x < w && y + 1 < h && vin[q + XR_W] == 1) { nu += sf[q + XR_W]; count += 1; }
-      if (x > 0 && y - 1 > 0 && vin[q - XR_W] == 1) { nu += sf[q - XR_W]; count += 1; }
-      if (count > 0) {
-        nu /= (float)count;
-        sf[q] = nu;
-        vout[q] = 1;
-      } else {
-        vout[q] = 0;
-      }
-    }
     __syncthreads();
   }
   for (int t = threadIdx.x; t < XT_W * XT_H; t += NT) {
@@ -775,9 +770,23 @@ __global__ __launch_bounds__(NT) void k_extrapolate_fused(Batch b) {
     const size_t id = (size_t)y * w + x;
     const int q = ly * XR_W + lx;
     F[id] = sf[q];
-    V0[id] = sv[0][q];
-    V1[id] = sv[1][q];
+    V0[id] = sv0[q];
+    V1[id] = sv1[q];
   }
+}
+
+// Batch::level of a slot names the component (0 = u, 1 = v): both run in
+// one launch
+__global__ __launch_bounds__(NT) void k_extrapolate_fused(Batch b) {
+  __shared__ float sf[XR_N];
+  __shared__ uint8_t sv[2][XR_N];
+  __shared__ float sl[XL_N];
+  const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  int i = 0;
+  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  const int lb = blk - b.start[i];
+  if (b.level[i] == 0) extrapolate_tile<0>(b, i, lb, sf, sv[0], sv[1], sl);
+  else extrapolate_tile<1>(b, i, lb, sf, sv[0], sv[1], sl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1449,12 +1458,12 @@ void launch_prolong(hipStream_t st, const Batch& b, int pcg) { EFL_LAUNCH(k_prol
 void launch_jacobi(hipStream_t st, const Batch& b, int reverse) { EFL_LAUNCH(k_jacobi, b, st, reverse); }
 void launch_solve_u(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_solve_u, b, st); }
 void launch_solve_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_solve_v, b, st); }
-// valid mask + 10 passes of one component; b tiles the coverage in 64 x 16 tiles
-void launch_extrapolate_fused(hipStream_t st, const Batch& b, int comp) {
+// validity mask + 10 passes; b tiles the coverage in 64 x 16 tiles, level
+// = component of each slot
+void launch_extrapolate_fused(hipStream_t st, const Batch& b) {
   const int nb = b.start[b.n];
   if (nb <= 0) return;
-  if (comp == 0) hipLaunchKernelGGL(k_extrapolate_fused<0>, dim3(nb), dim3(NT), 0, st, b);
-  else hipLaunchKernelGGL(k_extrapolate_fused<1>, dim3(nb), dim3(NT), 0, st, b);
+  hipLaunchKernelGGL(k_extrapolate_fused, dim3(nb), dim3(NT), 0, st, b);
 }
 int extrapolate_tile_w() { return XT_W; }
 int extrapolate_tile_h() { return XT_H; }

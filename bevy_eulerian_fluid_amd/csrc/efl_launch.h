@@ -28,7 +28,7 @@ void launch_prolong(hipStream_t st, const Batch& b, int pcg);
 void launch_jacobi(hipStream_t st, const Batch& b, int reverse);
 void launch_solve_u(hipStream_t st, const Batch& b);
 void launch_solve_v(hipStream_t st, const Batch& b);
-void launch_extrapolate_fused(hipStream_t st, const Batch& b, int comp);
+void launch_extrapolate_fused(hipStream_t st, const Batch& b);
 int extrapolate_tile_w();
 int extrapolate_tile_h();
 int extrapolate_passes();
@@ -60,7 +60,7 @@ namespace efl {
 // fused multigrid (efl_mg.hip)
 int mg_tile_w();
 int mg_tile_h();
-int mg_bottom_cells();
+int bottom_cells_for_device(int device);
 int mg_setup_tile_w();
 int mg_setup_tile_h();
 void launch_mg_setup(hipStream_t st, const Batch& b);

@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "efl_internal.h"
 #include "efl_launch.h"
@@ -48,7 +49,6 @@ namespace efl {
 constexpr int MT_W = 64;   // tile width (one wavefront per tile row)
 constexpr int MT_H = 16;   // tile height
 constexpr int MNT = 256;   // threads per block
-constexpr int BOTTOM_CELLS = 1024;
 static_assert(MNT == RED_NT, "the r . z reduction uses RED_NT threads per block");
 
 // Level::fl bits (efl_internal.h)
@@ -460,31 +460,160 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
 }
 
 // ---------------------------------------------------------------------------
-// Bottom of the V-cycle in one workgroup of 1024 threads (32 x 32): levels
-// l0 .. nlev-1 staged in LDS (stencil, x, b); the level table lives in LDS
-// too.  One workgroup owns the whole level, so x is read from and written
-// back to the buffer Batch::sel names.
+// Bottom of the V-cycle in one workgroup of 1024 threads: levels l0 .. nlev-1
+// staged in LDS (stencil, x, b); the level table lives in LDS too.  Levels of
+// more than BOT_SMALL cells are swept by the whole workgroup (a barrier after
+// every half-sweep); the smaller ones by wavefront 0 alone, which needs only
+// an LDS fence between half-sweeps, while the other wavefronts wait at one
+// barrier.  One workgroup owns the whole level, so x is read from and
+// written back to the buffer Batch::sel names.
 // ---------------------------------------------------------------------------
+constexpr int BOT_NT = 1024;
+constexpr int BOT_SMALL = 256;
+
 struct BotLds {
   float4* c;
-  float *x, *b, *dn;
+  float *x, *b;
   int* fl;
 };
 
+// the Gauss-Seidel denominator from the stencil: the in-level coefficients
+// summed in neighbour order, as k_mg_setup forms it
+__device__ __forceinline__ float b_denominator(float4 c, int fl) {
+  float d = 0.0f;
+  if (fl & 1) d += c.x;
+  if (fl & 2) d += c.y;
+  if (fl & 4) d += c.z;
+  if (fl & 8) d += c.w;
+  return d;
+}
+
+// G threads (the workgroup, or wavefront 0) take part
+template <int G>
+__device__ __forceinline__ void b_sync() {
+  if (G == BOT_NT) {
+    __syncthreads();
+  } else {  // one wavefront: its LDS accesses complete before any later one
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// G threads walk a level as a TX x TY grid (no index divisions)
+template <int G>
+struct BGrid {
+  static constexpr int TX = G == BOT_NT ? 32 : 8, TY = G / TX;
+  static __device__ __forceinline__ int tx() { return (int)threadIdx.x % TX; }
+  static __device__ __forceinline__ int ty() { return (int)threadIdx.x / TX; }
+};
+
+template <int G>
 __device__ __forceinline__ void b_sweep(int w, int h, int off, const BotLds& m, int parity, float c0) {
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int gy = ty; gy < h; gy += 32)
-    for (int gx = tx; gx < w; gx += 32) {
-      if (((gx + gy) & 1) != parity) continue;
+  using T = BGrid<G>;
+  const int hw = (w + 1) / 2;  // cells of one colour in a row, at most
+  for (int gy = T::ty(); gy < h; gy += T::TY)
+    for (int i = T::tx(); i < hw; i += T::TX) {
+      const int gx = 2 * i + ((parity + gy) & 1);
+      if (gx >= w) continue;
       const int q = off + gy * w + gx;
       const int j0 = gx > 0 ? q - 1 : q, j1 = gx + 1 < w ? q + 1 : q;
       const int j2 = gy > 0 ? q - w : q, j3 = gy + 1 < h ? q + w : q;
-      m.x[q] = gs_upd(m.b[q], m.c[q], m.dn[q], m.fl[q], m.x[q], m.x[j0], m.x[j1], m.x[j2], m.x[j3], 1.0f, c0);
+      const float4 c = m.c[q];
+      const int fl = m.fl[q];
+      m.x[q] = gs_upd(m.b[q], c, b_denominator(c, fl), fl, m.x[q], m.x[j0], m.x[j1], m.x[j2], m.x[j3], 1.0f, c0);
     }
-  __syncthreads();
+  b_sync<G>();
 }
 
-__global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric) {
+// residual + restriction of b into the next level (each coarse cell forms
+// the residuals of its four children); the coarse x starts at 0
+template <int G>
+__device__ __forceinline__ void b_restrict(int w, int h, int off, int cw, int ch, int coff, const BotLds& m,
+                                           float factor) {
+  using T = BGrid<G>;
+  for (int cy = T::ty(); cy < ch; cy += T::TY)
+    for (int cx = T::tx(); cx < cw; cx += T::TX) {
+      const int dxs[4] = {0, 0, 1, 1}, dys[4] = {0, 1, 0, 1};  // (0,0) (0,1) (1,0) (1,1)
+      float r4[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int gx = 2 * cx + dxs[k], gy = 2 * cy + dys[k];
+        const bool in = mg_inb(gx, gy, w, h);
+        const int q = in ? off + gy * w + gx : off;
+        const int j0 = gx > 0 ? q - 1 : q, j1 = gx + 1 < w ? q + 1 : q;
+        const int j2 = gy > 0 ? q - w : q, j3 = gy + 1 < h ? q + w : q;
+        const float r = res_upd(m.b[q], m.c[q], m.fl[q], m.x[q], m.x[j0], m.x[j1], m.x[j2], m.x[j3], factor);
+        r4[k] = in ? r : 0.0f;
+      }
+      const int q = coff + cy * cw + cx;
+      m.b[q] = 0.25f * (((r4[0] + r4[1]) + r4[2]) + r4[3]);
+      m.x[q] = 0.0f;
+    }
+  b_sync<G>();
+}
+
+template <int G>
+__device__ __forceinline__ void b_prolong(int w, int h, int off, int cw, int ch, int coff, const BotLds& m) {
+  using T = BGrid<G>;
+  for (int gy = T::ty(); gy < h; gy += T::TY)
+    for (int gx = T::tx(); gx < w; gx += T::TX) {
+      const int cx = gx >> 1, cy = gy >> 1;
+      const int q = off + gy * w + gx;
+      if (cx < cw && cy < ch && (m.fl[q] & F_LSNEG)) m.x[q] = m.x[coff + cy * cw + cx] + m.x[q];
+    }
+  b_sync<G>();
+}
+
+struct BotCfg {
+  const DevInst* I;
+  const int *lw, *lh, *loff;
+  int nl, pre, post, coarse, symmetric;
+};
+
+// the coarsest level: coarse red-black sweeps (and as many black-red ones
+// when the cycle is symmetric)
+template <int G>
+__device__ __forceinline__ void b_coarsest(const BotCfg& g, const BotLds& m) {
+  const int l = g.nl - 1;
+  const float c0 = gs_c0(g.I, (float)(1 << l));
+  for (int k = 0; k < g.coarse; ++k) {
+    b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 1, c0);
+    b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 0, c0);
+  }
+  if (g.symmetric)
+    for (int k = 0; k < g.coarse; ++k) {
+      b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 0, c0);
+      b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 1, c0);
+    }
+}
+// pre-smoothing of level l and restriction into l + 1
+template <int G>
+__device__ __forceinline__ void b_down(const BotCfg& g, const BotLds& m, int l) {
+  const float scale = (float)(1 << l);
+  const float c0 = gs_c0(g.I, scale);
+  for (int k = 0; k < g.pre; ++k) {
+    b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 1, c0);
+    b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 0, c0);
+  }
+  b_restrict<G>(g.lw[l], g.lh[l], g.loff[l], g.lw[l + 1], g.lh[l + 1], g.loff[l + 1], m, resid_factor(g.I, scale));
+}
+// prolongation from l + 1 into l and post-smoothing of level l
+template <int G>
+__device__ __forceinline__ void b_up(const BotCfg& g, const BotLds& m, int l) {
+  b_prolong<G>(g.lw[l], g.lh[l], g.loff[l], g.lw[l + 1], g.lh[l + 1], g.loff[l + 1], m);
+  const float c0 = gs_c0(g.I, (float)(1 << l));
+  for (int k = 0; k < g.post; ++k) {
+    if (g.symmetric) {
+      b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 0, c0);
+      b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 1, c0);
+    } else {
+      b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 1, c0);
+      b_sweep<G>(g.lw[l], g.lh[l], g.loff[l], m, 0, c0);
+    }
+  }
+}
+
+__global__ __launch_bounds__(BOT_NT) void k_bottom(Batch b, int pcg, int symmetric) {
   extern __shared__ float4 smem_f4[];
   __shared__ int lw[kMaxLevels + 1], lh[kMaxLevels + 1], loff[kMaxLevels + 1];
   const int slot = blockIdx.x;
@@ -492,7 +621,6 @@ __global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric
   if (pcg && cg_done(I)) return;
   const int l0 = b.level[slot];
   const int nl = I->nlev;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   if (threadIdx.x == 0) {
     int total = 0;
     for (int l = l0; l < nl; ++l) {
@@ -510,15 +638,13 @@ __global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric
   m.c = smem_f4;
   m.x = (float*)(m.c + total);
   m.b = m.x + total;
-  m.dn = m.b + total;
-  m.fl = (int*)(m.dn + total);
+  m.fl = (int*)(m.b + total);
   const bool xzero = (b.sel[slot] & 4) != 0;
   for (int l = l0; l < nl; ++l) {
     const Level L = mg_level(I, l, pcg);
     const int off = loff[l];
-    for (int t = threadIdx.x; t < L.w * L.h; t += 1024) {
+    for (int t = threadIdx.x; t < L.w * L.h; t += BOT_NT) {
       m.c[off + t] = ldg(L.cf, t);
-      m.dn[off + t] = ldg(L.dn, t);
       m.fl[off + t] = ldg(L.fl, t);
       if (l == l0) {
         m.x[t] = xzero ? 0.0f : ldg((b.sel[slot] & 1) ? L.xs : L.x, t);
@@ -528,75 +654,33 @@ __global__ __launch_bounds__(1024) void k_bottom(Batch b, int pcg, int symmetric
   }
   __syncthreads();
   // sweep counts ride in the batch: ex = pre, ey = post | coarsest << 16
-  const int pre = b.ex[slot], post = b.ey[slot] & 0xffff, coarse = b.ey[slot] >> 16;
-  for (int l = l0; l < nl; ++l) {
-    const int w = lw[l], h = lh[l], off = loff[l];
-    const float scale = (float)(1 << l);
-    const float c0 = gs_c0(I, scale);
-    if (l == nl - 1) {
-      for (int k = 0; k < coarse; ++k) {
-        b_sweep(w, h, off, m, 1, c0);
-        b_sweep(w, h, off, m, 0, c0);
-      }
-      if (symmetric)
-        for (int k = 0; k < coarse; ++k) {
-          b_sweep(w, h, off, m, 0, c0);
-          b_sweep(w, h, off, m, 1, c0);
-        }
-      break;
+  BotCfg g;
+  g.I = I;
+  g.lw = lw;
+  g.lh = lh;
+  g.loff = loff;
+  g.nl = nl;
+  g.pre = b.ex[slot];
+  g.post = b.ey[slot] & 0xffff;
+  g.coarse = b.ey[slot] >> 16;
+  g.symmetric = symmetric;
+  int ls = l0;  // first level small enough for wavefront 0
+  while (ls < nl && lw[ls] * lh[ls] > BOT_SMALL) ++ls;
+  for (int l = l0; l < ls && l < nl - 1; ++l) b_down<BOT_NT>(g, m, l);
+  if (ls >= nl) {
+    b_coarsest<BOT_NT>(g, m);  // the coarsest level is itself large
+  } else {
+    if (threadIdx.x < 64) {
+      for (int l = ls; l < nl - 1; ++l) b_down<64>(g, m, l);
+      b_coarsest<64>(g, m);
+      for (int l = nl - 2; l >= ls; --l) b_up<64>(g, m, l);
     }
-    for (int k = 0; k < pre; ++k) {
-      b_sweep(w, h, off, m, 1, c0);
-      b_sweep(w, h, off, m, 0, c0);
-    }
-    // residual + restriction of b into level l + 1 (each coarse cell forms
-    // the residuals of its four children); the coarse x starts at 0
-    const int cw = lw[l + 1], ch = lh[l + 1], coff = loff[l + 1];
-    const float factor = resid_factor(I, scale);
-    for (int cy = ty; cy < ch; cy += 32)
-      for (int cx = tx; cx < cw; cx += 32) {
-        const int dxs[4] = {0, 0, 1, 1}, dys[4] = {0, 1, 0, 1};  // (0,0) (0,1) (1,0) (1,1)
-        float r4[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int gx = 2 * cx + dxs[k], gy = 2 * cy + dys[k];
-          const bool in = mg_inb(gx, gy, w, h);
-          const int q = in ? off + gy * w + gx : off;
-          const int j0 = gx > 0 ? q - 1 : q, j1 = gx + 1 < w ? q + 1 : q;
-          const int j2 = gy > 0 ? q - w : q, j3 = gy + 1 < h ? q + w : q;
-          const float r = res_upd(m.b[q], m.c[q], m.fl[q], m.x[q], m.x[j0], m.x[j1], m.x[j2], m.x[j3], factor);
-          r4[k] = in ? r : 0.0f;
-        }
-        const int q = coff + cy * cw + cx;
-        m.b[q] = 0.25f * (((r4[0] + r4[1]) + r4[2]) + r4[3]);
-        m.x[q] = 0.0f;
-      }
     __syncthreads();
   }
-  for (int l = nl - 2; l >= l0; --l) {
-    const int w = lw[l], h = lh[l], off = loff[l];
-    const int cw = lw[l + 1], ch = lh[l + 1], coff = loff[l + 1];
-    for (int gy = ty; gy < h; gy += 32)
-      for (int gx = tx; gx < w; gx += 32) {
-        const int cx = gx >> 1, cy = gy >> 1;
-        const int q = off + gy * w + gx;
-        if (cx < cw && cy < ch && (m.fl[q] & F_LSNEG)) m.x[q] = m.x[coff + cy * cw + cx] + m.x[q];
-      }
-    __syncthreads();
-    const float c0 = gs_c0(I, (float)(1 << l));
-    for (int k = 0; k < post; ++k) {
-      if (symmetric) {
-        b_sweep(w, h, off, m, 0, c0);
-        b_sweep(w, h, off, m, 1, c0);
-      } else {
-        b_sweep(w, h, off, m, 1, c0);
-        b_sweep(w, h, off, m, 0, c0);
-      }
-    }
-  }
+  for (int l = min(ls, nl - 1) - 1; l >= l0; --l) b_up<BOT_NT>(g, m, l);
   const Level L = mg_level(I, l0, pcg);
   float* xb = (b.sel[slot] & 1) ? L.xs : L.x;
-  for (int t = threadIdx.x; t < L.w * L.h; t += 1024) stg(xb, t, m.x[t]);
+  for (int t = threadIdx.x; t < L.w * L.h; t += BOT_NT) stg(xb, t, m.x[t]);
 }
 
 // ---------------------------------------------------------------------------
@@ -636,18 +720,36 @@ void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int firs
   }
 }
 
-// per cell: stencil (16) + x, b, denominator (12) + flags (4)
-size_t bottom_lds_bytes(int cells) { return (size_t)cells * 32; }
+// per cell: stencil (16) + x, b (8) + flags (4)
+size_t bottom_lds_bytes(int cells) { return (size_t)cells * 28; }
+
+// The bottom takes every level of at most 1024 cells (a 32 x 32 level and
+// the ones below it).  EFL_BOTTOM_CELLS=4096 also gives it the 64 x 64 level
+// where a workgroup may hold the ~150 KB of LDS that needs; at 1024^2 the two
+// cost the same (one CU sweeps 4096 cells about as fast as two tiled launches
+// with their gaps), so the smaller footprint is the default.
+int bottom_cells_for_device(int device) {
+  const char* e = getenv("EFL_BOTTOM_CELLS");
+  if (!e || atoi(e) != 4096) return 1024;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 0;
+  const size_t need = bottom_lds_bytes(4096 * 4 / 3 + 64) + 1024;
+  if ((size_t)lds >= need &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(k_bottom), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)bottom_lds_bytes(4096 * 4 / 3 + 64)) == hipSuccess)
+    return 4096;
+  (void)hipGetLastError();
+  return 1024;
+}
 
 void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells) {
   if (b.n <= 0) return;
-  hipLaunchKernelGGL(k_bottom, dim3(b.n), dim3(1024), bottom_lds_bytes(max_cells), st, b, pcg, symmetric);
+  hipLaunchKernelGGL(k_bottom, dim3(b.n), dim3(BOT_NT), bottom_lds_bytes(max_cells), st, b, pcg, symmetric);
 }
 
 int mg_tile_w() { return MT_W; }
 int mg_tile_h() { return MT_H; }
 int mg_setup_tile_w() { return SU_W; }
 int mg_setup_tile_h() { return SU_H; }
-int mg_bottom_cells() { return BOTTOM_CELLS; }
 
 }  // namespace efl
