@@ -140,6 +140,91 @@ def gpu_run(args):
                 sweep_ms=sweep_ms, sweep_bytes=alg_bytes, achieved_gbs=achieved, traffic=traffic)
 
 
+def run_config(args):
+    """The other workloads of the benchmark suite, one JSON line each:
+      1  dam break 256^2, Gauss-Seidel 100 sweeps (weight 1.9), no bodies
+      3  64 independent instances, sides log-uniform in [128, 1024], 0-8
+         two-way coupled bodies each, one multigrid V-cycle per step
+      5  4096^2 pool + 200 droplets (an initial level set), 24 bodies, 64
+         local forces resampled every 30 steps, MG-preconditioned CG to
+         1e-5, stepped together with 8 instances of 512^2"""
+    from bevy_eulerian_fluid_amd import native, scenes
+    ctx = native.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
+    insts, forces, desc = [], None, {}
+    if args.config == 1:
+        sc = scenes.dam_break(256)
+        i = ctx.instance(sc["settings"])
+        i.set_sources(sc["init_sources"], True)
+        insts = [i]
+        desc = {"workload": "dam break 256x256, GS 100 sweeps w=1.9, FIM(10)", "grid": [256, 256], "instances": 1}
+    elif args.config == 3:
+        for sc in scenes.batch(64):
+            i = ctx.instance(sc["settings"])
+            i.set_sources(sc["init_sources"], True)
+            i.set_obstacles(sc["obstacles"], scenes.bodies_for(sc["obstacles"], sc["densities"]))
+            insts.append(i)
+        cells = sum(i.W * i.H for i in insts)
+        desc = {"workload": "64 instances, sides log-uniform [128, 1024], 0-8 coupled bodies each, "
+                            "one multigrid V-cycle per step", "instances": 64, "cells": cells}
+    elif args.config == 5:
+        sc = scenes.combined(4096)
+        big = ctx.instance(sc["settings"])
+        big.init_levelset(sc["levelset"])
+        ctx.set_obstacles(sc["obstacles"])
+        ctx.set_bodies(scenes.bodies_for(sc["obstacles"], sc["densities"]))
+        insts = [big]
+        for k in range(8):
+            s2 = scenes.coupled_tank(512, seed=scenes.SEED_BASE + 50 + k, n_each=1,
+                                     proj=scenes.projection("mgpcg", max_cycles=200, rel_tol=1e-5))
+            i = ctx.instance(s2["settings"])
+            i.set_sources(s2["init_sources"], True)
+            i.set_obstacles(s2["obstacles"], scenes.bodies_for(s2["obstacles"], s2["densities"]))
+            insts.append(i)
+        forces = (sc["forces"], sc["rng"])
+        desc = {"workload": "4096x4096 pool + 200 droplets, 24 coupled bodies, 64 local forces, "
+                            "MG-preconditioned CG to 1e-5, + 8 instances of 512x512", "instances": 9,
+                "grid": [4096, 4096]}
+    else:
+        raise SystemExit(f"unknown --config {args.config}")
+    cells = sum(i.W * i.H for i in insts)
+    frc = forces[0] if forces else None
+
+    def step(k):
+        nonlocal frc
+        if forces:
+            if k % 30 == 0 and k > 0:
+                frc = scenes.random_forces(forces[1], 4096, 4096, 64)
+            insts[0].set_forces(frc)
+        ctx.step(insts, DT)
+
+    log(f"[bench] config {args.config}: {len(insts)} instance(s), {cells} cells; init + {args.warmup} warm-up steps")
+    step(0)
+    for k in range(args.warmup):
+        step(k + 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(args.warmup + 1 + k)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    stats = [i.stats() for i in insts]
+    out = {
+        "metric": f"config {args.config} steps/s", "value": round(args.steps / wall, 3), "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": desc, "cell_steps_per_s": round(cells * args.steps / wall, 1),
+        "instance_steps_per_s": round(len(insts) * args.steps / wall, 3),
+        "cycles_last_step": [int(s.cycles) for s in stats[:9]],
+        "roofline": None, "cpu_baseline": None,
+    }
+    for i in insts:
+        i.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(args):
     """The CPU oracle on the same scene, bounded sample (a few steps)."""
     from oracle.oracle import OracleInstance, set_threads
@@ -181,6 +266,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel instead of replaying graphs")
+    ap.add_argument("--config", type=int, default=2,
+                    help="workload: 2 = coupled tank 1024^2 (the headline line); 1, 3, 5 = the other suite configs")
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the roofline kernel (for profiling it alone)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -195,6 +282,9 @@ def main():
         return
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.config != 2:
+        run_config(args)
         return
     g = gpu_run(args)
     cpu = None if args.no_cpu else cpu_baseline(args)

@@ -282,3 +282,54 @@ def batch_instance(rng, proj=None):
 def batch(n=64, seed=SEED_BASE + 3, proj=None):
     """Config 3: n independent instances of uneven size."""
     return [batch_instance(SplitMix64(seed, i), proj=proj) for i in range(n)]
+
+
+def droplet_levelset(rng, width, height, n_drops, r_lo=4.0, r_hi=32.0, pool=0.2):
+    """Air level set of a pool filling the bottom `pool` of the grid plus
+    n_drops circular droplets (radius U[r_lo, r_hi]) above it, in grid cells
+    (row 0 = top, negative = fluid); for efl_init_levelset, which takes
+    initial fluid beyond the 16-source cap.  Returns a list of rows of floats
+    (numpy.float32 array when numpy is importable)."""
+    import numpy as np
+    ys = np.arange(height, dtype=np.float32)[:, None]
+    xs = np.arange(width, dtype=np.float32)[None, :]
+    surface = (1.0 - pool) * height
+    ls = np.broadcast_to(np.float32(surface) - ys, (height, width)).copy()
+    top = max(r_hi + 1.0, surface - r_hi - 2.0)
+    for _ in range(n_drops):
+        r = rng.uniform(r_lo, r_hi)
+        cx = rng.uniform(r + 1.0, width - r - 1.0)
+        cy = rng.uniform(r + 1.0, top)
+        x0, x1 = int(max(0, cx - r - 2)), int(min(width, cx + r + 3))
+        y0, y1 = int(max(0, cy - r - 2)), int(min(height, cy + r + 3))
+        d = np.hypot(xs[:, x0:x1] - np.float32(cx), ys[y0:y1, :] - np.float32(cy)) - np.float32(r)
+        ls[y0:y1, x0:x1] = np.minimum(ls[y0:y1, x0:x1], d.astype(np.float32))
+    return ls
+
+
+def random_forces(rng, width, height, n, f_max=50.0):
+    """n LocalForces: magnitude U[0, f_max] m/s^2 in a uniform direction at
+    uniform positions (pixels, as ForceToFluid takes them)."""
+    out = []
+    for _ in range(n):
+        a = rng.uniform(0.0, 2 * math.pi)
+        m = rng.uniform(0.0, f_max)
+        out.append(force((m * math.cos(a), m * math.sin(a)), (rng.uniform(0, width), rng.uniform(0, height))))
+    return out
+
+
+def combined(size=4096, seed=SEED_BASE + 5, n_drops=200, n_each=6, n_forces=64, proj=None, rel_tol=1e-5):
+    """Config 5: size^2 grid with a pool in the bottom 20% and n_drops
+    droplets above it (an initial level set, not sources), 4 * n_each mixed
+    bodies and n_forces local forces (resampled by the caller every 30
+    steps); MG-preconditioned CG to rel_tol."""
+    rng = SplitMix64(seed, 0)
+    if proj is None:
+        proj = projection("mgpcg", max_cycles=200, rel_tol=rel_tol, check_every=1)
+    s = settings(size, size, proj=proj)
+    ls = droplet_levelset(rng, size, size, n_drops)
+    obs, dens = random_bodies(rng, size, size, {k: n_each for k in ("circle", "rectangle", "capsule", "triangle")},
+                              rho=99.7)
+    frc = random_forces(rng, size, size, n_forces)
+    return dict(settings=s, init_sources=[], sources=[], levelset=ls, obstacles=obs, densities=dens, forces=frc,
+                rng=rng, dt=1.0 / 60.0, seed=seed)
