@@ -29,6 +29,7 @@ PROJECTION_JACOBI = 0
 PROJECTION_GAUSS_SEIDEL = 1
 PROJECTION_MULTIGRID = 2
 PROJECTION_MGPCG = 3
+PROJECTION_LEARNED = 4
 
 # efl_reinit_kind
 REINIT_JUMP_FLOODING = 0
@@ -52,6 +53,7 @@ STAGE_NAMES = [
     "ADVECT_VELOCITY", "APPLY_FORCES", "DIVERGENCE", "PROJECTION", "SOLVE_VELOCITY",
     "EXTRAPOLATE_VELOCITY", "ADVECT_LEVELSET", "SOURCE", "REINITIALIZE", "LEVELSET_GRADIENT",
     "FLUID_TO_SOLID", "GS_SWEEP", "MG_VCYCLE", "MG_RESIDUAL", "MG_SMOOTH", "GS_SWEEP_SPLIT",
+    "NN_PRECONDITION",
 ]
 STAGE = {n: i for i, n in enumerate(STAGE_NAMES)}
 STAGE_COUNT = len(STAGE_NAMES)

@@ -18,6 +18,7 @@
 #include "../../include/efl.h"
 #include "efl_internal.h"
 #include "efl_launch.h"
+#include "efl_nn.h"
 #include "efl_reduce.h"
 
 using efl::Batch;
@@ -90,6 +91,22 @@ struct efl_instance {
   // per multigrid level: 1 while x is known to be zero (not read)
   uint8_t xzero[efl::kMaxLevels + 1] = {};
   uint32_t cg_last = 0;  // CG iterations of the previous solve
+  // learned preconditioner (efl_set_learned): the network and this
+  // instance's activations (padded to multiples of 8)
+  efl_nn* nn = nullptr;
+  efl::nn::Activations nn_act{};
+  uint32_t nn_iters = 0, nn_fallback = 0;  // last learned solve
+  bool nn_in_fallback = false;             // MGPCG is finishing a learned solve
+};
+
+struct efl_nn {
+  efl_ctx* ctx = nullptr;
+  efl::nn::Trainer tr;
+  efl::nn::Packed pk{};
+  float* d_tmp = nullptr;  // two sample planes (rhs, target) for efl_nn_add_sample
+  size_t tmp_floats = 0;
+  double* d_hist = nullptr;
+  uint32_t hist_cap = 0;
 };
 
 namespace {
@@ -179,10 +196,12 @@ int upload(efl_instance* in) {
   }
   const efl_projection& pj = in->s.projection;
   const uint32_t maxc = pj.max_cycles ? pj.max_cycles : 1, every = pj.check_every ? pj.check_every : 1;
-  if (in->h.cg_tol != pj.rel_tol || in->h.cg_maxc != maxc || in->h.cg_every != every) {
+  const uint32_t flex = pj.kind == EFL_PROJECTION_LEARNED && !in->nn_in_fallback ? 1u : 0u;
+  if (in->h.cg_tol != pj.rel_tol || in->h.cg_maxc != maxc || in->h.cg_every != every || in->h.cg_flex != flex) {
     in->h.cg_tol = pj.rel_tol;
     in->h.cg_maxc = maxc;
     in->h.cg_every = every;
+    in->h.cg_flex = flex;
     in->dirty = true;
   }
   if (!in->dirty) return EFL_OK;
@@ -467,6 +486,7 @@ bool mg_needs_host(const efl_instance* i) {  // multigrid repeated to a residual
   return i->s.projection.kind == EFL_PROJECTION_MULTIGRID && i->s.projection.rel_tol > 0.0f;
 }
 bool is_cg(const efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_MGPCG; }
+bool is_learned(const efl_instance* i) { return i->s.projection.kind == EFL_PROJECTION_LEARNED; }
 uint32_t max_cycles(const efl_instance* i) { return i->s.projection.max_cycles ? i->s.projection.max_cycles : 1; }
 
 // The solves that need no host decision: Gauss-Seidel (gauss_seidel.rs:
@@ -602,9 +622,91 @@ int cg_run(efl_ctx* c, const IV& cg, hipGraphExec_t iter) {
   return EFL_OK;
 }
 
+// ---- learned preconditioner ---------------------------------------------------
+void nn_free_activations(efl::nn::Activations& a) {
+  for (int k = 0; k < efl::nn::kTensors; ++k) {
+    if (a.t[k]) (void)hipFree(a.t[k]);
+    if (a.g[k]) (void)hipFree(a.g[k]);
+    a.t[k] = a.g[k] = nullptr;
+  }
+  if (a.scratch) (void)hipFree(a.scratch);
+  a.scratch = nullptr;
+}
+
+void nn_precondition(hipStream_t st, efl_instance* i) {
+  efl::nn::precondition(st, i->nn->pk, i->nn_act, i->d, i->s.width, i->s.height, i->h.scal + efl::SC_DONE);
+}
+
+// Flexible CG preconditioned by the network.  The host reads the residual
+// every few iterations; when it has fallen by less than a factor 0.7 over the
+// last 8 iterations (stall), or max_cycles pass without convergence, the
+// solve continues as MGPCG from the current iterate.
+int proj_learned(efl_ctx* c, efl_instance* i) {
+  if (!i->nn) return fail(c, EFL_E_STATE, "learned projection without a network (efl_set_learned)");
+  hipStream_t st = c->stream;
+  IV v{i};
+  mg_setup(st, v, true);  // stencil and masks (and the fallback's pyramid)
+  Batch full = cg_batch(v);
+  efl::launch_mask_p(st, full);
+  efl::launch_pcg_init(st, full);  // r = b - A p0, ||r||, the initial test
+  nn_precondition(st, i);
+  const uint32_t cap = max_cycles(i);
+  std::vector<std::pair<uint32_t, double>> seen;  // (iteration, ||r|| / ||b||) at each read
+  uint32_t n = std::min(std::max(i->nn_iters, 4u), cap);
+  bool converged = false, stalled = false;
+  uint32_t k = 0;
+  double rel = 0.0;
+  for (;;) {
+    for (uint32_t j = 0; j < n; ++j) {
+      efl::launch_pcg_apply(st, full);
+      efl::launch_pcg_axpy(st, full);
+      efl::launch_cg_check(st, full);
+      nn_precondition(st, i);
+    }
+    int rc = read_scalars(c, v);
+    if (rc) return rc;
+    const double* sc = c->h_scal;
+    k = (uint32_t)sc[efl::SC_ITER];
+    rel = rel_of(sc);
+    const bool done = sc[efl::SC_DONE] != 0.0;
+    converged = done && sc[efl::SC_CONV] != 0.0 && sc[efl::SC_REL] <= (double)i->s.projection.rel_tol;
+    if (done) break;
+    for (auto it = seen.rbegin(); it != seen.rend(); ++it)
+      if (k >= it->first + 8) {
+        stalled = rel > 0.7 * it->second;
+        break;
+      }
+    if (stalled) break;
+    seen.emplace_back(k, rel);
+    n = std::min(4u, cap > k ? cap - k : 1u);
+  }
+  i->nn_iters = k;
+  i->nn_fallback = 0;
+  i->stats.cycles = k;
+  i->stats.converged = converged;
+  i->stats.rel_residual = (float)rel;
+  if (converged) return EFL_OK;
+  // fallback: MGPCG from the current iterate
+  i->nn_in_fallback = true;
+  int rc = upload(i);
+  if (rc) return rc;
+  efl::launch_pcg_init(st, full);
+  precondition(st, v);
+  const uint32_t keep = i->cg_last;
+  rc = cg_run(c, v, nullptr);
+  i->nn_in_fallback = false;
+  if (rc) return rc;
+  rc = upload(i);
+  if (rc) return rc;
+  i->nn_fallback = i->stats.cycles;
+  i->cg_last = keep;
+  i->stats.cycles += k;
+  return EFL_OK;
+}
+
 int stage_projection(efl_ctx* c, const IV& v) {
   hipStream_t st = c->stream;
-  IV fixed = filter(v, [](efl_instance* i) { return !is_cg(i) && !mg_needs_host(i); });
+  IV fixed = filter(v, [](efl_instance* i) { return !is_cg(i) && !mg_needs_host(i) && !is_learned(i); });
   IV mgt = filter(v, [](efl_instance* i) { return mg_needs_host(i); });
   IV cg = filter(v, [](efl_instance* i) { return is_cg(i); });
   proj_fixed_stats(v);
@@ -618,6 +720,11 @@ int stage_projection(efl_ctx* c, const IV& v) {
     int rc = cg_run(c, cg, nullptr);
     if (rc) return rc;
   }
+  for (auto* i : v)
+    if (is_learned(i)) {
+      int rc = proj_learned(c, i);
+      if (rc) return rc;
+    }
   return EFL_OK;
 }
 
@@ -850,7 +957,7 @@ int run_step_graph(efl_ctx* c, const IV& v, float dt, bool* used) {
   *used = false;
   if (!c->graphs_on || c->timing) return EFL_OK;
   for (auto* i : v)
-    if (i->diag || mg_needs_host(i)) return EFL_OK;
+    if (i->diag || mg_needs_host(i) || is_learned(i)) return EFL_OK;
   uint32_t db = 0;
   std::memcpy(&db, &dt, 4);
   StepGraph* g = nullptr;
@@ -957,7 +1064,7 @@ FieldDesc field_desc(efl_instance* in, int f) {
 }
 
 bool valid_projection(const efl_projection& p) {
-  return p.kind >= EFL_PROJECTION_JACOBI && p.kind <= EFL_PROJECTION_MGPCG && p.rel_tol >= 0.0f;
+  return p.kind >= EFL_PROJECTION_JACOBI && p.kind <= EFL_PROJECTION_LEARNED && p.rel_tol >= 0.0f;
 }
 bool valid_reinit(const efl_reinit& r) {
   return r.kind == EFL_REINIT_JUMP_FLOODING || r.kind == EFL_REINIT_FAST_ITERATIVE;
@@ -1154,7 +1261,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
     o_lcf[l] = take(m * 16); o_ldn[l] = take(m * 4); o_lfl[l] = take(m * 4);
   }
   size_t o_pr = take(n * 4), o_pz = take(n * 4), o_pp = take(n * 4), o_pp1 = take(n * 4), o_pq = take(n * 4);
-  size_t o_red = take((size_t)3 * nblk * 8), o_scal = take(16 * 8), o_cnt = take(16), o_diag = take(16);
+  size_t o_red = take((size_t)4 * nblk * 8), o_scal = take(16 * 8), o_cnt = take(16), o_diag = take(16);
   size_t o_src0 = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES), o_src = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES);
   size_t o_desc = take(sizeof(DevInst));
   in->bytes = off;
@@ -1237,6 +1344,7 @@ int efl_instance_destroy(efl_instance* in) {
   if (in->d_obs_own) hipFree(in->d_obs_own);
   if (in->d_bodies_own) hipFree(in->d_bodies_own);
   if (in->mem) hipFree(in->mem);
+  nn_free_activations(in->nn_act);
   c->insts.erase(std::remove(c->insts.begin(), c->insts.end(), in), c->insts.end());
   drop_graphs(c);  // their launches name this instance
   c->gversion++;
@@ -1393,6 +1501,10 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
   IV v{in};
   Batch center = make_batch(v, EX_CENTER), bx = make_batch(v, EX_XMAJOR), by = make_batch(v, EX_YMAJOR);
   switch (stage) {
+    case EFL_STAGE_NN_PRECONDITION:
+      if (!in->nn || !is_learned(in)) return fail(c, EFL_E_STATE, "NN_PRECONDITION needs efl_set_learned and a learned projection");
+      nn_precondition(st, in);
+      break;
     case EFL_STAGE_INIT_GRID_CENTER: efl::launch_init_center(st, center); break;
     case EFL_STAGE_INIT_GRID_EDGE: efl::launch_init_edge(st, bx); break;
     case EFL_STAGE_SOURCE_ONESHOT: efl::launch_fluid_source(st, center, 1); break;
@@ -1562,6 +1674,261 @@ int efl_measure(efl_instance* in, float dt, float* rel, float* mdiv) {
 int efl_get_stats(efl_instance* in, efl_stats* out) {
   if (!in || !out) return EFL_E_INVALID;
   *out = in->stats;
+  return EFL_OK;
+}
+
+// ---- learned preconditioner -------------------------------------------------
+
+size_t efl_nn_param_count(void) { return efl::nn::param_count(); }
+double efl_nn_flops_per_pixel(void) { return efl::nn::flops_per_pixel(); }
+
+int efl_nn_create(efl_ctx* c, efl_nn** out) {
+  if (!c || !out) return EFL_E_INVALID;
+  *out = nullptr;
+  HIPCHK(c, hipSetDevice(c->device));
+  efl_nn* nn = new efl_nn();
+  nn->ctx = c;
+  const size_t np = efl::nn::param_count();
+  efl::nn::Trainer& t = nn->tr;
+  bool ok = hipMalloc(&t.params, np * 4) == hipSuccess && hipMalloc(&t.grads, np * 4) == hipSuccess &&
+            hipMalloc(&t.m, np * 4) == hipSuccess && hipMalloc(&t.v, np * 4) == hipSuccess &&
+            hipMalloc(&t.loss, sizeof(double)) == hipSuccess;
+  for (int k = 0; ok && k < efl::nn::kLayers; ++k) {
+    const efl::nn::LayerDef& L = efl::nn::kNet[k];
+    const size_t nw = 9 * (size_t)L.cout * (L.ca + L.cb);
+    ok = hipMalloc(&nn->pk.wf[k], nw * 2) == hipSuccess && hipMalloc(&nn->pk.wb[k], nw * 2) == hipSuccess;
+  }
+  if (!ok) {
+    efl_nn_destroy(nn);
+    return fail(c, EFL_E_OOM, "network parameters");
+  }
+  HIPCHK(c, hipMemsetAsync(t.params, 0, np * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(t.grads, 0, np * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(t.m, 0, np * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(t.v, 0, np * 4, c->stream));
+  efl::nn::pack(c->stream, t.params, nn->pk);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = nn;
+  return EFL_OK;
+}
+
+int efl_nn_destroy(efl_nn* nn) {
+  if (!nn) return EFL_E_INVALID;
+  efl_ctx* c = nn->ctx;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto* i : c->insts)
+    if (i->nn == nn) {
+      i->nn = nullptr;
+      nn_free_activations(i->nn_act);
+    }
+  efl::nn::Trainer& t = nn->tr;
+  for (void* p : {(void*)t.params, (void*)t.grads, (void*)t.m, (void*)t.v, (void*)t.loss, (void*)t.data,
+                  (void*)t.partial, (void*)t.idx, (void*)t.target, (void*)t.mask, (void*)nn->d_tmp,
+                  (void*)nn->d_hist})
+    if (p) (void)hipFree(p);
+  nn_free_activations(t.act);
+  for (int k = 0; k < efl::nn::kLayers; ++k) {
+    if (nn->pk.wf[k]) (void)hipFree(nn->pk.wf[k]);
+    if (nn->pk.wb[k]) (void)hipFree(nn->pk.wb[k]);
+  }
+  delete nn;
+  return EFL_OK;
+}
+
+int efl_nn_set_params(efl_nn* nn, const float* params, size_t n) {
+  if (!nn || !params) return EFL_E_INVALID;
+  efl_ctx* c = nn->ctx;
+  if (n != efl::nn::param_count()) return fail(c, EFL_E_INVALID, "%zu parameters, expected %zu", n, efl::nn::param_count());
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(nn->tr.params, params, n * 4, hipMemcpyHostToDevice, c->stream));
+  efl::nn::pack(c->stream, nn->tr.params, nn->pk);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_nn_get_params(efl_nn* nn, float* params, size_t n) {
+  if (!nn || !params) return EFL_E_INVALID;
+  efl_ctx* c = nn->ctx;
+  if (n != efl::nn::param_count()) return fail(c, EFL_E_INVALID, "%zu parameters, expected %zu", n, efl::nn::param_count());
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(params, nn->tr.params, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_nn_init_params(efl_nn* nn, uint64_t seed) {
+  if (!nn) return EFL_E_INVALID;
+  std::vector<float> h(efl::nn::param_count());
+  efl::nn::init_params(h.data(), seed);
+  nn->tr.adam_t = 0;
+  efl_ctx* c = nn->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemsetAsync(nn->tr.m, 0, h.size() * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(nn->tr.v, 0, h.size() * 4, c->stream));
+  return efl_nn_set_params(nn, h.data(), h.size());
+}
+
+int efl_nn_dataset_reserve(efl_nn* nn, uint32_t w, uint32_t h, uint32_t capacity) {
+  if (!nn) return EFL_E_INVALID;
+  efl_ctx* c = nn->ctx;
+  if (w == 0 || h == 0 || w % 8 || h % 8 || capacity == 0)
+    return fail(c, EFL_E_INVALID, "dataset samples must be multiples of 8 on a side");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  efl::nn::Trainer& t = nn->tr;
+  if (t.data) (void)hipFree(t.data);
+  t.data = nullptr;
+  t.count = t.capacity = 0;
+  t.act_n = 0;  // batch buffers follow the sample size
+  nn_free_activations(t.act);
+  t.w = (int)w;
+  t.h = (int)h;
+  if (hipMalloc(&t.data, (size_t)capacity * 5 * w * h * 4) != hipSuccess) return fail(c, EFL_E_OOM, "dataset");
+  t.capacity = capacity;
+  if (nn->tmp_floats < 2ull * w * h) {
+    if (nn->d_tmp) (void)hipFree(nn->d_tmp);
+    if (hipMalloc(&nn->d_tmp, 2ull * w * h * 4) != hipSuccess) return fail(c, EFL_E_OOM, "dataset staging");
+    nn->tmp_floats = 2ull * w * h;
+  }
+  return EFL_OK;
+}
+
+int efl_nn_add_sample(efl_nn* nn, efl_instance* in, const float* rhs, const float* target) {
+  if (!nn || !in || !rhs || !target) return EFL_E_INVALID;
+  efl_ctx* c = nn->ctx;
+  efl::nn::Trainer& t = nn->tr;
+  if (in->ctx != c) return fail(c, EFL_E_INVALID, "instance of another context");
+  if ((int)in->s.width != t.w || (int)in->s.height != t.h)
+    return fail(c, EFL_E_INVALID, "sample %ux%u, dataset %dx%d", in->s.width, in->s.height, t.w, t.h);
+  if (t.count >= t.capacity) return fail(c, EFL_E_RANGE, "dataset full (%u samples)", t.capacity);
+  const size_t n = (size_t)t.w * t.h;
+  double ss = 0.0;
+  for (size_t k = 0; k < n; ++k) ss += (double)rhs[k] * rhs[k];
+  const float sigma = ss > 0.0 ? (float)std::sqrt(ss / (double)n) : 1.0f;
+  const float factor = in->h.dt / in->h.rho / in->h.dx / in->h.dx;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(nn->d_tmp, rhs, n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(nn->d_tmp + n, target, n * 4, hipMemcpyHostToDevice, c->stream));
+  efl::nn::make_sample(c->stream, t, t.count, in->d, t.w, t.h, nn->d_tmp, nn->d_tmp + n, sigma, factor);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  t.count += 1;
+  return EFL_OK;
+}
+
+int efl_nn_sample_count(const efl_nn* nn, uint32_t* count) {
+  if (!nn || !count) return EFL_E_INVALID;
+  *count = nn->tr.count;
+  return EFL_OK;
+}
+
+int efl_nn_get_sample(const efl_nn* nn, uint32_t index, float* planes) {
+  if (!nn || !planes) return EFL_E_INVALID;
+  efl_ctx* c = nn->ctx;
+  const efl::nn::Trainer& t = nn->tr;
+  if (index >= t.count) return fail(c, EFL_E_RANGE, "sample %u of %u", index, t.count);
+  const size_t n = 5ull * t.w * t.h;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(planes, t.data + (size_t)index * n, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_nn_train(efl_nn* nn, uint32_t iterations, uint32_t batch, float lr, uint64_t seed, float* loss_history) {
+  if (!nn) return EFL_E_INVALID;
+  efl_ctx* c = nn->ctx;
+  efl::nn::Trainer& t = nn->tr;
+  if (t.count == 0 || batch == 0) return fail(c, EFL_E_STATE, "no training samples");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (efl::nn::trainer_reserve_batch(t, (int)batch)) return fail(c, EFL_E_OOM, "training batch");
+  if (nn->hist_cap < iterations) {
+    if (nn->d_hist) (void)hipFree(nn->d_hist);
+    if (hipMalloc(&nn->d_hist, (size_t)iterations * 8) != hipSuccess) return fail(c, EFL_E_OOM, "loss history");
+    nn->hist_cap = iterations;
+  }
+  uint64_t s = seed;
+  std::vector<int> idx((size_t)iterations * batch);  // every batch's samples, SplitMix64
+  for (int& x : idx) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x = (int)(z % t.count);
+  }
+  for (uint32_t it = 0; it < iterations; ++it) {
+    efl::nn::eval_batch(c->stream, t, nn->pk, idx.data() + (size_t)it * batch, (int)batch, true);
+    HIPCHK(c, hipMemcpyAsync(nn->d_hist + it, t.loss, 8, hipMemcpyDeviceToDevice, c->stream));
+    efl::nn::adam_step(c->stream, t, lr);
+    efl::nn::pack(c->stream, t.params, nn->pk);
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (loss_history) {
+    std::vector<double> hist(iterations);
+    HIPCHK(c, hipMemcpy(hist.data(), nn->d_hist, (size_t)iterations * 8, hipMemcpyDeviceToHost));
+    for (uint32_t it = 0; it < iterations; ++it) loss_history[it] = (float)hist[it];
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, EFL_E_HIP, "training: %s", hipGetErrorString(e));
+  return EFL_OK;
+}
+
+int efl_nn_eval(efl_nn* nn, uint32_t first, uint32_t count, float* loss, float* grads, float* out) {
+  if (!nn || count == 0) return EFL_E_INVALID;
+  efl_ctx* c = nn->ctx;
+  efl::nn::Trainer& t = nn->tr;
+  if ((uint64_t)first + count > t.count) return fail(c, EFL_E_RANGE, "samples [%u, %u) of %u", first, first + count, t.count);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (efl::nn::trainer_reserve_batch(t, (int)count)) return fail(c, EFL_E_OOM, "evaluation batch");
+  std::vector<int> idx(count);
+  for (uint32_t k = 0; k < count; ++k) idx[k] = (int)(first + k);
+  efl::nn::eval_batch(c->stream, t, nn->pk, idx.data(), (int)count, grads != nullptr);
+  double l = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&l, t.loss, 8, hipMemcpyDeviceToHost, c->stream));
+  if (grads)
+    HIPCHK(c, hipMemcpyAsync(grads, t.grads, efl::nn::param_count() * 4, hipMemcpyDeviceToHost, c->stream));
+  if (out) {
+    const size_t npx = (size_t)t.w * t.h * count;
+    float* d_out = nullptr;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (hipMalloc(&d_out, npx * 4) != hipSuccess) return fail(c, EFL_E_OOM, "outputs");
+    efl::nn::read_output(c->stream, t.act, (int)count, d_out);
+    hipError_t e = hipMemcpyAsync(out, d_out, npx * 4, hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(c, EFL_E_HIP, "outputs: %s", hipGetErrorString(e));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (loss) *loss = (float)l;
+  return EFL_OK;
+}
+
+int efl_set_learned(efl_instance* in, efl_nn* nn) {
+  if (!in) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  if (nn && nn->ctx != c) return fail(c, EFL_E_INVALID, "network of another context");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  nn_free_activations(in->nn_act);
+  in->nn = nn;
+  in->nn_iters = in->nn_fallback = 0;
+  if (nn) {
+    efl::nn::Activations& a = in->nn_act;
+    a.s = efl::nn::Shapes{1, (int)((in->s.width + 7) / 8 * 8), (int)((in->s.height + 7) / 8 * 8)};
+    for (int k = 0; k < efl::nn::kTensors; ++k)
+      if (hipMalloc(&a.t[k], a.s.elems(k) * 2) != hipSuccess) {
+        nn_free_activations(a);
+        in->nn = nullptr;
+        return fail(c, EFL_E_OOM, "network activations");
+      }
+  }
+  c->gversion++;
+  return EFL_OK;
+}
+
+int efl_nn_last_solve(efl_instance* in, uint32_t out[2]) {
+  if (!in || !out) return EFL_E_INVALID;
+  out[0] = in->nn_iters;
+  out[1] = in->nn_fallback;
   return EFL_OK;
 }
 

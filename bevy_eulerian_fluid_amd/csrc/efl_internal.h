@@ -70,13 +70,14 @@ struct DevInst {
   // PCG vectors (W x H) and scalars
   float *pcg_r, *pcg_z, *pcg_q;
   float* pcg_p[2];  // search direction of iteration k in pcg_p[k & 1]
-  double* red;         // reduction partials: 3 x red_stride doubles
+  double* red;         // reduction partials: 4 x red_stride doubles
   int red_stride;      // >= blocks of one full-grid launch
   double* scal;        // scalars, slots SC_* (efl_reduce.h)
   uint32_t* counter;   // last-block counter (efl_reduce.h)
   // CG stopping rule, evaluated on the device (projection settings)
   float cg_tol;
   uint32_t cg_maxc, cg_every;
+  uint32_t cg_flex;    // flexible beta (learned, nonlinear preconditioner)
   float* diag;         // [0] max |D| (as float bits via atomic max), ...
 };
 

@@ -1242,9 +1242,17 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
   const DevInst* I = s.I;
   if (cg_done(I)) return;
   const int iter = (int)I->scal[SC_ITER] + 1;
-  double rz[1];
-  fold_partials<1>(I, RED_RZ, (int)I->scal[SC_NRZ], false, rz);
-  const double beta = iter == 1 ? 0.0 : rz[0] / I->scal[SC_RZ0 + ((iter - 2) & 1)];
+  double rz[2] = {0.0, 0.0};
+  double beta = 0.0;
+  if (I->cg_flex) {  // flexible CG: beta = -(z . q_prev) / (p . q)_prev
+    fold_partials<2>(I, RED_RZ, (int)I->scal[SC_NRZ], false, rz);
+    if (iter > 1 && I->scal[SC_PQ] != 0.0) beta = -rz[1] / I->scal[SC_PQ];
+  } else {
+    double t[1];
+    fold_partials<1>(I, RED_RZ, (int)I->scal[SC_NRZ], false, t);
+    rz[0] = t[0];
+    if (iter > 1) beta = rz[0] / I->scal[SC_RZ0 + ((iter - 2) & 1)];
+  }
   if (s.lb == 0 && threadIdx.x == 0) {
     I->scal[SC_RZ0 + ((iter - 1) & 1)] = rz[0];
     I->scal[SC_BETA] = beta;

@@ -21,6 +21,7 @@ constexpr int RED_NT = 256;  // threads of every block that takes part
 
 // partial components (red[])
 constexpr int RED_PQ = 0, RED_RR = 1, RED_BB = 2, RED_RZ = 2;  // bb is folded before any r . z exists
+constexpr int RED_ZQ = 3;  // z . q_prev of the flexible (learned) preconditioner
 
 // scal[] slots
 constexpr int SC_PQ = 1, SC_RR = 2, SC_BB = 3, SC_ALPHA = 4, SC_BETA = 5, SC_DONE = 6;
@@ -29,6 +30,7 @@ constexpr int SC_NRR = 8;            // number of ||r||^2 partials of the last k
 constexpr int SC_ITER = 9, SC_REL = 10, SC_CONV = 11;
 constexpr int SC_RZ0 = 12;           // r . z of iteration k at SC_RZ0 + (k & 1)
 constexpr int SC_KCUR = 14;          // the iteration k_pcg_axpy just ran
+constexpr int SC_STALL = 15;         // flexible CG: relative residual at the last stall checkpoint
 
 // Fold the nblk partials of components c0 .. c0 + N - 1 (all threads of a
 // block of RED_NT call it; every thread gets the sums): thread t sums

@@ -23,6 +23,10 @@ EXPORTS = [
     "efl_set_field", "efl_field_device_ptr", "efl_get_solid_forces", "efl_set_diagnostics",
     "efl_measure", "efl_get_stats", "efl_time_stage", "efl_set_bodies", "efl_get_obstacles",
     "efl_set_instance_obstacles", "efl_get_instance_obstacles",
+    "efl_nn_param_count", "efl_nn_flops_per_pixel", "efl_nn_create", "efl_nn_destroy", "efl_nn_init_params",
+    "efl_nn_set_params", "efl_nn_get_params", "efl_nn_dataset_reserve", "efl_nn_add_sample",
+    "efl_nn_sample_count", "efl_nn_get_sample", "efl_nn_train", "efl_nn_eval", "efl_set_learned",
+    "efl_nn_last_solve",
 ]
 
 
@@ -84,6 +88,19 @@ def load():
         "efl_get_obstacles": [P, P, C.c_uint32],
         "efl_set_instance_obstacles": [P, P, P, C.c_uint32],
         "efl_get_instance_obstacles": [P, P, C.c_uint32],
+        "efl_nn_create": [P, PP],
+        "efl_nn_destroy": [P],
+        "efl_nn_init_params": [P, C.c_uint64],
+        "efl_nn_set_params": [P, P, C.c_size_t],
+        "efl_nn_get_params": [P, P, C.c_size_t],
+        "efl_nn_dataset_reserve": [P, C.c_uint32, C.c_uint32, C.c_uint32],
+        "efl_nn_add_sample": [P, P, P, P],
+        "efl_nn_sample_count": [P, C.POINTER(C.c_uint32)],
+        "efl_nn_get_sample": [P, C.c_uint32, P],
+        "efl_nn_train": [P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, P],
+        "efl_nn_eval": [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), P, P],
+        "efl_set_learned": [P, P],
+        "efl_nn_last_solve": [P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -91,6 +108,10 @@ def load():
         fn.restype = C.c_int
     L.efl_last_error.argtypes = [P]
     L.efl_last_error.restype = C.c_char_p
+    L.efl_nn_param_count.argtypes = []
+    L.efl_nn_param_count.restype = C.c_size_t
+    L.efl_nn_flops_per_pixel.argtypes = []
+    L.efl_nn_flops_per_pixel.restype = C.c_double
     _lib = L
     return L
 
@@ -298,7 +319,92 @@ class Instance:
         self._chk(self.L.efl_measure(self.h, C.c_float(dt), C.byref(r), C.byref(d)), "efl_measure")
         return r.value, d.value
 
+    def set_learned(self, net):
+        self._chk(self.L.efl_set_learned(self.h, net.h if net is not None else None), "efl_set_learned")
+
+    def learned_iterations(self):
+        a = (C.c_uint32 * 2)()
+        self._chk(self.L.efl_nn_last_solve(self.h, a), "efl_nn_last_solve")
+        return int(a[0]), int(a[1])
+
     def stats(self):
         s = abi.efl_stats()
         self._chk(self.L.efl_get_stats(self.h, C.byref(s)), "efl_get_stats")
         return s
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Network:
+    """efl_nn: the learned preconditioner's parameters, training set and
+    trainer on one context's device."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.L = load()
+        h = C.c_void_p()
+        ctx._chk(self.L.efl_nn_create(ctx.h, C.byref(h)), "efl_nn_create")
+        self.h = h
+        self.w = self.h_ = 0
+
+    @staticmethod
+    def param_count():
+        return int(load().efl_nn_param_count())
+
+    @staticmethod
+    def flops_per_pixel():
+        return float(load().efl_nn_flops_per_pixel())
+
+    def close(self):
+        if self.h:
+            self.L.efl_nn_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc, what):
+        self.ctx._chk(rc, what)
+
+    def init_params(self, seed):
+        self._chk(self.L.efl_nn_init_params(self.h, seed), "efl_nn_init_params")
+
+    def set_params(self, p):
+        a = _f32(p)
+        self._chk(self.L.efl_nn_set_params(self.h, a.ctypes.data, a.size), "efl_nn_set_params")
+
+    def get_params(self):
+        a = np.zeros(self.param_count(), dtype=np.float32)
+        self._chk(self.L.efl_nn_get_params(self.h, a.ctypes.data, a.size), "efl_nn_get_params")
+        return a
+
+    def reserve(self, w, h, capacity):
+        self._chk(self.L.efl_nn_dataset_reserve(self.h, w, h, capacity), "efl_nn_dataset_reserve")
+        self.w, self.h_ = w, h
+
+    def add_sample(self, inst, rhs, target):
+        r, t = _f32(rhs), _f32(target)
+        self._chk(self.L.efl_nn_add_sample(self.h, inst.h, r.ctypes.data, t.ctypes.data), "efl_nn_add_sample")
+
+    def count(self):
+        n = C.c_uint32()
+        self._chk(self.L.efl_nn_sample_count(self.h, C.byref(n)), "efl_nn_sample_count")
+        return n.value
+
+    def sample(self, index):
+        a = np.zeros((5, self.h_, self.w), dtype=np.float32)
+        self._chk(self.L.efl_nn_get_sample(self.h, index, a.ctypes.data), "efl_nn_get_sample")
+        return a
+
+    def train(self, iterations, batch, lr, seed):
+        hist = np.zeros(iterations, dtype=np.float32)
+        self._chk(self.L.efl_nn_train(self.h, iterations, batch, lr, seed, hist.ctypes.data), "efl_nn_train")
+        return hist
+
+    def eval(self, first, count, grads=False, outputs=False):
+        loss = C.c_float()
+        g = np.zeros(self.param_count(), dtype=np.float32) if grads else None
+        o = np.zeros((count, self.h_, self.w), dtype=np.float32) if outputs else None
+        self._chk(self.L.efl_nn_eval(self.h, first, count, C.byref(loss),
+                                     g.ctypes.data if grads else None, o.ctypes.data if outputs else None),
+                  "efl_nn_eval")
+        return loss.value, g, o

@@ -38,7 +38,8 @@ class SplitMix64:
 def projection(kind="gauss_seidel", gs_iterations=20, pre=2, post=2, coarsest=3,
                max_cycles=1, rel_tol=0.0, check_every=1):
     kinds = {"jacobi": abi.PROJECTION_JACOBI, "gauss_seidel": abi.PROJECTION_GAUSS_SEIDEL,
-             "multigrid": abi.PROJECTION_MULTIGRID, "mgpcg": abi.PROJECTION_MGPCG}
+             "multigrid": abi.PROJECTION_MULTIGRID, "mgpcg": abi.PROJECTION_MGPCG,
+             "learned": abi.PROJECTION_LEARNED}
     p = abi.efl_projection()
     p.kind = kinds[kind]
     p.gs_iterations = gs_iterations

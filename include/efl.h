@@ -60,7 +60,9 @@ typedef enum {
   EFL_PROJECTION_JACOBI = 0,        /* 50 x (jacobi, jacobi_reverse), projection.rs:135-143 */
   EFL_PROJECTION_GAUSS_SEIDEL = 1,  /* SOR red-black GS, weight 1.9, gauss_seidel.rs:60-148 */
   EFL_PROJECTION_MULTIGRID = 2,     /* V-cycle(s) with RBGS smoother, multi_grid.rs:390-475 */
-  EFL_PROJECTION_MGPCG = 3          /* no counterpart: CG preconditioned by one symmetric V-cycle */
+  EFL_PROJECTION_MGPCG = 3,         /* no counterpart: CG preconditioned by one symmetric V-cycle */
+  EFL_PROJECTION_LEARNED = 4        /* no counterpart: flexible CG preconditioned by the learned
+                                       network (efl_set_learned), MGPCG as the fallback */
 } efl_projection_kind;
 
 /* ReinitializeMethod (reinitialize_levelset.rs:36-46). */
@@ -225,6 +227,8 @@ typedef enum {
   EFL_STAGE_MG_RESIDUAL,          /* finest-level residual into EFL_FIELD_MG_R0 */
   EFL_STAGE_MG_SMOOTH,            /* two red-black sweeps at weight 1 on level 0 (pre-smoothing) */
   EFL_STAGE_GS_SWEEP_SPLIT,       /* GS_SWEEP dispatched one colour per kernel */
+  EFL_STAGE_NN_PRECONDITION,      /* one application of the learned preconditioner to the CG
+                                     residual (needs efl_set_learned and a learned projection) */
   EFL_STAGE_COUNT
 } efl_stage;
 
@@ -343,6 +347,54 @@ int efl_set_diagnostics(efl_instance* inst, int enabled);
 /* Measure ||r||/||b|| and dt*max|D| of the current fields (forces a sync). */
 int efl_measure(efl_instance* inst, float dt, float* rel_residual, float* max_norm_divergence);
 int efl_get_stats(efl_instance* inst, efl_stats* out);
+
+/* ---- learned preconditioner (no counterpart in the reference) -----------
+ *
+ * A 4-level convolutional U-Net (3x3 convolutions, 16/32/64/64 channels,
+ * BF16 weights and activations, FP32 accumulation on MFMA) maps the CG
+ * residual r (divided by sigma = rms(r)), the solid fraction
+ * 1 - mean(area fractions) and clamp(phi_air / 8, -1, 1) to an
+ * approximation of A^-1 r.  EFL_PROJECTION_LEARNED runs flexible CG with it
+ * (beta = -(z . q_prev) / (p . q)_prev); when the relative residual stalls
+ * (less than a factor 0.7 over 8 iterations) or max_cycles pass, the solve
+ * continues as MGPCG from the current iterate.  Training runs on the device
+ * with the library's own forward and backward kernels and Adam, on samples
+ * (rhs, solution) that the plain solver produces.  Parameters are FP32
+ * master weights, layer by layer: W[cout][cin][3][3] then bias[cout]. */
+typedef struct efl_nn efl_nn;
+size_t efl_nn_param_count(void);
+double efl_nn_flops_per_pixel(void);   /* forward FLOPs per grid cell */
+int efl_nn_create(efl_ctx* ctx, efl_nn** out);
+int efl_nn_destroy(efl_nn* nn);
+/* He-normal initialisation from a seed (SplitMix64). */
+int efl_nn_init_params(efl_nn* nn, uint64_t seed);
+int efl_nn_set_params(efl_nn* nn, const float* params, size_t n);
+int efl_nn_get_params(efl_nn* nn, float* params, size_t n);
+/* Training set of `capacity` samples of width x height (multiples of 8). */
+int efl_nn_dataset_reserve(efl_nn* nn, uint32_t width, uint32_t height, uint32_t capacity);
+/* Append a sample: rhs and target are W x H host arrays (the right-hand side
+ * of a pressure solve and its solution, or a residual and the error it
+ * leaves); the features come from the instance's current area fractions,
+ * level set and pressure-unknown mask (set by its last projection), whose
+ * size must match the dataset's. */
+int efl_nn_add_sample(efl_nn* nn, efl_instance* inst, const float* rhs, const float* target);
+int efl_nn_sample_count(const efl_nn* nn, uint32_t* count);
+/* Copy sample `index` to the host: 5 planes of W x H floats (r / sigma,
+ * solid fraction, clamped level set, mask, target * factor / sigma with
+ * factor = dt / (rho dx^2)). */
+int efl_nn_get_sample(const efl_nn* nn, uint32_t index, float* planes);
+/* iterations Adam steps on random batches of `batch` samples (seeded);
+ * loss_history (may be NULL) receives each step's loss. */
+int efl_nn_train(efl_nn* nn, uint32_t iterations, uint32_t batch, float lr, uint64_t seed, float* loss_history);
+/* Loss of samples [first, first + count) without updating; grads (may be
+ * NULL, efl_nn_param_count() floats) receives d loss / d params, out (may
+ * be NULL, count * W * H floats) the network outputs. */
+int efl_nn_eval(efl_nn* nn, uint32_t first, uint32_t count, float* loss, float* grads, float* out);
+/* Use nn for this instance's EFL_PROJECTION_LEARNED solves (NULL detaches). */
+int efl_set_learned(efl_instance* inst, efl_nn* nn);
+/* Iterations of the last learned solve: [0] flexible-CG iterations with the
+ * network, [1] MGPCG iterations of the fallback (0 if it was not needed). */
+int efl_nn_last_solve(efl_instance* inst, uint32_t out[2]);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "efl.h")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(efl_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*|size_t|double)\s+(efl_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_expected_entry_points():

@@ -1,0 +1,162 @@
+"""The learned preconditioner on the GPU against its CPU oracle
+(oracle/nn_oracle.py), and inside the projection.
+
+- forward: the MFMA U-Net's outputs against the oracle's BF16-rounded forward
+  on the same dataset samples and parameters (FP32 accumulation order differs
+  from the oracle's exact sums, which can move a BF16 rounding: relative L2
+  error <= 2e-2);
+- backward: the loss and every layer's parameter gradients against the
+  oracle's float64 backward through the same rounded forward (the GPU keeps
+  gradients in BF16: relative L2 error per layer <= 6e-2, cosine >= 0.998);
+- training lowers the loss on solver-generated samples;
+- EFL_PROJECTION_LEARNED converges to the tolerance (through the MGPCG
+  fallback when the network does not help) and agrees with MGPCG.
+"""
+import numpy as np
+import pytest
+
+from bevy_eulerian_fluid_amd import native, scenes
+from oracle import nn_oracle as O
+from tests.parity_util import small_scene
+
+pytestmark = pytest.mark.gpu
+DT = 1.0 / 60.0
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _instance(ctx, size, proj, bodies=True, steps=3):
+    sc = small_scene(size, size, proj=proj, bodies=bodies)
+    inst = ctx.instance(sc["settings"])
+    inst.set_sources(sc["init_sources"], True)
+    inst.set_sources(sc["sources"], False)
+    ctx.set_obstacles(sc["obstacles"])
+    for _ in range(steps):
+        inst.step(DT)
+    return inst
+
+
+@pytest.fixture(scope="module")
+def setup():
+    ctx = native.Context(0)
+    inst = _instance(ctx, 64, scenes.projection("mgpcg", max_cycles=60, rel_tol=1e-6))
+    net = native.Network(ctx)
+    net.init_params(11)
+    net.reserve(64, 64, 8)
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        net.add_sample(inst, rng.standard_normal(64 * 64), rng.standard_normal(64 * 64))
+    yield ctx, inst, net
+    net.close()
+    inst.close()
+    ctx.close()
+
+
+def test_sample_features(setup):
+    _say("nn: dataset sample features")
+    ctx, inst, net = setup
+    s = net.sample(0)
+    afs = inst.get("AREA_FRACTION").reshape(64, 64, 4)
+    solid = np.float32(1.0) - np.float32(0.25) * (((afs[..., 0] + afs[..., 1]) + afs[..., 2]) + afs[..., 3])
+    assert np.array_equal(s[1], solid)
+    ls = inst.get("LS_AIR0").reshape(64, 64)
+    assert np.array_equal(s[2], np.clip(ls * np.float32(0.125), -1, 1))
+    mask = (ls < 0) & ~(afs == 0).all(axis=-1)
+    assert np.array_equal(s[3] != 0, mask)
+    assert abs(np.sqrt(np.mean(s[0].astype(np.float64) ** 2)) - 1.0) < 1e-5
+
+
+def test_forward_matches_oracle(setup):
+    _say("nn: forward 2 x 64x64 vs oracle")
+    ctx, inst, net = setup
+    loss, _, out = net.eval(0, 2, outputs=True)
+    planes = np.stack([net.sample(i) for i in range(2)]).astype(np.float64)
+    t, _ = O.forward(net.get_params(), O.input_tensor(planes))
+    ref = t[14][..., 0]
+    err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert err <= 2e-2, err
+    d = np.where(planes[:, 3] != 0, ref - planes[:, 4], 0.0)
+    assert abs(loss - (d * d).mean()) <= 2e-2 * (d * d).mean()
+
+
+def test_gradients_match_oracle(setup):
+    _say("nn: parameter gradients 2 x 64x64 vs oracle")
+    ctx, inst, net = setup
+    loss, grads, _ = net.eval(0, 2, grads=True)
+    planes = np.stack([net.sample(i) for i in range(2)]).astype(np.float64)
+    loss_o, grads_o, _ = O.loss_and_grads(net.get_params(), planes)
+    assert abs(loss - loss_o) <= 2e-2 * loss_o
+    off = 0
+    for k, (ca, cb, co, *_r) in enumerate(O.NET):
+        n = co * (ca + cb) * 9 + co
+        g, r = grads[off:off + n], grads_o[off:off + n]
+        if np.linalg.norm(r) > 0:
+            err = np.linalg.norm(g - r) / np.linalg.norm(r)
+            assert err <= 6e-2, (k, err)
+        off += n
+    cos = grads @ grads_o / (np.linalg.norm(grads) * np.linalg.norm(grads_o))
+    assert cos >= 0.998, cos
+
+
+def test_training_lowers_loss():
+    _say("nn: 150 Adam steps on 64x64 solver samples")
+    ctx = native.Context(0)
+    try:
+        inst = _instance(ctx, 64, scenes.projection("mgpcg", max_cycles=80, rel_tol=1e-7), steps=1)
+        net = native.Network(ctx)
+        net.init_params(12)
+        net.reserve(64, 64, 12)
+        for _ in range(12):
+            inst.step(DT)
+            net.add_sample(inst, inst.get("DIV"), inst.get("P0"))
+        hist = net.train(150, 4, 2e-3, 99)
+        assert np.isfinite(hist).all()
+        first, last = hist[:10].mean(), hist[-10:].mean()
+        assert last < 0.5 * first, (first, last)
+        net.close()
+        inst.close()
+    finally:
+        ctx.close()
+
+
+def test_learned_projection_converges_and_matches_mgpcg():
+    """With an untrained network the flexible solve stalls and MGPCG takes
+    over from its iterate; the result must meet the tolerance and agree with
+    a plain MGPCG solve of the same state."""
+    _say("nn: learned projection (untrained network, fallback) vs MGPCG, 128x128")
+    ctx = native.Context(0)
+    try:
+        sc = small_scene(128, 128, proj=scenes.projection("mgpcg", max_cycles=100, rel_tol=1e-5))
+        a = ctx.instance(sc["settings"])
+        b = ctx.instance(sc["settings"])
+        for i in (a, b):
+            i.set_sources(sc["init_sources"], True)
+            i.set_sources(sc["sources"], False)
+        ctx.set_obstacles(sc["obstacles"])
+        for _ in range(3):
+            a.step(DT)
+            b.step(DT)
+        net = native.Network(ctx)
+        net.init_params(13)
+        a.set_learned(net)
+        a.set_projection(scenes.projection("learned", max_cycles=100, rel_tol=1e-5))
+        for k in range(3):
+            a.step(DT)
+            b.step(DT)
+            st = a.stats()
+            assert st.converged == 1 and st.rel_residual <= 1e-5, (k, st.rel_residual)
+            nn_it, fb = a.learned_iterations()
+            _say(f"  step {k}: {nn_it} flexible-CG iterations, {fb} MGPCG fallback iterations")
+            assert nn_it >= 1 and st.cycles == nn_it + fb
+            pa, pb = a.get("P0"), b.get("P0")
+            assert np.abs(pa - pb).max() <= 2e-3 * np.abs(pb).max(), k
+            # bring both back to the same state for the next step
+            for f in ("U0", "V0", "P0", "LS_AIR0"):
+                a.set(f, b.get(f))
+        net.close()
+        a.close()
+        b.close()
+    finally:
+        ctx.close()
