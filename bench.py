@@ -149,6 +149,8 @@ def run_config(args):
          local forces resampled every 30 steps, MG-preconditioned CG to
          1e-5, stepped together with 8 instances of 512^2"""
     from bevy_eulerian_fluid_amd import native, scenes
+    if args.config == 4:
+        return run_learned(args)
     ctx = native.Context(0)
     if args.no_graphs:
         ctx.set_graphs(False)
@@ -225,6 +227,71 @@ def run_config(args):
     print(json.dumps(out), flush=True)
 
 
+def run_learned(args):
+    """Config 4: the learned preconditioner (shipped weights, trained at
+    512^2) at args.size^2 on the coupled tank: steps/s with
+    EFL_PROJECTION_LEARNED (flexible CG with the network, MGPCG fallback)
+    beside MGPCG's, the per-step solver comparison on identical systems, and
+    one network application timed with hipEvents against the dense BF16 MFMA
+    peak."""
+    from bevy_eulerian_fluid_amd import learned, native, scenes
+    w = learned.load_weights()
+    if w is None:
+        raise SystemExit("no shipped weights: bevy_eulerian_fluid_amd/data/learned_preconditioner.f32")
+    size = args.size
+    ctx = native.Context(0)
+    log(f"[bench] config 4: learned vs MGPCG on identical systems, {size}^2")
+    cmp = learned.evaluate(ctx, w, size=size, steps=10)
+    rates = {}
+    for kind in ("learned", "mgpcg"):
+        sc = scenes.coupled_tank(size, proj=scenes.projection(kind, max_cycles=200, rel_tol=args.rel_tol))
+        i = ctx.instance(sc["settings"])
+        i.set_sources(sc["init_sources"], True)
+        i.set_obstacles(sc["obstacles"], scenes.bodies_for(sc["obstacles"], sc["densities"]))
+        net = None
+        if kind == "learned":
+            net = native.Network(ctx)
+            net.set_params(w)
+            i.set_learned(net)
+        for _ in range(6):
+            i.step(DT)
+        ctx.sync()
+        n = 20
+        t0 = time.perf_counter()
+        for _ in range(n):
+            i.step(DT)
+        ctx.sync()
+        rates[kind] = n / (time.perf_counter() - t0)
+        if kind == "learned":
+            reps = max(20, args.kernel_reps // 4)
+            ms = i.time_stage("NN_PRECONDITION", DT, reps)
+            wp, hp = (size + 7) // 8 * 8, (size + 7) // 8 * 8
+            flops = native.Network.flops_per_pixel() * wp * hp
+            tflops = flops / (ms * 1e-3) / 1e12
+            nn_timing = dict(ms=round(ms, 4), gflop=round(flops / 1e9, 2), tflops=round(tflops, 2))
+            net.close()
+        i.close()
+    ctx.close()
+    peak = 2500.0
+    out = {
+        "metric": "config 4 steps/s (learned preconditioner)", "value": round(rates["learned"], 3), "unit": "steps/s",
+        "n_gpus": 1, "steps": 20, "warmup": 6, "ms_per_step": round(1e3 / rates["learned"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 network, f32 solve",
+        "data": "synthetic", "config": {"workload": f"coupled tank {size}x{size}, 16 two-way coupled bodies, "
+                                                    f"pressure to ||r||/||b||<={args.rel_tol}, learned U-Net "
+                                                    "preconditioner (4 levels, 272k params, trained at 512x512)",
+                                        "grid": [size, size], "projection": "learned"},
+        "mgpcg_steps_per_s": round(rates["mgpcg"], 3),
+        "solver": {k: v for k, v in cmp.items() if k != "rows"},
+        "network_application": nn_timing,
+        "roofline": {"bound": "mfma", "kernel": "U-Net forward (15 k_conv3x3 + 3 k_pool2 + features/output)",
+                     "achieved": nn_timing["tflops"], "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(nn_timing["tflops"] / peak, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(args):
     """The CPU oracle on the same scene, bounded sample (a few steps)."""
     from oracle.oracle import OracleInstance, set_threads
@@ -267,7 +334,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel instead of replaying graphs")
     ap.add_argument("--config", type=int, default=2,
-                    help="workload: 2 = coupled tank 1024^2 (the headline line); 1, 3, 5 = the other suite configs")
+                    help="workload: 2 = coupled tank 1024^2 (the headline line); 1, 3, 4 (learned), 5 = the other suite configs")
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the roofline kernel (for profiling it alone)")
     ap.add_argument("--copy-probe", action="store_true",

@@ -97,6 +97,7 @@ struct efl_instance {
   efl::nn::Activations nn_act{};
   uint32_t nn_iters = 0, nn_fallback = 0;  // last learned solve
   bool nn_in_fallback = false;             // MGPCG is finishing a learned solve
+  float nn_rel = 0.0f;                     // ||r|| / ||b|| when the network's part ended
 };
 
 struct efl_nn {
@@ -633,8 +634,10 @@ void nn_free_activations(efl::nn::Activations& a) {
   a.scratch = nullptr;
 }
 
-void nn_precondition(hipStream_t st, efl_instance* i) {
-  efl::nn::precondition(st, i->nn->pk, i->nn_act, i->d, i->s.width, i->s.height, i->h.scal + efl::SC_DONE);
+// forced = true runs the network even when the solve has stopped (timing)
+void nn_precondition(hipStream_t st, efl_instance* i, bool forced = false) {
+  efl::nn::precondition(st, i->nn->pk, i->nn_act, i->d, i->s.width, i->s.height,
+                        forced ? nullptr : i->h.scal + efl::SC_DONE);
 }
 
 // Flexible CG preconditioned by the network.  The host reads the residual
@@ -652,7 +655,8 @@ int proj_learned(efl_ctx* c, efl_instance* i) {
   nn_precondition(st, i);
   const uint32_t cap = max_cycles(i);
   std::vector<std::pair<uint32_t, double>> seen;  // (iteration, ||r|| / ||b||) at each read
-  uint32_t n = std::min(std::max(i->nn_iters, 4u), cap);
+  // queue what the last solve needed when it converged without help, else 4
+  uint32_t n = std::min(std::max(i->nn_fallback == 0 ? i->nn_iters : 0u, 4u), cap);
   bool converged = false, stalled = false;
   uint32_t k = 0;
   double rel = 0.0;
@@ -671,7 +675,8 @@ int proj_learned(efl_ctx* c, efl_instance* i) {
     const bool done = sc[efl::SC_DONE] != 0.0;
     converged = done && sc[efl::SC_CONV] != 0.0 && sc[efl::SC_REL] <= (double)i->s.projection.rel_tol;
     if (done) break;
-    for (auto it = seen.rbegin(); it != seen.rend(); ++it)
+    stalled = !std::isfinite(rel);
+    for (auto it = seen.rbegin(); !stalled && it != seen.rend(); ++it)
       if (k >= it->first + 8) {
         stalled = rel > 0.7 * it->second;
         break;
@@ -682,14 +687,16 @@ int proj_learned(efl_ctx* c, efl_instance* i) {
   }
   i->nn_iters = k;
   i->nn_fallback = 0;
+  i->nn_rel = (float)rel;
   i->stats.cycles = k;
   i->stats.converged = converged;
   i->stats.rel_residual = (float)rel;
   if (converged) return EFL_OK;
-  // fallback: MGPCG from the current iterate
+  // fallback: MGPCG from the current iterate (from zero if it is not finite)
   i->nn_in_fallback = true;
   int rc = upload(i);
   if (rc) return rc;
+  if (!std::isfinite(rel)) HIPCHK(c, hipMemsetAsync(i->h.p0, 0, (size_t)i->s.width * i->s.height * 4, st));
   efl::launch_pcg_init(st, full);
   precondition(st, v);
   const uint32_t keep = i->cg_last;
@@ -1503,7 +1510,7 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
   switch (stage) {
     case EFL_STAGE_NN_PRECONDITION:
       if (!in->nn || !is_learned(in)) return fail(c, EFL_E_STATE, "NN_PRECONDITION needs efl_set_learned and a learned projection");
-      nn_precondition(st, in);
+      nn_precondition(st, in, true);
       break;
     case EFL_STAGE_INIT_GRID_CENTER: efl::launch_init_center(st, center); break;
     case EFL_STAGE_INIT_GRID_EDGE: efl::launch_init_edge(st, bx); break;
@@ -1803,14 +1810,19 @@ int efl_nn_add_sample(efl_nn* nn, efl_instance* in, const float* rhs, const floa
     return fail(c, EFL_E_INVALID, "sample %ux%u, dataset %dx%d", in->s.width, in->s.height, t.w, t.h);
   if (t.count >= t.capacity) return fail(c, EFL_E_RANGE, "dataset full (%u samples)", t.capacity);
   const size_t n = (size_t)t.w * t.h;
-  double ss = 0.0;
-  for (size_t k = 0; k < n; ++k) ss += (double)rhs[k] * rhs[k];
+  double ss = 0.0, tt = 0.0;
+  for (size_t k = 0; k < n; ++k) {
+    if (!std::isfinite(rhs[k]) || !std::isfinite(target[k]))
+      return fail(c, EFL_E_INVALID, "non-finite sample value at cell %zu", k);
+    ss += (double)rhs[k] * rhs[k];
+    tt += (double)target[k] * target[k];
+  }
   const float sigma = ss > 0.0 ? (float)std::sqrt(ss / (double)n) : 1.0f;
-  const float factor = in->h.dt / in->h.rho / in->h.dx / in->h.dx;
+  const float trms = tt > 0.0 ? (float)std::sqrt(tt / (double)n) : 1.0f;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(nn->d_tmp, rhs, n * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(nn->d_tmp + n, target, n * 4, hipMemcpyHostToDevice, c->stream));
-  efl::nn::make_sample(c->stream, t, t.count, in->d, t.w, t.h, nn->d_tmp, nn->d_tmp + n, sigma, factor);
+  efl::nn::make_sample(c->stream, t, t.count, in->d, t.w, t.h, nn->d_tmp, nn->d_tmp + n, sigma, trms);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   t.count += 1;
   return EFL_OK;
@@ -1925,10 +1937,11 @@ int efl_set_learned(efl_instance* in, efl_nn* nn) {
   return EFL_OK;
 }
 
-int efl_nn_last_solve(efl_instance* in, uint32_t out[2]) {
+int efl_nn_last_solve(efl_instance* in, uint32_t out[2], float* learned_rel) {
   if (!in || !out) return EFL_E_INVALID;
   out[0] = in->nn_iters;
   out[1] = in->nn_fallback;
+  if (learned_rel) *learned_rel = in->nn_rel;
   return EFL_OK;
 }
 

@@ -76,7 +76,7 @@ void precondition(hipStream_t st, const Packed& p, Activations& a, const DevInst
 struct Trainer {
   int w = 0, h = 0;        // sample size
   uint32_t capacity = 0, count = 0;
-  float* data = nullptr;   // per sample: 5 planes (r, solid, phi, mask, target) fp32
+  float* data = nullptr;   // per sample: 5 planes (r, solid, phi, mask, target), fp32
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
   float* partial = nullptr;  // weight-gradient partials
   size_t partial_floats = 0;
@@ -98,9 +98,9 @@ void pack(hipStream_t st, const float* params, Packed& p);
 void eval_batch(hipStream_t st, Trainer& t, const Packed& p, const int* host_idx, int n, bool grads);
 void adam_step(hipStream_t st, Trainer& t, float lr);
 // Build one dataset sample on the device from an instance's fields and two
-// host planes (already uploaded to dev_r / dev_t).
+// device planes: r / sigma and target / trms (sigma, trms: their rms).
 void make_sample(hipStream_t st, Trainer& t, uint32_t slot, const DevInst* I, int W, int H,
-                 const float* dev_r, const float* dev_t, float sigma, float factor);
+                 const float* dev_r, const float* dev_t, float sigma, float trms);
 void read_output(hipStream_t st, const Activations& a, int n, float* dev_out);
 void init_params(float* host, uint64_t seed);
 

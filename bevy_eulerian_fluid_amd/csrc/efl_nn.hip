@@ -108,7 +108,10 @@ struct ConvArgs {
   const double* stop; // CG done flag: skip the launch when set
 };
 
-template <int CA, int CB, int COUT, bool UPA, bool RELU>
+// One wavefront: MBW x 16 consecutive pixels of row y and every output
+// channel (each input fragment feeds COUT / 16 MFMAs); MBW = 1 on the small
+// levels so that they still launch enough workgroups.
+template <int CA, int CB, int COUT, bool UPA, bool RELU, int MBW>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs g) {
   constexpr int CIN = CA + CB, NB = COUT / 16;
   if (stopped(g.stop)) return;
@@ -116,14 +119,14 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs g) {
   const int lr = lane & 15, lq = lane >> 4;
   const int n = blockIdx.z;
   const int y = blockIdx.y * 4 + wave;
-  const int x0 = blockIdx.x * 64;
+  const int x0 = blockIdx.x * (16 * MBW);
   if (y >= g.H) return;  // no barriers below: a whole wavefront leaves
   const uint16_t* A = g.a + n * g.sa;
   const uint16_t* B = CB ? g.b + n * g.sb : nullptr;
   const int Wa = UPA ? g.W >> 1 : g.W;
-  f32x4 acc[4][NB];
+  f32x4 acc[MBW][NB];
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb)
+  for (int mb = 0; mb < MBW; ++mb)
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int tap = 0; tap < 9; ++tap) {
@@ -138,7 +141,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs g) {
       for (int nb = 0; nb < NB; ++nb)
         bw[nb] = *(const bf16x4*)(g.w + ((size_t)(tap * COUT + nb * 16 + lr) * CIN + c0 + 4 * lq));
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb) {
+      for (int mb = 0; mb < MBW; ++mb) {
         const int xx = x0 + mb * 16 + lr + dx;
         bf16x4 av = {0, 0, 0, 0};
         if (rowok && xx >= 0 && xx < g.W) {
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs g) {
     const int co = nb * 16 + lr;
     const float bv = g.bias ? g.bias[co] : 0.0f;
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+    for (int mb = 0; mb < MBW; ++mb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int x = x0 + mb * 16 + 4 * lq + i;
@@ -411,9 +414,11 @@ __global__ __launch_bounds__(256) void k_adam(float* p, const float* g, float* m
 // ---------------------------------------------------------------- data
 
 // A dataset sample: planes r / sigma, solid fraction, clamped level set,
-// mask (pressure unknown), target * factor / sigma.
+// mask (pressure unknown), target / rms(target).  Flexible CG fixes the
+// length of every step itself, so only the shape of A^-1 r matters and each
+// target is scaled to unit rms.
 __global__ __launch_bounds__(256) void k_make_sample(float* dst, const DevInst* I, int W, int H, const float* r,
-                                                     const float* t, float inv_sigma, float factor) {
+                                                     const float* t, float inv_sigma, float inv_trms) {
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   const long n = (long)W * H;
   if (p >= n) return;
@@ -425,7 +430,7 @@ __global__ __launch_bounds__(256) void k_make_sample(float* dst, const DevInst* 
   dst[n + p] = 1.0f - 0.25f * (a.x + a.y + a.z + a.w);
   dst[2 * n + p] = fminf(fmaxf(I->ls0[p] * kLsScale, -1.0f), 1.0f);
   dst[3 * n + p] = act ? 1.0f : 0.0f;
-  dst[4 * n + p] = t[p] * factor * inv_sigma;
+  dst[4 * n + p] = t[p] * inv_trms;
 }
 
 // batch input tensor, target and mask from dataset samples idx[n]
@@ -484,8 +489,8 @@ __device__ __forceinline__ bool active_cell(const DevInst* I, size_t id) {
 
 // input tensor (Wp x Hp, zero outside W x H) from the CG residual:
 // sigma = sqrt(||r||^2 / (W H)) from the stopping test of this iteration
-__global__ __launch_bounds__(256) void k_features(const DevInst* I, uint16_t* x, int Wp, int Hp) {
-  if (cg_done(I)) return;
+__global__ __launch_bounds__(256) void k_features(const DevInst* I, uint16_t* x, int Wp, int Hp, const double* stop) {
+  if (stopped(stop)) return;
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   if (p >= (long)Wp * Hp) return;
   const int xp = (int)(p % Wp), yp = (int)(p / Wp);
@@ -508,9 +513,9 @@ __global__ __launch_bounds__(256) void k_features(const DevInst* I, uint16_t* x,
 
 // z = sigma / factor * out (unknowns only), partials of r . z (RED_RZ) and
 // z . q_prev (RED_ZQ, for the flexible beta) in blocks of 1024 cells
-__global__ __launch_bounds__(256) void k_output(const DevInst* I, const uint16_t* o, int Wp) {
+__global__ __launch_bounds__(256) void k_output(const DevInst* I, const uint16_t* o, int Wp, const double* stop) {
   __shared__ double sh[2][4];
-  if (cg_done(I)) return;
+  if (stopped(stop)) return;
   const double rr = I->scal[SC_RR];
   const float sigma = rr > 0.0 ? (float)sqrt(rr / ((double)I->W * I->H)) : 1.0f;
   const float factor = I->dt / I->rho / I->dx / I->dx;
@@ -549,10 +554,17 @@ namespace {
 
 inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
+// 64-pixel strips while that gives >= 256 workgroups (one per CU), else
+// 16-pixel strips (measured at 1024^2: 16-pixel strips cost 1.7x on the
+// 256^2 level and save 2x on the 128^2 level)
 template <int CA, int CB, int COUT, bool UPA, bool RELU>
 void conv_t(hipStream_t st, const ConvArgs& g, int n) {
-  dim3 grid((g.W + 63) / 64, (g.H + 3) / 4, n);
-  hipLaunchKernelGGL((k_conv3x3<CA, CB, COUT, UPA, RELU>), grid, dim3(256), 0, st, g);
+  const unsigned rows = (unsigned)(g.H + 3) / 4;
+  if ((unsigned)((g.W + 63) / 64) * rows * (unsigned)n >= 256) {
+    hipLaunchKernelGGL((k_conv3x3<CA, CB, COUT, UPA, RELU, 4>), dim3((g.W + 63) / 64, rows, n), dim3(256), 0, st, g);
+  } else {
+    hipLaunchKernelGGL((k_conv3x3<CA, CB, COUT, UPA, RELU, 1>), dim3((g.W + 15) / 16, rows, n), dim3(256), 0, st, g);
+  }
 }
 
 // every (ca, cb, cout, upa, relu) the network and its input gradient use
@@ -666,9 +678,11 @@ void forward(hipStream_t st, const Packed& p, Activations& a) { forward_impl(st,
 
 void precondition(hipStream_t st, const Packed& p, Activations& a, const DevInst* I, int W, int H,
                   const double* done_flag) {
-  hipLaunchKernelGGL(k_features, dim3(blocks((long)a.s.w * a.s.h)), dim3(256), 0, st, I, a.t[kTX], a.s.w, a.s.h);
+  hipLaunchKernelGGL(k_features, dim3(blocks((long)a.s.w * a.s.h)), dim3(256), 0, st, I, a.t[kTX], a.s.w, a.s.h,
+                     done_flag);
   forward_impl(st, p, a, done_flag);
-  hipLaunchKernelGGL(k_output, dim3((unsigned)(((long)W * H + 1023) / 1024)), dim3(256), 0, st, I, a.t[14], a.s.w);
+  hipLaunchKernelGGL(k_output, dim3((unsigned)(((long)W * H + 1023) / 1024)), dim3(256), 0, st, I, a.t[14], a.s.w,
+                     done_flag);
 }
 
 void pack(hipStream_t st, const float* params, Packed& p) {
@@ -786,10 +800,10 @@ void adam_step(hipStream_t st, Trainer& t, float lr) {
 }
 
 void make_sample(hipStream_t st, Trainer& t, uint32_t slot, const DevInst* I, int W, int H, const float* dev_r,
-                 const float* dev_t, float sigma, float factor) {
+                 const float* dev_t, float sigma, float trms) {
   float* dst = t.data + (size_t)slot * 5 * W * H;
   hipLaunchKernelGGL(k_make_sample, dim3(blocks((long)W * H)), dim3(256), 0, st, dst, I, W, H, dev_r, dev_t,
-                     1.0f / sigma, factor);
+                     1.0f / sigma, 1.0f / trms);
 }
 
 void read_output(hipStream_t st, const Activations& a, int n, float* dev_out) {

@@ -100,7 +100,7 @@ def load():
         "efl_nn_train": [P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, P],
         "efl_nn_eval": [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), P, P],
         "efl_set_learned": [P, P],
-        "efl_nn_last_solve": [P, P],
+        "efl_nn_last_solve": [P, P, C.POINTER(C.c_float)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -322,10 +322,11 @@ class Instance:
     def set_learned(self, net):
         self._chk(self.L.efl_set_learned(self.h, net.h if net is not None else None), "efl_set_learned")
 
-    def learned_iterations(self):
+    def learned_iterations(self, with_rel=False):
         a = (C.c_uint32 * 2)()
-        self._chk(self.L.efl_nn_last_solve(self.h, a), "efl_nn_last_solve")
-        return int(a[0]), int(a[1])
+        rel = C.c_float()
+        self._chk(self.L.efl_nn_last_solve(self.h, a, C.byref(rel)), "efl_nn_last_solve")
+        return (int(a[0]), int(a[1]), rel.value) if with_rel else (int(a[0]), int(a[1]))
 
     def stats(self):
         s = abi.efl_stats()

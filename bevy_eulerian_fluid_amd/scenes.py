@@ -252,14 +252,16 @@ def coupled_tank(size=1024, seed=SEED_BASE + 2, proj=None, rel_tol=1e-5, n_each=
                 dt=1.0 / 60.0, seed=seed)
 
 
-def batch_instance(rng, proj=None):
+def batch_instance(rng, proj=None, size=None):
     """One instance of config 3: side lengths log-uniform in [128, 1024]
-    rounded to a multiple of 32, random gravity direction, 0-8 bodies and a
-    fluid AABB covering 30-70% of the area."""
+    rounded to a multiple of 32 (or size x size), random gravity direction,
+    0-8 bodies and a fluid AABB covering 30-70% of the area."""
     def side():
         v = math.exp(rng.uniform(math.log(128.0), math.log(1024.0)))
         return int(max(128, min(1024, round(v / 32.0) * 32)))
     W, H = side(), side()
+    if size is not None:
+        W = H = size
     ang = rng.uniform(0.0, 2 * math.pi)
     g = (9.8 * math.cos(ang), 9.8 * math.sin(ang))
     frac = rng.uniform(0.3, 0.7)

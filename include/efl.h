@@ -379,9 +379,9 @@ int efl_nn_dataset_reserve(efl_nn* nn, uint32_t width, uint32_t height, uint32_t
  * size must match the dataset's. */
 int efl_nn_add_sample(efl_nn* nn, efl_instance* inst, const float* rhs, const float* target);
 int efl_nn_sample_count(const efl_nn* nn, uint32_t* count);
-/* Copy sample `index` to the host: 5 planes of W x H floats (r / sigma,
- * solid fraction, clamped level set, mask, target * factor / sigma with
- * factor = dt / (rho dx^2)). */
+/* Copy sample `index` to the host: 5 planes of W x H floats (rhs / rms(rhs),
+ * solid fraction, clamped level set, mask, target / rms(target): flexible CG
+ * fixes every step length itself, so only the shape of A^-1 r is learned). */
 int efl_nn_get_sample(const efl_nn* nn, uint32_t index, float* planes);
 /* iterations Adam steps on random batches of `batch` samples (seeded);
  * loss_history (may be NULL) receives each step's loss. */
@@ -393,8 +393,9 @@ int efl_nn_eval(efl_nn* nn, uint32_t first, uint32_t count, float* loss, float* 
 /* Use nn for this instance's EFL_PROJECTION_LEARNED solves (NULL detaches). */
 int efl_set_learned(efl_instance* inst, efl_nn* nn);
 /* Iterations of the last learned solve: [0] flexible-CG iterations with the
- * network, [1] MGPCG iterations of the fallback (0 if it was not needed). */
-int efl_nn_last_solve(efl_instance* inst, uint32_t out[2]);
+ * network, [1] MGPCG iterations of the fallback (0 if it was not needed);
+ * learned_rel (may be NULL): ||r|| / ||b|| when the network's part ended. */
+int efl_nn_last_solve(efl_instance* inst, uint32_t out[2], float* learned_rel);
 
 #ifdef __cplusplus
 }
