@@ -94,6 +94,12 @@ def gpu_run(args):
     inst.step(DT)  # initialisation dispatch
     for _ in range(args.warmup):
         inst.step(DT)
+    tuning = None
+    if args.autotune:
+        tuning = inst.autotune(DT, apply=True)
+        log(f"[bench] autotuned: {tuning}")
+        for _ in range(3):
+            inst.step(DT)
     ctx.sync()
     log(f"[bench] timing {args.steps} steps")
     cycles = []
@@ -137,7 +143,7 @@ def gpu_run(args):
                 cycles_mean=sum(cycles) / len(cycles), cycles_max=max(cycles),
                 pressure_solve_ms=sum(proj_ms) / len(proj_ms), device_step_ms=sum(step_ms) / len(step_ms),
                 max_norm_divergence=max(divs), rel_residual=max(rels),
-                sweep_ms=sweep_ms, sweep_bytes=alg_bytes, achieved_gbs=achieved, traffic=traffic)
+                sweep_ms=sweep_ms, sweep_bytes=alg_bytes, achieved_gbs=achieved, traffic=traffic, tuning=tuning)
 
 
 def run_config(args):
@@ -333,6 +339,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel instead of replaying graphs")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick sweep counts and the bottom-solve size with efl_autotune before timing")
     ap.add_argument("--config", type=int, default=2,
                     help="workload: 2 = coupled tank 1024^2 (the headline line); 1, 3, 4 (learned), 5 = the other suite configs")
     ap.add_argument("--roofline-only", action="store_true",
@@ -371,7 +379,9 @@ def main():
         "config": {"workload": f"coupled tank {args.size}x{args.size}, lower 55% fluid, 16 two-way coupled bodies, "
                                f"dt=1/60, pressure to ||r||/||b||<={args.rel_tol:g} (MG-preconditioned CG)",
                    "grid": [args.size, args.size], "bodies": 16, "projection": "mgpcg",
-                   "rel_tol": args.rel_tol, "reinit": "fast_iterative(10)"},
+                   "rel_tol": args.rel_tol, "reinit": "fast_iterative(10)",
+                   "autotuned": None if g["tuning"] is None else
+                   {k: g["tuning"][k] for k in ("pre", "post", "coarsest", "bottom_cells")}},
         "pressure_solve_ms": round(g["pressure_solve_ms"], 4),
         "device_step_ms": round(g["device_step_ms"], 4),
         "cg_iterations_mean": round(g["cycles_mean"], 2),

@@ -136,9 +136,23 @@ class efl_stats(C.Structure):
     ]
 
 
+class efl_tuning(C.Structure):
+    _fields_ = [
+        ("pre", C.c_uint32),
+        ("post", C.c_uint32),
+        ("coarsest", C.c_uint32),
+        ("bottom_cells", C.c_uint32),
+        ("nn_strips", C.c_uint32),
+        ("trials", C.c_uint32),
+        ("best_ms", C.c_float),
+        ("default_ms", C.c_float),
+    ]
+
+
 STRUCT_SIZES = {
     "efl_projection": 32, "efl_reinit": 8, "efl_settings": 128, "efl_source": 32,
     "efl_force": 16, "efl_obstacle": 176, "efl_body": 16, "efl_solid_force": 16, "efl_stats": 32,
+    "efl_tuning": 32,
 }
 
 

@@ -32,6 +32,7 @@ static_assert(sizeof(efl_force) == 16, "efl_force layout");
 static_assert(sizeof(efl_obstacle) == 176, "efl_obstacle layout (std430 SolidObstacle)");
 static_assert(sizeof(efl_solid_force) == 16, "efl_solid_force layout");
 static_assert(sizeof(efl_stats) == 32, "efl_stats layout");
+static_assert(sizeof(efl_tuning) == 32, "efl_tuning layout");
 
 struct efl_instance;
 struct StepGraph {
@@ -98,6 +99,7 @@ struct efl_instance {
   uint32_t nn_iters = 0, nn_fallback = 0;  // last learned solve
   bool nn_in_fallback = false;             // MGPCG is finishing a learned solve
   float nn_rel = 0.0f;                     // ||r|| / ||b|| when the network's part ended
+  int bottom_cells = 1024;                 // the bottom solve takes levels of at most this many cells
 };
 
 struct efl_nn {
@@ -385,11 +387,11 @@ uint32_t cnt_post(efl_instance* i) { return i->s.projection.mg_post_iterations; 
 uint32_t cnt_coarse(efl_instance* i) { return i->s.projection.mg_coarsest_iterations; }
 uint32_t cnt_gs(efl_instance* i) { return i->s.projection.gs_iterations; }
 
-// First level whose grid holds at most ctx->bottom_cells cells: from there
-// the rest of the cycle runs in one workgroup.  -1 if none.
+// First level whose grid holds at most the instance's bottom_cells cells:
+// from there the rest of the cycle runs in one workgroup.  -1 if none.
 int bottom_level(const efl_instance* i) {
   for (int l = 0; l < i->h.nlev; ++l)
-    if (i->h.lev[l].w * i->h.lev[l].h <= i->ctx->bottom_cells) return l;
+    if (i->h.lev[l].w * i->h.lev[l].h <= i->bottom_cells) return l;
   return -1;
 }
 
@@ -1237,6 +1239,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   if (!in) return EFL_E_OOM;
   in->ctx = c;
   in->s = *s;
+  in->bottom_cells = c->bottom_cells;
   size_t W = s->width, H = s->height, n = W * H, nu = (W + 1) * H, nv = W * (H + 1);
   // multigrid levels, multi_grid.rs:95-97
   int mn = (int)std::min(W, H);
@@ -1681,6 +1684,132 @@ int efl_measure(efl_instance* in, float dt, float* rel, float* mdiv) {
 int efl_get_stats(efl_instance* in, efl_stats* out) {
   if (!in || !out) return EFL_E_INVALID;
   *out = in->stats;
+  return EFL_OK;
+}
+
+// ---- autotuner -----------------------------------------------------------------
+
+int efl_autotune(efl_instance* in, float dt, int apply, efl_tuning* res) {
+  if (!in) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  const efl_projection orig = in->s.projection;
+  const int orig_bottom = in->bottom_cells;
+  const bool learned = orig.kind == EFL_PROJECTION_LEARNED;
+  const bool tunable = orig.kind == EFL_PROJECTION_MGPCG || learned ||
+                       (orig.kind == EFL_PROJECTION_MULTIGRID && orig.rel_tol > 0.0f);
+  if (!tunable) return fail(c, EFL_E_STATE, "autotune needs a projection with a residual target");
+  if (!in->initialized) return fail(c, EFL_E_STATE, "autotune needs an initialised instance");
+  if (learned && !in->nn) return fail(c, EFL_E_STATE, "learned projection without a network");
+  if (!(dt > 0.0f)) return fail(c, EFL_E_INVALID, "dt must be positive");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (in->h.dt != dt) {
+    in->h.dt = dt;
+    in->dirty = true;
+  }
+  int rc = upload(in);
+  if (rc) return rc;
+  hipStream_t st = c->stream;
+  IV v{in};
+  const size_t n = (size_t)in->s.width * in->s.height;
+  float* saved = nullptr;
+  if (hipMalloc(&saved, n * 4) != hipSuccess) return fail(c, EFL_E_OOM, "autotune");
+  struct Free {
+    float* p;
+    ~Free() { (void)hipFree(p); }
+  } guard{saved};
+  HIPCHK(c, hipMemcpyAsync(saved, in->h.p0, n * 4, hipMemcpyDeviceToDevice, st));
+  phase_pre(st, v);  // a fresh right-hand side from the current state
+  uint32_t trials = 0;
+  // one solve from the saved initial guess, best of two; +inf if it misses rel_tol
+  auto solve_ms = [&](float* ms) -> int {
+    float best = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+      HIPCHK(c, hipMemcpyAsync(in->h.p0, saved, n * 4, hipMemcpyDeviceToDevice, st));
+      HIPCHK(c, hipEventRecord(c->ev[0], st));
+      int r = stage_projection(c, v);
+      if (r) return r;
+      HIPCHK(c, hipEventRecord(c->ev[1], st));
+      HIPCHK(c, hipEventSynchronize(c->ev[1]));
+      float t = 0.0f;
+      HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+      if (!in->stats.converged) t = 1e30f;
+      best = std::min(best, t);
+      ++trials;
+    }
+    *ms = best;
+    return EFL_OK;
+  };
+  auto set = [&](uint32_t pre, uint32_t post, uint32_t coarse, int bottom) {
+    in->s.projection.mg_pre_iterations = pre;
+    in->s.projection.mg_post_iterations = post;
+    in->s.projection.mg_coarsest_iterations = coarse;
+    in->bottom_cells = bottom;
+    c->gversion++;
+  };
+  float def_ms = 0.0f;
+  rc = solve_ms(&def_ms);
+  if (rc) return rc;
+  uint32_t best[3] = {orig.mg_pre_iterations, orig.mg_post_iterations, orig.mg_coarsest_iterations};
+  int best_bottom = in->bottom_cells;
+  float best_ms = def_ms;
+  static const uint32_t kSweeps[][3] = {{2, 2, 3}, {1, 1, 2}, {1, 1, 3}, {2, 2, 2}, {2, 2, 4},
+                                        {3, 3, 3}, {1, 2, 3}, {2, 1, 3}, {3, 3, 4}};
+  const int big = efl::bottom_cells_enable(c->device, 4096);
+  const int bottoms[2] = {1024, big};
+  for (int bi = 0; bi < (big > 1024 ? 2 : 1); ++bi)
+    for (const auto& sw : kSweeps) {
+      set(sw[0], sw[1], sw[2], bottoms[bi]);
+      float ms = 0.0f;
+      rc = solve_ms(&ms);
+      if (rc) return rc;
+      if (ms < 0.97f * best_ms) {  // a clear gain only: single-solve timings carry a few % of noise
+        best_ms = ms;
+        best[0] = sw[0];
+        best[1] = sw[1];
+        best[2] = sw[2];
+        best_bottom = bottoms[bi];
+      }
+    }
+  set(best[0], best[1], best[2], best_bottom);
+  // the network's convolution strips, one level at a time
+  int strips_before[efl::nn::kLevels];
+  uint32_t strip_bits = 0;
+  if (learned) {
+    for (int l = 0; l < efl::nn::kLevels; ++l) strips_before[l] = in->nn_act.strip[l];
+    for (int l = 0; l < efl::nn::kLevels; ++l) {
+      float t[2] = {0.0f, 0.0f};
+      const int choice[2] = {64, 16};
+      for (int k = 0; k < 2; ++k) {
+        in->nn_act.strip[l] = choice[k];
+        nn_precondition(st, in, true);  // warm
+        HIPCHK(c, hipEventRecord(c->ev[0], st));
+        for (int r = 0; r < 5; ++r) nn_precondition(st, in, true);
+        HIPCHK(c, hipEventRecord(c->ev[1], st));
+        HIPCHK(c, hipEventSynchronize(c->ev[1]));
+        HIPCHK(c, hipEventElapsedTime(&t[k], c->ev[0], c->ev[1]));
+        trials += 5;
+      }
+      in->nn_act.strip[l] = t[0] <= t[1] ? 64 : 16;
+      if (t[0] <= t[1]) strip_bits |= 1u << l;
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(in->h.p0, saved, n * 4, hipMemcpyDeviceToDevice, st));
+  if (!apply) {
+    set(orig.mg_pre_iterations, orig.mg_post_iterations, orig.mg_coarsest_iterations, orig_bottom);
+    if (learned)
+      for (int l = 0; l < efl::nn::kLevels; ++l) in->nn_act.strip[l] = strips_before[l];
+  }
+  HIPCHK(c, hipStreamSynchronize(st));
+  if (res) {
+    res->pre = best[0];
+    res->post = best[1];
+    res->coarsest = best[2];
+    res->bottom_cells = (uint32_t)best_bottom;
+    res->nn_strips = strip_bits;
+    res->trials = trials;
+    res->best_ms = best_ms;
+    res->default_ms = def_ms;
+  }
   return EFL_OK;
 }
 

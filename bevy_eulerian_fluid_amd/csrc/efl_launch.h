@@ -61,6 +61,7 @@ namespace efl {
 int mg_tile_w();
 int mg_tile_h();
 int bottom_cells_for_device(int device);
+int bottom_cells_enable(int device, int cells);
 int mg_setup_tile_w();
 int mg_setup_tile_h();
 void launch_mg_setup(hipStream_t st, const Batch& b);

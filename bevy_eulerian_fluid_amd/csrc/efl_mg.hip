@@ -742,6 +742,22 @@ int bottom_cells_for_device(int device) {
   return 1024;
 }
 
+// Let the bottom solve take levels of up to `cells` cells on this device
+// (4096: the 64 x 64 level too), raising k_bottom's dynamic LDS limit.
+// Returns the size granted: `cells`, or 1024 if the device refuses.
+int bottom_cells_enable(int device, int cells) {
+  if (cells <= 1024) return 1024;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 0;
+  const size_t need = bottom_lds_bytes(cells * 4 / 3 + 64);
+  if ((size_t)lds >= need + 1024 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(k_bottom), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)need) == hipSuccess)
+    return cells;
+  (void)hipGetLastError();
+  return 1024;
+}
+
 void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells) {
   if (b.n <= 0) return;
   hipLaunchKernelGGL(k_bottom, dim3(b.n), dim3(BOT_NT), bottom_lds_bytes(max_cells), st, b, pcg, symmetric);

@@ -60,6 +60,7 @@ struct Activations {
   uint16_t* t[kTensors];
   uint16_t* g[kTensors];   // null unless allocated for training
   uint16_t* scratch;       // concat-gradient buffer (largest level-0 concat)
+  int strip[kLevels];      // convolution strip per level: 64 or 16 pixels, 0 = by size
 };
 
 // Forward pass from act.t[kTX] to act.t[14].

@@ -106,6 +106,7 @@ struct ConvArgs {
   int W, H;
   long sa, sb, sy;    // per-sample strides (elements)
   const double* stop; // CG done flag: skip the launch when set
+  int strip;          // host side: pixels per wavefront strip (64 or 16), 0 = by size
 };
 
 // One wavefront: MBW x 16 consecutive pixels of row y and every output
@@ -560,7 +561,8 @@ inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 template <int CA, int CB, int COUT, bool UPA, bool RELU>
 void conv_t(hipStream_t st, const ConvArgs& g, int n) {
   const unsigned rows = (unsigned)(g.H + 3) / 4;
-  if ((unsigned)((g.W + 63) / 64) * rows * (unsigned)n >= 256) {
+  const bool wide = g.strip ? g.strip == 64 : (unsigned)((g.W + 63) / 64) * rows * (unsigned)n >= 256;
+  if (wide) {
     hipLaunchKernelGGL((k_conv3x3<CA, CB, COUT, UPA, RELU, 4>), dim3((g.W + 63) / 64, rows, n), dim3(256), 0, st, g);
   } else {
     hipLaunchKernelGGL((k_conv3x3<CA, CB, COUT, UPA, RELU, 1>), dim3((g.W + 15) / 16, rows, n), dim3(256), 0, st, g);
@@ -668,6 +670,7 @@ void forward_impl(hipStream_t st, const Packed& p, Activations& a, const double*
     g.sb = L.in_b >= 0 ? stride(s, L.in_b) : 0;
     g.sy = stride(s, k);
     g.stop = stop;
+    g.strip = a.strip[L.level];
     conv(st, L.ca, L.cb, L.cout, L.upa, L.relu, g, s.n);
   }
 }

@@ -26,7 +26,7 @@ EXPORTS = [
     "efl_nn_param_count", "efl_nn_flops_per_pixel", "efl_nn_create", "efl_nn_destroy", "efl_nn_init_params",
     "efl_nn_set_params", "efl_nn_get_params", "efl_nn_dataset_reserve", "efl_nn_add_sample",
     "efl_nn_sample_count", "efl_nn_get_sample", "efl_nn_train", "efl_nn_eval", "efl_set_learned",
-    "efl_nn_last_solve",
+    "efl_nn_last_solve", "efl_autotune",
 ]
 
 
@@ -101,6 +101,7 @@ def load():
         "efl_nn_eval": [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), P, P],
         "efl_set_learned": [P, P],
         "efl_nn_last_solve": [P, P, C.POINTER(C.c_float)],
+        "efl_autotune": [P, C.c_float, C.c_int, C.POINTER(abi.efl_tuning)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -318,6 +319,12 @@ class Instance:
         r, d = C.c_float(), C.c_float()
         self._chk(self.L.efl_measure(self.h, C.c_float(dt), C.byref(r), C.byref(d)), "efl_measure")
         return r.value, d.value
+
+    def autotune(self, dt, apply=True):
+        """efl_autotune: the chosen settings and the solve times as a dict."""
+        t = abi.efl_tuning()
+        self._chk(self.L.efl_autotune(self.h, C.c_float(dt), int(bool(apply)), C.byref(t)), "efl_autotune")
+        return {f: getattr(t, f) for f, _ in abi.efl_tuning._fields_}
 
     def set_learned(self, net):
         self._chk(self.L.efl_set_learned(self.h, net.h if net is not None else None), "efl_set_learned")

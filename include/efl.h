@@ -348,6 +348,30 @@ int efl_set_diagnostics(efl_instance* inst, int enabled);
 int efl_measure(efl_instance* inst, float dt, float* rel_residual, float* max_norm_divergence);
 int efl_get_stats(efl_instance* inst, efl_stats* out);
 
+/* ---- autotuner (no counterpart in the reference) -----------------------
+ *
+ * efl_autotune times this instance's pressure solve on its own current
+ * system for each candidate and keeps the fastest: multigrid sweep counts
+ * (pre, post, coarsest) from a fixed list, the size of the bottom solve
+ * (levels of at most 1024 or 4096 cells in one workgroup) and, for a learned
+ * projection, the convolution strip width of each network level.  It runs
+ * the first stages of a step (solids to divergence) to form a fresh system,
+ * restores the initial guess before every trial, and leaves velocities,
+ * level set and pressure as they were; every candidate meets the same
+ * rel_tol, so the answer's accuracy does not change.  A candidate replaces the
+ * current choice only if it is at least 3 % faster.  Only projections with a
+ * residual target are tuned (MGPCG, MULTIGRID with rel_tol > 0, LEARNED);
+ * others return EFL_E_STATE.  With apply = 0 the settings are only reported. */
+typedef struct {
+  uint32_t pre, post, coarsest;   /* chosen sweep counts */
+  uint32_t bottom_cells;          /* 1024 or 4096 */
+  uint32_t nn_strips;             /* bit l: 64-pixel strips on network level l (else 16) */
+  uint32_t trials;                /* timed solves */
+  float best_ms;                  /* projection time of the chosen settings */
+  float default_ms;               /* projection time of the settings on entry */
+} efl_tuning;
+int efl_autotune(efl_instance* inst, float dt, int apply, efl_tuning* result);
+
 /* ---- learned preconditioner (no counterpart in the reference) -----------
  *
  * A 4-level convolutional U-Net (3x3 convolutions, 16/32/64/64 channels,

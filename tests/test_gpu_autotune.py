@@ -1,0 +1,89 @@
+"""efl_autotune: it times the instance's own solve per candidate, leaves the
+simulation state untouched, keeps the fastest candidate that meets the same
+tolerance (never slower than the settings it started from, which are one of
+the candidates), and the tuned instance keeps converging."""
+import numpy as np
+import pytest
+
+from bevy_eulerian_fluid_amd import abi, native, scenes
+
+pytestmark = pytest.mark.gpu
+DT = 1.0 / 60.0
+STATE = ("U0", "V0", "P0", "LS_AIR0", "LS_AIR1")  # derived fields (solids, divergence) are rebuilt by the next step
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _tank(ctx, size, proj):
+    sc = scenes.coupled_tank(size, n_each=2, proj=proj)
+    i = ctx.instance(sc["settings"])
+    i.set_sources(sc["init_sources"], True)
+    i.set_obstacles(sc["obstacles"], scenes.bodies_for(sc["obstacles"], sc["densities"]))
+    for _ in range(6):
+        i.step(DT)
+    return i
+
+
+def test_autotune_mgpcg():
+    _say("autotune: MGPCG, coupled tank 256x256")
+    ctx = native.Context(0)
+    try:
+        i = _tank(ctx, 256, scenes.projection("mgpcg", max_cycles=100, rel_tol=1e-5))
+        before = {f: i.get(f).copy() for f in STATE}
+        r = i.autotune(DT, apply=False)
+        _say(f"  report only: {r}")
+        for f in STATE:
+            assert np.array_equal(i.get(f).view(np.uint32), before[f].view(np.uint32)), f
+        assert r["best_ms"] <= r["default_ms"]
+        assert r["trials"] >= 2 * 10
+        assert r["bottom_cells"] in (1024, 4096)
+        r = i.autotune(DT, apply=True)
+        _say(f"  applied: {r}")
+        for _ in range(3):
+            i.step(DT)
+            st = i.stats()
+            assert st.converged == 1 and st.rel_residual <= 1e-5
+        i.close()
+    finally:
+        ctx.close()
+
+
+def test_autotune_rejects_fixed_solves():
+    ctx = native.Context(0)
+    try:
+        i = _tank(ctx, 128, scenes.projection("gauss_seidel", gs_iterations=20))
+        with pytest.raises(native.EflError) as e:
+            i.autotune(DT)
+        assert e.value.code == abi.E_STATE
+        i.close()
+    finally:
+        ctx.close()
+
+
+def test_autotune_learned_strips():
+    _say("autotune: learned projection, network strips, 128x128")
+    from bevy_eulerian_fluid_amd import learned
+    ctx = native.Context(0)
+    try:
+        i = _tank(ctx, 128, scenes.projection("mgpcg", max_cycles=100, rel_tol=1e-5))
+        net = native.Network(ctx)
+        w = learned.load_weights()
+        if w is None:
+            net.init_params(3)
+        else:
+            net.set_params(w)
+        i.set_learned(net)
+        i.set_projection(scenes.projection("learned", max_cycles=100, rel_tol=1e-5))
+        i.step(DT)
+        r = i.autotune(DT, apply=True)
+        _say(f"  applied: {r}")
+        assert 0 <= r["nn_strips"] < 16
+        i.step(DT)
+        st = i.stats()
+        assert st.converged == 1 and st.rel_residual <= 1e-5
+        net.close()
+        i.close()
+    finally:
+        ctx.close()
