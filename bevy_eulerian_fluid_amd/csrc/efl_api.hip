@@ -1511,6 +1511,11 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
   IV v{in};
   Batch center = make_batch(v, EX_CENTER), bx = make_batch(v, EX_XMAJOR), by = make_batch(v, EX_YMAJOR);
   switch (stage) {
+    case EFL_STAGE_MG_BOTTOM:
+      if (bottom_level(in) < 0) return fail(c, EFL_E_STATE, "no bottom level at this size");
+      mg_setup(st, v, true);
+      v_cycle(st, v, bottom_level(in), 0, true);
+      break;
     case EFL_STAGE_NN_PRECONDITION:
       if (!in->nn || !is_learned(in)) return fail(c, EFL_E_STATE, "NN_PRECONDITION needs efl_set_learned and a learned projection");
       nn_precondition(st, in, true);
@@ -1569,6 +1574,18 @@ int efl_time_stage(efl_instance* in, int stage, float dt, uint32_t reps, float* 
   HIPCHK(c, hipSetDevice(c->device));
   int rc = efl_run_stage(in, stage, dt);  // warm (and validate the stage id)
   if (rc) return rc;
+  if (stage == EFL_STAGE_MG_BOTTOM) {  // the bottom launch alone, reps times back to back
+    hipStream_t st = c->stream;
+    IV v{in};
+    HIPCHK(c, hipEventRecord(c->ev[0], st));
+    for (uint32_t k = 0; k < reps; ++k) v_cycle(st, v, bottom_level(in), 0, true);
+    HIPCHK(c, hipEventRecord(c->ev[1], st));
+    HIPCHK(c, hipEventSynchronize(c->ev[1]));
+    float t = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+    *ms = t / (float)reps;
+    return EFL_OK;
+  }
   if (stage == EFL_STAGE_GS_SWEEP || stage == EFL_STAGE_MG_SMOOTH) {
     // the tiled smoother alone: reps launches back to back (ping-ponging the
     // x pair), without the stencil setup and the final copy-back

@@ -613,6 +613,108 @@ __device__ __forceinline__ void b_up(const BotCfg& g, const BotLds& m, int l) {
   }
 }
 
+// The last two levels when they are 16 x 16 and 8 x 8 (square power-of-two
+// grids), on wavefront 0 in registers: lane (lx, ly) = (lane & 7, lane >> 3)
+// owns the 2 x 2 block (2 lx + dx, 2 ly + dy) of the 16 x 16 level, kept as
+// k = 2 dy + dx, and cell (lx, ly) of the 8 x 8 level; the neighbours across
+// a block edge come from the adjacent lanes by cross-lane shuffles.  The
+// updates are the ones of b_down / b_coarsest / b_up with the same operands
+// in the same order (a neighbour outside the level is the cell itself, as in
+// b_sweep), so the results equal the LDS path's bit for bit; only the LDS
+// round trip and the fence per half-sweep are gone.
+__device__ __forceinline__ float b_shfl(float v, int src) { return __shfl(v, src & 63, 64); }
+
+__device__ void b_tail16(const BotCfg& g, const BotLds& m, int ls) {
+  const int lane = threadIdx.x & 63, lx = lane & 7, ly = lane >> 3;
+  const int off = g.loff[ls], coff = g.loff[ls + 1];
+  const bool hasL = lx > 0, hasR = lx < 7, hasU = ly > 0, hasD = ly < 7;
+  float x[4], bb[4], dn[4];
+  float4 c[4];
+  int fl[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int q = off + (2 * ly + (k >> 1)) * 16 + 2 * lx + (k & 1);
+    x[k] = m.x[q];
+    bb[k] = m.b[q];
+    c[k] = m.c[q];
+    fl[k] = m.fl[q];
+    dn[k] = b_denominator(c[k], fl[k]);
+  }
+  const int cq = coff + ly * 8 + lx;
+  const float4 cc = m.c[cq];
+  const int cfl = m.fl[cq];
+  const float cdn = b_denominator(cc, cfl);
+  const float c0f = gs_c0(g.I, (float)(1 << ls)), c0c = gs_c0(g.I, (float)(1 << (ls + 1)));
+  // one colour of the 16 x 16 level: parity 0 = cells k 0, 3; parity 1 = k 1, 2
+  auto fine = [&](int parity) {
+    if (parity == 0) {
+      const float l1 = b_shfl(x[1], lane - 1), u2 = b_shfl(x[2], lane - 8);
+      const float r2 = b_shfl(x[2], lane + 1), d1 = b_shfl(x[1], lane + 8);
+      const float n0 = gs_upd(bb[0], c[0], dn[0], fl[0], x[0], hasL ? l1 : x[0], x[1], hasU ? u2 : x[0], x[2], 1.0f, c0f);
+      const float n3 = gs_upd(bb[3], c[3], dn[3], fl[3], x[3], x[2], hasR ? r2 : x[3], x[1], hasD ? d1 : x[3], 1.0f, c0f);
+      x[0] = n0;
+      x[3] = n3;
+    } else {
+      const float r0 = b_shfl(x[0], lane + 1), u3 = b_shfl(x[3], lane - 8);
+      const float l3 = b_shfl(x[3], lane - 1), d0 = b_shfl(x[0], lane + 8);
+      const float n1 = gs_upd(bb[1], c[1], dn[1], fl[1], x[1], x[0], hasR ? r0 : x[1], hasU ? u3 : x[1], x[3], 1.0f, c0f);
+      const float n2 = gs_upd(bb[2], c[2], dn[2], fl[2], x[2], hasL ? l3 : x[2], x[3], x[0], hasD ? d0 : x[2], 1.0f, c0f);
+      x[1] = n1;
+      x[2] = n2;
+    }
+  };
+  for (int k = 0; k < g.pre; ++k) {
+    fine(1);
+    fine(0);
+  }
+  // residuals of the four children, restricted in the order (0,0) (0,1) (1,0) (1,1)
+  float cb, cx = 0.0f;
+  {
+    const float rf = resid_factor(g.I, (float)(1 << ls));
+    const float l1 = b_shfl(x[1], lane - 1), l3 = b_shfl(x[3], lane - 1);
+    const float r0 = b_shfl(x[0], lane + 1), r2 = b_shfl(x[2], lane + 1);
+    const float u2 = b_shfl(x[2], lane - 8), u3 = b_shfl(x[3], lane - 8);
+    const float d0 = b_shfl(x[0], lane + 8), d1 = b_shfl(x[1], lane + 8);
+    const float ra = res_upd(bb[0], c[0], fl[0], x[0], hasL ? l1 : x[0], x[1], hasU ? u2 : x[0], x[2], rf);
+    const float rb = res_upd(bb[2], c[2], fl[2], x[2], hasL ? l3 : x[2], x[3], x[0], hasD ? d0 : x[2], rf);
+    const float rc = res_upd(bb[1], c[1], fl[1], x[1], x[0], hasR ? r0 : x[1], hasU ? u3 : x[1], x[3], rf);
+    const float rd = res_upd(bb[3], c[3], fl[3], x[3], x[2], hasR ? r2 : x[3], x[1], hasD ? d1 : x[3], rf);
+    cb = 0.25f * (((ra + rb) + rc) + rd);
+  }
+  // the coarsest level: lane (lx, ly) is cell (lx, ly), colour (lx + ly) & 1
+  auto coarse = [&](int parity) {
+    const float l = b_shfl(cx, lane - 1), r = b_shfl(cx, lane + 1), u = b_shfl(cx, lane - 8), d = b_shfl(cx, lane + 8);
+    const float n = gs_upd(cb, cc, cdn, cfl, cx, hasL ? l : cx, hasR ? r : cx, hasU ? u : cx, hasD ? d : cx, 1.0f, c0c);
+    if (((lx + ly) & 1) == parity) cx = n;
+  };
+  for (int k = 0; k < g.coarse; ++k) {
+    coarse(1);
+    coarse(0);
+  }
+  if (g.symmetric)
+    for (int k = 0; k < g.coarse; ++k) {
+      coarse(0);
+      coarse(1);
+    }
+  // prolongation and post-smoothing
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (fl[k] & F_LSNEG) x[k] = cx + x[k];
+  for (int k = 0; k < g.post; ++k) {
+    if (g.symmetric) {
+      fine(0);
+      fine(1);
+    } else {
+      fine(1);
+      fine(0);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) m.x[off + (2 * ly + (k >> 1)) * 16 + 2 * lx + (k & 1)] = x[k];
+  m.x[cq] = cx;
+  m.b[cq] = cb;
+}
+
 __global__ __launch_bounds__(BOT_NT) void k_bottom(Batch b, int pcg, int symmetric) {
   extern __shared__ float4 smem_f4[];
   __shared__ int lw[kMaxLevels + 1], lh[kMaxLevels + 1], loff[kMaxLevels + 1];
@@ -670,10 +772,15 @@ __global__ __launch_bounds__(BOT_NT) void k_bottom(Batch b, int pcg, int symmetr
   if (ls >= nl) {
     b_coarsest<BOT_NT>(g, m);  // the coarsest level is itself large
   } else {
+    const bool tail16 = nl == ls + 2 && lw[ls] == 16 && lh[ls] == 16 && lw[ls + 1] == 8 && lh[ls + 1] == 8;
     if (threadIdx.x < 64) {
-      for (int l = ls; l < nl - 1; ++l) b_down<64>(g, m, l);
-      b_coarsest<64>(g, m);
-      for (int l = nl - 2; l >= ls; --l) b_up<64>(g, m, l);
+      if (tail16) {
+        b_tail16(g, m, ls);
+      } else {
+        for (int l = ls; l < nl - 1; ++l) b_down<64>(g, m, l);
+        b_coarsest<64>(g, m);
+        for (int l = nl - 2; l >= ls; --l) b_up<64>(g, m, l);
+      }
     }
     __syncthreads();
   }

@@ -229,6 +229,7 @@ typedef enum {
   EFL_STAGE_GS_SWEEP_SPLIT,       /* GS_SWEEP dispatched one colour per kernel */
   EFL_STAGE_NN_PRECONDITION,      /* one application of the learned preconditioner to the CG
                                      residual (needs efl_set_learned and a learned projection) */
+  EFL_STAGE_MG_BOTTOM,            /* the V-cycle's bottom (the levels one workgroup solves) */
   EFL_STAGE_COUNT
 } efl_stage;
 
@@ -326,7 +327,8 @@ int efl_run_stage(efl_instance* inst, int stage, float dt);
  * hipEvents; *ms = mean device time per repetition.  GS_SWEEP and MG_SMOOTH
  * time their tiled smoother launch alone: reps launches back to back after
  * one full run of the stage (stencil setup and the copy of x back to its
- * buffer stay outside the events). */
+ * buffer stay outside the events); MG_BOTTOM likewise times the bottom
+ * launch alone. */
 int efl_time_stage(efl_instance* inst, int stage, float dt, uint32_t reps, float* ms);
 
 /* ---- results ------------------------------------------------------------ */
