@@ -55,7 +55,7 @@ struct efl_ctx {
   // gversion changes whenever launch arguments would
   std::vector<StepGraph> graphs;
   uint64_t gversion = 1;
-  bool graphs_on = true;
+  bool graphs_on = false;  // eager launches measured faster than replay on this runtime
   int bottom_cells = 1024;  // the bottom solve takes the levels of at most this many cells
   double* h_scal = nullptr;  // pinned, 16 doubles per batch slot
   std::vector<efl_instance*> insts;
@@ -256,11 +256,18 @@ void stage_reinit(hipStream_t st, const IV& v) {  // reinitialize_levelset.rs:16
     efl::launch_fim_active(st, make_batch(fim, EX_CENTER));
     uint32_t M = 0;
     for (auto* i : fim) M = std::max(M, i->s.reinit.fim_iterations);
-    for (uint32_t k = 0; k < M; ++k) {
-      IV part = filter(fim, [k](efl_instance* i) { return i->s.reinit.fim_iterations > k; });
-      Batch b = make_batch(part, EX_CENTER);
-      efl::launch_fim_phase_a(st, b);
-      efl::launch_fim_phase_b(st, b);
+    if (M == 0) return;
+    // an even number of temporally blocked launches (the last one writes
+    // ls0/lab), each instance's iterations spread over them
+    const uint32_t per = (uint32_t)efl::fim_iterations_per_launch();
+    const uint32_t L = 2 * ((M + 2 * per - 1) / (2 * per));
+    for (uint32_t j = 0; j < L; ++j) {
+      Batch b = make_batch(fim, EX_FULL, 0, nullptr, efl::fim_tile_w(), efl::fim_tile_h());
+      for (int n = 0; n < b.n; ++n) {
+        const uint32_t m = fim[n]->s.reinit.fim_iterations;
+        b.level[n] = (int)(m / L + (j < m % L ? 1 : 0));
+      }
+      efl::launch_fim_fused(st, b, (int)(j & 1));
     }
   }
 }

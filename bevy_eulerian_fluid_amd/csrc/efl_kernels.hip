@@ -926,65 +926,121 @@ __global__ __launch_bounds__(NT) void k_fim_active(Batch b) {
   }
   I->lab[id] = out;
 }
-__device__ __forceinline__ float far_abs(const float* phi, int W, int H, int x, int y) {
-  return inb(x, y, W, H) ? fabsf(phi[(size_t)y * W + x]) : FIM_FAR;
+// The iterations, temporally blocked: a 64 x 16 tile with a halo of 2 FK
+// cells runs up to FK iterations (phases A and B, A converging a cell when
+// it moved by at most 1e-6) in LDS.  Each phase reads only the previous
+// phase's values, and a phase's exact region shrinks by one cell, so after
+// FK iterations the tile itself holds what FK grid-wide rounds give.  Tiles
+// read one buffer pair and write the other (ls0/lab <-> phi_tmp/lab_tmp,
+// `src` selects), so no tile reads a halo another tile has already advanced;
+// an even number of launches ends in ls0/lab.  Batch::level = this
+// instance's iterations in this launch.  Tiles cover the whole grid so every
+// cell reaches the output buffer; only cells inside the reference's dispatch
+// cover (8 floor(W/8) x 8 floor(H/8)) are updated, the others pass their
+// values through (phase A's copy of them is their current value and they
+// never trigger).
+constexpr int FK = 5, FH = 2 * FK;
+constexpr int FT_W = 64, FT_H = 16;
+constexpr int FR_W = FT_W + 2 * FH, FR_H = FT_H + 2 * FH, FR_N = FR_W * FR_H;
+
+__device__ __forceinline__ float eikonal_lds(const float* sp, int c, bool l, bool r, bool u, bool d) {
+  const float R2 = 1.41421354f;
+  float xm = fminf(l ? fabsf(sp[c - 1]) : FIM_FAR, r ? fabsf(sp[c + 1]) : FIM_FAR);
+  float ym = fminf(u ? fabsf(sp[c - FR_W]) : FIM_FAR, d ? fabsf(sp[c + FR_W]) : FIM_FAR);
+  float dd = xm - ym;
+  if (dd > R2) return ym + 1.0f;
+  if (dd < -R2) return xm + 1.0f;
+  return 0.5f * ((xm + ym) + sqrtf(2.0f - dd * dd));
 }
-__device__ __forceinline__ float eikonal(const float* phi, int W, int H, int x, int y) {
-  const float R2 = 1.41421354f;  // sqrt(2.0) rounded to f32
-  float xm = fminf(far_abs(phi, W, H, x - 1, y), far_abs(phi, W, H, x + 1, y));
-  float ym = fminf(far_abs(phi, W, H, x, y - 1), far_abs(phi, W, H, x, y + 1));
-  float d = xm - ym;
-  if (d > R2) return ym + 1.0f;
-  if (d < -R2) return xm + 1.0f;
-  return 0.5f * ((xm + ym) + sqrtf(2.0f - d * d));
-}
-// phase A: ls0/lab -> phi_tmp/lab_tmp
-__global__ __launch_bounds__(NT) void k_fim_phase_a(Batch b) {
-  Site s = locate(b);
-  if (!s.ok) return;
-  const DevInst* I = s.I;
-  int W = I->W, H = I->H, x = s.x, y = s.y;
-  size_t id = (size_t)y * W + x;
-  uint32_t l = I->lab[id];
-  float p = I->ls0[id];
-  if (l != L_ACTIVE) {
-    I->phi_tmp[id] = p;
-    I->lab_tmp[id] = l;
-    return;
+
+__global__ __launch_bounds__(NT) void k_fim_fused(Batch b, int src) {
+  __shared__ float sp[2][FR_N];
+  __shared__ uint8_t sl[2][FR_N];
+  const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  int i = 0;
+  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  const int lb = blk - b.start[i];
+  const DevInst* I = b.inst[i];
+  const int iters = b.level[i];
+  const int W = I->W, H = I->H, cw = (W / 8) * 8, ch = (H / 8) * 8;
+  const int ox = (lb % b.tiles_x[i]) * FT_W - FH, oy = (lb / b.tiles_x[i]) * FT_H - FH;
+  const float* pin = src ? I->phi_tmp : I->ls0;
+  const uint32_t* lin = src ? I->lab_tmp : I->lab;
+  float* pout = src ? I->ls0 : I->phi_tmp;
+  uint32_t* lout = src ? I->lab : I->lab_tmp;
+  for (int c = threadIdx.x; c < FR_N; c += NT) {
+    const int x = ox + c % FR_W, y = oy + c / FR_W;
+    float p = 0.0f;
+    uint8_t l = (uint8_t)L_NONE;
+    if (inb(x, y, W, H)) {
+      p = pin[(size_t)y * W + x];
+      l = (uint8_t)lin[(size_t)y * W + x];
+    }
+    sp[0][c] = p;
+    sl[0][c] = l;
   }
-  float q = eikonal(I->ls0, W, H, x, y) * sgn(p);
-  I->phi_tmp[id] = q;
-  I->lab_tmp[id] = (fabsf(p - q) > 1e-6f) ? L_ACTIVE : L_CONVERGED;
-}
-// phase B: phi_tmp/lab_tmp (+ ls0 for the trigger's sign) -> ls0/lab
-__global__ __launch_bounds__(NT) void k_fim_phase_b(Batch b) {
-  Site s = locate(b);
-  if (!s.ok) return;
-  const DevInst* I = s.I;
-  int W = I->W, H = I->H, x = s.x, y = s.y;
-  size_t id = (size_t)y * W + x;
-  uint32_t l = I->lab_tmp[id];
-  if (l == L_CONVERGED) { I->lab[id] = L_NONE; I->ls0[id] = I->phi_tmp[id]; return; }
-  if (l != L_NONE) { I->ls0[id] = I->phi_tmp[id]; return; }
-  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
-  int trig = -1;
+  __syncthreads();
+  const int nb[4] = {-1, 1, -FR_W, FR_W};
+  for (int it = 0; it < iters; ++it) {
+    const int ma = 2 * it + 1, mb = 2 * it + 2;
+    for (int c = threadIdx.x; c < FR_N; c += NT) {  // phase A
+      const int cx = c % FR_W, cy = c / FR_W, x = ox + cx, y = oy + cy;
+      if (cx < ma || cx >= FR_W - ma || cy < ma || cy >= FR_H - ma || !inb(x, y, W, H)) continue;
+      const uint32_t l = sl[0][c];
+      const float p = sp[0][c];
+      if (l != L_ACTIVE || x >= cw || y >= ch) {
+        sp[1][c] = p;
+        sl[1][c] = (uint8_t)l;
+      } else {
+        const float q = eikonal_lds(sp[0], c, x > 0, x < W - 1, y > 0, y < H - 1) * sgn(p);
+        sp[1][c] = q;
+        sl[1][c] = (uint8_t)((fabsf(p - q) > 1e-6f) ? L_ACTIVE : L_CONVERGED);
+      }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < FR_N; c += NT) {  // phase B
+      const int cx = c % FR_W, cy = c / FR_W, x = ox + cx, y = oy + cy;
+      if (cx < mb || cx >= FR_W - mb || cy < mb || cy >= FR_H - mb || !inb(x, y, W, H)) continue;
+      if (x >= cw || y >= ch) {  // outside the dispatch cover: unchanged
+        sp[0][c] = sp[1][c];
+        continue;
+      }
+      const uint32_t l = sl[1][c];
+      if (l == L_CONVERGED) {
+        sl[0][c] = (uint8_t)L_NONE;
+        sp[0][c] = sp[1][c];
+        continue;
+      }
+      if (l != L_NONE) {
+        sp[0][c] = sp[1][c];
+        continue;
+      }
+      const bool inn[4] = {x > 0, x < W - 1, y > 0, y < H - 1};
+      int trig = -1;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int nx = x + ox[k], ny = y + oy[k];
-    if (trig < 0 && inb(nx, ny, W, H) && I->lab_tmp[(size_t)ny * W + nx] == L_CONVERGED) trig = k;
+      for (int k = 0; k < 4; ++k)
+        if (trig < 0 && inn[k] && sl[1][c + nb[k]] == L_CONVERGED) trig = k;
+      const float pa = sp[1][c];
+      if (trig < 0) {
+        sp[0][c] = pa;
+        continue;
+      }
+      const float qn = eikonal_lds(sp[1], c, inn[0], inn[1], inn[2], inn[3]);
+      if (fabsf(pa) > qn) {
+        sp[0][c] = qn * sgn(sp[1][c + nb[trig]]);
+        sl[0][c] = (uint8_t)L_ACTIVE;
+      } else {
+        sp[0][c] = pa;
+      }
+    }
+    __syncthreads();
   }
-  float pa = I->phi_tmp[id];
-  if (trig < 0) { I->ls0[id] = pa; return; }
-  float qn = eikonal(I->phi_tmp, W, H, x, y);
-  if (fabsf(pa) > qn) {
-    // The trigger's phase-A value has the sign of its pre-A value (the
-    // eikonal solution is > 0), so phi_tmp gives sign(p) without reading
-    // ls0, which this phase is writing.
-    float sg = sgn(I->phi_tmp[(size_t)(y + oy[trig]) * W + (x + ox[trig])]);
-    I->ls0[id] = qn * sg;
-    I->lab[id] = L_ACTIVE;
-  } else {
-    I->ls0[id] = pa;
+  for (int t = threadIdx.x; t < FT_W * FT_H; t += NT) {
+    const int c = (FH + t / FT_W) * FR_W + FH + t % FT_W;
+    const int x = ox + FH + t % FT_W, y = oy + FH + t / FT_W;
+    if (!inb(x, y, W, H)) continue;
+    pout[(size_t)y * W + x] = sp[0][c];
+    lout[(size_t)y * W + x] = sl[0][c];
   }
 }
 
@@ -1482,8 +1538,14 @@ void launch_jfa_step(hipStream_t st, const Batch& b, int step) { EFL_LAUNCH(k_jf
 void launch_jfa_sdf(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_jfa_sdf, b, st); }
 void launch_fim_init(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_init, b, st); }
 void launch_fim_active(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_active, b, st); }
-void launch_fim_phase_a(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_phase_a, b, st); }
-void launch_fim_phase_b(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_phase_b, b, st); }
+void launch_fim_fused(hipStream_t st, const Batch& b, int src) {
+  const int nb = b.start[b.n];
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_fim_fused, dim3(nb), dim3(NT), 0, st, b, src);
+}
+int fim_tile_w() { return FT_W; }
+int fim_tile_h() { return FT_H; }
+int fim_iterations_per_launch() { return FK; }
 void launch_ls_gradient(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_ls_gradient, b, st); }
 void launch_clear_bins(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_clear_bins, b, st); }
 void launch_sample_forces(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_sample_forces, b, st); }

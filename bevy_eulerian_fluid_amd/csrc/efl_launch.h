@@ -38,8 +38,10 @@ void launch_jfa_step(hipStream_t st, const Batch& b, int step);  // Batch.level 
 void launch_jfa_sdf(hipStream_t st, const Batch& b);  // Batch.level = final seed buffer
 void launch_fim_init(hipStream_t st, const Batch& b);
 void launch_fim_active(hipStream_t st, const Batch& b);
-void launch_fim_phase_a(hipStream_t st, const Batch& b);
-void launch_fim_phase_b(hipStream_t st, const Batch& b);
+void launch_fim_fused(hipStream_t st, const Batch& b, int src);  // Batch::level = iterations
+int fim_tile_w();
+int fim_tile_h();
+int fim_iterations_per_launch();
 void launch_ls_gradient(hipStream_t st, const Batch& b);
 void launch_clear_bins(hipStream_t st, const Batch& b);
 void launch_sample_forces(hipStream_t st, const Batch& b);

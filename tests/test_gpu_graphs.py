@@ -1,7 +1,7 @@
 """Replayed steps (HIP graphs) against eager launches.
 
 Two contexts run the same scenes side by side, one replaying its steps as
-HIP graphs (the default), one launching every kernel; after every step -
+HIP graphs, one launching every kernel (the default); after every step -
 with host reads and synchronisation in between - every field and the solver
 statistics must agree bit for bit.  Covers the MG-preconditioned CG (a
 graph for the CG iteration replayed a data-dependent number of times), a
@@ -46,6 +46,7 @@ def _same(a, b, where):
 @pytest.fixture()
 def two_ctx():
     g = native.Context(0)
+    g.set_graphs(True)
     e = native.Context(0)
     e.set_graphs(False)
     yield g, e
