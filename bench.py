@@ -84,8 +84,8 @@ def gpu_run(args):
     from bevy_eulerian_fluid_amd import native, scenes
     sc = build_scene(args.size, args.rel_tol)
     ctx = native.Context(0)
-    if args.no_graphs:
-        ctx.set_graphs(False)
+    if args.graphs:
+        ctx.set_graphs(True)
     inst = ctx.instance(sc["settings"])
     inst.set_sources(sc["init_sources"], True)
     ctx.set_obstacles(sc["obstacles"])
@@ -158,8 +158,8 @@ def run_config(args):
     if args.config == 4:
         return run_learned(args)
     ctx = native.Context(0)
-    if args.no_graphs:
-        ctx.set_graphs(False)
+    if args.graphs:
+        ctx.set_graphs(True)
     insts, forces, desc = [], None, {}
     if args.config == 1:
         sc = scenes.dam_break(256)
@@ -338,7 +338,7 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=25.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel instead of replaying graphs")
+    ap.add_argument("--graphs", action="store_true", help="replay steps as HIP graphs instead of launching every kernel")
     ap.add_argument("--autotune", action="store_true",
                     help="pick sweep counts and the bottom-solve size with efl_autotune before timing")
     ap.add_argument("--config", type=int, default=2,

@@ -153,7 +153,7 @@ class Context:
         self._chk(self.L.efl_set_timing(self.h, int(bool(on))), "efl_set_timing")
 
     def set_graphs(self, on):
-        """Replay steps as HIP graphs (default) or launch every kernel eagerly."""
+        """Replay steps as HIP graphs, or launch every kernel eagerly (default)."""
         self._chk(self.L.efl_set_graphs(self.h, int(bool(on))), "efl_set_graphs")
 
     def set_obstacles(self, obstacles):
