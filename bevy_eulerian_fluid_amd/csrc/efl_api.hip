@@ -1268,14 +1268,14 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   size_t o_grad = take(n * 8), o_s0 = take(n * 8), o_s1 = take(n * 8), o_afs = take(n * 16);
   size_t o_bins = take(4 * EFL_MAX_SOLIDS * 4), o_fts = take(EFL_MAX_SOLIDS * 16);
   size_t o_r0 = take(n * 4), o_xs0 = take(n * 4);
-  std::vector<size_t> o_lx(nl), o_lb(nl), o_lls(nl), o_laf(nl), o_lr(nl), o_lxs(nl), o_lcf(nl), o_ldn(nl), o_lfl(nl);
+  std::vector<size_t> o_lx(nl), o_lb(nl), o_lls(nl), o_laf(nl), o_lr(nl), o_lxs(nl), o_lcf(nl), o_lfl(nl);
   for (int l = 0; l < nl; ++l) {
     size_t m = (size_t)lv[l].first * lv[l].second;
     if (l > 0) {
       o_lx[l] = take(m * 4); o_lb[l] = take(m * 4); o_lls[l] = take(m * 4); o_laf[l] = take(m * 16);
       o_lr[l] = take(m * 4); o_lxs[l] = take(m * 4);
     }
-    o_lcf[l] = take(m * 16); o_ldn[l] = take(m * 4); o_lfl[l] = take(m * 4);
+    o_lcf[l] = take(m * 16); o_lfl[l] = take(m * 4);
   }
   size_t o_pr = take(n * 4), o_pz = take(n * 4), o_pp = take(n * 4), o_pp1 = take(n * 4), o_pq = take(n * 4);
   size_t o_red = take((size_t)4 * nblk * 8), o_scal = take(16 * 8), o_cnt = take(16), o_diag = take(16);
@@ -1319,11 +1319,11 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   h.bins = (int32_t*)(base + o_bins); h.fts = (float4*)(base + o_fts);
   h.nlev = nl;
   h.lev[0] = {(int)W, (int)H, h.p0, h.div, h.ls0, h.afs, (float*)(base + o_r0), (float*)(base + o_xs0),
-              (float4*)(base + o_lcf[0]), (float*)(base + o_ldn[0]), (int32_t*)(base + o_lfl[0])};
+              (float4*)(base + o_lcf[0]), (int32_t*)(base + o_lfl[0])};
   for (int l = 1; l < nl; ++l)
     h.lev[l] = {lv[l].first, lv[l].second, (float*)(base + o_lx[l]), (float*)(base + o_lb[l]), (float*)(base + o_lls[l]),
                 (float4*)(base + o_laf[l]), (float*)(base + o_lr[l]), (float*)(base + o_lxs[l]),
-                (float4*)(base + o_lcf[l]), (float*)(base + o_ldn[l]), (int32_t*)(base + o_lfl[l])};
+                (float4*)(base + o_lcf[l]), (int32_t*)(base + o_lfl[l])};
   h.pcg_r = (float*)(base + o_pr); h.pcg_z = (float*)(base + o_pz); h.pcg_p[0] = (float*)(base + o_pp); h.pcg_p[1] = (float*)(base + o_pp1); h.pcg_q = (float*)(base + o_pq);
   h.lev_pcg0 = h.lev[0];  // z over r, same level set, area fractions and stencil
   h.lev_pcg0.x = h.pcg_z;

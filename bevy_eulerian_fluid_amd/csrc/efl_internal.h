@@ -27,9 +27,9 @@ struct Level {  // one multigrid level (multi_grid.rs:99-118)
   float* r;
   float* xs;  // second buffer of x: the tiled smoothers read one and write the other
   // per-cell stencil, fixed during a solve (k_mg_setup): neighbour
-  // coefficients, Gauss-Seidel denominator, flags
+  // coefficients and flags (the Gauss-Seidel denominator is their in-order
+  // sum, formed where it is used)
   float4* cf;
-  float* dn;
   int32_t* fl;
 };
 

@@ -3,11 +3,12 @@
 // CG preconditioner) in two launches per level plus one for the bottom.
 //
 //   mg_setup   once per projection and level: the stencil of every cell
-//              (the 4 neighbour coefficients and the denominator that
+//              (the 4 neighbour coefficients and the flag word that
 //              gauss_seidel.wgsl:44-88 and residual.wgsl:12-57 derive from
 //              the air level set and the area fractions, which do not change
-//              during a solve) and, for the next level, restriction.wgsl's
-//              level set and area fractions.
+//              during a solve; the denominator is the coefficients' in-order
+//              sum) and, for the next level, restriction.wgsl's level set and
+//              area fractions.
 //   mg_down    a block owns a 64 x 16 tile of one level; it stages x of the
 //              tile plus a halo in LDS and keeps the stencils of its cells in
 //              registers, runs the pre-smoothing half-sweeps (each one
@@ -99,6 +100,17 @@ __device__ __forceinline__ float resid_factor(const DevInst* I, float scale) {
   return I->dt / I->rho / dx / dx;
 }
 
+// the Gauss-Seidel denominator from the stencil: the in-level coefficients
+// summed in neighbour order, as k_mg_setup forms it (F_IN0 is bit 0)
+__device__ __forceinline__ float b_denominator(float4 c, int fl) {
+  float d = 0.0f;
+  if (fl & 1) d += c.x;
+  if (fl & 2) d += c.y;
+  if (fl & 4) d += c.z;
+  if (fl & 8) d += c.w;
+  return d;
+}
+
 // gauss_seidel.wgsl:44-88 with the stencil precomputed: nume = c0 b plus the
 // fluid neighbours' c_k x_k in neighbour order, then nume / denominator and
 // the relaxation.  A skipped term leaves the sum untouched.
@@ -176,7 +188,6 @@ __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
   if (lc < 0.0f) fl |= F_LSNEG;
   if (closed) fl |= F_CLOSED;
   stg(L.cf, id, make_float4(cv[0], cv[1], cv[2], cv[3]));
-  stg(L.dn, id, denom);
   stg(L.fl, id, fl);
   // restriction.wgsl:10-60 of the level set and the area fractions, by the
   // thread of the coarse cell's (0, 0) child
@@ -248,7 +259,7 @@ struct Tile {
   static constexpr int NC = HW * RH;                 // cells of one colour
   static constexpr int NIT = (NC + MNT - 1) / MNT;   // slots per colour
   static constexpr int NSL = 2 * NIT;
-  float b[NSL], x[NSL], dn[NSL];
+  float b[NSL], x[NSL];
   float4 c[NSL];
   int fl[NSL];
   int lx[NSL], ly[NSL];
@@ -269,24 +280,24 @@ struct Tile {
   }
 
   // stencils, b and x from the level (x = 0 when xzero); with C, the
-  // prolongation adds the coarse x (cxin) on fluid cells that have a parent
+  // prolongation adds the coarse x (cxin) on fluid cells that have a parent.
+  // Loads are issued in the order they are consumed (x, the coarse x and
+  // the flags for the LDS copy, then b and the coefficients of the colour
+  // swept first, then those of the other colour), so the LDS copy and the
+  // first half-sweep wait only for their own data.
   __device__ __forceinline__ void load(const Level& L, const float* xin, bool xzero, const Level* C,
                                        const float* cxin, int ox, int oy, float* sx) {
     float xc[NSL];
     bool par[NSL];
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
-      b[s] = x[s] = dn[s] = xc[s] = 0.0f;
-      c[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      x[s] = xc[s] = 0.0f;
       fl[s] = 0;
       par[s] = false;
       if (inl[s]) {
         const int gx = ox + lx[s], gy = oy + ly[s];
         const size_t id = (size_t)gy * L.w + gx;
         if (!xzero) x[s] = ldg(xin, id);
-        b[s] = ldg(L.b, id);
-        c[s] = ldg(L.cf, id);
-        dn[s] = ldg(L.dn, id);
         fl[s] = ldg(L.fl, id);
         if (C) {
           const int cx = gx >> 1, cy = gy >> 1;
@@ -295,6 +306,16 @@ struct Tile {
             xc[s] = ldg(cxin, (size_t)cy * C->w + cx);
           }
         }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      b[s] = 0.0f;
+      c[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (inl[s]) {
+        const size_t id = (size_t)(oy + ly[s]) * L.w + ox + lx[s];
+        b[s] = ldg(L.b, id);
+        c[s] = ldg(L.cf, id);
       }
     }
 #pragma unroll
@@ -326,7 +347,8 @@ struct Tile {
         // an inactive slot evaluates an interior dummy cell and discards it
         const int q = act[k] ? ly[s] * RW + lx[s] : RW + 1;
         qq[k] = q;
-        nv[k] = gs_upd(b[s], c[s], dn[s], fl[s], x[s], sx[q - 1], sx[q + 1], sx[q - RW], sx[q + RW], weight, c0);
+        nv[k] = gs_upd(b[s], c[s], b_denominator(c[s], fl[s]), fl[s], x[s], sx[q - 1], sx[q + 1], sx[q - RW],
+                       sx[q + RW], weight, c0);
       }
 #pragma unroll
       for (int k = 0; k < NIT; ++k)
@@ -476,17 +498,6 @@ struct BotLds {
   float *x, *b;
   int* fl;
 };
-
-// the Gauss-Seidel denominator from the stencil: the in-level coefficients
-// summed in neighbour order, as k_mg_setup forms it
-__device__ __forceinline__ float b_denominator(float4 c, int fl) {
-  float d = 0.0f;
-  if (fl & 1) d += c.x;
-  if (fl & 2) d += c.y;
-  if (fl & 4) d += c.z;
-  if (fl & 8) d += c.w;
-  return d;
-}
 
 // G threads (the workgroup, or wavefront 0) take part
 template <int G>
