@@ -614,20 +614,22 @@ int cg_run(efl_ctx* c, const IV& cg, hipGraphExec_t iter) {
     int rc = read_scalars(c, act);
     if (rc) return rc;
     IV keep;
-    for (size_t j = 0; j < act.size(); ++j)
-      if (c->h_scal[16 * j + efl::SC_DONE] == 0.0) keep.push_back(act[j]);
+    for (size_t j = 0; j < act.size(); ++j) {
+      const double* sc = c->h_scal + 16 * j;
+      efl_instance* i = act[j];
+      if (sc[efl::SC_DONE] == 0.0) {
+        keep.push_back(i);
+        continue;
+      }
+      // once the done flag is up every kernel of the iteration returns at
+      // once, so these are the solve's final values
+      i->stats.cycles = (uint32_t)sc[efl::SC_ITER];
+      i->stats.converged = sc[efl::SC_CONV] != 0.0;
+      i->stats.rel_residual = (float)sc[efl::SC_REL];
+      i->cg_last = i->stats.cycles;
+    }
     if (iter) keep = keep.empty() ? keep : act;  // the graph advances the whole set
     act.swap(keep);
-  }
-  int rc = read_scalars(c, cg);
-  if (rc) return rc;
-  for (size_t j = 0; j < cg.size(); ++j) {
-    const double* sc = c->h_scal + 16 * j;
-    efl_instance* i = cg[j];
-    i->stats.cycles = (uint32_t)sc[efl::SC_ITER];
-    i->stats.converged = sc[efl::SC_CONV] != 0.0;
-    i->stats.rel_residual = (float)sc[efl::SC_REL];
-    i->cg_last = i->stats.cycles;
   }
   return EFL_OK;
 }

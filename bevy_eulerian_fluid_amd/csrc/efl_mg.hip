@@ -100,14 +100,26 @@ __device__ __forceinline__ float resid_factor(const DevInst* I, float scale) {
   return I->dt / I->rho / dx / dx;
 }
 
+// bit `bit` of fl set ? a : b, as v_bfe_i32 (the bit spread to a lane mask)
+// and v_bfi_b32 in one block: the flag tests stay per-lane VALU work, where
+// the compiler would otherwise hoist them out of the unrolled sweeps as
+// 64-bit lane masks and spill those through VGPR lanes.
+static_assert(F_IN0 == 0 && F_GSZ == 1 << 8 && F_RZ == 1 << 9, "flag bits fsel is called with");
+template <int BIT>
+__device__ __forceinline__ float fsel(int fl, float a, float b) {
+  float r;
+  asm("v_bfe_i32 %0, %1, %2, 1\n\tv_bfi_b32 %0, %0, %3, %4" : "=&v"(r) : "v"(fl), "I"(BIT), "v"(a), "v"(b));
+  return r;
+}
+
 // the Gauss-Seidel denominator from the stencil: the in-level coefficients
 // summed in neighbour order, as k_mg_setup forms it (F_IN0 is bit 0)
 __device__ __forceinline__ float b_denominator(float4 c, int fl) {
   float d = 0.0f;
-  if (fl & 1) d += c.x;
-  if (fl & 2) d += c.y;
-  if (fl & 4) d += c.z;
-  if (fl & 8) d += c.w;
+  d = fsel<0>(fl, d + c.x, d);
+  d = fsel<1>(fl, d + c.y, d);
+  d = fsel<2>(fl, d + c.z, d);
+  d = fsel<3>(fl, d + c.w, d);
   return d;
 }
 
@@ -116,34 +128,26 @@ __device__ __forceinline__ float b_denominator(float4 c, int fl) {
 // the relaxation.  A skipped term leaves the sum untouched.
 __device__ __forceinline__ float gs_upd(float b, float4 c, float dn, int fl, float xo, float x0, float x1, float x2,
                                         float x3, float weight, float c0) {
-  const float cv[4] = {c.x, c.y, c.z, c.w};
-  const float xv[4] = {x0, x1, x2, x3};
   float nume = c0 * b;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const float t = nume + cv[k] * xv[k];
-    nume = ((fl >> (F_FL0 + k)) & 1) ? t : nume;
-  }
+  nume = fsel<F_FL0 + 0>(fl, nume + c.x * x0, nume);
+  nume = fsel<F_FL0 + 1>(fl, nume + c.y * x1, nume);
+  nume = fsel<F_FL0 + 2>(fl, nume + c.z * x2, nume);
+  nume = fsel<F_FL0 + 3>(fl, nume + c.w * x3, nume);
   const float pn = nume / dn;
   const float v = weight * pn + (1.0f - weight) * xo;
-  return (fl & F_GSZ) ? 0.0f : v;
+  return fsel<8>(fl, 0.0f, v);  // F_GSZ
 }
 
 // residual.wgsl:12-57 with the stencil precomputed: fluid neighbours
 // subtract c_k (x - x_k) factor, air neighbours c_k x factor.
 __device__ __forceinline__ float res_upd(float b, float4 c, int fl, float xc, float x0, float x1, float x2, float x3,
                                          float factor) {
-  const float cv[4] = {c.x, c.y, c.z, c.w};
-  const float xv[4] = {x0, x1, x2, x3};
   float res = b;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const float r_fl = res - cv[k] * (xc - xv[k]) * factor;
-    const float r_air = res - cv[k] * xc * factor;
-    const bool in = (fl >> (F_IN0 + k)) & 1, fluid = (fl >> (F_FL0 + k)) & 1;
-    res = in ? (fluid ? r_fl : r_air) : res;
-  }
-  return (fl & F_RZ) ? 0.0f : res;
+  res = fsel<F_IN0 + 0>(fl, fsel<F_FL0 + 0>(fl, res - c.x * (xc - x0) * factor, res - c.x * xc * factor), res);
+  res = fsel<F_IN0 + 1>(fl, fsel<F_FL0 + 1>(fl, res - c.y * (xc - x1) * factor, res - c.y * xc * factor), res);
+  res = fsel<F_IN0 + 2>(fl, fsel<F_FL0 + 2>(fl, res - c.z * (xc - x2) * factor, res - c.z * xc * factor), res);
+  res = fsel<F_IN0 + 3>(fl, fsel<F_FL0 + 3>(fl, res - c.w * (xc - x3) * factor, res - c.w * xc * factor), res);
+  return fsel<9>(fl, 0.0f, res);  // F_RZ
 }
 
 // ---------------------------------------------------------------------------
@@ -337,25 +341,20 @@ struct Tile {
       const int m = hs + 1;
       const int lo_x = max(m, bx0), hi_x = min(RW - m, bx1), lo_y = max(m, by0), hi_y = min(RH - m, by1);
       const int s0 = (hs & 1) ? NIT : 0;
-      float nv[NIT];
-      int qq[NIT];
-      bool act[NIT];
+      // The cells a half-sweep updates and the neighbours they read are of
+      // different colours, so each slot updates and stores in turn; a
+      // wavefront with no active cell in a slot skips it.
 #pragma unroll
       for (int k = 0; k < NIT; ++k) {
         const int s = s0 + k;
-        act[k] = valid[s] && lx[s] >= lo_x && lx[s] < hi_x && ly[s] >= lo_y && ly[s] < hi_y;
-        // an inactive slot evaluates an interior dummy cell and discards it
-        const int q = act[k] ? ly[s] * RW + lx[s] : RW + 1;
-        qq[k] = q;
-        nv[k] = gs_upd(b[s], c[s], b_denominator(c[s], fl[s]), fl[s], x[s], sx[q - 1], sx[q + 1], sx[q - RW],
-                       sx[q + RW], weight, c0);
-      }
-#pragma unroll
-      for (int k = 0; k < NIT; ++k)
-        if (act[k]) {
-          x[s0 + k] = nv[k];
-          sx[qq[k]] = nv[k];
+        if (valid[s] && lx[s] >= lo_x && lx[s] < hi_x && ly[s] >= lo_y && ly[s] < hi_y) {
+          const int q = ly[s] * RW + lx[s];
+          const float nv = gs_upd(b[s], c[s], b_denominator(c[s], fl[s]), fl[s], x[s], sx[q - 1], sx[q + 1],
+                                  sx[q - RW], sx[q + RW], weight, c0);
+          x[s] = nv;
+          sx[q] = nv;
         }
+      }
       __syncthreads();
     }
   }
