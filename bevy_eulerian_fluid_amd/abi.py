@@ -136,6 +136,15 @@ class efl_stats(C.Structure):
     ]
 
 
+class efl_fluid_diagnostics(C.Structure):
+    _fields_ = [
+        ("volume", C.c_float),
+        ("min_velocity", C.c_float),
+        ("max_velocity", C.c_float),
+        ("reserved", C.c_float),
+    ]
+
+
 class efl_tuning(C.Structure):
     _fields_ = [
         ("pre", C.c_uint32),
@@ -152,7 +161,7 @@ class efl_tuning(C.Structure):
 STRUCT_SIZES = {
     "efl_projection": 32, "efl_reinit": 8, "efl_settings": 128, "efl_source": 32,
     "efl_force": 16, "efl_obstacle": 176, "efl_body": 16, "efl_solid_force": 16, "efl_stats": 32,
-    "efl_tuning": 32,
+    "efl_tuning": 32, "efl_fluid_diagnostics": 16,
 }
 
 

@@ -1281,6 +1281,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   }
   size_t o_pr = take(n * 4), o_pz = take(n * 4), o_pp = take(n * 4), o_pp1 = take(n * 4), o_pq = take(n * 4);
   size_t o_red = take((size_t)4 * nblk * 8), o_scal = take(16 * 8), o_cnt = take(16), o_diag = take(16);
+  size_t o_fdp = take((size_t)3 * (W / 16) * (H / 16) * 4), o_fdr = take(16);
   size_t o_src0 = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES), o_src = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES);
   size_t o_desc = take(sizeof(DevInst));
   in->bytes = off;
@@ -1335,6 +1336,8 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   h.scal = (double*)(base + o_scal);
   h.counter = (uint32_t*)(base + o_cnt);
   h.diag = (float*)(base + o_diag);
+  h.fdp = (float*)(base + o_fdp);
+  h.fdr = (float*)(base + o_fdr);
   h.src_init = (efl_source*)(base + o_src0);
   h.src = (efl_source*)(base + o_src);
   h.obs = c->d_obs;
@@ -1680,6 +1683,24 @@ int efl_get_solid_forces(efl_instance* in, efl_solid_force* out, uint32_t n) {
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyAsync(out, in->h.fts, sizeof(efl_solid_force) * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_get_fluid_diagnostics(efl_instance* in, efl_fluid_diagnostics* out) {
+  if (!in || !out) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = upload(in);
+  if (rc) return rc;
+  efl::launch_fluid_diagnostics(c->stream, in->d, in->h.W, in->h.H);
+  HIPCHK(c, hipGetLastError());
+  float r[4];
+  HIPCHK(c, hipMemcpyAsync(r, in->h.fdr, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out->volume = r[0];
+  out->min_velocity = r[1];
+  out->max_velocity = r[2];
+  out->reserved = 0.0f;
   return EFL_OK;
 }
 

@@ -79,6 +79,10 @@ struct DevInst {
   uint32_t cg_maxc, cg_every;
   uint32_t cg_flex;    // flexible beta (learned, nonlinear preconditioner)
   float* diag;         // [0] max |D| (as float bits via atomic max), ...
+  // fluid diagnostics: per 16 x 16 block volume, max and min speed
+  // (3 arrays of (W/16)(H/16) floats), then {volume, min, max}
+  float* fdp;
+  float* fdr;
 };
 
 // Arguments of the rigid-body integrator (one launch per obstacle list).

@@ -1421,6 +1421,70 @@ __global__ __launch_bounds__(NT) void k_max_divergence(Batch b) {
 }
 
 // ---------------------------------------------------------------------------
+// Fluid diagnostics (diagnostics/calculate_volume.wgsl:12-75,
+// max_velocity.wgsl:13-81, min_velocity.wgsl:13-87): the volume (cells with
+// ls0 < 0) and the largest and smallest cell-centred speed (the smallest over
+// fluid cells), one 16 x 16 block per workgroup over the floor(W/16) x
+// floor(H/16) cover as the reference dispatches them.  Counting, max and min
+// are order independent, so the block folds use wavefront reductions.  The
+// final pass keeps the reference's second stage: lane l of 512 adds the
+// block values l, l + 512, ... in order (the speed passes add their block
+// maxima and minima too), then the lanes fold by halving strides 256..1.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fluid_diag_blocks(const DevInst* I) {
+  const int W = I->W, gx = W / 16;
+  const int x = (int)blockIdx.x * 16 + (int)threadIdx.x % 16, y = (int)blockIdx.y * 16 + (int)threadIdx.x / 16;
+  const float level = I->ls0[(size_t)y * W + x];
+  const float uc = 0.5f * (I->u0[(size_t)y * (W + 1) + x] + I->u0[(size_t)y * (W + 1) + x + 1]);
+  const float vc = 0.5f * (I->v0[(size_t)y * W + x] + I->v0[(size_t)(y + 1) * W + x]);
+  const float mag = sqrtf(uc * uc + vc * vc);
+  float mx = mag, mn = level < 0.0f ? mag : 1e30f;
+  const int cnt = __syncthreads_count(level < 0.0f);
+  for (int off = 32; off > 0; off >>= 1) {
+    mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    mn = fminf(mn, __shfl_xor(mn, off, 64));
+  }
+  __shared__ float wmx[4], wmn[4];
+  if ((threadIdx.x & 63) == 0) {
+    wmx[threadIdx.x >> 6] = mx;
+    wmn[threadIdx.x >> 6] = mn;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const size_t np = (size_t)gx * (I->H / 16), wid = blockIdx.x + (size_t)blockIdx.y * gx;
+  I->fdp[wid] = (float)cnt;
+  I->fdp[np + wid] = fmaxf(fmaxf(wmx[0], wmx[1]), fmaxf(wmx[2], wmx[3]));
+  I->fdp[2 * np + wid] = fminf(fminf(wmn[0], wmn[1]), fminf(wmn[2], wmn[3]));
+}
+__global__ __launch_bounds__(512) void k_fluid_diag_final(const DevInst* I) {
+  const int np = (I->W / 16) * (I->H / 16), l = (int)threadIdx.x;
+  __shared__ float tv[512], tx[512], tn[512];
+  float sv = 0.0f, sx = 0.0f, sn = 0.0f;
+  for (int i = l; i < np; i += 512) {
+    sv += I->fdp[i];
+    sx += I->fdp[(size_t)np + i];
+    sn += I->fdp[2 * (size_t)np + i];
+  }
+  tv[l] = sv;
+  tx[l] = sx;
+  tn[l] = sn;
+  __syncthreads();
+  for (int stride = 256; stride > 0; stride >>= 1) {
+    if (l < stride) {
+      tv[l] += tv[l + stride];
+      tx[l] = fmaxf(tx[l], tx[l + stride]);
+      tn[l] = fminf(tn[l], tn[l + stride]);
+    }
+    __syncthreads();
+  }
+  if (l == 0) {
+    I->fdr[0] = tv[0];
+    I->fdr[1] = tn[0];
+    I->fdr[2] = tx[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Rigid bodies: semi-implicit Euler of the dynamic obstacles, one thread per
 // body (the role avian2d plays for the reference).  The fluid forces of the
 // contributing instances are summed; gravity, walls, fluid density and dx
@@ -1565,6 +1629,10 @@ void launch_cg_check(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_cg_
 void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_mask_p, b, st); }
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
   if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);
+}
+void launch_fluid_diagnostics(hipStream_t st, const DevInst* dev, int W, int H) {
+  if (W >= 16 && H >= 16) hipLaunchKernelGGL(k_fluid_diag_blocks, dim3(W / 16, H / 16), dim3(256), 0, st, dev);
+  hipLaunchKernelGGL(k_fluid_diag_final, dim3(1), dim3(512), 0, st, dev);
 }
 void launch_max_divergence(hipStream_t st, const Batch& b) {
   EFL_LAUNCH(k_max_divergence, b, st);

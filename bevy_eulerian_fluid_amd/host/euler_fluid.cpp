@@ -451,6 +451,14 @@ bool FluidWorld::update() {
           "set_bodies");
     bodies_dirty_ = false;
   }
+  // DiagnosticsNode, ahead of the fluid node every frame
+  if (diagnostics_on_)
+    for (Fluid& f : fluids_) {
+      if (!f.alive) continue;
+      efl_fluid_diagnostics d{};
+      check(efl_get_fluid_diagnostics(f.inst, &d), "DiagnosticsNode");
+      f.diag = FluidDiagnostics{d.volume, d.min_velocity, d.max_velocity};
+    }
   // EulerFluidNode::update: idle unless a new physics step happened.
   if (step_number_ == rendered_step_) return false;
   rendered_step_ = step_number_;
@@ -498,6 +506,8 @@ void FluidWorld::read_field(Entity e, efl_field field, std::vector<float>& out) 
   out.resize(count);
   check(efl_get_field(f.inst, field, out.data(), count), "read_field");
 }
+
+FluidDiagnostics FluidWorld::diagnostics(Entity e) { return fluid(e).diag; }
 
 efl_stats FluidWorld::stats(Entity e) {
   efl_stats s{};

@@ -291,6 +291,14 @@ using Entity = uint32_t;
 
 // One Bevy App with FluidPlugin: the fluid entities, their child sources,
 // the solids, the physics clock and the render node, on one device.
+// diagnostics/component.rs: FluidVolume, FluidMinVelocityMagnitude,
+// FluidMaxVelocityMagnitude of a fluid
+struct FluidDiagnostics {
+  float volume = 0.0f;
+  float min_velocity = 0.0f;
+  float max_velocity = 0.0f;
+};
+
 class FluidWorld {
  public:
   FluidWorld(const FluidPlugin& plugin, int device = 0);
@@ -338,6 +346,13 @@ class FluidWorld {
   // Copy a field to the host (efl_field ids); resizes out.
   void read_field(Entity fluid, efl_field field, std::vector<float>& out);
   efl_stats stats(Entity fluid);
+  // FluidDiagnosticsPlugin (diagnostics/plugin.rs:20-48): with it on, every
+  // update() runs the volume and min / max speed passes for each fluid before
+  // the fluid node, as the reference's DiagnosticsNode does each frame
+  // (render_graph edge DiagnosticsLabel -> FluidLabel); diagnostics() returns
+  // the last values (the reference reads them back frames later).
+  void enable_diagnostics(bool on) { diagnostics_on_ = on; }
+  FluidDiagnostics diagnostics(Entity fluid);
   uint64_t physics_step_number() const { return step_number_; }
   float time_step() const { return time_step_; }
   efl_ctx* context() { return ctx_; }
@@ -361,6 +376,7 @@ class FluidWorld {
     FluidStatus status = FluidStatus::Reset;
     std::vector<Source> sources;
     ForcesToFluid forces;
+    FluidDiagnostics diag;
   };
   Fluid& fluid(Entity e);
   const Fluid& fluid(Entity e) const;
@@ -373,6 +389,7 @@ class FluidWorld {
   std::vector<SolidBody> bodies_;
   bool integrate_ = false;
   bool bodies_dirty_ = false;
+  bool diagnostics_on_ = false;
   uint64_t step_number_ = 0;    // PhysicsFrameInfo.step_number
   uint64_t rendered_step_ = 0;  // CurrentPhysicsStepNumberRenderWorld
   float time_step_ = 0.0f;      // FluidTimeStep

@@ -7,9 +7,10 @@
 //   host_check scene <file>  a 128 x 96 fluid driven frame by frame through
 //                            FluidWorld on device 0 (sources incl. an inactive
 //                            one, bodies incl. an unsupported shape, local
-//                            forces, idle frames); writes the obstacle records,
-//                            the frame outcomes, every field and the solid
-//                            forces to <file> for comparison with the oracle.
+//                            forces, idle frames, a final frame with the
+//                            diagnostics plugin on); writes the obstacle
+//                            records, the frame outcomes, every field, the
+//                            solid forces and the diagnostics to <file>.
 //
 // Record format of <file>: 32-byte NUL-padded name, uint64 byte count, bytes.
 #include <hip/hip_runtime_api.h>
@@ -232,6 +233,12 @@ int run_scene(const char* path) {
   world.physics_step(dt);
   frames.push_back(world.update());
 
+  // a frame with the diagnostics plugin on and no physics step: the passes
+  // run on the final state, the fluid node idles
+  world.enable_diagnostics(true);
+  const bool idle = !world.update();
+  const FluidDiagnostics diag = world.diagnostics(fluid);
+
   // FluidTextures.u is the device field U0
   const FluidTextures tex = world.textures(fluid);
   std::vector<float> u;
@@ -258,6 +265,8 @@ int run_scene(const char* path) {
   w.put("forces_to_solid", fs.data(), fs.size() * sizeof(FluidToSolidForce));
   const efl_stats st = world.stats(fluid);
   w.put("stats", &st, sizeof(st));
+  const float dg[4] = {diag.volume, diag.min_velocity, diag.max_velocity, idle ? 1.0f : 0.0f};
+  w.put("diagnostics", dg, sizeof(dg));
   for (int fld = 0; fld < EFL_FIELD_COUNT; ++fld) {
     std::vector<float> data;
     world.read_field(fluid, (efl_field)fld, data);

@@ -26,7 +26,7 @@ EXPORTS = [
     "efl_nn_param_count", "efl_nn_flops_per_pixel", "efl_nn_create", "efl_nn_destroy", "efl_nn_init_params",
     "efl_nn_set_params", "efl_nn_get_params", "efl_nn_dataset_reserve", "efl_nn_add_sample",
     "efl_nn_sample_count", "efl_nn_get_sample", "efl_nn_train", "efl_nn_eval", "efl_set_learned",
-    "efl_nn_last_solve", "efl_autotune",
+    "efl_nn_last_solve", "efl_autotune", "efl_get_fluid_diagnostics",
 ]
 
 
@@ -83,6 +83,7 @@ def load():
         "efl_set_diagnostics": [P, C.c_int],
         "efl_measure": [P, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)],
         "efl_get_stats": [P, C.POINTER(abi.efl_stats)],
+        "efl_get_fluid_diagnostics": [P, C.POINTER(abi.efl_fluid_diagnostics)],
         "efl_time_stage": [P, C.c_int, C.c_float, C.c_uint32, C.POINTER(C.c_float)],
         "efl_set_bodies": [P, P, C.c_uint32],
         "efl_get_obstacles": [P, P, C.c_uint32],
@@ -339,6 +340,13 @@ class Instance:
         s = abi.efl_stats()
         self._chk(self.L.efl_get_stats(self.h, C.byref(s)), "efl_get_stats")
         return s
+
+    def fluid_diagnostics(self):
+        """(volume, min speed, max speed) of the current fields, as the
+        reference's diagnostics passes compute them."""
+        d = abi.efl_fluid_diagnostics()
+        self._chk(self.L.efl_get_fluid_diagnostics(self.h, C.byref(d)), "efl_get_fluid_diagnostics")
+        return d.volume, d.min_velocity, d.max_velocity
 
 
 def _f32(a):

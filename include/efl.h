@@ -172,6 +172,15 @@ typedef struct {
   float step_ms;              /* device time of the whole step (if timing on) */
 } efl_stats;
 
+/* Fluid diagnostics (diagnostics/component.rs: FluidVolume,
+ * FluidMinVelocityMagnitude, FluidMaxVelocityMagnitude). */
+typedef struct {
+  float volume;        /* cells with levelset_air < 0 (calculate_volume.wgsl) */
+  float min_velocity;  /* min_velocity.wgsl: smallest cell-centred speed over fluid cells, cells/s */
+  float max_velocity;  /* max_velocity.wgsl: largest cell-centred speed, cells/s */
+  float reserved;
+} efl_fluid_diagnostics;
+
 /* Field identifiers for efl_get_field / efl_set_field / efl_field_info. */
 typedef enum {
   EFL_FIELD_U0 = 0,           /* (W+1) x H f32; FluidTextures.u */
@@ -352,6 +361,14 @@ int efl_set_diagnostics(efl_instance* inst, int enabled);
 /* Measure ||r||/||b|| and dt*max|D| of the current fields (forces a sync). */
 int efl_measure(efl_instance* inst, float dt, float* rel_residual, float* max_norm_divergence);
 int efl_get_stats(efl_instance* inst, efl_stats* out);
+/* The reference's diagnostics passes (diagnostics/plugin.rs:20-104; its node
+ * runs them on u0 / v0 / levelset_air before each frame's step) on the
+ * current fields: the same 16 x 16 blocks over the floor(W/16) x floor(H/16)
+ * cover and the same second stage, where each of 512 lanes adds its block
+ * values l, l + 512, ... before the lanes are folded, so the speeds are a
+ * max / min of sums when there are more than 512 blocks and min_velocity is 0
+ * when there are fewer.  Synchronous (one device round trip). */
+int efl_get_fluid_diagnostics(efl_instance* inst, efl_fluid_diagnostics* out);
 
 /* ---- autotuner (no counterpart in the reference) -----------------------
  *

@@ -1617,6 +1617,72 @@ int orc_measure(orc_inst* o, float dt, float* rel, float* mdiv) {
   return 0;
 }
 
+// Fluid diagnostics (diagnostics/plugin.rs:20-104): the volume
+// (calculate_volume.wgsl:12-75: cells with levelset_air < 0), and the largest
+// and smallest cell-centred speed (max_velocity.wgsl:13-81, min_velocity.wgsl:
+// 13-87: |(u(x)+u(x+1), v(y)+v(y+1)) / 2|, the minimum over cells with
+// levelset_air < 0, others 1e30), each in two passes.  Pass 1 runs one 16 x 16
+// workgroup per block of the floor(W/16) x floor(H/16) cover
+// (calculate_volume.rs:51,133; max_velocity.rs:53,135; min_velocity.rs:55,
+// 138) and folds its 256 values by halving strides 128..1.  Pass 2 is one
+// workgroup of 512: lane l adds the partials l, l + 512, ... in order (the
+// velocity passes add their partial maxima and minima as well) and the 512
+// lane values fold by halving strides 256..1 (sum for the volume, max / min
+// for the speeds).  FluidTextures.u / v / levelset_air are u0 / v0 / ls0
+// (setup_components.rs:115-122).
+int orc_fluid_diagnostics(const orc_inst* o, float* out) {
+  const int W = o->W, H = o->H, gx = W / 16, gy = H / 16, np = gx * gy;
+  std::vector<float> pv(np), px(np), pn(np);
+  for (int by = 0; by < gy; ++by)
+    for (int bx = 0; bx < gx; ++bx) {
+      float wv[256], wx[256], wn[256];
+      for (int lid = 0; lid < 256; ++lid) {
+        const int x = bx * 16 + lid % 16, y = by * 16 + lid / 16;
+        const float level = o->ls0.at(x, y);
+        const float uc = 0.5f * (o->u0.at(x, y) + o->u0.at(x + 1, y));
+        const float vc = 0.5f * (o->v0.at(x, y) + o->v0.at(x, y + 1));
+        const float mag = std::sqrt(uc * uc + vc * vc);
+        wv[lid] = level < 0.0f ? 1.0f : 0.0f;
+        wx[lid] = mag;
+        wn[lid] = level < 0.0f ? mag : 1e30f;
+      }
+      for (int stride = 128;; stride /= 2) {
+        for (int lid = 0; lid < stride; ++lid) {
+          wv[lid] += wv[lid + stride];
+          wx[lid] = std::fmax(wx[lid], wx[lid + stride]);
+          wn[lid] = std::fmin(wn[lid], wn[lid + stride]);
+        }
+        if (stride == 1) break;
+      }
+      const int wid = bx + by * gx;
+      pv[wid] = wv[0];
+      px[wid] = wx[0];
+      pn[wid] = wn[0];
+    }
+  float tv[512], tx[512], tn[512];
+  for (int l = 0; l < 512; ++l) {
+    float sv = 0.0f, sx = 0.0f, sn = 0.0f;
+    for (int i = l; i < np; i += 512) {
+      sv += pv[i];
+      sx += px[i];
+      sn += pn[i];
+    }
+    tv[l] = sv;
+    tx[l] = sx;
+    tn[l] = sn;
+  }
+  for (int stride = 256; stride > 0; stride /= 2)
+    for (int l = 0; l < stride; ++l) {
+      tv[l] += tv[l + stride];
+      tx[l] = std::fmax(tx[l], tx[l + stride]);
+      tn[l] = std::fmin(tn[l], tn[l + stride]);
+    }
+  out[0] = tv[0];
+  out[1] = tn[0];
+  out[2] = tx[0];
+  return 0;
+}
+
 int orc_residual_of(orc_inst* o, float dt, const float* x, float* r_out) {
   o->uni.dt = dt;
   Lvl<float> L{o->W, o->H, const_cast<float*>(x), o->div.d.data(), o->ls0.d.data(), o->afs.d.data(), r_out, 1.0f};

@@ -51,6 +51,9 @@ int orc_solve_fp64(orc_inst* o, float dt, double rel_tol, double* p_out, double*
  * for a caller-given x (W x H), b = the current div. */
 int orc_residual_of(orc_inst* o, float dt, const float* x, float* r_out);
 /* Single-threaded switch (1 = serial, otherwise OpenMP threads if built). */
+/* volume, smallest and largest speed as the reference's diagnostics passes
+ * compute them from u0 / v0 / ls0: out[3] = {volume, min, max} */
+int orc_fluid_diagnostics(const orc_inst* o, float* out);
 int orc_set_threads(int n);
 
 #ifdef __cplusplus

@@ -62,6 +62,7 @@ def lib():
             ("orc_set_diagnostics", [P, C.c_int]),
             ("orc_set_bodies", [P, C.c_void_p, C.c_uint32]),
             ("orc_get_obstacles", [P, C.c_void_p, C.c_uint32]),
+            ("orc_fluid_diagnostics", [P, C.c_void_p]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -192,6 +193,12 @@ class OracleInstance:
         r, d = C.c_float(), C.c_float()
         _check(lib().orc_measure(self.h, dt, C.byref(r), C.byref(d)), "measure")
         return r.value, d.value
+
+    def fluid_diagnostics(self):
+        """(volume, min speed, max speed) as the reference's diagnostics passes give them."""
+        out = np.zeros(3, np.float32)
+        _check(lib().orc_fluid_diagnostics(self.h, out.ctypes.data), "fluid_diagnostics")
+        return float(out[0]), float(out[1]), float(out[2])
 
     def solve_fp64(self, dt, rel_tol=1e-12):
         out = np.zeros(self.W * self.H, dtype=np.float64)

@@ -126,6 +126,11 @@ def test_host_mirror_scene_matches_direct_abi(tmp_path):
         ref = g.get("SOLID_FORCES").reshape(-1, 4)[: forces.shape[0], :3]
         assert forces.shape[0] == 5 and np.array_equal(forces, ref)
         assert np.abs(ref[[0, 1, 3, 4]]).max() > 0 and not ref[2].any()
+        # the final frame ran the diagnostics passes (FluidDiagnosticsPlugin)
+        # on the final state and left the fluid node idle
+        dg = np.frombuffer(rec["diagnostics"], dtype=np.float32)
+        assert dg[3] == 1.0, "the diagnostics frame advanced the fluid"
+        assert np.array_equal(dg[:3], np.float32(g.fluid_diagnostics())) and dg[0] > 0
         g.close()
     finally:
         ctx.close()
