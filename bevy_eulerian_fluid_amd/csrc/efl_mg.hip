@@ -74,18 +74,6 @@ __device__ __forceinline__ float4 ldg(const float4* p, size_t i) {
   const f4v v = ((const __attribute__((address_space(1))) f4v*)p)[i];
   return make_float4(v.x, v.y, v.z, v.w);
 }
-// two adjacent elements (i, i + 1) in one load; the pair need only be
-// 4-byte aligned (rows of odd width)
-typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
-typedef int i2u __attribute__((ext_vector_type(2), aligned(4)));
-__device__ __forceinline__ float2 ldg2(const float* p, size_t i) {
-  const f2u v = *(const __attribute__((address_space(1))) f2u*)(p + i);
-  return make_float2(v.x, v.y);
-}
-__device__ __forceinline__ int2 ldg2(const int32_t* p, size_t i) {
-  const i2u v = *(const __attribute__((address_space(1))) i2u*)(p + i);
-  return make_int2(v.x, v.y);
-}
 __device__ __forceinline__ void stg(float* p, size_t i, float v) {
   ((__attribute__((address_space(1))) float*)p)[i] = v;
 }
@@ -271,15 +259,9 @@ __device__ __forceinline__ MSite mg_locate(const Batch& b) {
 // LDS for the neighbours.
 template <int HALO>
 struct Tile {
-  // The x halo is HALO rounded up to even: region rows then start on an even
-  // column and hold whole pairs of cells (2i, 2i + 1), which are staged with
-  // one load per field.  The extra column is smoothed like the rest of the
-  // halo and only widens the region whose values are exact.
-  static constexpr int HX = (HALO + 1) & ~1;
-  static constexpr int RW = MT_W + 2 * HX, RH = MT_H + 2 * HALO;
-  static constexpr int NPR = RW / 2;                 // pairs per region row
-  static constexpr int NP = NPR * RH;                // pairs of the region
-  static constexpr int NIT = (NP + MNT - 1) / MNT;   // pairs per thread = slots per colour
+  static constexpr int RW = MT_W + 2 * HALO, RH = MT_H + 2 * HALO, HW = RW / 2;
+  static constexpr int NC = HW * RH;                 // cells of one colour
+  static constexpr int NIT = (NC + MNT - 1) / MNT;   // slots per colour
   static constexpr int NSL = 2 * NIT;
   float b[NSL], x[NSL];
   float4 c[NSL];
@@ -287,88 +269,62 @@ struct Tile {
   int lx[NSL], ly[NSL];
   bool valid[NSL], inl[NSL];
 
-  // ox, oy: level coordinates of LDS cell (0, 0), ox even.  Pair k of a
-  // thread is region pair threadIdx.x + k MNT; slot k holds its cell of the
-  // colour swept first, slot NIT + k the other one.
+  // ox, oy: level coordinates of LDS cell (0, 0)
   __device__ __forceinline__ void place(int ox, int oy, int first, int w, int h) {
     const int base = (ox + oy) & 1;
 #pragma unroll
-    for (int k = 0; k < NIT; ++k) {
-      const int u = (int)threadIdx.x + k * MNT;
-      const int py = u / NPR, px = 2 * (u - py * NPR);
-      const int e = (first + py + base) & 1;  // element of the pair with colour `first`
-      valid[k] = valid[NIT + k] = u < NP;
-      ly[k] = ly[NIT + k] = py;
-      lx[k] = px + e;
-      lx[NIT + k] = px + 1 - e;
-      inl[k] = valid[k] && mg_inb(ox + lx[k], oy + py, w, h);
-      inl[NIT + k] = valid[k] && mg_inb(ox + lx[NIT + k], oy + py, w, h);
+    for (int s = 0; s < NSL; ++s) {
+      const int colour = s < NIT ? first : 1 - first;
+      const int u = (int)threadIdx.x + (s % NIT) * MNT;
+      valid[s] = u < NC;
+      ly[s] = u / HW;
+      lx[s] = 2 * (u - ly[s] * HW) + ((colour + ly[s] + base) & 1);
+      inl[s] = valid[s] && mg_inb(ox + lx[s], oy + ly[s], w, h);
     }
   }
 
   // stencils, b and x from the level (x = 0 when xzero); with C, the
-  // prolongation adds the coarse x (cxin) on fluid cells that have a parent
-  // (both cells of a pair have the same one).  A pair whose second cell is
-  // outside the level (odd width) loads its first cell alone.  x, the
-  // coarse x and the flags (for the LDS copy) are loaded before b and the
-  // coefficients.
+  // prolongation adds the coarse x (cxin) on fluid cells that have a parent.
+  // Loads are issued in the order they are consumed (x, the coarse x and
+  // the flags for the LDS copy, then b and the coefficients of the colour
+  // swept first, then those of the other colour), so the LDS copy and the
+  // first half-sweep wait only for their own data.
   __device__ __forceinline__ void load(const Level& L, const float* xin, bool xzero, const Level* C,
                                        const float* cxin, int ox, int oy, float* sx) {
-    float xc[NIT];
-    bool par[NIT];
+    float xc[NSL];
+    bool par[NSL];
 #pragma unroll
-    for (int k = 0; k < NIT; ++k) {
-      const int s0 = k, s1 = NIT + k;
-      const int e = lx[s0] & 1, gx = ox + (lx[s0] & ~1), gy = oy + ly[s0];
-      const bool in0 = e ? inl[s1] : inl[s0], in1 = e ? inl[s0] : inl[s1];
-      const size_t id = (size_t)gy * L.w + gx;
-      float2 xv = make_float2(0.f, 0.f);
-      int2 fv = make_int2(0, 0);
-      if (in1) {
-        if (!xzero) xv = ldg2(xin, id);
-        fv = ldg2(L.fl, id);
-      } else if (in0) {
-        if (!xzero) xv.x = ldg(xin, id);
-        fv.x = ldg(L.fl, id);
-      }
-      xc[k] = 0.0f;
-      par[k] = false;
-      if (C && in0) {
-        const int cx = gx >> 1, cy = gy >> 1;
-        if (cx < C->w && cy < C->h) {
-          par[k] = true;
-          xc[k] = ldg(cxin, (size_t)cy * C->w + cx);
+    for (int s = 0; s < NSL; ++s) {
+      x[s] = xc[s] = 0.0f;
+      fl[s] = 0;
+      par[s] = false;
+      if (inl[s]) {
+        const int gx = ox + lx[s], gy = oy + ly[s];
+        const size_t id = (size_t)gy * L.w + gx;
+        if (!xzero) x[s] = ldg(xin, id);
+        fl[s] = ldg(L.fl, id);
+        if (C) {
+          const int cx = gx >> 1, cy = gy >> 1;
+          if (cx < C->w && cy < C->h) {
+            par[s] = true;
+            xc[s] = ldg(cxin, (size_t)cy * C->w + cx);
+          }
         }
       }
-      x[s0] = e ? xv.y : xv.x;
-      x[s1] = e ? xv.x : xv.y;
-      fl[s0] = e ? fv.y : fv.x;
-      fl[s1] = e ? fv.x : fv.y;
-    }
-#pragma unroll
-    for (int k = 0; k < NIT; ++k) {
-      const int s0 = k, s1 = NIT + k;
-      const int e = lx[s0] & 1, gx = ox + (lx[s0] & ~1), gy = oy + ly[s0];
-      const bool in0 = e ? inl[s1] : inl[s0], in1 = e ? inl[s0] : inl[s1];
-      const size_t id = (size_t)gy * L.w + gx;
-      float2 bv = make_float2(0.f, 0.f);
-      float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
-      if (in1) {
-        bv = ldg2(L.b, id);
-        c0 = ldg(L.cf, id);
-        c1 = ldg(L.cf, id + 1);
-      } else if (in0) {
-        bv.x = ldg(L.b, id);
-        c0 = ldg(L.cf, id);
-      }
-      b[s0] = e ? bv.y : bv.x;
-      b[s1] = e ? bv.x : bv.y;
-      c[s0] = e ? c1 : c0;
-      c[s1] = e ? c0 : c1;
     }
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
-      if (C && par[s % NIT] && (fl[s] & F_LSNEG)) x[s] = xc[s % NIT] + x[s];
+      b[s] = 0.0f;
+      c[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (inl[s]) {
+        const size_t id = (size_t)(oy + ly[s]) * L.w + ox + lx[s];
+        b[s] = ldg(L.b, id);
+        c[s] = ldg(L.cf, id);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      if (C && par[s] && (fl[s] & F_LSNEG)) x[s] = xc[s] + x[s];
       if (valid[s]) sx[ly[s] * RW + lx[s]] = x[s];
     }
   }
@@ -404,7 +360,7 @@ struct Tile {
   }
 
   __device__ __forceinline__ bool in_tile(int s) const {
-    return valid[s] && lx[s] >= HX && lx[s] < HX + MT_W && ly[s] >= HALO && ly[s] < HALO + MT_H;
+    return valid[s] && lx[s] >= HALO && lx[s] < HALO + MT_W && ly[s] >= HALO && ly[s] < HALO + MT_H;
   }
 };
 
@@ -412,13 +368,13 @@ struct Tile {
 // complete copy of x after the launch (row-major from LDS: coalesced).
 template <int HALO>
 __device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, int y0, const float* sx) {
-  constexpr int RW = Tile<HALO>::RW, HX = Tile<HALO>::HX;
+  constexpr int RW = MT_W + 2 * HALO;
 #pragma unroll
   for (int k = 0; k < (MT_W * MT_H) / MNT; ++k) {
     int t = threadIdx.x + k * MNT;
     int lx = t % MT_W, ly = t / MT_W;
     int gx = x0 + lx, gy = y0 + ly;
-    if (gx < L.w && gy < L.h) stg(xout, (size_t)gy * L.w + gx, sx[(ly + HALO) * RW + lx + HX]);
+    if (gx < L.w && gy < L.h) stg(xout, (size_t)gy * L.w + gx, sx[(ly + HALO) * RW + lx + HALO]);
   }
 }
 
@@ -444,7 +400,7 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   const Level L = mg_level(I, s.lvl, pcg);
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
-  const int ox = s.tx0 - T::HX, oy = s.ty0 - HALO;
+  const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
   T t;
   t.place(ox, oy, first, L.w, L.h);
   t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, sx);
@@ -461,7 +417,7 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
     const float r = t.inl[k] ? res_upd(t.b[k], t.c[k], t.fl[k], t.x[k], sx[q - 1], sx[q + 1], sx[q - T::RW],
                                        sx[q + T::RW], factor)
                              : 0.0f;
-    sr[(t.ly[k] - HALO) * MT_W + t.lx[k] - T::HX] = r;
+    sr[(t.ly[k] - HALO) * MT_W + t.lx[k] - HALO] = r;
   }
   __syncthreads();
   // one coarse cell per thread: 32 x 8 coarse cells under the 64 x 16 tile
@@ -492,7 +448,7 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   const Level C = I->lev[s.lvl + 1];
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
-  const int ox = s.tx0 - T::HX, oy = s.ty0 - HALO;
+  const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
   T t;
   t.place(ox, oy, first, L.w, L.h);
   t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, sx);
