@@ -688,6 +688,7 @@ __global__ __launch_bounds__(NT) void k_solve_v(Batch b) {
 // the first pass before any pass reads it.
 // ---------------------------------------------------------------------------
 constexpr int XP = 10;  // passes = halo
+constexpr int XNT = 1024;  // threads per tile: a tile's passes are its critical path
 constexpr int XT_W = 64, XT_H = 16;
 constexpr int XR_W = XT_W + 2 * XP, XR_H = XT_H + 2 * XP, XR_N = XR_W * XR_H;
 constexpr int XL_W = XR_W + 2, XL_N = XL_W * (XR_H + 2);
@@ -704,9 +705,10 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
   int32_t* V0 = COMP == 0 ? I->uval0 : I->vval0;
   int32_t* V1 = COMP == 0 ? I->uval1 : I->vval1;
   // level-set cells [ox - 1, ox + XR_W + 1) x [oy - 1, oy + XR_H + 1)
-  for (int q = threadIdx.x; q < XL_N; q += NT) sl[q] = ldz(I->ls0, W, H, ox - 1 + q % XL_W, oy - 1 + q / XL_W);
+  for (int q = threadIdx.x; q < XL_N; q += XNT) sl[q] = ldz(I->ls0, W, H, ox - 1 + q % XL_W, oy - 1 + q / XL_W);
   __syncthreads();
-  for (int q = threadIdx.x; q < XR_N; q += NT) {
+  bool any_valid = false, any_open = false;
+  for (int q = threadIdx.x; q < XR_N; q += XNT) {
     const int lx = q % XR_W, ly = q / XR_W, x = ox + lx, y = oy + ly;
     float f = 0.0f;
     uint8_t v0 = 0, v1 = 0;
@@ -731,17 +733,38 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
     sf[q] = f;
     sv0[q] = v0;
     sv1[q] = v1;
+    const bool covered = inb(x, y, w, h) && x < ex && y < ey;
+    any_valid |= v0 == 1 || (v1 == 1 && !covered);  // uncovered faces keep both planes
+    any_open |= v0 == 0 && covered;
   }
-  __syncthreads();
   const int cx1 = min(ex, w) - ox, cy1 = min(ey, h) - oy;  // coverage in local coordinates
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads walk the window
+  // The passes change nothing unless the region holds a valid face (in the
+  // recomputed plane, or in either staged plane of a face outside the
+  // coverage, which even and odd passes read) and a covered invalid one:
+  // with no valid face no face can become valid, with no covered invalid
+  // face none is written.  Then every covered face of the tile ends with
+  // both planes equal to its recomputed mask and keeps its velocity.
+  const int has_valid = __syncthreads_or(any_valid ? 1 : 0);
+  const int has_open = __syncthreads_or(any_open ? 1 : 0);
+  if (!(has_valid && has_open)) {
+    for (int t = threadIdx.x; t < XT_W * XT_H; t += XNT) {
+      const int lx = XP + t % XT_W, ly = XP + t / XT_W, x = ox + lx, y = oy + ly;
+      if (!inb(x, y, w, h)) continue;
+      const size_t id = (size_t)y * w + x;
+      const int q = ly * XR_W + lx;
+      V0[id] = sv0[q];
+      if (lx < cx1 && ly < cy1) V1[id] = sv0[q];
+    }
+    return;
+  }
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 16 threads walk the window
   for (int p = 0; p < XP; ++p) {
     const uint8_t* vin = (p & 1) ? sv1 : sv0;  // pass 0: valid0 -> valid1, then back
     uint8_t* vout = (p & 1) ? sv0 : sv1;
     const int m = p + 1;
     const int lo_x = max(m, -ox), hi_x = min(XR_W - m, cx1), lo_y = max(m, -oy), hi_y = min(XR_H - m, cy1);
-    for (int ly = lo_y + ty; ly < hi_y; ly += 8)
-      for (int lx = lo_x + tx; lx < hi_x; lx += 32) {
+    for (int ly = lo_y + ty; ly < hi_y; ly += 16)
+      for (int lx = lo_x + tx; lx < hi_x; lx += 64) {
         const int q = ly * XR_W + lx;
         if (vin[q] == 1) {
           vout[q] = 1;
@@ -764,7 +787,7 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
       }
     __syncthreads();
   }
-  for (int t = threadIdx.x; t < XT_W * XT_H; t += NT) {
+  for (int t = threadIdx.x; t < XT_W * XT_H; t += XNT) {
     const int lx = XP + t % XT_W, ly = XP + t / XT_W, x = ox + lx, y = oy + ly;
     if (!inb(x, y, w, h)) continue;
     const size_t id = (size_t)y * w + x;
@@ -777,7 +800,7 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
 
 // Batch::level of a slot names the component (0 = u, 1 = v): both run in
 // one launch
-__global__ __launch_bounds__(NT) void k_extrapolate_fused(Batch b) {
+__global__ __launch_bounds__(XNT) void k_extrapolate_fused(Batch b) {
   __shared__ float sf[XR_N];
   __shared__ uint8_t sv[2][XR_N];
   __shared__ float sl[XL_N];
@@ -941,6 +964,7 @@ __global__ __launch_bounds__(NT) void k_fim_active(Batch b) {
 // never trigger).
 constexpr int FK = 5, FH = 2 * FK;
 constexpr int FT_W = 64, FT_H = 16;
+constexpr int FNT = 1024;  // threads per tile: a tile's iterations are its critical path
 constexpr int FR_W = FT_W + 2 * FH, FR_H = FT_H + 2 * FH, FR_N = FR_W * FR_H;
 
 __device__ __forceinline__ float eikonal_lds(const float* sp, int c, bool l, bool r, bool u, bool d) {
@@ -953,7 +977,7 @@ __device__ __forceinline__ float eikonal_lds(const float* sp, int c, bool l, boo
   return 0.5f * ((xm + ym) + sqrtf(2.0f - dd * dd));
 }
 
-__global__ __launch_bounds__(NT) void k_fim_fused(Batch b, int src) {
+__global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
   __shared__ float sp[2][FR_N];
   __shared__ uint8_t sl[2][FR_N];
   const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
@@ -968,7 +992,8 @@ __global__ __launch_bounds__(NT) void k_fim_fused(Batch b, int src) {
   const uint32_t* lin = src ? I->lab_tmp : I->lab;
   float* pout = src ? I->ls0 : I->phi_tmp;
   uint32_t* lout = src ? I->lab : I->lab_tmp;
-  for (int c = threadIdx.x; c < FR_N; c += NT) {
+  bool active = false;
+  for (int c = threadIdx.x; c < FR_N; c += FNT) {
     const int x = ox + c % FR_W, y = oy + c / FR_W;
     float p = 0.0f;
     uint8_t l = (uint8_t)L_NONE;
@@ -978,12 +1003,16 @@ __global__ __launch_bounds__(NT) void k_fim_fused(Batch b, int src) {
     }
     sp[0][c] = p;
     sl[0][c] = l;
+    active |= l == L_ACTIVE && x < cw && y < ch;
   }
-  __syncthreads();
+  // Only an ACTIVE cell inside the cover starts a change (stored labels are
+  // never CONVERGED): a region without one passes through unchanged, and the
+  // tile is copied to the output pair as it is.
+  const int n_it = __syncthreads_or(active ? 1 : 0) ? iters : 0;
   const int nb[4] = {-1, 1, -FR_W, FR_W};
-  for (int it = 0; it < iters; ++it) {
+  for (int it = 0; it < n_it; ++it) {
     const int ma = 2 * it + 1, mb = 2 * it + 2;
-    for (int c = threadIdx.x; c < FR_N; c += NT) {  // phase A
+    for (int c = threadIdx.x; c < FR_N; c += FNT) {  // phase A
       const int cx = c % FR_W, cy = c / FR_W, x = ox + cx, y = oy + cy;
       if (cx < ma || cx >= FR_W - ma || cy < ma || cy >= FR_H - ma || !inb(x, y, W, H)) continue;
       const uint32_t l = sl[0][c];
@@ -998,7 +1027,7 @@ __global__ __launch_bounds__(NT) void k_fim_fused(Batch b, int src) {
       }
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < FR_N; c += NT) {  // phase B
+    for (int c = threadIdx.x; c < FR_N; c += FNT) {  // phase B
       const int cx = c % FR_W, cy = c / FR_W, x = ox + cx, y = oy + cy;
       if (cx < mb || cx >= FR_W - mb || cy < mb || cy >= FR_H - mb || !inb(x, y, W, H)) continue;
       if (x >= cw || y >= ch) {  // outside the dispatch cover: unchanged
@@ -1035,7 +1064,7 @@ __global__ __launch_bounds__(NT) void k_fim_fused(Batch b, int src) {
     }
     __syncthreads();
   }
-  for (int t = threadIdx.x; t < FT_W * FT_H; t += NT) {
+  for (int t = threadIdx.x; t < FT_W * FT_H; t += FNT) {
     const int c = (FH + t / FT_W) * FR_W + FH + t % FT_W;
     const int x = ox + FH + t % FT_W, y = oy + FH + t / FT_W;
     if (!inb(x, y, W, H)) continue;
@@ -1591,7 +1620,7 @@ void launch_solve_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_solve_v, b, s
 void launch_extrapolate_fused(hipStream_t st, const Batch& b) {
   const int nb = b.start[b.n];
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_extrapolate_fused, dim3(nb), dim3(NT), 0, st, b);
+  hipLaunchKernelGGL(k_extrapolate_fused, dim3(nb), dim3(XNT), 0, st, b);
 }
 int extrapolate_tile_w() { return XT_W; }
 int extrapolate_tile_h() { return XT_H; }
@@ -1605,7 +1634,7 @@ void launch_fim_active(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_active
 void launch_fim_fused(hipStream_t st, const Batch& b, int src) {
   const int nb = b.start[b.n];
   if (nb <= 0) return;
-  hipLaunchKernelGGL(k_fim_fused, dim3(nb), dim3(NT), 0, st, b, src);
+  hipLaunchKernelGGL(k_fim_fused, dim3(nb), dim3(FNT), 0, st, b, src);
 }
 int fim_tile_w() { return FT_W; }
 int fim_tile_h() { return FT_H; }
