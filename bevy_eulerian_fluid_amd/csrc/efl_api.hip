@@ -179,6 +179,7 @@ Batch make_batch(const std::vector<efl_instance*>& v, Extent k, int lvl = 0, con
   }
   b.n = n;
   b.start[n] = start;
+  b.nblocks = start;
   return b;
 }
 
@@ -772,6 +773,7 @@ void stage_extrapolate(hipStream_t st, const IV& v) {
   }
   b.n = bu.n + bv.n;
   b.start[b.n] = base + bv.start[bv.n];
+  b.nblocks = b.start[b.n];
   efl::launch_extrapolate_fused(st, b);
 }
 

@@ -118,6 +118,7 @@ __device__ __forceinline__ int xcd_tile(int blk, int nb) {
 // per-instance ranges; inside a range, blocks tile the instance's extent.
 struct Batch {
   int n;
+  int nblocks;  // start[n], beside n so that one scalar load brings both
   int start[kMaxBatch + 1];
   int tiles_x[kMaxBatch];
   int ex[kMaxBatch], ey[kMaxBatch];

@@ -670,10 +670,11 @@ __global__ __launch_bounds__(NT) void k_solve_v(Batch b) {
 // initialize_v_valid.wgsl: a face is valid (1) where the open fraction of the
 // air level set averaged to its two end corners is 0) and its 10
 // extrapolation passes (extrapolate_u.wgsl / extrapolate_v.wgsl, loop
-// render_node.rs:540-608) in one launch.  A 64 x 16 tile of faces stages its faces plus a 10-face
-// halo (one per pass) in LDS, with both validity planes and the level set
-// cells the mask needs; pass p is exact on faces at least p + 1 from the
-// region border, so the tile's faces are exact after 10.  Each pass updates
+// render_node.rs:540-608) in one launch.  A 64 x 16 tile of faces (1024
+// threads) stages its faces plus a 10-face halo (one per pass) in LDS: the
+// level set cells the mask needs, the validity, and - only if its passes can
+// change anything - the velocities; pass p is exact on faces at least p + 1
+// from the region border, so the tile's faces are exact after 10.  Each pass updates
 // only faces inside the dispatch coverage [0, ex) x [0, ey), so faces in the
 // reference's coverage holes keep their values, as there.  Within a pass the
 // face values change in place: a pass writes only faces that were invalid
@@ -707,15 +708,16 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
   // level-set cells [ox - 1, ox + XR_W + 1) x [oy - 1, oy + XR_H + 1)
   for (int q = threadIdx.x; q < XL_N; q += XNT) sl[q] = ldz(I->ls0, W, H, ox - 1 + q % XL_W, oy - 1 + q / XL_W);
   __syncthreads();
+  // Validity of the region: recomputed on covered faces, staged (both
+  // planes) elsewhere.  A covered face's second plane is written by pass 0
+  // before any pass reads it, so it is not staged; velocities are staged only
+  // by tiles whose passes run.
   bool any_valid = false, any_open = false;
   for (int q = threadIdx.x; q < XR_N; q += XNT) {
     const int lx = q % XR_W, ly = q / XR_W, x = ox + lx, y = oy + ly;
-    float f = 0.0f;
     uint8_t v0 = 0, v1 = 0;
     if (inb(x, y, w, h)) {
       const size_t id = (size_t)y * w + x;
-      f = F[id];
-      v1 = (uint8_t)V1[id];
       if (x < ex && y < ey) {  // the validity kernels cover the same faces
         const int c = (ly + 1) * XL_W + lx + 1;  // cell (x, y)
         float c0, c1, c2, c3, c4, c5;
@@ -728,9 +730,9 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
         v0 = face_open(vm, vp) > 0.0f ? 0 : 1;
       } else {
         v0 = (uint8_t)V0[id];
+        v1 = (uint8_t)V1[id];
       }
     }
-    sf[q] = f;
     sv0[q] = v0;
     sv1[q] = v1;
     const bool covered = inb(x, y, w, h) && x < ex && y < ey;
@@ -757,6 +759,11 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
     }
     return;
   }
+  for (int q = threadIdx.x; q < XR_N; q += XNT) {
+    const int x = ox + q % XR_W, y = oy + q / XR_W;
+    sf[q] = inb(x, y, w, h) ? F[(size_t)y * w + x] : 0.0f;
+  }
+  __syncthreads();
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 16 threads walk the window
   for (int p = 0; p < XP; ++p) {
     const uint8_t* vin = (p & 1) ? sv1 : sv0;  // pass 0: valid0 -> valid1, then back
@@ -804,7 +811,7 @@ __global__ __launch_bounds__(XNT) void k_extrapolate_fused(Batch b) {
   __shared__ float sf[XR_N];
   __shared__ uint8_t sv[2][XR_N];
   __shared__ float sl[XL_N];
-  const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int blk = xcd_tile((int)blockIdx.x, b.nblocks);
   int i = 0;
   while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
   const int lb = blk - b.start[i];
@@ -980,7 +987,7 @@ __device__ __forceinline__ float eikonal_lds(const float* sp, int c, bool l, boo
 __global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
   __shared__ float sp[2][FR_N];
   __shared__ uint8_t sl[2][FR_N];
-  const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int blk = xcd_tile((int)blockIdx.x, b.nblocks);
   int i = 0;
   while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
   const int lb = blk - b.start[i];

@@ -235,9 +235,10 @@ struct MSite {
 };
 
 __device__ __forceinline__ MSite mg_locate(const Batch& b) {
-  const int blk = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int n = b.n, nb = b.nblocks;  // one load
+  const int blk = xcd_tile((int)blockIdx.x, nb);
   int i = 0;
-  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  while (i + 1 < n && blk >= b.start[i + 1]) ++i;
   int lb = blk - b.start[i];
   MSite s;
   s.blk = blk;
@@ -391,12 +392,10 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   __shared__ float sr[MT_W * MT_H];
   const MSite s = mg_locate(b);
   const DevInst* I = s.I;
-  if (pcg && cg_done(I)) return;
-  if (cg_check) {  // the CG stopping test of the iteration that just ran
-    double rr[1];
-    fold_partials<1>(I, RED_RR, (int)I->scal[SC_NRR], false, rr);
-    if (cg_stop(I, rr[0], s.blk == b.start[s.slot] && threadIdx.x == 0)) return;
-  }
+  // The done flag (written by an earlier launch, usually not in this XCD's
+  // L2) is read first but tested only after the tile loads are in flight, so
+  // the two latencies overlap; returning then is uniform for the block.
+  const double done_flag = I->scal[SC_DONE];
   const Level L = mg_level(I, s.lvl, pcg);
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
@@ -404,6 +403,12 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   T t;
   t.place(ox, oy, first, L.w, L.h);
   t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, sx);
+  if (pcg && done_flag != 0.0) return;
+  if (cg_check) {  // the CG stopping test of the iteration that just ran
+    double rr[1];
+    fold_partials<1>(I, RED_RR, (int)I->scal[SC_NRR], false, rr);
+    if (cg_stop(I, rr[0], s.blk == b.start[s.slot] && threadIdx.x == 0)) return;
+  }
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
   t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(I, scale), sx);
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   __shared__ double wsum[MNT / 64];
   const MSite s = mg_locate(b);
   const DevInst* I = s.I;
-  if (pcg && cg_done(I)) return;
+  const double done_flag = I->scal[SC_DONE];  // tested once the tile loads are in flight
   const Level L = mg_level(I, s.lvl, pcg);
   const Level C = I->lev[s.lvl + 1];
   const float* xin = (s.sel & 1) ? L.xs : L.x;
@@ -452,6 +457,7 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   T t;
   t.place(ox, oy, first, L.w, L.h);
   t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, sx);
+  if (pcg && done_flag != 0.0) return;
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
   t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(I, scale), sx);
