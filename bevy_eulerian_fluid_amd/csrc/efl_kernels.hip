@@ -365,15 +365,42 @@ __global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
   int32_t id_s = -1;
   // the faces' sample points lie within fd of the centre
   const float fd = fmaxf(hyp(pu.x - pc.x, pu.y - pc.y), hyp(pv.x - pc.x, pv.y - pc.y));
+  // lb = |pc - bounding centre| - radius - 1 bounds a body's level at the
+  // centre from below, and lb - fd its level at the faces.  The body with
+  // the smallest bound is evaluated first, so that `level` (the minimum,
+  // order independent) is tight before the pass in index order, which keeps
+  // the reference's rule that the last body in index order sets the id and
+  // the face velocities; a body is evaluated at the centre only if it could
+  // lower the level or set the id, at the faces only if it could reach 0.5.
+  int kb = -1;
+  float lbest = 3.0e38f;
   for (uint32_t k = 0; k < I->nobs; ++k) {
     const float4 bd = bnd[k];
     if (bd.z < 0.0f) continue;  // level 1e6 everywhere: changes nothing
     const float lb = hyp(pc.x - bd.x, pc.y - bd.y) - bd.z - 1.0f;
-    if (lb >= level && lb >= 0.5f && lb - fd >= 0.5f) continue;
+    if (lb < lbest) {
+      lbest = lb;
+      kb = (int)k;
+    }
+  }
+  float lc_best = 0.0f;
+  if (kb >= 0) {
+    lc_best = obstacle_level(I->obs[kb], pc.x, pc.y);
+    if (lc_best < level) level = lc_best;
+  }
+  for (uint32_t k = 0; k < I->nobs; ++k) {
+    const float4 bd = bnd[k];
+    if (bd.z < 0.0f) continue;
+    const float lb = hyp(pc.x - bd.x, pc.y - bd.y) - bd.z - 1.0f;
+    const bool centre = (int)k == kb || lb < level || lb < 0.5f, faces = lb - fd < 0.5f;
+    if (!centre && !faces) continue;
     const efl_obstacle& o = I->obs[k];
-    float lc = obstacle_level(o, pc.x, pc.y);
-    if (lc < level) level = lc;
-    if (lc < 0.5f) id_s = (int32_t)o.entity_id;
+    if (centre) {
+      const float lc = (int)k == kb ? lc_best : obstacle_level(o, pc.x, pc.y);
+      if (lc < level) level = lc;
+      if (lc < 0.5f) id_s = (int32_t)o.entity_id;
+    }
+    if (!faces) continue;
     if (obstacle_level(o, pu.x, pu.y) < 0.5f) {
       float ry = pu.y - o.transform[13];
       uu = o.linear_velocity[0] + (-o.angular_velocity * ry);
@@ -1113,9 +1140,14 @@ __global__ __launch_bounds__(NT) void k_sample_forces(Batch b) {
   int q[4] = {0, 0, 0, 0};
   if (s.ok) {
     int W = I->W, H = I->H, x = s.x, y = s.y;
-    float4 f = fractions_at(I->lss, W, H, x, y);
     size_t id = (size_t)y * W + x;
-    int cand = all_closed(f) ? -1 : I->sid[id];
+    // a cell without a solid id contributes nothing whatever its fractions
+    int cand = I->sid[id];
+    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (cand >= 0 && cand < EFL_MAX_SOLIDS) {
+      f = fractions_at(I->lss, W, H, x, y);
+      if (all_closed(f)) cand = -1;
+    }
     if (cand >= 0 && cand < EFL_MAX_SOLIDS) {
       float p = I->p0[id];
       float fx = ((-(f.y - f.x)) * p) * I->dx;
