@@ -234,22 +234,38 @@ struct MSite {
   int sel;       // buffer selectors (Batch::sel)
 };
 
+// Slot 0's fields are read together with the slot count, unconditionally, so
+// that a launch of one instance (the usual case) has its tile origin and
+// instance after one round of scalar loads; other slots are looked up after.
 __device__ __forceinline__ MSite mg_locate(const Batch& b) {
-  const int n = b.n, nb = b.nblocks;  // one load
-  const int blk = xcd_tile((int)blockIdx.x, nb);
-  int i = 0;
-  while (i + 1 < n && blk >= b.start[i + 1]) ++i;
-  int lb = blk - b.start[i];
+  const int n = b.n, nb = b.nblocks;
+  int st = b.start[0], tx = b.tiles_x[0];
   MSite s;
+  s.I = b.inst[0];
+  s.lvl = b.level[0];
+  s.ex = b.ex[0];
+  s.ey = b.ey[0];
+  s.sel = b.sel[0];
+  s.slot = 0;
+  // keeps the compiler from sinking these loads into the paths below
+  asm volatile("" ::"s"(st), "s"(tx), "s"(s.I), "s"(s.lvl), "s"(s.ex), "s"(s.ey), "s"(s.sel));
+  const int blk = xcd_tile((int)blockIdx.x, nb);
+  if (n > 1) {
+    int i = 0;
+    while (i + 1 < n && blk >= b.start[i + 1]) ++i;
+    st = b.start[i];
+    tx = b.tiles_x[i];
+    s.I = b.inst[i];
+    s.lvl = b.level[i];
+    s.ex = b.ex[i];
+    s.ey = b.ey[i];
+    s.sel = b.sel[i];
+    s.slot = i;
+  }
+  const int lb = blk - st;
   s.blk = blk;
-  s.I = b.inst[i];
-  s.tx0 = (lb % b.tiles_x[i]) * MT_W;
-  s.ty0 = (lb / b.tiles_x[i]) * MT_H;
-  s.lvl = b.level[i];
-  s.ex = b.ex[i];
-  s.ey = b.ey[i];
-  s.slot = i;
-  s.sel = b.sel[i];
+  s.tx0 = (lb % tx) * MT_W;
+  s.ty0 = (lb / tx) * MT_H;
   return s;
 }
 
@@ -736,7 +752,7 @@ __global__ __launch_bounds__(BOT_NT) void k_bottom(Batch b, int pcg, int symmetr
   __shared__ int lw[kMaxLevels + 1], lh[kMaxLevels + 1], loff[kMaxLevels + 1];
   const int slot = blockIdx.x;
   const DevInst* I = b.inst[slot];
-  if (pcg && cg_done(I)) return;
+  const double done_flag = I->scal[SC_DONE];  // tested once the staging loads are in flight
   const int l0 = b.level[slot];
   const int nl = I->nlev;
   if (threadIdx.x == 0) {
@@ -770,6 +786,7 @@ __global__ __launch_bounds__(BOT_NT) void k_bottom(Batch b, int pcg, int symmetr
       }
     }
   }
+  if (pcg && done_flag != 0.0) return;
   __syncthreads();
   // sweep counts ride in the batch: ex = pre, ey = post | coarsest << 16
   BotCfg g;
