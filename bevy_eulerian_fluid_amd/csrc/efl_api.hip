@@ -448,7 +448,14 @@ void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
 // unknowns, rz = r . z and beta.  Instances whose level-0 post-smoothing
 // fits one launch get the mask and the r . z reduction fused into it; the
 // others take the separate k_pcg_rz pass.
-bool check_in_smoother(const efl_instance* i) { return bottom_level(i) != 0 && i->h.nlev > 1; }
+// Instances with DevInst::prefold take k_cg_check (one block folds ||r||^2)
+// rather than have every block of the launch fold it.
+bool check_in_smoother(const efl_instance* i) { return !i->h.prefold && bottom_level(i) != 0 && i->h.nlev > 1; }
+bool prefolds(const efl_instance* i) { return i->h.prefold != 0; }
+void prefold(hipStream_t st, const IV& v, int kind) {
+  IV pf = filter(v, prefolds);
+  if (!pf.empty()) efl::launch_prefold(st, cg_batch(pf), kind);
+}
 
 // with cg_check, the first tiled launch of level 0 runs the CG stopping test
 // (instances that start in the bottom solver have run k_cg_check instead)
@@ -478,6 +485,7 @@ void precondition(hipStream_t st, const IV& v, bool cg_check = false) {
   mg_settle(st, v, 1);
   IV sep = filter(v, [&](efl_instance* i) { return std::find(fused.begin(), fused.end(), i) == fused.end(); });
   if (!sep.empty()) efl::launch_pcg_rz(st, cg_batch(sep));
+  prefold(st, v, 0);
 }
 
 int read_scalars(efl_ctx* c, const IV& v) {
@@ -593,6 +601,7 @@ void cg_begin(hipStream_t st, const IV& cg) {
 void cg_iteration(hipStream_t st, const IV& cg) {
   Batch b = cg_batch(cg);
   efl::launch_pcg_apply(st, b);  // p = z + beta p, q = A p
+  prefold(st, cg, 1);
   efl::launch_pcg_axpy(st, b);   // x, r, ||r||^2
   IV sep = filter(cg, [](efl_instance* i) { return !check_in_smoother(i); });
   if (!sep.empty()) efl::launch_cg_check(st, cg_batch(sep));
@@ -650,6 +659,7 @@ void nn_free_activations(efl::nn::Activations& a) {
 void nn_precondition(hipStream_t st, efl_instance* i, bool forced = false) {
   efl::nn::precondition(st, i->nn->pk, i->nn_act, i->d, i->s.width, i->s.height,
                         forced ? nullptr : i->h.scal + efl::SC_DONE);
+  if (!forced) prefold(st, IV{i}, 0);
 }
 
 // Flexible CG preconditioned by the network.  The host reads the residual
@@ -675,6 +685,7 @@ int proj_learned(efl_ctx* c, efl_instance* i) {
   for (;;) {
     for (uint32_t j = 0; j < n; ++j) {
       efl::launch_pcg_apply(st, full);
+      prefold(st, v, 1);
       efl::launch_pcg_axpy(st, full);
       efl::launch_cg_check(st, full);
       nn_precondition(st, i);
@@ -1282,7 +1293,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
     o_lcf[l] = take(m * 16); o_lfl[l] = take(m * 4);
   }
   size_t o_pr = take(n * 4), o_pz = take(n * 4), o_pp = take(n * 4), o_pp1 = take(n * 4), o_pq = take(n * 4);
-  size_t o_red = take((size_t)4 * nblk * 8), o_scal = take(16 * 8), o_cnt = take(16), o_diag = take(16);
+  size_t o_red = take((size_t)4 * nblk * 8), o_scal = take(efl::kScalars * 8), o_cnt = take(16), o_diag = take(16);
   size_t o_fdp = take((size_t)3 * (W / 16) * (H / 16) * 4), o_fdr = take(16);
   size_t o_src0 = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES), o_src = take(sizeof(efl_source) * EFL_MAX_FLUID_SOURCES);
   size_t o_desc = take(sizeof(DevInst));
@@ -1335,6 +1346,12 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   h.lev_pcg0.b = h.pcg_r;
   h.red = (double*)(base + o_red);
   h.red_stride = nblk;
+  {  // CG tiles (64 x 16), multigrid tiles and network blocks all cover ~1024 cells
+    const long cg_tiles = (long)((W + efl::tile_w() - 1) / efl::tile_w()) * ((H + efl::cg_tile_h() - 1) / efl::cg_tile_h());
+    const long mg_tiles = (long)((W + efl::mg_tile_w() - 1) / efl::mg_tile_w()) * ((H + efl::mg_tile_h() - 1) / efl::mg_tile_h());
+    const long nn_blocks = ((long)W * H + 1023) / 1024;
+    h.prefold = std::max(cg_tiles, std::max(mg_tiles, nn_blocks)) > efl::kInlineFold;
+  }
   h.scal = (double*)(base + o_scal);
   h.counter = (uint32_t*)(base + o_cnt);
   h.diag = (float*)(base + o_diag);

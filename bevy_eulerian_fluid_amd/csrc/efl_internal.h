@@ -78,6 +78,7 @@ struct DevInst {
   float cg_tol;
   uint32_t cg_maxc, cg_every;
   uint32_t cg_flex;    // flexible beta (learned, nonlinear preconditioner)
+  uint32_t prefold;    // more CG partials than kInlineFold: k_prefold folds them
   float* diag;         // [0] max |D| (as float bits via atomic max), ...
   // fluid diagnostics: per 16 x 16 block volume, max and min speed
   // (3 arrays of (W/16)(H/16) floats), then {volume, min, max}

@@ -1337,6 +1337,35 @@ __global__ __launch_bounds__(NT) void k_cg_check(Batch b, int init) {
   if (threadIdx.x == 0) cg_stop(I, t[0], true);
 }
 
+// One block per instance with DevInst::prefold folds the partials its CG
+// consumers would otherwise each fold for themselves (the same fold_partials,
+// so the scalar is the same either way): kind 0, after a preconditioner:
+// r . z (and z . q_prev for flexible CG) -> scal[SC_RZF..];  kind 1, after
+// k_pcg_apply: p . q over the Batch's tiles -> scal[SC_PQF].
+__global__ __launch_bounds__(NT) void k_prefold(Batch b, int kind) {
+  const int slot = blockIdx.x;
+  const DevInst* I = b.inst[slot];
+  if (!I->prefold || cg_done(I)) return;
+  if (kind == 0) {
+    double t[2] = {0.0, 0.0};
+    if (I->cg_flex) {
+      fold_partials<2>(I, RED_RZ, (int)I->scal[SC_NRZ], false, t);
+    } else {
+      double u[1];
+      fold_partials<1>(I, RED_RZ, (int)I->scal[SC_NRZ], false, u);
+      t[0] = u[0];
+    }
+    if (threadIdx.x == 0) {
+      I->scal[SC_RZF] = t[0];
+      I->scal[SC_RZF + 1] = t[1];
+    }
+  } else {
+    double t[1];
+    fold_partials<1>(I, RED_PQ, inst_blocks(b, slot), false, t);
+    if (threadIdx.x == 0) I->scal[SC_PQF] = t[0];
+  }
+}
+
 // z masked to active cells, partials of r . z (RED_RZ)
 __global__ __launch_bounds__(NT) void k_pcg_rz(Batch b) {
   const CSite s = locate_cg(b);
@@ -1368,13 +1397,19 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
   const int iter = (int)I->scal[SC_ITER] + 1;
   double rz[2] = {0.0, 0.0};
   double beta = 0.0;
-  if (I->cg_flex) {  // flexible CG: beta = -(z . q_prev) / (p . q)_prev
+  if (I->prefold) {
+    rz[0] = I->scal[SC_RZF];
+    rz[1] = I->scal[SC_RZF + 1];
+  } else if (I->cg_flex) {
     fold_partials<2>(I, RED_RZ, (int)I->scal[SC_NRZ], false, rz);
-    if (iter > 1 && I->scal[SC_PQ] != 0.0) beta = -rz[1] / I->scal[SC_PQ];
   } else {
     double t[1];
     fold_partials<1>(I, RED_RZ, (int)I->scal[SC_NRZ], false, t);
     rz[0] = t[0];
+  }
+  if (I->cg_flex) {  // flexible CG: beta = -(z . q_prev) / (p . q)_prev
+    if (iter > 1 && I->scal[SC_PQ] != 0.0) beta = -rz[1] / I->scal[SC_PQ];
+  } else {
     if (iter > 1) beta = rz[0] / I->scal[SC_RZ0 + ((iter - 2) & 1)];
   }
   if (s.lb == 0 && threadIdx.x == 0) {
@@ -1429,7 +1464,8 @@ __global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
   if (cg_done(I)) return;
   const int iter = (int)I->scal[SC_ITER] + 1;
   double t[1];
-  fold_partials<1>(I, RED_PQ, inst_blocks(b, s.slot), false, t);
+  if (I->prefold) t[0] = I->scal[SC_PQF];
+  else fold_partials<1>(I, RED_PQ, inst_blocks(b, s.slot), false, t);
   const double alpha = t[0] > 0.0 ? I->scal[SC_RZ0 + ((iter - 1) & 1)] / t[0] : 0.0;
   if (s.lb == 0 && threadIdx.x == 0) {
     I->scal[SC_PQ] = t[0];
@@ -1694,6 +1730,7 @@ void launch_pcg_rz(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_rz, b, st)
 void launch_pcg_apply(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_apply, b, st); }
 void launch_pcg_axpy(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_axpy, b, st); }
 void launch_cg_check(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_cg_check, b, st, 0); }
+void launch_prefold(hipStream_t st, const Batch& b, int kind) { EFL_LAUNCH_PER_INST(k_prefold, b, st, kind); }
 void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_mask_p, b, st); }
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
   if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);

@@ -52,6 +52,9 @@ void launch_pcg_rz(hipStream_t st, const Batch& b);
 void launch_pcg_apply(hipStream_t st, const Batch& b);
 void launch_pcg_axpy(hipStream_t st, const Batch& b);
 void launch_cg_check(hipStream_t st, const Batch& b);
+// kind 0: r . z after a preconditioner, 1: p . q after k_pcg_apply; Batch =
+// the CG tiles of the instances with DevInst::prefold
+void launch_prefold(hipStream_t st, const Batch& b, int kind);
 void launch_mask_p(hipStream_t st, const Batch& b);
 void launch_max_divergence(hipStream_t st, const Batch& b);
 void launch_fluid_diagnostics(hipStream_t st, const DevInst* dev, int W, int H);

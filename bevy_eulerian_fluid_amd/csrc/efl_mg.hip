@@ -420,7 +420,7 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   t.place(ox, oy, first, L.w, L.h);
   t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, sx);
   if (pcg && done_flag != 0.0) return;
-  if (cg_check) {  // the CG stopping test of the iteration that just ran
+  if (cg_check && !I->prefold) {  // the CG stopping test of the iteration that just ran
     double rr[1];
     fold_partials<1>(I, RED_RR, (int)I->scal[SC_NRR], false, rr);
     if (cg_stop(I, rr[0], s.blk == b.start[s.slot] && threadIdx.x == 0)) return;

@@ -31,21 +31,37 @@ constexpr int SC_ITER = 9, SC_REL = 10, SC_CONV = 11;
 constexpr int SC_RZ0 = 12;           // r . z of iteration k at SC_RZ0 + (k & 1)
 constexpr int SC_KCUR = 14;          // the iteration k_pcg_axpy just ran
 constexpr int SC_STALL = 15;         // flexible CG: relative residual at the last stall checkpoint
+constexpr int SC_RZF = 16;           // k_prefold: r . z (and z . q_prev) of the last preconditioner
+constexpr int SC_PQF = 18;           // k_prefold: p . q of the last k_pcg_apply
+constexpr int kScalars = 24;         // scal[] length (the host reads the first 16)
+
+// Partial counts up to this are folded by every block that needs the
+// scalar; an instance with more (DevInst::prefold) has them folded once by
+// k_prefold, one block per instance, and its consumers read scal[].
+constexpr int kInlineFold = 2048;
 
 // Fold the nblk partials of components c0 .. c0 + N - 1 (all threads of a
 // block of RED_NT call it; every thread gets the sums): thread t sums
 // partials t, t + 256, ... in order, a butterfly over the wavefront's lanes
-// (the same tree in every lane), then the 4 wavefront sums in order.
+// (the same tree in every lane), then the 4 wavefront sums in order.  The
+// partials are loaded FOLD_U at a time (absent ones as 0, which leaves a sum
+// or a max of non-negative values unchanged: the accumulator starts at +0
+// and never becomes -0), so one round trip covers up to FOLD_U * 256.
+constexpr int FOLD_U = 8;
 template <int N>
 __device__ __forceinline__ void fold_partials(const DevInst* I, int c0, int nblk, bool is_max, double (&out)[N]) {
   __shared__ double sh[N][RED_NT / 64];
   const int t = threadIdx.x;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
+    const double* red = I->red + (size_t)(c0 + k) * I->red_stride;
     double acc = 0.0;
-    for (int i = t; i < nblk; i += RED_NT) {
-      const double v = I->red[(size_t)(c0 + k) * I->red_stride + i];
-      acc = is_max ? fmax(acc, v) : acc + v;
+    for (int i0 = t; i0 < nblk; i0 += RED_NT * FOLD_U) {
+      double v[FOLD_U];
+#pragma unroll
+      for (int u = 0; u < FOLD_U; ++u) v[u] = i0 + u * RED_NT < nblk ? red[i0 + u * RED_NT] : 0.0;
+#pragma unroll
+      for (int u = 0; u < FOLD_U; ++u) acc = is_max ? fmax(acc, v[u]) : acc + v[u];
     }
     for (int off = 32; off > 0; off >>= 1) {
       const double o = __shfl_xor(acc, off, 64);
