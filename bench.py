@@ -171,14 +171,14 @@ def run_config(args):
         insts = [i]
         desc = {"workload": "dam break 256x256, GS 100 sweeps w=1.9, FIM(10)", "grid": [256, 256], "instances": 1}
     elif args.config == 3:
-        for sc in scenes.batch(64):
+        for sc in scenes.batch(64, proj=scenes.projection("mgpcg", max_cycles=200, rel_tol=args.rel_tol)):
             i = ctx.instance(sc["settings"])
             i.set_sources(sc["init_sources"], True)
             i.set_obstacles(sc["obstacles"], scenes.bodies_for(sc["obstacles"], sc["densities"]))
             insts.append(i)
         cells = sum(i.W * i.H for i in insts)
         desc = {"workload": "64 instances, sides log-uniform [128, 1024], 0-8 coupled bodies each, "
-                            "one multigrid V-cycle per step", "instances": 64, "cells": cells}
+                            f"MG-preconditioned CG to {args.rel_tol}", "instances": 64, "cells": cells}
     elif args.config == 5:
         sc = scenes.combined(4096)
         big = ctx.instance(sc["settings"])

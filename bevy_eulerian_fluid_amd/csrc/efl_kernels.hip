@@ -1328,8 +1328,9 @@ __global__ __launch_bounds__(NT) void k_cg_check(Batch b, int init) {
     sc[SC_ITER] = 0.0;
     const double rel = t[1] > 0.0 ? sqrt(t[0] / t[1]) : sqrt(t[0]);
     sc[SC_REL] = rel;
-    sc[SC_CONV] = 1.0;
-    sc[SC_DONE] = (I->cg_tol > 0.0f && rel <= (double)I->cg_tol) ? 1.0 : 0.0;
+    const bool ok = I->cg_tol > 0.0f && rel <= (double)I->cg_tol;
+    sc[SC_CONV] = (I->cg_tol > 0.0f && !ok) ? 0.0 : 1.0;  // until a later test records one
+    sc[SC_DONE] = ok ? 1.0 : 0.0;
     return;
   }
   double t[1];
@@ -1458,6 +1459,9 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
 
 // Second half of the same iteration: every block folds p . q itself,
 // alpha = rz / pq; x += alpha p, r -= alpha q, partials of ||r||^2 (RED_RR).
+// A breakdown (p . q not positive, e.g. a non-finite state) ends the solve
+// with the iterate and iteration count of the previous iteration; every
+// block sees the same p . q, so all of them return.
 __global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
   const CSite s = locate_cg(b);
   const DevInst* I = s.I;
@@ -1466,7 +1470,11 @@ __global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
   double t[1];
   if (I->prefold) t[0] = I->scal[SC_PQF];
   else fold_partials<1>(I, RED_PQ, inst_blocks(b, s.slot), false, t);
-  const double alpha = t[0] > 0.0 ? I->scal[SC_RZ0 + ((iter - 1) & 1)] / t[0] : 0.0;
+  if (!(t[0] > 0.0)) {  // breakdown (p . q <= 0 or not a number): stop before the update
+    if (s.lb == 0 && threadIdx.x == 0) I->scal[SC_DONE] = 1.0;
+    return;
+  }
+  const double alpha = I->scal[SC_RZ0 + ((iter - 1) & 1)] / t[0];
   if (s.lb == 0 && threadIdx.x == 0) {
     I->scal[SC_PQ] = t[0];
     I->scal[SC_ALPHA] = alpha;

@@ -224,3 +224,37 @@ def test_bodies_two_way_parity(gpu_ctx):
         gpu_ctx.set_obstacles([])
         g.close()
         o.close()
+
+
+@pytest.mark.parametrize("size", [(64, 64), (200, 136)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_mgpcg_stops_on_non_finite_state(gpu_ctx, size):
+    """A NaN in the pressure guess at an unknown makes r, and then p . q, not
+    a number: the oracle's CG loop breaks before its first update, so the GPU
+    solve must stop there too (0 iterations, not converged, pressure left as
+    it was) instead of running max_cycles iterations."""
+    _say(f"mgpcg non-finite state {size}")
+    proj = scenes.projection("mgpcg", max_cycles=50, rel_tol=1e-5)
+    sc = small_scene(*size, proj=proj)
+    g, o = make_pair(gpu_ctx, sc, OracleInstance)
+    try:
+        _warm_oracle(o, sc)
+        for st in STEP_ORDER[:STEP_ORDER.index("PROJECTION")]:
+            o.run_stage(st, DT)
+        ls, af = o.get("LS_AIR0"), o.get("AREA_FRACTION")
+        open_fluid = (ls < -2.0) & (af.sum(axis=2) == 4.0)
+        y, x = np.argwhere(open_fluid)[0]
+        p = o.get("P0")
+        p[y, x] = np.nan
+        o.set("P0", p)
+        copy_state(o, g)
+        o.run_stage("PROJECTION", DT)
+        g.run_stage("PROJECTION", DT)
+        so, sg = o.stats(), g.stats()
+        assert int(so.cycles) == 0 and so.converged == 0
+        assert int(sg.cycles) == int(so.cycles) and sg.converged == so.converged
+        assert np.isnan(sg.rel_residual)
+        errs = compare_all(g, o, ["P0"])
+        assert not errs, "\n".join(errs)
+    finally:
+        g.close()
+        o.close()
