@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -1346,11 +1347,15 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   h.lev_pcg0.b = h.pcg_r;
   h.red = (double*)(base + o_red);
   h.red_stride = nblk;
-  {  // CG tiles (64 x 16), multigrid tiles and network blocks all cover ~1024 cells
+  {  // CG tiles (64 x 16), multigrid tiles and network blocks all cover ~1024 cells.
+     // EFL_INLINE_FOLD_MAX lowers the threshold (0: every instance takes k_prefold);
+     // the scalars are the same either way.
     const long cg_tiles = (long)((W + efl::tile_w() - 1) / efl::tile_w()) * ((H + efl::cg_tile_h() - 1) / efl::cg_tile_h());
     const long mg_tiles = (long)((W + efl::mg_tile_w() - 1) / efl::mg_tile_w()) * ((H + efl::mg_tile_h() - 1) / efl::mg_tile_h());
     const long nn_blocks = ((long)W * H + 1023) / 1024;
-    h.prefold = std::max(cg_tiles, std::max(mg_tiles, nn_blocks)) > efl::kInlineFold;
+    long limit = efl::kInlineFold;
+    if (const char* e = getenv("EFL_INLINE_FOLD_MAX")) limit = std::min<long>(limit, std::max(0L, atol(e)));
+    h.prefold = std::max(cg_tiles, std::max(mg_tiles, nn_blocks)) > limit;
   }
   h.scal = (double*)(base + o_scal);
   h.counter = (uint32_t*)(base + o_cnt);
