@@ -93,6 +93,11 @@ struct efl_instance {
   // per multigrid level: 1 while x is known to be zero (not read)
   uint8_t xzero[efl::kMaxLevels + 1] = {};
   uint32_t cg_last = 0;  // CG iterations of the previous solve
+  // fused CG update (k_mg_down's SEL_AXPY): the residual buffer that holds
+  // the current r (0: pcg_r, 1: pcg_r2), and the launch that carries the
+  // update next (1: the level-0 k_mg_down, 2: the level-1 one, which runs
+  // the stopping test; 0: none)
+  uint8_t rpar = 0, fax = 0;
   // learned preconditioner (efl_set_learned): the network and this
   // instance's activations (padded to multiples of 8)
   efl_nn* nn = nullptr;
@@ -294,7 +299,15 @@ Batch mg_batch(const IV& v, Extent ext, int lvl, bool with_coarse) {
   Batch b = make_batch(v, EX_LEVEL, lvl, nullptr, efl::mg_tile_w(), efl::mg_tile_h());
   for (int n = 0; n < b.n; ++n) {
     extent_of(v[n], ext, lvl, b.ex[n], b.ey[n]);
-    b.sel[n] = v[n]->xsel[lvl] | (with_coarse ? v[n]->xsel[lvl + 1] << 1 : 0) | (v[n]->xzero[lvl] << 2);
+    efl_instance* i = v[n];
+    int sel = i->xsel[lvl] | (with_coarse ? i->xsel[lvl + 1] << 1 : 0) | (i->xzero[lvl] << 2);
+    if (lvl == 0 && i->rpar) sel |= efl::SEL_R2;
+    if (!with_coarse && i->fax == lvl + 1) {  // a down launch that carries the CG update
+      sel |= efl::SEL_AXPY;
+      if (lvl == 0) i->rpar ^= 1;  // r is written to the other buffer
+      i->fax = lvl == 0 ? 2 : 0;
+    }
+    b.sel[n] = sel;
   }
   return b;
 }
@@ -453,6 +466,16 @@ void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
 // rather than have every block of the launch fold it.
 bool check_in_smoother(const efl_instance* i) { return !i->h.prefold && bottom_level(i) != 0 && i->h.nlev > 1; }
 bool prefolds(const efl_instance* i) { return i->h.prefold != 0; }
+// the fused CG update (cg_iteration): level 0 pre-smooths in one 2-sweep
+// launch and level 1 is a tiled down level too (not the bottom or the
+// coarsest), post-smoothing in one launch (so r . z comes from k_mg_up)
+bool fused_axpy(efl_instance* i) {
+  const int b = bottom_level(i);
+  return check_in_smoother(i) && cnt_pre(i) == 2 && cnt_post(i) <= 3 && (b > 1 || (b < 0 && i->h.nlev > 2));
+}
+void reset_fused(const IV& v) {  // r is in pcg_r after k_residual_norm(pcg_init)
+  for (auto* i : v) i->rpar = i->fax = 0;
+}
 void prefold(hipStream_t st, const IV& v, int kind) {
   IV pf = filter(v, prefolds);
   if (!pf.empty()) efl::launch_prefold(st, cg_batch(pf), kind);
@@ -597,13 +620,23 @@ void cg_begin(hipStream_t st, const IV& cg) {
   Batch full = cg_batch(cg);
   efl::launch_mask_p(st, full);
   efl::launch_pcg_init(st, full);  // r = b - A p0, the initial test
+  reset_fused(cg);
   precondition(st, cg);
 }
-void cg_iteration(hipStream_t st, const IV& cg) {
+// With fuse (eager launches: the host tracks the residual buffer of each
+// iteration, which a replayed graph could not), instances whose first two
+// down launches are tiled 2-sweep launches take k_pcg_axpy's update in the
+// level-0 one and the stopping test in the level-1 one.
+void cg_iteration(hipStream_t st, const IV& cg, bool fuse = true) {
   Batch b = cg_batch(cg);
   efl::launch_pcg_apply(st, b);  // p = z + beta p, q = A p
   prefold(st, cg, 1);
-  efl::launch_pcg_axpy(st, b);   // x, r, ||r||^2
+  IV fx = fuse ? filter(cg, fused_axpy) : IV{};
+  if (fx.size() < cg.size()) {
+    IV plain = filter(cg, [&](efl_instance* i) { return std::find(fx.begin(), fx.end(), i) == fx.end(); });
+    efl::launch_pcg_axpy(st, fx.empty() ? b : cg_batch(plain));  // x, r, ||r||^2
+  }
+  for (auto* i : fx) i->fax = 1;
   IV sep = filter(cg, [](efl_instance* i) { return !check_in_smoother(i); });
   if (!sep.empty()) efl::launch_cg_check(st, cg_batch(sep));
   precondition(st, cg, true);  // the stopping test, z = M r, r . z
@@ -675,6 +708,7 @@ int proj_learned(efl_ctx* c, efl_instance* i) {
   Batch full = cg_batch(v);
   efl::launch_mask_p(st, full);
   efl::launch_pcg_init(st, full);  // r = b - A p0, ||r||, the initial test
+  reset_fused(v);
   nn_precondition(st, i);
   const uint32_t cap = max_cycles(i);
   std::vector<std::pair<uint32_t, double>> seen;  // (iteration, ||r|| / ||b||) at each read
@@ -722,6 +756,7 @@ int proj_learned(efl_ctx* c, efl_instance* i) {
   if (rc) return rc;
   if (!std::isfinite(rel)) HIPCHK(c, hipMemsetAsync(i->h.p0, 0, (size_t)i->s.width * i->s.height * 4, st));
   efl::launch_pcg_init(st, full);
+  reset_fused(v);
   precondition(st, v);
   const uint32_t keep = i->cg_last;
   rc = cg_run(c, v, nullptr);
@@ -1014,7 +1049,7 @@ int run_step_graph(efl_ctx* c, const IV& v, float dt, bool* used) {
       proj_fixed(st, fixed);
       if (!cg.empty()) cg_begin(st, cg);
     });
-    if (!cg.empty()) n.iter = capture(c, [&] { cg_iteration(st, cg); });
+    if (!cg.empty()) n.iter = capture(c, [&] { cg_iteration(st, cg, false); });
     n.post = capture(c, [&] {
       phase_velocity(st, v);
       phase_post(st, v);
@@ -1293,6 +1328,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
     }
     o_lcf[l] = take(m * 16); o_lfl[l] = take(m * 4);
   }
+  size_t o_pr2 = take(n * 4);
   size_t o_pr = take(n * 4), o_pz = take(n * 4), o_pp = take(n * 4), o_pp1 = take(n * 4), o_pq = take(n * 4);
   size_t o_red = take((size_t)4 * nblk * 8), o_scal = take(efl::kScalars * 8), o_cnt = take(16), o_diag = take(16);
   size_t o_fdp = take((size_t)3 * (W / 16) * (H / 16) * 4), o_fdr = take(16);
@@ -1342,6 +1378,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
                 (float4*)(base + o_laf[l]), (float*)(base + o_lr[l]), (float*)(base + o_lxs[l]),
                 (float4*)(base + o_lcf[l]), (int32_t*)(base + o_lfl[l])};
   h.pcg_r = (float*)(base + o_pr); h.pcg_z = (float*)(base + o_pz); h.pcg_p[0] = (float*)(base + o_pp); h.pcg_p[1] = (float*)(base + o_pp1); h.pcg_q = (float*)(base + o_pq);
+  h.pcg_r2 = (float*)(base + o_pr2);
   h.lev_pcg0 = h.lev[0];  // z over r, same level set, area fractions and stencil
   h.lev_pcg0.x = h.pcg_z;
   h.lev_pcg0.b = h.pcg_r;
@@ -1356,6 +1393,7 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
     long limit = efl::kInlineFold;
     if (const char* e = getenv("EFL_INLINE_FOLD_MAX")) limit = std::min<long>(limit, std::max(0L, atol(e)));
     h.prefold = std::max(cg_tiles, std::max(mg_tiles, nn_blocks)) > limit;
+    h.cg_tiles = (int)cg_tiles;
   }
   h.scal = (double*)(base + o_scal);
   h.counter = (uint32_t*)(base + o_cnt);

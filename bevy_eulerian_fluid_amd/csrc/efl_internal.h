@@ -69,6 +69,7 @@ struct DevInst {
   Level lev_pcg0;
   // PCG vectors (W x H) and scalars
   float *pcg_r, *pcg_z, *pcg_q;
+  float* pcg_r2;    // second residual buffer of the fused x / r update (Batch::sel SEL_R2)
   float* pcg_p[2];  // search direction of iteration k in pcg_p[k & 1]
   double* red;         // reduction partials: 4 x red_stride doubles
   int red_stride;      // >= blocks of one full-grid launch
@@ -79,6 +80,7 @@ struct DevInst {
   uint32_t cg_maxc, cg_every;
   uint32_t cg_flex;    // flexible beta (learned, nonlinear preconditioner)
   uint32_t prefold;    // more CG partials than kInlineFold: k_prefold folds them
+  int cg_tiles;        // 64 x 16 tiles of the CG kernels (partials of one of their reductions)
   float* diag;         // [0] max |D| (as float bits via atomic max), ...
   // fluid diagnostics: per 16 x 16 block volume, max and min speed
   // (3 arrays of (W/16)(H/16) floats), then {volume, min, max}
@@ -117,6 +119,14 @@ __device__ __forceinline__ int xcd_tile(int blk, int nb) {
 
 // Launch map of a batched kernel: the flat grid of blocks is split into
 // per-instance ranges; inside a range, blocks tile the instance's extent.
+// Batch::sel bits of the level-0 / level-1 CG preconditioner launches (bits
+// 0..2 are the fused multigrid's buffer selection): SEL_AXPY on the first
+// level-0 k_mg_down of an iteration: it also does k_pcg_axpy's update (r
+// read from the buffer SEL_R2 names, written to the other); on the level-1
+// k_mg_down after it: it runs the stopping test.  SEL_R2: level-0 b (the
+// residual) is in pcg_r2.
+constexpr int SEL_AXPY = 8, SEL_R2 = 16;
+
 struct Batch {
   int n;
   int nblocks;  // start[n], beside n so that one scalar load brings both

@@ -39,6 +39,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 
 #include "efl_internal.h"
@@ -301,13 +302,15 @@ struct Tile {
   }
 
   // stencils, b and x from the level (x = 0 when xzero); with C, the
-  // prolongation adds the coarse x (cxin) on fluid cells that have a parent.
+  // prolongation adds the coarse x (cxin) on fluid cells that have a parent;
+  // with qv, qo[s] = qv at the slot's cell (0 outside the level).
   // Loads are issued in the order they are consumed (x, the coarse x and
   // the flags for the LDS copy, then b and the coefficients of the colour
   // swept first, then those of the other colour), so the LDS copy and the
   // first half-sweep wait only for their own data.
   __device__ __forceinline__ void load(const Level& L, const float* xin, bool xzero, const Level* C,
-                                       const float* cxin, int ox, int oy, float* sx) {
+                                       const float* cxin, int ox, int oy, float* sx, const float* qv = nullptr,
+                                       float* qo = nullptr) {
     float xc[NSL];
     bool par[NSL];
 #pragma unroll
@@ -333,9 +336,11 @@ struct Tile {
     for (int s = 0; s < NSL; ++s) {
       b[s] = 0.0f;
       c[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (qv) qo[s] = 0.0f;
       if (inl[s]) {
         const size_t id = (size_t)(oy + ly[s]) * L.w + ox + lx[s];
         b[s] = ldg(L.b, id);
+        if (qv) qo[s] = ldg(qv, id);
         c[s] = ldg(L.cf, id);
       }
     }
@@ -400,27 +405,97 @@ __device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, 
 // one cell wider than the sweeps need so that the residual of every tile
 // cell sees exact neighbours.
 // ---------------------------------------------------------------------------
-template <int NS, int HALO>
+//
+// With SEL_AXPY at level 0 (the AX instantiation) the launch first finishes the CG iteration that
+// k_pcg_apply began (k_pcg_axpy's work): every block folds p . q, alpha =
+// rz / pq, stages b = r - alpha q (r from the buffer SEL_R2 names) over its
+// region, writes the new r of its tile to the other buffer and x += alpha p,
+// and leaves the partial of ||r||^2; the stopping test then runs in the
+// level-1 launch that follows (SEL_AXPY there).  A neighbour's halo reads the
+// old r, which no block of the launch writes, so the tiles need no order.
+template <int NS, int HALO, bool AX = false>
 __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, float weight, int restrict_,
                                                  int cg_check) {
   using T = Tile<HALO>;
   __shared__ float sx[T::RW * T::RH];
   __shared__ float sr[MT_W * MT_H];
+  __shared__ double wsum[MNT / 64];
   const MSite s = mg_locate(b);
   const DevInst* I = s.I;
   // The done flag (written by an earlier launch, usually not in this XCD's
   // L2) is read first but tested only after the tile loads are in flight, so
   // the two latencies overlap; returning then is uniform for the block.
   const double done_flag = I->scal[SC_DONE];
-  const Level L = mg_level(I, s.lvl, pcg);
+  Level L = mg_level(I, s.lvl, pcg);
+  if (s.sel & SEL_R2) L.b = I->pcg_r2;
+  const bool fax = (s.sel & SEL_AXPY) != 0;
+  const bool axpy = AX && fax && s.lvl == 0;  // AX: the instantiation that carries the update
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
   const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
   T t;
+  float qv[T::NSL];
+  constexpr int XU = (MT_W * MT_H) / MNT;  // tile cells per thread, row-major over the tile
+  float pv[XU], xv[XU];
   t.place(ox, oy, first, L.w, L.h);
-  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, sx);
+  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, sx, axpy ? I->pcg_q : nullptr, qv);
+  int iter = 0;
+  if (axpy) {  // p and x of the tile's cells (coalesced)
+    iter = (int)I->scal[SC_ITER] + 1;
+    const float* p = I->pcg_p[iter & 1];
+#pragma unroll
+    for (int k = 0; k < XU; ++k) {
+      const int u = (int)threadIdx.x + k * MNT, gx = s.tx0 + u % MT_W, gy = s.ty0 + u / MT_W;
+      pv[k] = xv[k] = 0.0f;
+      if (gx < L.w && gy < L.h) {
+        pv[k] = ldg(p, (size_t)gy * L.w + gx);
+        xv[k] = ldg(I->p0, (size_t)gy * L.w + gx);
+      }
+    }
+  }
   if (pcg && done_flag != 0.0) return;
-  if (cg_check && !I->prefold) {  // the CG stopping test of the iteration that just ran
+  if (axpy) {
+    const int tile = s.blk - b.start[s.slot];
+    double pq[1];
+    fold_partials<1>(I, RED_PQ, I->cg_tiles, false, pq);
+    if (!(pq[0] > 0.0)) {  // breakdown: stop before the update (as k_pcg_axpy)
+      if (tile == 0 && threadIdx.x == 0) I->scal[SC_DONE] = 1.0;
+      return;
+    }
+    const double alpha = I->scal[SC_RZ0 + ((iter - 1) & 1)] / pq[0];
+    const float fa = (float)alpha;
+    float* rout = (s.sel & SEL_R2) ? I->pcg_r : I->pcg_r2;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < T::NSL; ++k) {
+      if (!t.inl[k]) continue;
+      const float rv = t.b[k] - fa * qv[k];
+      t.b[k] = rv;
+      if (!t.in_tile(k)) continue;
+      stg(rout, (size_t)(oy + t.ly[k]) * L.w + ox + t.lx[k], rv);
+      acc += (double)rv * (double)rv;
+    }
+#pragma unroll
+    for (int k = 0; k < XU; ++k) {
+      const int u = (int)threadIdx.x + k * MNT, gx = s.tx0 + u % MT_W, gy = s.ty0 + u / MT_W;
+      if (gx < L.w && gy < L.h) stg(I->p0, (size_t)gy * L.w + gx, xv[k] + fa * pv[k]);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = wsum[0];
+      for (int w = 1; w < MNT / 64; ++w) tot += wsum[w];
+      I->red[(size_t)RED_RR * I->red_stride + tile] = tot;
+      if (tile == 0) {
+        I->scal[SC_PQ] = pq[0];
+        I->scal[SC_ALPHA] = alpha;
+        I->scal[SC_KCUR] = (double)iter;
+        I->scal[SC_NRR] = (double)(b.start[s.slot + 1] - b.start[s.slot]);
+      }
+    }
+  }
+  if (fax ? s.lvl == 1 : (cg_check && !I->prefold)) {  // the CG stopping test of the iteration that just ran
     double rr[1];
     fold_partials<1>(I, RED_RR, (int)I->scal[SC_NRR], false, rr);
     if (cg_stop(I, rr[0], s.blk == b.start[s.slot] && threadIdx.x == 0)) return;
@@ -465,7 +540,8 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   const MSite s = mg_locate(b);
   const DevInst* I = s.I;
   const double done_flag = I->scal[SC_DONE];  // tested once the tile loads are in flight
-  const Level L = mg_level(I, s.lvl, pcg);
+  Level L = mg_level(I, s.lvl, pcg);
+  if (s.sel & SEL_R2) L.b = I->pcg_r2;
   const Level C = I->lev[s.lvl + 1];
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
@@ -838,12 +914,21 @@ void launch_mg_setup(hipStream_t st, const Batch& b) {
 // with cg_check, the launch first runs the CG stopping test (every block)
 void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_,
                     int cg_check) {
+  bool axpy = false;  // the fused CG update needs the AX instantiation (2 sweeps only)
+  for (int n = 0; n < b.n; ++n) axpy = axpy || ((b.sel[n] & SEL_AXPY) && b.level[n] == 0);
+  if (axpy && nsweeps != 2) {  // the host marks only 2-sweep instances; anything else is a bug
+    fprintf(stderr, "efl: launch_mg_down: fused CG update with %d sweeps\n", nsweeps);
+    abort();
+  }
   int nb = b.start[b.n];
   if (nb <= 0) return;
   switch (nsweeps) {
     case 0: MG_LAUNCH((k_mg_down<0, 1>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
     case 1: MG_LAUNCH((k_mg_down<1, 3>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
-    case 2: MG_LAUNCH((k_mg_down<2, 5>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+    case 2:
+      if (axpy) MG_LAUNCH((k_mg_down<2, 5, true>), nb, st, b, pcg, first, weight, restrict_, cg_check);
+      else MG_LAUNCH((k_mg_down<2, 5>), nb, st, b, pcg, first, weight, restrict_, cg_check);
+      break;
     default: MG_LAUNCH((k_mg_down<3, 7>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
   }
 }
