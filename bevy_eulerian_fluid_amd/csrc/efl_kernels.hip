@@ -1403,9 +1403,9 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
     rz[1] = I->scal[SC_RZF + 1];
   } else if (I->cg_flex) {
     fold_partials<2>(I, RED_RZ, (int)I->scal[SC_NRZ], false, rz);
-  } else {
+  } else {  // r . z of k_mg_up or k_pcg_rz: one partial per 64 x 16 tile
     double t[1];
-    fold_partials<1>(I, RED_RZ, (int)I->scal[SC_NRZ], false, t);
+    fold_partials<1>(I, RED_RZ, I->cg_tiles, false, t);
     rz[0] = t[0];
   }
   if (I->cg_flex) {  // flexible CG: beta = -(z . q_prev) / (p . q)_prev

@@ -496,8 +496,8 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
     }
   }
   if (fax ? s.lvl == 1 : (cg_check && !I->prefold)) {  // the CG stopping test of the iteration that just ran
-    double rr[1];
-    fold_partials<1>(I, RED_RR, (int)I->scal[SC_NRR], false, rr);
+    double rr[1];  // one partial per 64 x 16 tile (k_pcg_axpy's, or this kernel's at level 0)
+    fold_partials<1>(I, RED_RR, I->cg_tiles, false, rr);
     if (cg_stop(I, rr[0], s.blk == b.start[s.slot] && threadIdx.x == 0)) return;
   }
   __syncthreads();
