@@ -384,7 +384,14 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
         if (mode == DOWN && last) {
           IV fin = filter(part, [&](efl_instance* i) { return k + 1 == chunks(i); });
           IV more = filter(part, [&](efl_instance* i) { return k + 1 != chunks(i); });
-          efl::launch_mg_down(st, mg_batch(fin, ext, lvl, false), pcg, (int)n, first, weight, 1, cg_check && k == 0);
+          // instances that carry the fused CG update get a launch of their own (the
+          // update's instantiation is slower for the others: 332 vs 222 us at 4096^2)
+          IV fin_ax = filter(fin, [&](efl_instance* i) { return lvl == 0 && i->fax == 1; });
+          IV fin_pl = filter(fin, [&](efl_instance* i) { return !(lvl == 0 && i->fax == 1); });
+          for (const IV* grp : {&fin_ax, &fin_pl})
+            if (!grp->empty())
+              efl::launch_mg_down(st, mg_batch(*grp, ext, lvl, false), pcg, (int)n, first, weight, 1,
+                                  cg_check && k == 0);
           mg_flip(fin, lvl);
           for (auto* i : fin) {  // the coarse x starts at zero (not stored)
             i->xsel[lvl + 1] = 0;
