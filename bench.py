@@ -337,7 +337,7 @@ def main():
     ap.add_argument("--rel-tol", type=float, default=1e-5)
     ap.add_argument("--diag-steps", type=int, default=10)
     ap.add_argument("--kernel-reps", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=64)
     ap.add_argument("--cpu-budget-s", type=float, default=25.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
