@@ -6,7 +6,10 @@ with host reads and synchronisation in between - every field and the solver
 statistics must agree bit for bit.  Covers the MG-preconditioned CG (a
 graph for the CG iteration replayed a data-dependent number of times), a
 fixed Gauss-Seidel solve, two-way coupled bodies, and a batch of instances
-of different sizes.
+of different sizes.  The replayed CG iteration keeps k_pcg_axpy, while eager
+launches fuse its update into the first level-0 smoother launch wherever an
+instance's first two levels are tiled (128x96, 96x80 and 128x64 here), so
+these tests also hold the fused update to the unfused one bit for bit.
 """
 import numpy as np
 import pytest
