@@ -118,6 +118,20 @@ struct efl_nn {
   uint32_t hist_cap = 0;
 };
 
+namespace efl {
+namespace {
+thread_local const char* g_launch_error = nullptr;
+}
+void set_launch_error(const char* msg) {
+  if (!g_launch_error) g_launch_error = msg;
+}
+const char* take_launch_error() {
+  const char* m = g_launch_error;
+  g_launch_error = nullptr;
+  return m;
+}
+}  // namespace efl
+
 namespace {
 
 int fail(efl_ctx* c, int code, const char* fmt, ...) {
@@ -130,6 +144,18 @@ int fail(efl_ctx* c, int code, const char* fmt, ...) {
     c->err = buf;
   }
   return code;
+}
+
+// After a batch of launches: an invariant a launcher found broken, else the
+// runtime's launch error.
+int launch_status(efl_ctx* c, const char* what) {
+  if (const char* m = efl::take_launch_error()) {
+    (void)hipGetLastError();
+    return fail(c, EFL_E_STATE, "%s: %s", what, m);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, EFL_E_HIP, "%s: %s", what, hipGetErrorString(e));
+  return EFL_OK;
 }
 
 #define HIPCHK(ctx, call)                                                                   \
@@ -294,15 +320,18 @@ enum SweepMode { PLAIN, DOWN, UP };
 
 // Batch of a tiled multigrid launch: the tiles cover the whole level (each
 // launch writes a complete x into the other buffer of the pair), `ext` only
-// bounds the updated cells; sel carries the instances' buffer state.
-Batch mg_batch(const IV& v, Extent ext, int lvl, bool with_coarse) {
+// bounds the updated cells; sel carries the instances' buffer state.  The
+// CG bits (SEL_R2, SEL_AXPY) go only to launches of the preconditioner
+// (pcg): a pressure launch's b is the divergence whatever state a finished
+// CG solve left behind.
+Batch mg_batch(const IV& v, Extent ext, int lvl, bool with_coarse, int pcg) {
   Batch b = make_batch(v, EX_LEVEL, lvl, nullptr, efl::mg_tile_w(), efl::mg_tile_h());
   for (int n = 0; n < b.n; ++n) {
     extent_of(v[n], ext, lvl, b.ex[n], b.ey[n]);
     efl_instance* i = v[n];
     int sel = i->xsel[lvl] | (with_coarse ? i->xsel[lvl + 1] << 1 : 0) | (i->xzero[lvl] << 2);
-    if (lvl == 0 && i->rpar) sel |= efl::SEL_R2;
-    if (!with_coarse && i->fax == lvl + 1) {  // a down launch that carries the CG update
+    if (pcg && lvl == 0 && i->rpar) sel |= efl::SEL_R2;
+    if (pcg && !with_coarse && i->fax == lvl + 1) {  // a down launch that carries the CG update
       sel |= efl::SEL_AXPY;
       if (lvl == 0) i->rpar ^= 1;  // r is written to the other buffer
       i->fax = lvl == 0 ? 2 : 0;
@@ -368,14 +397,14 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
         IV fuse = filter(part, [&](efl_instance* i) { return rz && chunks(i) == 1; });
         IV rest = filter(part, [&](efl_instance* i) { return !(rz && chunks(i) == 1); });
         if (!fuse.empty()) {
-          Batch bf = mg_batch(fuse, ext, lvl, true);
+          Batch bf = mg_batch(fuse, ext, lvl, true, pcg);
           efl::launch_mg_up(st, bf, pcg, (int)n, first, weight, 1);
           mg_flip(fuse, lvl);
           if (rz_done) *rz_done = fuse;
           if (rz_batch) *rz_batch = bf;
         }
         if (!rest.empty()) {
-          efl::launch_mg_up(st, mg_batch(rest, ext, lvl, true), pcg, (int)n, first, weight, 0);
+          efl::launch_mg_up(st, mg_batch(rest, ext, lvl, true, pcg), pcg, (int)n, first, weight, 0);
           mg_flip(rest, lvl);
         }
       } else {
@@ -386,11 +415,11 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
           IV more = filter(part, [&](efl_instance* i) { return k + 1 != chunks(i); });
           // instances that carry the fused CG update get a launch of their own (the
           // update's instantiation is slower for the others: 332 vs 222 us at 4096^2)
-          IV fin_ax = filter(fin, [&](efl_instance* i) { return lvl == 0 && i->fax == 1; });
-          IV fin_pl = filter(fin, [&](efl_instance* i) { return !(lvl == 0 && i->fax == 1); });
+          IV fin_ax = filter(fin, [&](efl_instance* i) { return pcg && lvl == 0 && i->fax == 1; });
+          IV fin_pl = filter(fin, [&](efl_instance* i) { return !(pcg && lvl == 0 && i->fax == 1); });
           for (const IV* grp : {&fin_ax, &fin_pl})
             if (!grp->empty())
-              efl::launch_mg_down(st, mg_batch(*grp, ext, lvl, false), pcg, (int)n, first, weight, 1,
+              efl::launch_mg_down(st, mg_batch(*grp, ext, lvl, false, pcg), pcg, (int)n, first, weight, 1,
                                   cg_check && k == 0);
           mg_flip(fin, lvl);
           for (auto* i : fin) {  // the coarse x starts at zero (not stored)
@@ -398,12 +427,12 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
             i->xzero[lvl + 1] = 1;
           }
           if (!more.empty()) {
-            efl::launch_mg_down(st, mg_batch(more, ext, lvl, false), pcg, (int)n, first, weight, 0,
+            efl::launch_mg_down(st, mg_batch(more, ext, lvl, false, pcg), pcg, (int)n, first, weight, 0,
                                 cg_check && k == 0);
             mg_flip(more, lvl);
           }
         } else {
-          efl::launch_mg_down(st, mg_batch(part, ext, lvl, false), pcg, (int)n, first, weight, 0, cg_check && k == 0);
+          efl::launch_mg_down(st, mg_batch(part, ext, lvl, false, pcg), pcg, (int)n, first, weight, 0, cg_check && k == 0);
           mg_flip(part, lvl);
         }
       }
@@ -678,6 +707,7 @@ int cg_run(efl_ctx* c, const IV& cg, hipGraphExec_t iter) {
       i->stats.converged = sc[efl::SC_CONV] != 0.0;
       i->stats.rel_residual = (float)sc[efl::SC_REL];
       i->cg_last = i->stats.cycles;
+      i->rpar = i->fax = 0;  // the next solve starts from k_pcg_init's pcg_r
     }
     if (iter) keep = keep.empty() ? keep : act;  // the graph advances the whole set
     act.swap(keep);
@@ -1573,9 +1603,7 @@ int efl_step(efl_ctx* c, efl_instance* const* insts, uint32_t n, float dt) {
       if (rc) return rc;
     }
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(c, EFL_E_HIP, "kernel launch: %s", hipGetErrorString(e));
-  return EFL_OK;
+  return launch_status(c, "kernel launch");
 }
 
 int efl_run_stage(efl_instance* in, int stage, float dt) {
@@ -1619,14 +1647,14 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
     case EFL_STAGE_FLUID_TO_SOLID: stage_fluid_to_solid(st, v); break;
     case EFL_STAGE_GS_SWEEP: {  // one red + black sweep, weight 1.9, gauss_seidel.rs:142-147
       mg_setup(st, v, false);
-      efl::launch_mg_down(st, mg_batch(v, EX_CENTER, 0, false), 0, 1, 1, 1.9f, 0);
+      efl::launch_mg_down(st, mg_batch(v, EX_CENTER, 0, false, 0), 0, 1, 1, 1.9f, 0);
       mg_flip(v, 0);
       mg_settle(st, v, 0);
       break;
     }
     case EFL_STAGE_MG_SMOOTH: {  // two red-black sweeps at weight 1 on level 0 in one tiled launch
       mg_setup(st, v, false);
-      efl::launch_mg_down(st, mg_batch(v, EX_LEVEL, 0, false), 0, 2, 1, 1.0f, 0);
+      efl::launch_mg_down(st, mg_batch(v, EX_LEVEL, 0, false, 0), 0, 2, 1, 1.0f, 0);
       mg_flip(v, 0);
       mg_settle(st, v, 0);
       break;
@@ -1644,9 +1672,7 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
     default: return fail(c, EFL_E_INVALID, "unknown stage %d", stage);
   }
   if (rc) return rc;
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(c, EFL_E_HIP, "kernel launch: %s", hipGetErrorString(e));
-  return EFL_OK;
+  return launch_status(c, "kernel launch");
 }
 
 int efl_time_stage(efl_instance* in, int stage, float dt, uint32_t reps, float* ms) {
@@ -1673,7 +1699,7 @@ int efl_time_stage(efl_instance* in, int stage, float dt, uint32_t reps, float* 
     hipStream_t st = c->stream;
     IV v{in};
     const bool gs = stage == EFL_STAGE_GS_SWEEP;
-    Batch b = mg_batch(v, gs ? EX_CENTER : EX_LEVEL, 0, false);
+    Batch b = mg_batch(v, gs ? EX_CENTER : EX_LEVEL, 0, false, 0);
     HIPCHK(c, hipEventRecord(c->ev[0], st));
     for (uint32_t k = 0; k < reps; ++k) {
       b.sel[0] = in->xsel[0];
@@ -2124,9 +2150,7 @@ int efl_nn_train(efl_nn* nn, uint32_t iterations, uint32_t batch, float lr, uint
     HIPCHK(c, hipMemcpy(hist.data(), nn->d_hist, (size_t)iterations * 8, hipMemcpyDeviceToHost));
     for (uint32_t it = 0; it < iterations; ++it) loss_history[it] = (float)hist[it];
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(c, EFL_E_HIP, "training: %s", hipGetErrorString(e));
-  return EFL_OK;
+  return launch_status(c, "training");
 }
 
 int efl_nn_eval(efl_nn* nn, uint32_t first, uint32_t count, float* loss, float* grads, float* out) {
@@ -2155,6 +2179,8 @@ int efl_nn_eval(efl_nn* nn, uint32_t first, uint32_t count, float* loss, float* 
     if (e != hipSuccess) return fail(c, EFL_E_HIP, "outputs: %s", hipGetErrorString(e));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc = launch_status(c, "evaluation");
+  if (rc) return rc;
   if (loss) *loss = (float)l;
   return EFL_OK;
 }

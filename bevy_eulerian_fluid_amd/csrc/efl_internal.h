@@ -16,6 +16,11 @@
 namespace efl {
 
 constexpr int kMaxLevels = 14;
+// Reduction tile: kernels that leave one partial per tile (the CG kernels,
+// and the level-0 multigrid launches that fuse r . z or ||r||^2) all tile the
+// grid this way, so a fold over DevInst::cg_tiles partials reads exactly
+// what any of them wrote.  Each side checks it with a static_assert.
+constexpr int kRedTileW = 64, kRedTileH = 16;
 constexpr int kMaxBatch = EFL_MAX_BATCH;
 
 struct Level {  // one multigrid level (multi_grid.rs:99-118)

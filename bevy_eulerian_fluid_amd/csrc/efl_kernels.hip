@@ -1246,6 +1246,7 @@ __device__ __forceinline__ int local_block(const Batch& b, int slot) { return (i
 // The CG kernels work on 64 x 16 tiles (one partial per tile): thread t
 // takes column t % 64 of rows t / 64 + 4 r, r = 0..3.
 constexpr int CG_TH = 16, CG_ROWS = CG_TH / TY;
+static_assert(TX == kRedTileW && CG_TH == kRedTileH, "CG partials are counted in DevInst::cg_tiles");
 struct CSite {
   const DevInst* I;
   int x, y0, ex, ey, slot, lb;

@@ -7,6 +7,12 @@
 
 namespace efl {
 
+// A launcher that finds an internal invariant broken skips its launch and
+// records why; the entry point that queued it returns EFL_E_STATE with the
+// message (nothing aborts across the C ABI).  One per host thread.
+void set_launch_error(const char* msg);
+const char* take_launch_error();  // nullptr when none; clears it
+
 int tile_w();
 int tile_h();
 int cg_tile_h();

@@ -52,6 +52,8 @@ constexpr int MT_W = 64;   // tile width (one wavefront per tile row)
 constexpr int MT_H = 16;   // tile height
 constexpr int MNT = 256;   // threads per block
 static_assert(MNT == RED_NT, "the r . z reduction uses RED_NT threads per block");
+static_assert(MT_W == kRedTileW && MT_H == kRedTileH,
+              "k_mg_up / k_mg_down leave one r . z / ||r||^2 partial per tile; folds count DevInst::cg_tiles");
 
 // Level::fl bits (efl_internal.h)
 constexpr int F_IN0 = MG_IN0, F_FL0 = MG_FL0, F_GSZ = MG_GSZ, F_RZ = MG_RZ, F_LSNEG = MG_LSNEG, F_CLOSED = MG_CLOSED;
@@ -427,8 +429,8 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   // the two latencies overlap; returning then is uniform for the block.
   const double done_flag = I->scal[SC_DONE];
   Level L = mg_level(I, s.lvl, pcg);
-  if (s.sel & SEL_R2) L.b = I->pcg_r2;
-  const bool fax = (s.sel & SEL_AXPY) != 0;
+  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = I->pcg_r2;  // CG bits count in preconditioner launches only
+  const bool fax = pcg && (s.sel & SEL_AXPY) != 0;
   const bool axpy = AX && fax && s.lvl == 0;  // AX: the instantiation that carries the update
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
@@ -541,7 +543,7 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   const DevInst* I = s.I;
   const double done_flag = I->scal[SC_DONE];  // tested once the tile loads are in flight
   Level L = mg_level(I, s.lvl, pcg);
-  if (s.sel & SEL_R2) L.b = I->pcg_r2;
+  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = I->pcg_r2;
   const Level C = I->lev[s.lvl + 1];
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
@@ -917,8 +919,8 @@ void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int fi
   bool axpy = false;  // the fused CG update needs the AX instantiation (2 sweeps only)
   for (int n = 0; n < b.n; ++n) axpy = axpy || ((b.sel[n] & SEL_AXPY) && b.level[n] == 0);
   if (axpy && nsweeps != 2) {  // the host marks only 2-sweep instances; anything else is a bug
-    fprintf(stderr, "efl: launch_mg_down: fused CG update with %d sweeps\n", nsweeps);
-    abort();
+    set_launch_error("launch_mg_down: fused CG update on a launch without 2 sweeps");
+    return;
   }
   int nb = b.start[b.n];
   if (nb <= 0) return;

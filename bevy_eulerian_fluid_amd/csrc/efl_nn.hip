@@ -24,6 +24,7 @@
 #include <cstring>
 #include <vector>
 
+#include "efl_launch.h"
 #include "efl_reduce.h"
 
 namespace efl {
@@ -594,7 +595,7 @@ void conv(hipStream_t st, int ca, int cb, int cout, bool upa, bool relu, const C
   EFL_NN_CONV(64, 0, 128, false, false)
   EFL_NN_CONV(32, 0, 64, false, false)
 #undef EFL_NN_CONV
-  std::abort();  // a layer table / instantiation mismatch is a programming error
+  set_launch_error("nn conv: layer shape without an instantiation");  // a layer table mismatch
 }
 
 template <int CA, int CB, int COUT, bool UPA>
@@ -617,7 +618,7 @@ void wgrad(hipStream_t st, const LayerDef& L, const WgradArgs& g, int regions, i
   EFL_NN_WG(32, 0, 16, false)
   EFL_NN_WG(16, 16, 16, true)
 #undef EFL_NN_WG
-  std::abort();
+  set_launch_error("nn wgrad: layer shape without an instantiation");
 }
 
 void pool(hipStream_t st, int c, const uint16_t* in, uint16_t* out, int W, int H, long sin, long sout, int n,

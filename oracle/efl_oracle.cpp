@@ -1683,6 +1683,64 @@ int orc_fluid_diagnostics(const orc_inst* o, float* out) {
   return 0;
 }
 
+namespace {
+Tex<float>* mg_tex(orc_inst* o, int level, int which) {
+  if (level < 0 || level >= o->nlev) return nullptr;
+  if (level == 0) {
+    switch (which) {
+      case 0: return &o->p0;
+      case 1: return &o->div;
+      case 2: return &o->ls0;
+      case 3: return &o->afs;
+      case 4: return &o->mr[0];
+      default: return nullptr;
+    }
+  }
+  switch (which) {
+    case 0: return &o->mx[level];
+    case 1: return &o->mb[level];
+    case 2: return &o->mls[level];
+    case 3: return &o->mafs[level];
+    case 4: return &o->mr[level];
+    default: return nullptr;
+  }
+}
+}  // namespace
+
+int orc_mg_level_size(const orc_inst* o, int level, uint32_t* w, uint32_t* h) {
+  if (level < 0 || level >= o->nlev) return -1;
+  *w = (uint32_t)(level == 0 ? o->W : o->mx[level].w);
+  *h = (uint32_t)(level == 0 ? o->H : o->mx[level].h);
+  return 0;
+}
+
+int orc_mg_level(orc_inst* o, int level, int which, float* data, size_t count, int set) {
+  Tex<float>* t = mg_tex(o, level, which);
+  if (!t || count != t->d.size()) return -1;
+  if (set) std::copy(data, data + count, t->d.begin());
+  else std::copy(t->d.begin(), t->d.end(), data);
+  return 0;
+}
+
+int orc_mg_restrict(orc_inst* o, int level) {
+  if (level < 0 || level + 1 >= o->nlev) return -1;
+  Hier<float> Hh = float_hier(o, o->p0.d.data(), o->div.d.data(), o->mr[0].d.data());
+  Lvl<float> L{Hh.w[level], Hh.h[level], Hh.x[level], Hh.b[level], Hh.ls[level], Hh.af[level], Hh.r[level],
+               (float)(1 << level)};
+  restrict_level(L, Hh.w[level + 1], Hh.h[level + 1], Hh.bmut[level + 1], Hh.ls[level + 1], Hh.af[level + 1],
+                 Hh.x[level + 1]);
+  return 0;
+}
+
+int orc_mg_prolong(orc_inst* o, int level) {
+  if (level < 0 || level + 1 >= o->nlev) return -1;
+  Hier<float> Hh = float_hier(o, o->p0.d.data(), o->div.d.data(), o->mr[0].d.data());
+  Lvl<float> L{Hh.w[level], Hh.h[level], Hh.x[level], Hh.b[level], Hh.ls[level], Hh.af[level], Hh.r[level],
+               (float)(1 << level)};
+  prolong_level(L, Hh.w[level + 1], Hh.h[level + 1], Hh.x[level + 1]);
+  return 0;
+}
+
 int orc_residual_of(orc_inst* o, float dt, const float* x, float* r_out) {
   o->uni.dt = dt;
   Lvl<float> L{o->W, o->H, const_cast<float*>(x), o->div.d.data(), o->ls0.d.data(), o->afs.d.data(), r_out, 1.0f};

@@ -56,6 +56,15 @@ int orc_residual_of(orc_inst* o, float dt, const float* x, float* r_out);
 int orc_fluid_diagnostics(const orc_inst* o, float* out);
 int orc_set_threads(int n);
 
+/* Multigrid pyramid access for the pinning tests.  which: 0 x, 1 b, 2 level
+ * set, 3 area fractions (4 per cell), 4 residual; level 0 is p0 / div /
+ * levelset_air0 / area fractions / r0.  set = 1 copies `data` in, 0 out. */
+int orc_mg_level_size(const orc_inst* o, int level, uint32_t* w, uint32_t* h);
+int orc_mg_level(orc_inst* o, int level, int which, float* data, size_t count, int set);
+/* restriction.wgsl on level -> level + 1; prolongation.wgsl level + 1 -> level */
+int orc_mg_restrict(orc_inst* o, int level);
+int orc_mg_prolong(orc_inst* o, int level);
+
 #ifdef __cplusplus
 }
 #endif

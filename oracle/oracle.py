@@ -63,6 +63,10 @@ def lib():
             ("orc_set_bodies", [P, C.c_void_p, C.c_uint32]),
             ("orc_get_obstacles", [P, C.c_void_p, C.c_uint32]),
             ("orc_fluid_diagnostics", [P, C.c_void_p]),
+            ("orc_mg_level_size", [P, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+            ("orc_mg_level", [P, C.c_int, C.c_int, C.c_void_p, C.c_size_t, C.c_int]),
+            ("orc_mg_restrict", [P, C.c_int]),
+            ("orc_mg_prolong", [P, C.c_int]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -207,6 +211,31 @@ class OracleInstance:
         _check(lib().orc_solve_fp64(self.h, dt, rel_tol, out.ctypes.data, C.byref(rel), C.byref(it)),
                "solve_fp64")
         return out.reshape(self.H, self.W), rel.value, it.value
+
+    # multigrid pyramid (pinning tests): which = "x", "b", "ls", "af", "r"
+    _MG = {"x": 0, "b": 1, "ls": 2, "af": 3, "r": 4}
+
+    def mg_size(self, level):
+        w, h = C.c_uint32(), C.c_uint32()
+        _check(lib().orc_mg_level_size(self.h, level, C.byref(w), C.byref(h)), "mg_level_size")
+        return w.value, h.value
+
+    def mg_get(self, level, which):
+        w, h = self.mg_size(level)
+        ch = 4 if which == "af" else 1
+        out = np.zeros(w * h * ch, np.float32)
+        _check(lib().orc_mg_level(self.h, level, self._MG[which], out.ctypes.data, out.size, 0), "mg_level")
+        return out.reshape((h, w, 4) if ch == 4 else (h, w))
+
+    def mg_set(self, level, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32).reshape(-1)
+        _check(lib().orc_mg_level(self.h, level, self._MG[which], a.ctypes.data, a.size, 1), "mg_level")
+
+    def mg_restrict(self, level):
+        _check(lib().orc_mg_restrict(self.h, level), "mg_restrict")
+
+    def mg_prolong(self, level):
+        _check(lib().orc_mg_prolong(self.h, level), "mg_prolong")
 
     def residual_of(self, dt, x):
         a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)

@@ -258,3 +258,38 @@ def test_mgpcg_stops_on_non_finite_state(gpu_ctx, size):
     finally:
         g.close()
         o.close()
+
+
+@pytest.mark.parametrize("after", ["gauss_seidel", "multigrid", "mg_smooth", "gs_sweep"])
+@pytest.mark.parametrize("size", [(128, 128), (200, 136)], ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("cg_cycles", [1, 3])
+def test_projection_after_mgpcg_switch(gpu_ctx, size, after, cg_cycles):
+    """An MGPCG solve whose fused x / r update ran an odd number of times
+    leaves the CG residual in its second buffer.  A later Gauss-Seidel or
+    multigrid solve of the same instance must still smooth against the
+    divergence: switching away from MGPCG is bit-exact with the oracle."""
+    _say(f"projection after mgpcg ({cg_cycles} cycles) -> {after} {size}")
+    cg = scenes.projection("mgpcg", max_cycles=cg_cycles, rel_tol=1e-9)
+    sc = small_scene(*size, proj=cg)
+    g, o = make_pair(gpu_ctx, sc, OracleInstance)
+    try:
+        _warm_oracle(o, sc)
+        for st in STEP_ORDER[:STEP_ORDER.index("PROJECTION")]:
+            o.run_stage(st, DT)
+        copy_state(o, g)
+        g.run_stage("PROJECTION", DT)  # max_cycles odd iterations, each carrying the fused update
+        assert int(g.stats().cycles) == cg_cycles
+        stage = {"mg_smooth": "MG_SMOOTH", "gs_sweep": "GS_SWEEP"}.get(after, "PROJECTION")
+        proj = {"gauss_seidel": scenes.projection("gauss_seidel", gs_iterations=30),
+                "multigrid": scenes.projection("multigrid", max_cycles=2)}.get(after)
+        if proj is not None:
+            g.set_projection(proj)
+            o.set_projection(proj)
+        copy_state(o, g)
+        o.run_stage(stage, DT)
+        g.run_stage(stage, DT)
+        errs = compare_all(g, o, NO_R0)
+        assert not errs, "\n".join(errs)
+    finally:
+        g.close()
+        o.close()
