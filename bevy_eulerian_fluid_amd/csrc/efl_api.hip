@@ -807,6 +807,93 @@ int proj_learned(efl_ctx* c, efl_instance* i) {
   return EFL_OK;
 }
 
+// A fluid region with no Dirichlet term (no open face to air, to a fully
+// solid fluid cell, or out of the grid) makes the pressure system singular,
+// and inconsistent unless its divergence sums to zero; fluid trapped between
+// bodies that close on each other breaks that, and CG then diverges.  After
+// a solve that did not converge, the host labels the active cells (linked
+// across open faces) of the instance, subtracts the mean divergence of every
+// such sealed component (double sum in raster order, one float subtraction
+// per cell) and returns the number of components changed.  A rare path: it
+// reads the level set, area fractions and divergence back.
+int repair_sealed(efl_ctx* c, efl_instance* i, int* changed) {
+  *changed = 0;
+  const int W = (int)i->s.width, H = (int)i->s.height;
+  const size_t n = (size_t)W * H;
+  std::vector<float> ls(n), div(n), af(4 * n);
+  hipStream_t st = c->stream;
+  HIPCHK(c, hipMemcpyAsync(ls.data(), i->h.ls0, n * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(div.data(), i->h.div, n * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(af.data(), i->h.afs, n * 16, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  auto active = [&](size_t k) {
+    return ls[k] < 0.0f && !(af[4 * k] == 0.0f && af[4 * k + 1] == 0.0f && af[4 * k + 2] == 0.0f && af[4 * k + 3] == 0.0f);
+  };
+  std::vector<uint8_t> seen(n, 0);
+  std::vector<size_t> todo, cells;
+  const int dxs[4] = {-1, 1, 0, 0}, dys[4] = {0, 0, -1, 1};
+  for (size_t s0 = 0; s0 < n; ++s0) {
+    if (seen[s0] || !active(s0)) continue;
+    bool sealed = true;
+    cells.clear();
+    todo.assign(1, s0);
+    seen[s0] = 1;
+    while (!todo.empty()) {
+      const size_t k = todo.back();
+      todo.pop_back();
+      cells.push_back(k);
+      const int x = (int)(k % W), y = (int)(k / W);
+      for (int f = 0; f < 4; ++f) {
+        if (!(af[4 * k + f] > 0.0f)) continue;
+        const int jx = x + dxs[f], jy = y + dys[f];
+        if (jx < 0 || jy < 0 || jx >= W || jy >= H) continue;
+        const size_t j = (size_t)jy * W + jx;
+        if (!active(j)) {
+          sealed = false;
+          continue;
+        }
+        if (!seen[j]) {
+          seen[j] = 1;
+          todo.push_back(j);
+        }
+      }
+    }
+    if (!sealed) continue;
+    std::sort(cells.begin(), cells.end());
+    double sum = 0.0;
+    for (size_t k : cells) sum += (double)div[k];
+    const float mean = (float)(sum / (double)cells.size());
+    if (mean == 0.0f) continue;
+    for (size_t k : cells) div[k] = div[k] - mean;
+    ++*changed;
+  }
+  if (*changed) HIPCHK(c, hipMemcpyAsync(i->h.div, div.data(), n * 4, hipMemcpyHostToDevice, st));
+  return EFL_OK;
+}
+
+// MGPCG instances whose solve did not converge: repair sealed regions and
+// solve again from zero (the iteration counts add up).
+int resolve_sealed(efl_ctx* c, const IV& cg) {
+  IV again;
+  std::vector<uint32_t> first;
+  for (auto* i : cg) {
+    if (i->stats.converged) continue;
+    int changed = 0;
+    int rc = repair_sealed(c, i, &changed);
+    if (rc) return rc;
+    if (!changed) continue;
+    HIPCHK(c, hipMemsetAsync(i->h.p0, 0, (size_t)i->s.width * i->s.height * 4, c->stream));
+    again.push_back(i);
+    first.push_back(i->stats.cycles);
+  }
+  if (again.empty()) return EFL_OK;
+  cg_begin(c->stream, again);
+  int rc = cg_run(c, again, nullptr);
+  if (rc) return rc;
+  for (size_t k = 0; k < again.size(); ++k) again[k]->stats.cycles += first[k];
+  return EFL_OK;
+}
+
 int stage_projection(efl_ctx* c, const IV& v) {
   hipStream_t st = c->stream;
   IV fixed = filter(v, [](efl_instance* i) { return !is_cg(i) && !mg_needs_host(i) && !is_learned(i); });
@@ -821,6 +908,8 @@ int stage_projection(efl_ctx* c, const IV& v) {
   if (!cg.empty()) {
     cg_begin(st, cg);
     int rc = cg_run(c, cg, nullptr);
+    if (rc) return rc;
+    rc = resolve_sealed(c, cg);
     if (rc) return rc;
   }
   for (auto* i : v)
@@ -1122,6 +1211,8 @@ int run_step_graph(efl_ctx* c, const IV& v, float dt, bool* used) {
   HIPCHK(c, hipGraphLaunch(g->pre, st));
   if (!cg.empty()) {
     int rc = cg_run(c, cg, g->iter);
+    if (rc) return rc;
+    rc = resolve_sealed(c, cg);  // eager launches on this rare path
     if (rc) return rc;
   }
   HIPCHK(c, hipGraphLaunch(g->post, st));
@@ -1787,7 +1878,8 @@ int efl_get_fluid_diagnostics(efl_instance* in, efl_fluid_diagnostics* out) {
   HIPCHK(c, hipSetDevice(c->device));
   int rc = upload(in);
   if (rc) return rc;
-  efl::launch_fluid_diagnostics(c->stream, in->d, in->h.W, in->h.H);
+  efl::FluidDiagArgs da{in->h.ls0, in->h.u0, in->h.v0, in->h.W, in->h.H, in->h.fdp, in->h.fdr};
+  efl::launch_fluid_diagnostics(c->stream, da);
   HIPCHK(c, hipGetLastError());
   float r[4];
   HIPCHK(c, hipMemcpyAsync(r, in->h.fdr, sizeof(r), hipMemcpyDeviceToHost, c->stream));

@@ -93,6 +93,17 @@ struct DevInst {
   float* fdr;
 };
 
+// Fluid diagnostics (volume, min / max cell-centred speed) of one grid:
+// the fields, the per-16x16-block partials (3 (W/16)(H/16) floats) and the
+// 3 results, passed by value so that an instance's buffers or a caller's
+// fields with the library's static scratch can be used alike.
+struct FluidDiagArgs {
+  const float *ls, *u, *v;
+  int W, H;
+  float* part;
+  float* out;
+};
+
 // Arguments of the rigid-body integrator (one launch per obstacle list).
 struct BodyStep {
   efl_obstacle* obs;

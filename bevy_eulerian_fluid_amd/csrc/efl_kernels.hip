@@ -1419,7 +1419,9 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
     I->scal[SC_BETA] = beta;
   }
   const float fb = (float)beta;
-  const float* pold = I->pcg_p[(iter - 1) & 1];
+  // the first iteration takes p = z without reading the previous direction,
+  // which a solve that diverged before may have left non-finite (0 * inf)
+  const float* pold = iter > 1 ? I->pcg_p[(iter - 1) & 1] : nullptr;
   float* pnew = I->pcg_p[iter & 1];
   const float* z = I->pcg_z;
   const Level& L = I->lev[0];
@@ -1429,7 +1431,7 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
     const int y = s.y0 + TY * r;
     if (s.x >= s.ex || y >= s.ey) continue;
     const size_t id = (size_t)y * L.w + s.x;
-    const float pc = z[id] + fb * pold[id];
+    const float pc = pold ? z[id] + fb * pold[id] : z[id];
     pnew[id] = pc;
     // residual.wgsl's operator from the precomputed stencil (k_mg_setup):
     // fluid neighbours c_k (p - p_k), air neighbours c_k p, times factor
@@ -1445,7 +1447,7 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
         if (!((fl >> k) & 1)) continue;
         if ((fl >> (4 + k)) & 1) {
           const size_t jd = (size_t)((long)id + off[k]);
-          res = res - cv[k] * (pc - (z[jd] + fb * pold[jd])) * factor;
+          res = res - cv[k] * (pc - (pold ? z[jd] + fb * pold[jd] : z[jd])) * factor;
         } else {
           res = res - cv[k] * pc * factor;
         }
@@ -1544,12 +1546,12 @@ __global__ __launch_bounds__(NT) void k_max_divergence(Batch b) {
 // block values l, l + 512, ... in order (the speed passes add their block
 // maxima and minima too), then the lanes fold by halving strides 256..1.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fluid_diag_blocks(const DevInst* I) {
-  const int W = I->W, gx = W / 16;
+__global__ __launch_bounds__(256) void k_fluid_diag_blocks(FluidDiagArgs a) {
+  const int W = a.W, gx = W / 16;
   const int x = (int)blockIdx.x * 16 + (int)threadIdx.x % 16, y = (int)blockIdx.y * 16 + (int)threadIdx.x / 16;
-  const float level = I->ls0[(size_t)y * W + x];
-  const float uc = 0.5f * (I->u0[(size_t)y * (W + 1) + x] + I->u0[(size_t)y * (W + 1) + x + 1]);
-  const float vc = 0.5f * (I->v0[(size_t)y * W + x] + I->v0[(size_t)(y + 1) * W + x]);
+  const float level = a.ls[(size_t)y * W + x];
+  const float uc = 0.5f * (a.u[(size_t)y * (W + 1) + x] + a.u[(size_t)y * (W + 1) + x + 1]);
+  const float vc = 0.5f * (a.v[(size_t)y * W + x] + a.v[(size_t)(y + 1) * W + x]);
   const float mag = sqrtf(uc * uc + vc * vc);
   float mx = mag, mn = level < 0.0f ? mag : 1e30f;
   const int cnt = __syncthreads_count(level < 0.0f);
@@ -1564,19 +1566,19 @@ __global__ __launch_bounds__(256) void k_fluid_diag_blocks(const DevInst* I) {
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  const size_t np = (size_t)gx * (I->H / 16), wid = blockIdx.x + (size_t)blockIdx.y * gx;
-  I->fdp[wid] = (float)cnt;
-  I->fdp[np + wid] = fmaxf(fmaxf(wmx[0], wmx[1]), fmaxf(wmx[2], wmx[3]));
-  I->fdp[2 * np + wid] = fminf(fminf(wmn[0], wmn[1]), fminf(wmn[2], wmn[3]));
+  const size_t np = (size_t)gx * (a.H / 16), wid = blockIdx.x + (size_t)blockIdx.y * gx;
+  a.part[wid] = (float)cnt;
+  a.part[np + wid] = fmaxf(fmaxf(wmx[0], wmx[1]), fmaxf(wmx[2], wmx[3]));
+  a.part[2 * np + wid] = fminf(fminf(wmn[0], wmn[1]), fminf(wmn[2], wmn[3]));
 }
-__global__ __launch_bounds__(512) void k_fluid_diag_final(const DevInst* I) {
-  const int np = (I->W / 16) * (I->H / 16), l = (int)threadIdx.x;
+__global__ __launch_bounds__(512) void k_fluid_diag_final(FluidDiagArgs a) {
+  const int np = (a.W / 16) * (a.H / 16), l = (int)threadIdx.x;
   __shared__ float tv[512], tx[512], tn[512];
   float sv = 0.0f, sx = 0.0f, sn = 0.0f;
   for (int i = l; i < np; i += 512) {
-    sv += I->fdp[i];
-    sx += I->fdp[(size_t)np + i];
-    sn += I->fdp[2 * (size_t)np + i];
+    sv += a.part[i];
+    sx += a.part[(size_t)np + i];
+    sn += a.part[2 * (size_t)np + i];
   }
   tv[l] = sv;
   tx[l] = sx;
@@ -1591,9 +1593,9 @@ __global__ __launch_bounds__(512) void k_fluid_diag_final(const DevInst* I) {
     __syncthreads();
   }
   if (l == 0) {
-    I->fdr[0] = tv[0];
-    I->fdr[1] = tn[0];
-    I->fdr[2] = tx[0];
+    a.out[0] = tv[0];
+    a.out[1] = tn[0];
+    a.out[2] = tx[0];
   }
 }
 
@@ -1645,10 +1647,21 @@ __global__ void k_integrate_bodies(BodyStep a) {
   float vx = o.linear_velocity[0] + ax * a.dt;
   float vy = o.linear_velocity[1] + ay * a.dt;
   float w = o.angular_velocity + al * a.dt;
+  // a body moves at most one cell per step (translation, and rotation at its
+  // bounding radius): the area fractions and solid face velocities of the
+  // next step assume it, and faster bodies trap and squeeze fluid
+  float R = body_radius(o);
+  const float vmax = 1.0f / a.dt;
+  const float sp = sqrtf(vx * vx + vy * vy);
+  if (sp > vmax) {
+    const float sc = vmax / sp;
+    vx = vx * sc;
+    vy = vy * sc;
+  }
+  if (R > 0.0f && fabsf(w) * R > vmax) w = copysignf(vmax / R, w);
   float tx = o.transform[12] + vx * a.dt;
   float ty = o.transform[13] + vy * a.dt;
   float ang = atan2f(o.transform[1], o.transform[0]) + w * a.dt;
-  float R = body_radius(o);
   float lx = fmaxf(a.half_w - 1.0f - R, 0.0f), ly = fmaxf(a.half_h - 1.0f - R, 0.0f);
   if (tx > lx) { tx = lx; vx = fminf(vx, 0.0f); }
   if (tx < -lx) { tx = -lx; vx = fmaxf(vx, 0.0f); }
@@ -1744,9 +1757,9 @@ void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_mask_p, b, st)
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
   if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);
 }
-void launch_fluid_diagnostics(hipStream_t st, const DevInst* dev, int W, int H) {
-  if (W >= 16 && H >= 16) hipLaunchKernelGGL(k_fluid_diag_blocks, dim3(W / 16, H / 16), dim3(256), 0, st, dev);
-  hipLaunchKernelGGL(k_fluid_diag_final, dim3(1), dim3(512), 0, st, dev);
+void launch_fluid_diagnostics(hipStream_t st, const FluidDiagArgs& a) {
+  if (a.W >= 16 && a.H >= 16) hipLaunchKernelGGL(k_fluid_diag_blocks, dim3(a.W / 16, a.H / 16), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_fluid_diag_final, dim3(1), dim3(512), 0, st, a);
 }
 void launch_max_divergence(hipStream_t st, const Batch& b) {
   EFL_LAUNCH(k_max_divergence, b, st);

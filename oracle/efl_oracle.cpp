@@ -774,6 +774,107 @@ void precond(orc_inst* o, const float* r, float* z) {
     if (!active_cell(o, i)) z[i] = 0.0f;
 }
 
+// Fluid regions that the pressure system leaves without a Dirichlet term:
+// components of the active cells (linked across open faces, f > 0) none of
+// whose open faces leads to air, a fully solid fluid cell or a cell outside
+// the grid's stencil.  On such a component the operator is singular (constants
+// are its null space) and the system has a solution only if the divergence
+// sums to zero there, which fluid trapped between bodies that move toward
+// each other breaks.  This subtracts the component's mean divergence (double
+// sum over the cells in raster order, then one float subtraction per cell),
+// the compatibility condition of the Neumann problem.  Returns the number of
+// components changed.  (No reference counterpart: the reference's fixed
+// sweep counts never need the system to be solvable.)
+int repair_sealed(orc_inst* o) {
+  const int W = o->W, H = o->H;
+  const size_t n = (size_t)W * H;
+  std::vector<int32_t> comp(n, -1);
+  std::vector<size_t> stack, cells;
+  int repaired = 0, next = 0;
+  static const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+  for (size_t s0 = 0; s0 < n; ++s0) {
+    if (comp[s0] >= 0 || !active_cell(o, s0)) continue;
+    const int id = next++;
+    bool sealed = true;
+    cells.clear();
+    stack.assign(1, s0);
+    comp[s0] = id;
+    while (!stack.empty()) {
+      const size_t c = stack.back();
+      stack.pop_back();
+      cells.push_back(c);
+      const long cx = (long)(c % W), cy = (long)(c / W);
+      for (int k = 0; k < 4; ++k) {
+        if (!(o->afs.d[c * 4 + k] > 0.0f)) continue;  // closed face: no coupling
+        const long jx = cx + ox[k], jy = cy + oy[k];
+        if (jx < 0 || jy < 0 || jx >= W || jy >= H) continue;  // outside: no term
+        const size_t j = (size_t)jy * W + jx;
+        if (!active_cell(o, j)) { sealed = false; continue; }  // air or fully solid fluid: Dirichlet
+        if (comp[j] < 0) {
+          comp[j] = id;
+          stack.push_back(j);
+        }
+      }
+    }
+    if (!sealed) continue;
+    std::sort(cells.begin(), cells.end());
+    double sum = 0.0;
+    for (size_t c : cells) sum += (double)o->div.d[c];
+    const float mean = (float)(sum / (double)cells.size());
+    if (mean == 0.0f) continue;
+    for (size_t c : cells) o->div.d[c] = o->div.d[c] - mean;
+    ++repaired;
+  }
+  return repaired;
+}
+
+// MG-preconditioned CG from the current p0 (see k_projection); returns the
+// iteration count and sets *rel_out, *conv.
+uint32_t mgpcg(orc_inst* o, double* rel_out) {
+  const efl_projection& P = o->proj;
+  size_t n = (size_t)o->W * o->H;
+  float* x = o->p0.d.data();
+  float* r = o->pcg_r.d.data();
+  float* z = o->pcg_z.d.data();
+  float* p = o->pcg_p.d.data();
+  float* q = o->pcg_q.d.data();
+  for (size_t i = 0; i < n; ++i)
+    if (!active_cell(o, i)) x[i] = 0.0f;
+  apply_A(o, x, q);
+  for (size_t i = 0; i < n; ++i) r[i] = active_cell(o, i) ? o->div.d[i] - q[i] : 0.0f;
+  double bn = masked_norm2(o, o->div.d.data());
+  double rn = masked_norm2(o, r);
+  double rel = bn > 0 ? rn / bn : rn;
+  uint32_t maxc = P.max_cycles ? P.max_cycles : 1;
+  uint32_t k = 0;
+  if (!(P.rel_tol > 0.0f && rel <= P.rel_tol)) {
+    precond(o, r, z);
+    std::copy(z, z + n, p);
+    double rz = dotm(o, r, z);
+    for (; k < maxc;) {
+      apply_A(o, p, q);
+      double pq = dotm(o, p, q);
+      if (!(pq > 0.0)) break;
+      float alpha = (float)(rz / pq);
+      for (size_t i = 0; i < n; ++i) {
+        x[i] = x[i] + alpha * p[i];
+        r[i] = r[i] - alpha * q[i];
+      }
+      ++k;
+      rn = masked_norm2(o, r);
+      rel = bn > 0 ? rn / bn : rn;
+      if (P.rel_tol > 0.0f && rel <= P.rel_tol) break;
+      precond(o, r, z);
+      double rz2 = dotm(o, r, z);
+      float beta = (float)(rz2 / rz);
+      rz = rz2;
+      for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+  }
+  *rel_out = rel;
+  return k;
+}
+
 // ProjectionMethod dispatch, projection.rs:89-164, plus the residual-target
 // loop (MULTIGRID with max_cycles > 1 or rel_tol > 0) and MGPCG.
 void k_projection(orc_inst* o) {
@@ -853,48 +954,19 @@ void k_projection(orc_inst* o) {
     return;
   }
   if (P.kind == EFL_PROJECTION_MGPCG) {
-    size_t n = (size_t)o->W * o->H;
-    float* x = o->p0.d.data();
-    float* r = o->pcg_r.d.data();
-    float* z = o->pcg_z.d.data();
-    float* p = o->pcg_p.d.data();
-    float* q = o->pcg_q.d.data();
-    for (size_t i = 0; i < n; ++i)
-      if (!active_cell(o, i)) x[i] = 0.0f;
-    apply_A(o, x, q);
-    for (size_t i = 0; i < n; ++i) r[i] = active_cell(o, i) ? o->div.d[i] - q[i] : 0.0f;
-    double bn = masked_norm2(o, o->div.d.data());
-    double rn = masked_norm2(o, r);
-    double rel = bn > 0 ? rn / bn : rn;
-    uint32_t maxc = P.max_cycles ? P.max_cycles : 1;
-    uint32_t k = 0;
-    if (!(P.rel_tol > 0.0f && rel <= P.rel_tol)) {
-      precond(o, r, z);
-      std::copy(z, z + n, p);
-      double rz = dotm(o, r, z);
-      for (; k < maxc;) {
-        apply_A(o, p, q);
-        double pq = dotm(o, p, q);
-        if (!(pq > 0.0)) break;
-        float alpha = (float)(rz / pq);
-        for (size_t i = 0; i < n; ++i) {
-          x[i] = x[i] + alpha * p[i];
-          r[i] = r[i] - alpha * q[i];
-        }
-        ++k;
-        rn = masked_norm2(o, r);
-        rel = bn > 0 ? rn / bn : rn;
-        if (P.rel_tol > 0.0f && rel <= P.rel_tol) break;
-        precond(o, r, z);
-        double rz2 = dotm(o, r, z);
-        float beta = (float)(rz2 / rz);
-        rz = rz2;
-        for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
-      }
+    double rel = 0.0;
+    uint32_t k = mgpcg(o, &rel);
+    bool ok = P.rel_tol > 0.0f ? rel <= P.rel_tol : true;
+    // a solve that failed may have met a sealed, incompatible region: make
+    // the divergence compatible there and solve again from zero
+    if (!ok && repair_sealed(o) > 0) {
+      std::fill(o->p0.d.begin(), o->p0.d.end(), 0.0f);
+      k += mgpcg(o, &rel);
+      ok = P.rel_tol > 0.0f ? rel <= P.rel_tol : true;
     }
     o->stats.cycles = k;
     o->stats.rel_residual = (float)rel;
-    o->stats.converged = P.rel_tol > 0.0f ? rel <= P.rel_tol : 1;
+    o->stats.converged = ok;
     return;
   }
 }
@@ -1304,8 +1376,6 @@ void integrate_bodies(orc_inst* o, float dt) {
     float vx = ob.linear_velocity[0] + ax * dt;
     float vy = ob.linear_velocity[1] + ay * dt;
     float w = ob.angular_velocity + al * dt;
-    float tx = ob.transform[12] + vx * dt, ty = ob.transform[13] + vy * dt;
-    float ang = std::atan2(ob.transform[1], ob.transform[0]) + w * dt;
     const float* v = ob.values;
     float R = 0.0f;
     switch (ob.shape) {
@@ -1315,6 +1385,17 @@ void integrate_bodies(orc_inst* o, float dt) {
       case 4: R = std::fmax(std::fmax(wlen(v[0], v[1]), wlen(v[2], v[3])), wlen(v[4], v[5])); break;
       default: break;
     }
+    // at most one cell per step, translating and rotating at the bounding radius
+    const float vmax = 1.0f / dt;
+    const float sp = wlen(vx, vy);
+    if (sp > vmax) {
+      const float sc = vmax / sp;
+      vx = vx * sc;
+      vy = vy * sc;
+    }
+    if (R > 0.0f && std::fabs(w) * R > vmax) w = std::copysign(vmax / R, w);
+    float tx = ob.transform[12] + vx * dt, ty = ob.transform[13] + vy * dt;
+    float ang = std::atan2(ob.transform[1], ob.transform[0]) + w * dt;
     float lx = std::fmax(0.5f * (float)o->W - 1.0f - R, 0.0f), ly = std::fmax(0.5f * (float)o->H - 1.0f - R, 0.0f);
     if (tx > lx) { tx = lx; vx = std::fmin(vx, 0.0f); }
     if (tx < -lx) { tx = -lx; vx = std::fmax(vx, 0.0f); }
