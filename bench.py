@@ -12,17 +12,28 @@ target).  One step = one efl_step call: the whole render-node pipeline
 update, extrapolation, level-set advection, sources, redistancing, gradient,
 fluid-to-solid forces), inputs resident in HBM.
 
-Prints one JSON line.  `value` is steps/s; `pressure_solve_ms` and
-`max_norm_divergence` (dt * max|div u| after the projection) come with it.
-`roofline` prices the dominant kernel, the tiled multigrid smoother (two
-red-black Gauss-Seidel sweeps fused in one launch on the finest level, the
-MG_SMOOTH stage), at its algorithmic traffic: per sweep 8 FP32 words per cell
-(x, b, the air level set and 4 area fractions read once, x written once), for
-each of the 2 sweeps (the per-sweep unit, no credit for the reuse across the
-fused sweeps; `traffic` is the measured HBM bytes of one launch) against
-8 TB/s HBM, timed with hipEvents around that launch on the library's stream;
-`cpu_baseline` times the CPU oracle (the reference pipeline restated in C++)
-on a bounded sample of the same workload.
+Prints one JSON line.  `value` is steps/s over the timed window (simulation
+steps warmup+1 .. warmup+steps, named in `timed_steps`); `pressure_solve_ms`
+and `max_norm_divergence` (dt * max|div u| after the projection) come with it.
+`roofline` prices the kernel that dominates the timed step: k_mg_down<2,5,true>,
+the level-0 launch of every MGPCG iteration, which finishes the iteration's
+update (x += alpha p, r -= alpha q, ||r||^2 partials), runs the two red-black
+pre-smoothing sweeps of the preconditioner's V-cycle, forms the residual and
+restricts it.  Its algorithmic bytes follow the per-unit model of the
+reference dataflow (each kernel reads each input once and writes each output
+once, 4 B per word, no credit for fusing): the CG update 6 words per cell
+(x, p, r, q read; x, r written), 2 sweeps x 8 (x, b, air level set and 4 area
+fractions read, x written), the residual pass 8 (the same 7 inputs, r
+written) and the restriction of the residual 1.25 (r read, one coarse b per 4
+cells written) = 31.25 words per cell; it is timed with hipEvents around back-
+to-back launches of exactly that launch on the library's stream
+(efl_time_stage CG_SMOOTH); `traffic` is its measured HBM bytes per launch from
+the committed FETCH_SIZE / WRITE_SIZE profile.  `step_frac` prices the whole
+step: (176 + 69 k) words per cell for a step with k V-cycles (k = mean CG
+iterations, one V-cycle each) over ms_per_step.  `cpu_baseline` times the CPU
+oracle (the reference pipeline restated in C++) on a bounded sample of the same
+workload after 20 warm-up steps, on all host cores and on one core, and the
+reference's own projection (one 2-3-2 V-cycle per step) on all cores.
 """
 import argparse
 import json
@@ -36,6 +47,11 @@ sys.path.insert(0, ROOT)
 METRIC = "sim steps/s + pressure-solve ms at 1024² to ‖∇·u‖∞<1e-4; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
 SMOOTH_SWEEPS = 2  # red-black sweeps of one MG_SMOOTH launch
+# words per cell of the level-0 CG launch: update 6 + 2 sweeps x 8 + residual 8 + restriction 1.25
+CG_SMOOTH_WORDS = 6 + SMOOTH_SWEEPS * 8 + 8 + 1.25
+CG_SMOOTH_KERNEL = ("k_mg_down<2,5,true>: level-0 launch of each MGPCG iteration (x/r update, 2 red-black "
+                    "sweeps, residual, restriction)")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_mg_down_cg_level0_1024.json")
 DT = 1.0 / 60.0
 
 
@@ -50,8 +66,10 @@ def build_scene(size, rel_tol):
 
 
 def roofline_only(args):
-    """Initialise the workload, run two steps, then only the timed smoother
-    launches: a profile of this run holds the roofline kernel alone at level 0."""
+    """Initialise the workload, run two steps, then only the timed launches of
+    the roofline kernel (k_mg_down<2,5,true> as an MGPCG iteration issues it,
+    or with --roofline-kernel mg_smooth the standalone two-sweep smoother): a
+    profile of this run holds that kernel alone at level 0."""
     from bevy_eulerian_fluid_amd import native, scenes
     sc = build_scene(args.size, args.rel_tol)
     ctx = native.Context(0)
@@ -61,11 +79,15 @@ def roofline_only(args):
     inst.step(DT)
     inst.step(DT)
     ctx.sync()
-    ms = inst.time_stage("MG_SMOOTH", DT, args.kernel_reps)
     n = args.size * args.size
-    print(json.dumps({"kernel": "k_mg_down<2,5> level 0 (MG_SMOOTH)", "grid": [args.size, args.size],
-                      "reps": args.kernel_reps, "ms": ms, "algorithmic_bytes": SMOOTH_SWEEPS * 32 * n,
-                      "achieved_gbs": SMOOTH_SWEEPS * 32 * n / (ms * 1e-3) / 1e9}), flush=True)
+    if args.roofline_kernel == "mg_smooth":
+        ms = inst.time_stage("MG_SMOOTH", DT, args.kernel_reps)
+        name, nbytes = "k_mg_down<2,5> level 0 (MG_SMOOTH)", SMOOTH_SWEEPS * 32 * n
+    else:
+        ms = inst.time_stage("CG_SMOOTH", DT, args.kernel_reps)
+        name, nbytes = CG_SMOOTH_KERNEL, int(CG_SMOOTH_WORDS * 4 * n)
+    print(json.dumps({"kernel": name, "grid": [args.size, args.size], "reps": args.kernel_reps, "ms": ms,
+                      "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / (ms * 1e-3) / 1e9}), flush=True)
     inst.close()
     ctx.close()
 
@@ -129,24 +151,26 @@ def gpu_run(args):
     ctx.set_timing(False)
     inst.set_diagnostics(False)
 
-    # dominant kernel: the fused two-sweep smoother on level 0, timed with
-    # hipEvents around its launch on the library's stream
+    # the dominant kernel of the timed step: the level-0 launch of an MGPCG
+    # iteration, timed with hipEvents around back-to-back launches of it on the
+    # library's stream; beside it the standalone two-sweep smoother
     n = args.size * args.size
+    cg_ms = inst.time_stage("CG_SMOOTH", DT, args.kernel_reps)
+    cg_bytes = int(CG_SMOOTH_WORDS * 4 * n)
     sweep_ms = inst.time_stage("MG_SMOOTH", DT, args.kernel_reps)
-    alg_bytes = SMOOTH_SWEEPS * 8 * 4 * n
+    sweep_bytes = SMOOTH_SWEEPS * 8 * 4 * n
     traffic = None  # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE profile
-    prof = os.path.join(ROOT, "profiles", "k_mg_down_smooth_level0_1024.json")
-    if args.size == 1024 and os.path.exists(prof):
-        with open(prof) as f:
+    if args.size == 1024 and os.path.exists(TRAFFIC_PROFILE):
+        with open(TRAFFIC_PROFILE) as f:
             traffic = json.load(f).get("traffic_bytes")
-    achieved = alg_bytes / (sweep_ms * 1e-3) / 1e9
     inst.close()
     ctx.close()
     return dict(steps_per_s=steps_per_s, ms_per_step=1e3 * wall / args.steps,
                 cycles_mean=sum(cycles) / len(cycles), cycles_max=max(cycles),
                 pressure_solve_ms=sum(proj_ms) / len(proj_ms), device_step_ms=sum(step_ms) / len(step_ms),
                 max_norm_divergence=max(divs), rel_residual=max(rels),
-                sweep_ms=sweep_ms, sweep_bytes=alg_bytes, achieved_gbs=achieved, traffic=traffic, tuning=tuning)
+                cg_ms=cg_ms, cg_bytes=cg_bytes, sweep_ms=sweep_ms, sweep_bytes=sweep_bytes,
+                traffic=traffic, tuning=tuning)
 
 
 def run_config(args):
@@ -302,7 +326,10 @@ def run_learned(args):
 
 
 def cpu_baseline(args):
-    """The CPU oracle on the same scene, bounded sample (a few steps)."""
+    """The CPU oracle on the same scene: init, 20 warm-up steps on all cores,
+    then bounded samples of the same workload on all cores and on one core, and
+    of the reference's own projection (one 2-3-2 V-cycle per step)."""
+    from bevy_eulerian_fluid_amd import scenes
     from oracle.oracle import OracleInstance, set_threads
     cores = min(os.cpu_count() or 1, args.cpu_threads)
     set_threads(cores)
@@ -310,22 +337,36 @@ def cpu_baseline(args):
     o = OracleInstance(sc["settings"])
     o.set_sources(sc["init_sources"], True)
     o.set_obstacles(sc["obstacles"])
-    from bevy_eulerian_fluid_amd import scenes
     o.set_bodies(scenes.bodies_for(sc["obstacles"], sc["densities"]))
     o.step(DT)
-    o.step(DT)  # one warm-up step
-    t0 = time.perf_counter()
-    k = 0
-    while k < args.cpu_steps:
+    for _ in range(args.cpu_warmup):
         o.step(DT)
-        k += 1
-        if time.perf_counter() - t0 > args.cpu_budget_s:
-            break
-    el = time.perf_counter() - t0
+
+    def sample(budget):
+        t0 = time.perf_counter()
+        k = 0
+        while k < args.cpu_steps:
+            o.step(DT)
+            k += 1
+            if time.perf_counter() - t0 > budget:
+                break
+        return k, time.perf_counter() - t0
+
+    k, el = sample(args.cpu_budget_s)
+    set_threads(1)
+    k1, el1 = sample(args.cpu_budget_s)
+    set_threads(cores)
+    o.set_projection(scenes.projection("multigrid", max_cycles=1))
+    kv, elv = sample(args.cpu_budget_s / 2)
     o.close()
+    what = (f"{args.size}^2 scene after init + {args.cpu_warmup} warm-up steps, oracle (reference pipeline "
+            "restated in C++, OpenMP over rows)")
     return dict(value=k / el, unit="steps/s", cores=cores, kind="port",
-                sample=f"{k} steps of the same {args.size}^2 scene after init + 1 warm-up step, "
-                       f"oracle (reference pipeline restated in C++, OpenMP over rows), {el:.1f} s")
+                sample=f"{k} steps of the same {what}, {el:.1f} s",
+                one_core={"value": k1 / el1, "unit": "steps/s", "cores": 1, "sample": f"the next {k1} steps, {el1:.1f} s"},
+                reference_vcycle={"value": kv / elv, "unit": "steps/s", "cores": cores,
+                                  "sample": f"the next {kv} steps with the reference's projection (one 2-3-2 "
+                                            f"V-cycle per step, projection.rs:146-162), {elv:.1f} s"})
 
 
 def main():
@@ -338,7 +379,8 @@ def main():
     ap.add_argument("--diag-steps", type=int, default=10)
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=64)
-    ap.add_argument("--cpu-budget-s", type=float, default=25.0)
+    ap.add_argument("--cpu-warmup", type=int, default=20)
+    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", action="store_true", help="replay steps as HIP graphs instead of launching every kernel")
@@ -348,6 +390,7 @@ def main():
                     help="workload: 2 = coupled tank 1024^2 (the headline line); 1, 3, 4 (learned), 5 = the other suite configs")
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the roofline kernel (for profiling it alone)")
+    ap.add_argument("--roofline-kernel", choices=["cg_smooth", "mg_smooth"], default="cg_smooth")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -366,6 +409,10 @@ def main():
         return
     g = gpu_run(args)
     cpu = None if args.no_cpu else cpu_baseline(args)
+    n = args.size * args.size
+    cg_gbs = g["cg_bytes"] / (g["cg_ms"] * 1e-3) / 1e9
+    step_bytes = (176 + 69 * g["cycles_mean"]) * 4 * n  # step with k = mean CG iterations V-cycles
+    step_gbs = step_bytes / (g["ms_per_step"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(g["steps_per_s"], 3),
@@ -373,6 +420,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "timed_steps": [args.warmup + 1, args.warmup + args.steps],
         "ms_per_step": round(g["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -390,10 +438,18 @@ def main():
         "cg_iterations_mean": round(g["cycles_mean"], 2),
         "max_norm_divergence": g["max_norm_divergence"],
         "rel_residual": g["rel_residual"],
-        "roofline": {"bound": "hbm", "kernel": "k_mg_down<2,5>: 2 fused red-black GS sweeps, level 0 (MG_SMOOTH)",
-                     "achieved": round(g["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(g["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": g["traffic"],
-                     "algorithmic_bytes": g["sweep_bytes"], "ms": round(g["sweep_ms"], 5)},
+        "roofline": {"bound": "hbm", "kernel": CG_SMOOTH_KERNEL,
+                     "achieved": round(cg_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(cg_gbs / HBM_PEAK_GBS, 4), "traffic": g["traffic"],
+                     "traffic_source": "profiles/k_mg_down_cg_level0_1024.json (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
+                     "algorithmic_bytes": g["cg_bytes"], "words_per_cell": CG_SMOOTH_WORDS, "ms": round(g["cg_ms"], 5),
+                     "frac_sweeps_only": round(g["sweep_bytes"] / (g["cg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4), "step_achieved": round(step_gbs, 1),
+                     "step_bytes": int(step_bytes),
+                     "smoother_alone": {"kernel": "k_mg_down<2,5> level 0 (MG_SMOOTH)", "ms": round(g["sweep_ms"], 5),
+                                        "algorithmic_bytes": g["sweep_bytes"],
+                                        "frac": round(g["sweep_bytes"] / (g["sweep_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                                      4)}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -53,7 +53,7 @@ STAGE_NAMES = [
     "ADVECT_VELOCITY", "APPLY_FORCES", "DIVERGENCE", "PROJECTION", "SOLVE_VELOCITY",
     "EXTRAPOLATE_VELOCITY", "ADVECT_LEVELSET", "SOURCE", "REINITIALIZE", "LEVELSET_GRADIENT",
     "FLUID_TO_SOLID", "GS_SWEEP", "MG_VCYCLE", "MG_RESIDUAL", "MG_SMOOTH", "GS_SWEEP_SPLIT",
-    "NN_PRECONDITION", "MG_BOTTOM",
+    "NN_PRECONDITION", "MG_BOTTOM", "CG_SMOOTH",
 ]
 STAGE = {n: i for i, n in enumerate(STAGE_NAMES)}
 STAGE_COUNT = len(STAGE_NAMES)

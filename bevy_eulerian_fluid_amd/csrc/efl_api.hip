@@ -1760,16 +1760,77 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
       v_cycle(st, v, 0, 0, false);
       break;
     case EFL_STAGE_MG_RESIDUAL: efl::launch_residual(st, make_batch(v, EX_LEVEL, 0), 0); break;
+    case EFL_STAGE_CG_SMOOTH: return fail(c, EFL_E_INVALID, "CG_SMOOTH is a timing stage (efl_time_stage)");
     default: return fail(c, EFL_E_INVALID, "unknown stage %d", stage);
   }
   if (rc) return rc;
   return launch_status(c, "kernel launch");
 }
 
+namespace {
+// efl_time_stage(CG_SMOOTH): k_mg_down<2,5,true> on level 0 as an MGPCG
+// iteration launches it.  The solve starts from x = 0 (so the initial test
+// does not stop it and p . q > 0), k_pcg_apply forms p, q and the p . q
+// partials, then the same launch (r read from pcg_r, written to pcg_r2,
+// x += alpha p, z known to be zero, residual and restriction of b) runs reps
+// times between two events.  The pressure and the buffer-pair flags are put
+// back afterwards.
+int time_cg_smooth(efl_instance* in, float dt, uint32_t reps, float* ms) {
+  efl_ctx* c = in->ctx;
+  if (!is_cg(in) || !fused_axpy(in))
+    return fail(c, EFL_E_STATE, "CG_SMOOTH needs an MGPCG instance whose level-0 launch carries the CG update");
+  if (in->h.dt != dt) {
+    in->h.dt = dt;
+    in->dirty = true;
+  }
+  int rc = upload(in);
+  if (rc) return rc;
+  hipStream_t st = c->stream;
+  IV v{in};
+  const size_t bytes = (size_t)in->s.width * in->s.height * sizeof(float);
+  float* keep = nullptr;
+  HIPCHK(c, hipMalloc(&keep, bytes));
+  uint8_t xsel[efl::kMaxLevels + 1], xzero[efl::kMaxLevels + 1];
+  std::memcpy(xsel, in->xsel, sizeof(xsel));
+  std::memcpy(xzero, in->xzero, sizeof(xzero));
+  auto done = [&](int code) {
+    (void)hipMemcpyAsync(in->h.p0, keep, bytes, hipMemcpyDeviceToDevice, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(keep);
+    std::memcpy(in->xsel, xsel, sizeof(xsel));
+    std::memcpy(in->xzero, xzero, sizeof(xzero));
+    in->rpar = in->fax = 0;
+    return code;
+  };
+  if (hipMemcpyAsync(keep, in->h.p0, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+      hipMemsetAsync(in->h.p0, 0, bytes, st) != hipSuccess)
+    return done(fail(c, EFL_E_HIP, "CG_SMOOTH: saving the pressure"));
+  cg_begin(st, v);
+  efl::launch_pcg_apply(st, cg_batch(v));
+  prefold(st, v, 1);
+  in->xsel[0] = 0;  // the preconditioner's z starts at zero: not read
+  in->xzero[0] = 1;
+  in->rpar = 0;
+  in->fax = 1;
+  const Batch b = mg_batch(v, EX_LEVEL, 0, false, 1);  // SEL_AXPY: the update rides on this launch
+  efl::launch_mg_down(st, b, 1, 2, 1, 1.0f, 1, 1);       // warm
+  if (hipEventRecord(c->ev[0], st) != hipSuccess) return done(fail(c, EFL_E_HIP, "event"));
+  for (uint32_t k = 0; k < reps; ++k) efl::launch_mg_down(st, b, 1, 2, 1, 1.0f, 1, 1);
+  if (hipEventRecord(c->ev[1], st) != hipSuccess || hipEventSynchronize(c->ev[1]) != hipSuccess)
+    return done(fail(c, EFL_E_HIP, "event"));
+  float t = 0.0f;
+  if (hipEventElapsedTime(&t, c->ev[0], c->ev[1]) != hipSuccess) return done(fail(c, EFL_E_HIP, "event time"));
+  *ms = t / (float)reps;
+  rc = launch_status(c, "CG_SMOOTH");
+  return done(rc);
+}
+}  // namespace
+
 int efl_time_stage(efl_instance* in, int stage, float dt, uint32_t reps, float* ms) {
   if (!in || !ms || reps == 0) return EFL_E_INVALID;
   efl_ctx* c = in->ctx;
   HIPCHK(c, hipSetDevice(c->device));
+  if (stage == EFL_STAGE_CG_SMOOTH) return time_cg_smooth(in, dt, reps, ms);
   int rc = efl_run_stage(in, stage, dt);  // warm (and validate the stage id)
   if (rc) return rc;
   if (stage == EFL_STAGE_MG_BOTTOM) {  // the bottom launch alone, reps times back to back

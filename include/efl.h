@@ -239,6 +239,9 @@ typedef enum {
   EFL_STAGE_NN_PRECONDITION,      /* one application of the learned preconditioner to the CG
                                      residual (needs efl_set_learned and a learned projection) */
   EFL_STAGE_MG_BOTTOM,            /* the V-cycle's bottom (the levels one workgroup solves) */
+  EFL_STAGE_CG_SMOOTH,            /* timing only (efl_time_stage): the level-0 launch of an MGPCG
+                                     iteration that carries the x / r update (2 sweeps, residual,
+                                     restriction); needs an MGPCG instance with a tiled level 1 */
   EFL_STAGE_COUNT
 } efl_stage;
 
@@ -340,7 +343,11 @@ int efl_run_stage(efl_instance* inst, int stage, float dt);
  * time their tiled smoother launch alone: reps launches back to back after
  * one full run of the stage (stencil setup and the copy of x back to its
  * buffer stay outside the events); MG_BOTTOM likewise times the bottom
- * launch alone. */
+ * launch alone.  CG_SMOOTH starts an MGPCG solve of the current system from
+ * a zero guess, runs the first half of its first iteration and then times
+ * the level-0 launch that finishes it (as every iteration issues it) reps
+ * times; the pressure is restored afterwards and the next step solves as
+ * usual. */
 int efl_time_stage(efl_instance* inst, int stage, float dt, uint32_t reps, float* ms);
 
 /* ---- results ------------------------------------------------------------ */
