@@ -35,6 +35,15 @@ static_assert(sizeof(efl_solid_force) == 16, "efl_solid_force layout");
 static_assert(sizeof(efl_stats) == 32, "efl_stats layout");
 static_assert(sizeof(efl_tuning) == 32, "efl_tuning layout");
 
+// Scratch of efl_fluid_diagnostics_noalloc: the per-16x16-block partials
+// (3 floats per block).  A file-scope device array, so that entry point
+// allocates nothing; each GPU holds its own copy until the process ends.
+// Host code never passes the symbol itself: efl_ctx_create looks up its
+// address on the context's device (hipGetSymbolAddress) and keeps it in
+// efl_ctx::scratch_ptr.  It needs no clearing: every launch writes each
+// partial it reads.
+static __device__ __align__(256) float scratch[3 * EFL_NOALLOC_MAX_BLOCKS];
+
 struct efl_instance;
 struct StepGraph {
   std::vector<efl_instance*> insts;
@@ -62,6 +71,7 @@ struct efl_ctx {
   std::vector<efl_instance*> insts;
   efl_body* d_bodies = nullptr;  // EFL_MAX_SOLIDS records
   uint32_t nbodies = 0;          // 0 = no integration of the context's list
+  float* scratch_ptr = nullptr;  // device address of `scratch` on this context's GPU
 };
 
 struct efl_instance {
@@ -1292,6 +1302,14 @@ int efl_ctx_create(int device, efl_ctx** out) {
     delete c;
     return EFL_E_OOM;
   }
+  // the no-alloc entry point's scratch on this device (its own copy per GPU)
+  if (hipGetSymbolAddress((void**)&c->scratch_ptr, HIP_SYMBOL(scratch)) != hipSuccess || !c->scratch_ptr) {
+    hipFree(c->d_obs);
+    hipHostFree(c->h_scal);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return EFL_E_HIP;
+  }
   for (auto& e : c->ev) hipEventCreate(&e);
   c->bottom_cells = efl::bottom_cells_for_device(device);
   *out = c;
@@ -1950,6 +1968,21 @@ int efl_get_fluid_diagnostics(efl_instance* in, efl_fluid_diagnostics* out) {
   out->max_velocity = r[2];
   out->reserved = 0.0f;
   return EFL_OK;
+}
+
+int efl_fluid_diagnostics_noalloc(efl_ctx* c, const float* ls, const float* u, const float* v, uint32_t width,
+                                  uint32_t height, float* out) {
+  if (!c || !ls || !u || !v || !out) return EFL_E_INVALID;
+  if (width == 0 || height == 0 || width > 16384 || height > 16384)
+    return fail(c, EFL_E_INVALID, "grid size %ux%u outside [1, 16384]", width, height);
+  const size_t blocks = (size_t)(width / 16) * (height / 16);
+  if (blocks > (size_t)EFL_NOALLOC_MAX_BLOCKS)
+    return fail(c, EFL_E_RANGE, "%ux%u grid: %zu 16x16 blocks > EFL_NOALLOC_MAX_BLOCKS (%d)", width, height, blocks,
+                EFL_NOALLOC_MAX_BLOCKS);
+  HIPCHK(c, hipSetDevice(c->device));
+  efl::FluidDiagArgs da{ls, u, v, (int)width, (int)height, c->scratch_ptr, out};
+  efl::launch_fluid_diagnostics(c->stream, da);
+  return launch_status(c, "fluid diagnostics");
 }
 
 int efl_measure(efl_instance* in, float dt, float* rel, float* mdiv) {

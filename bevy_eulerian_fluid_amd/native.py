@@ -26,7 +26,7 @@ EXPORTS = [
     "efl_nn_param_count", "efl_nn_flops_per_pixel", "efl_nn_create", "efl_nn_destroy", "efl_nn_init_params",
     "efl_nn_set_params", "efl_nn_get_params", "efl_nn_dataset_reserve", "efl_nn_add_sample",
     "efl_nn_sample_count", "efl_nn_get_sample", "efl_nn_train", "efl_nn_eval", "efl_set_learned",
-    "efl_nn_last_solve", "efl_autotune", "efl_get_fluid_diagnostics",
+    "efl_nn_last_solve", "efl_autotune", "efl_get_fluid_diagnostics", "efl_fluid_diagnostics_noalloc",
 ]
 
 
@@ -103,6 +103,7 @@ def load():
         "efl_set_learned": [P, P],
         "efl_nn_last_solve": [P, P, C.POINTER(C.c_float)],
         "efl_autotune": [P, C.c_float, C.c_int, C.POINTER(abi.efl_tuning)],
+        "efl_fluid_diagnostics_noalloc": [P, P, P, P, C.c_uint32, C.c_uint32, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -172,6 +173,14 @@ class Context:
 
     def instance(self, settings):
         return Instance(self, settings)
+
+    def fluid_diagnostics_noalloc(self, ls_air, u, v, width, height, out):
+        """efl_fluid_diagnostics_noalloc on device pointers (ints): enqueues
+        {volume, min, max} into the 3-float device buffer `out`; read it after
+        sync().  Allocates nothing."""
+        self._chk(self.L.efl_fluid_diagnostics_noalloc(self.h, C.c_void_p(ls_air), C.c_void_p(u), C.c_void_p(v),
+                                                       int(width), int(height), C.c_void_p(out)),
+                  "efl_fluid_diagnostics_noalloc")
 
     def step(self, instances, dt):
         n = len(instances)

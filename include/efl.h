@@ -377,6 +377,21 @@ int efl_get_stats(efl_instance* inst, efl_stats* out);
  * when there are fewer.  Synchronous (one device round trip). */
 int efl_get_fluid_diagnostics(efl_instance* inst, efl_fluid_diagnostics* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same diagnostics as efl_get_fluid_diagnostics, over
+ * caller-owned device fields levelset_air (width x height), u ((width + 1) x
+ * height) and v (width x (height + 1)), written as {volume, min_velocity,
+ * max_velocity} to the caller's device buffer out (3 floats).  Enqueued on
+ * ctx's stream and asynchronous: read out after efl_sync.  Allocates nothing:
+ * the per-block partials go to a scratch array the library holds on each GPU
+ * (EFL_NOALLOC_MAX_BLOCKS 16 x 16 blocks, 768 KiB; grids up to 4096 x 4096),
+ * larger grids return EFL_E_RANGE.  Calls on one GPU share that array, so
+ * calls that overlap on one GPU (two contexts' streams) race. */
+#define EFL_NOALLOC_MAX_BLOCKS 65536
+int efl_fluid_diagnostics_noalloc(efl_ctx* ctx, const float* levelset_air, const float* u, const float* v,
+                                  uint32_t width, uint32_t height, float* out);
+
 /* ---- autotuner (no counterpart in the reference) -----------------------
  *
  * efl_autotune times this instance's pressure solve on its own current
