@@ -3,6 +3,7 @@ include/efl.h declares, and the ctypes mirror matches the C struct layouts."""
 import ctypes as C
 import os
 import re
+import struct
 import subprocess
 
 import pytest
@@ -34,11 +35,42 @@ def test_library_loads_and_exports_every_symbol():
     assert set(declared_functions()) <= exported
 
 
-def test_library_has_gfx950_code_object():
-    out = subprocess.run(["/opt/rocm/bin/roc-obj-ls", native.LIB_PATH], capture_output=True, text=True)
-    if out.returncode != 0:
-        pytest.skip("roc-obj-ls unavailable")
-    assert "gfx950" in out.stdout
+def _offload_targets(fatbin):
+    """Target triples of every clang offload bundle in a .hip_fatbin section
+    (each translation unit contributes one bundle: magic, entry count, then
+    per entry offset, size, triple length and triple)."""
+    magic = b"__CLANG_OFFLOAD_BUNDLE__"
+    out = []
+    at = fatbin.find(magic)
+    while at >= 0:
+        p = at + len(magic)
+        (n,) = struct.unpack_from("<Q", fatbin, p)
+        p += 8
+        triples = []
+        for _ in range(n):
+            _off, _size, tl = struct.unpack_from("<QQQ", fatbin, p)
+            p += 24
+            triples.append(fatbin[p:p + tl].decode())
+            p += tl
+        out.append(triples)
+        at = fatbin.find(magic, p)
+    return out
+
+
+def test_library_has_gfx950_code_object(tmp_path):
+    """libefl.so carries one device code object per HIP source, each for
+    gfx950 and nothing else."""
+    objcopy = "/opt/rocm/llvm/bin/llvm-objcopy"
+    if not os.path.exists(objcopy):
+        pytest.skip("llvm-objcopy unavailable")
+    sec = tmp_path / "fatbin"
+    subprocess.run([objcopy, "--dump-section", f".hip_fatbin={sec}", native.LIB_PATH, str(tmp_path / "lib.so")],
+                   check=True)
+    bundles = _offload_targets(sec.read_bytes())
+    assert len(bundles) == 4, bundles  # efl_kernels, efl_mg, efl_nn, efl_api
+    for triples in bundles:
+        dev = [t for t in triples if not t.startswith("host-")]
+        assert dev == ["hipv4-amdgcn-amd-amdhsa--gfx950"], triples
 
 
 def test_ctypes_layouts_match_c(tmp_path):
