@@ -126,14 +126,22 @@ def gpu_run(args):
         for _ in range(3):
             inst.step(DT)
     ctx.sync()
-    log(f"[bench] timing {args.steps} steps")
+    log(f"[bench] timing {args.steps} steps" + (f" (asynchronous, margin {args.async_margin})" if args.async_margin else ""))
     cycles = []
+    missed0 = inst.stats().unconverged_solves
+    if args.async_margin:
+        ctx.set_async(args.async_margin)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         inst.step(DT)
-        cycles.append(inst.stats().cycles)
+        if not args.async_margin:
+            cycles.append(inst.stats().cycles)
     ctx.sync()
     t1 = time.perf_counter()
+    if args.async_margin:
+        cycles.append(inst.stats().cycles)  # the last step's (statistics resolve lazily)
+        ctx.set_async(0)
+    unconverged = inst.stats().unconverged_solves - missed0
     wall = t1 - t0
     steps_per_s = args.steps / wall
 
@@ -166,7 +174,7 @@ def gpu_run(args):
     inst.close()
     ctx.close()
     return dict(steps_per_s=steps_per_s, ms_per_step=1e3 * wall / args.steps,
-                cycles_mean=sum(cycles) / len(cycles), cycles_max=max(cycles),
+                cycles_mean=sum(cycles) / len(cycles), cycles_max=max(cycles), unconverged=unconverged,
                 pressure_solve_ms=sum(proj_ms) / len(proj_ms), device_step_ms=sum(step_ms) / len(step_ms),
                 max_norm_divergence=max(divs), rel_residual=max(rels),
                 cg_ms=cg_ms, cg_bytes=cg_bytes, sweep_ms=sweep_ms, sweep_bytes=sweep_bytes,
@@ -384,6 +392,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", action="store_true", help="replay steps as HIP graphs instead of launching every kernel")
+    ap.add_argument("--async-margin", type=int, default=0,
+                    help="time the steps on an asynchronous context (efl_set_async): MGPCG budgets of the last "
+                         "iteration count + this margin, no read-back inside efl_step")
     ap.add_argument("--autotune", action="store_true",
                     help="pick sweep counts and the bottom-solve size with efl_autotune before timing")
     ap.add_argument("--config", type=int, default=2,
@@ -429,6 +440,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"coupled tank {args.size}x{args.size}, lower 55% fluid, 16 two-way coupled bodies, "
                                f"dt=1/60, pressure to ||r||/||b||<={args.rel_tol:g} (MG-preconditioned CG)",
+                   "async_margin": args.async_margin,
                    "grid": [args.size, args.size], "bodies": 16, "projection": "mgpcg",
                    "rel_tol": args.rel_tol, "reinit": "fast_iterative(10)",
                    "autotuned": None if g["tuning"] is None else
@@ -436,6 +448,7 @@ def main():
         "pressure_solve_ms": round(g["pressure_solve_ms"], 4),
         "device_step_ms": round(g["device_step_ms"], 4),
         "cg_iterations_mean": round(g["cycles_mean"], 2),
+        "unconverged_steps": g["unconverged"],
         "max_norm_divergence": g["max_norm_divergence"],
         "rel_residual": g["rel_residual"],
         "roofline": {"bound": "hbm", "kernel": CG_SMOOTH_KERNEL,

@@ -133,6 +133,8 @@ class efl_stats(C.Structure):
         ("max_norm_divergence", C.c_float),
         ("projection_ms", C.c_float),
         ("step_ms", C.c_float),
+        ("unconverged_solves", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -160,7 +162,7 @@ class efl_tuning(C.Structure):
 
 STRUCT_SIZES = {
     "efl_projection": 32, "efl_reinit": 8, "efl_settings": 128, "efl_source": 32,
-    "efl_force": 16, "efl_obstacle": 176, "efl_body": 16, "efl_solid_force": 16, "efl_stats": 32,
+    "efl_force": 16, "efl_obstacle": 176, "efl_body": 16, "efl_solid_force": 16, "efl_stats": 40,
     "efl_tuning": 32, "efl_fluid_diagnostics": 16,
 }
 

@@ -32,7 +32,7 @@ static_assert(sizeof(efl_source) == 32, "efl_source layout");
 static_assert(sizeof(efl_force) == 16, "efl_force layout");
 static_assert(sizeof(efl_obstacle) == 176, "efl_obstacle layout (std430 SolidObstacle)");
 static_assert(sizeof(efl_solid_force) == 16, "efl_solid_force layout");
-static_assert(sizeof(efl_stats) == 32, "efl_stats layout");
+static_assert(sizeof(efl_stats) == 40, "efl_stats layout");
 static_assert(sizeof(efl_tuning) == 32, "efl_tuning layout");
 
 // Scratch of efl_fluid_diagnostics_noalloc: the per-16x16-block partials
@@ -72,6 +72,11 @@ struct efl_ctx {
   efl_body* d_bodies = nullptr;  // EFL_MAX_SOLIDS records
   uint32_t nbodies = 0;          // 0 = no integration of the context's list
   float* scratch_ptr = nullptr;  // device address of `scratch` on this context's GPU
+  // efl_set_async: 0 = an MGPCG step reads its stopping state back (exact
+  // iteration counts); > 0 = it queues the previous solve's iterations plus
+  // this margin and reads nothing back (statistics resolve lazily)
+  uint32_t async_margin = 0;
+  uint64_t host_waits = 0;  // blocking waits for the device inside the step machinery (efl_host_waits)
 };
 
 struct efl_instance {
@@ -116,6 +121,18 @@ struct efl_instance {
   bool nn_in_fallback = false;             // MGPCG is finishing a learned solve
   float nn_rel = 0.0f;                     // ||r|| / ||b|| when the network's part ended
   int bottom_cells = 1024;                 // the bottom solve takes levels of at most this many cells
+  // asynchronous MGPCG steps: each solve's scalars are copied to one of two
+  // pinned slots (h_stat, kScalars doubles each) behind that slot's event
+  // and turned into stats when asked for or when the slot is needed again
+  double* h_stat = nullptr;
+  hipEvent_t stat_ev[2] = {nullptr, nullptr};
+  bool stat_pend[2] = {false, false};
+  uint64_t stat_seq[2] = {0, 0};
+  uint64_t stat_next = 0;     // sequence number of the next asynchronous solve
+  bool stat_pending = false;  // some slot is pending
+  uint32_t recent[4] = {};   // iteration counts of the last resolved asynchronous solves
+  uint32_t n_recent = 0;
+  uint32_t missed_seen = 0;  // the device tally (SC_MISSED) already added to stats.unconverged_solves
 };
 
 struct efl_nn {
@@ -173,6 +190,14 @@ int launch_status(efl_ctx* c, const char* what) {
     hipError_t e_ = (call);                                                                 \
     if (e_ != hipSuccess) return fail(ctx, EFL_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
+
+// A blocking wait for the context's stream inside the step machinery
+// (counted: an asynchronous MGPCG step makes none).
+int host_wait(efl_ctx* c) {
+  ++c->host_waits;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
 
 // ---- batches ---------------------------------------------------------------
 enum Extent { EX_CENTER, EX_XMAJOR, EX_YMAJOR, EX_FULL, EX_LEVEL, EX_COARSE };
@@ -561,8 +586,7 @@ void precondition(hipStream_t st, const IV& v, bool cg_check = false) {
 int read_scalars(efl_ctx* c, const IV& v) {
   for (size_t i = 0; i < v.size(); ++i)
     HIPCHK(c, hipMemcpyAsync(c->h_scal + 16 * i, v[i]->h.scal, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return EFL_OK;
+  return host_wait(c);
 }
 
 double rel_of(const double* sc) {
@@ -690,7 +714,91 @@ void cg_iteration(hipStream_t st, const IV& cg, bool fuse = true) {
 // Queue as many iterations as the previous solve took, then one at a time
 // until every instance has stopped; `iter` replays one iteration (a graph)
 // or is null (eager launches).
-int cg_run(efl_ctx* c, const IV& cg, hipGraphExec_t iter) {
+// The stopping state of the asynchronous solve in slot k, once its copy has
+// landed (block: wait for it).  Sets the statistics and the next budget's
+// estimate.
+int resolve_slot(efl_instance* i, int k, bool block) {
+  if (!i->stat_pend[k]) return EFL_OK;
+  efl_ctx* c = i->ctx;
+  if (block) {
+    HIPCHK(c, hipEventSynchronize(i->stat_ev[k]));
+  } else if (hipEventQuery(i->stat_ev[k]) != hipSuccess) {
+    (void)hipGetLastError();  // hipErrorNotReady is not an error here
+    return EFL_OK;
+  }
+  const double* sc = i->h_stat + (size_t)k * efl::kScalars;
+  i->stats.cycles = (uint32_t)sc[efl::SC_ITER];
+  i->stats.converged = sc[efl::SC_DONE] != 0.0 && sc[efl::SC_CONV] != 0.0;
+  i->stats.rel_residual = (float)sc[efl::SC_REL];
+  const uint32_t tally = (uint32_t)sc[efl::SC_MISSED];
+  i->stats.unconverged_solves += tally - i->missed_seen;
+  i->missed_seen = tally;
+  i->cg_last = i->stats.cycles;
+  i->recent[i->n_recent++ % 4] = i->stats.cycles;
+  i->stat_pend[k] = false;
+  i->stat_pending = i->stat_pend[0] || i->stat_pend[1];
+  return EFL_OK;
+}
+// Both slots, the older solve first.
+int resolve_stats(efl_instance* i, bool block) {
+  const int first = i->stat_seq[0] <= i->stat_seq[1] ? 0 : 1;
+  for (int k : {first, 1 - first}) {
+    int rc = resolve_slot(i, k, block);
+    if (rc) return rc;
+  }
+  return EFL_OK;
+}
+
+// Asynchronous MGPCG (efl_set_async): queue the largest of the last four
+// resolved iteration counts plus the margin (at most max_cycles) and copy the
+// stopping state to pinned memory behind an event.  The step never waits for
+// its own solve: iterations after the done flag rose return at once, so
+// whenever the budget covers the solve the result is the synchronous one; a
+// solve that needs more stops at the budget, unconverged (counted on the
+// device, k_cg_tally).  Each instance keeps at most two solves in flight: the
+// slot this solve reuses holds the solve before last, whose statistics are
+// taken first (a wait only if the device has not finished that step), which
+// keeps the estimate at most two steps old.
+int cg_run_async(efl_ctx* c, const IV& cg, hipGraphExec_t iter) {
+  hipStream_t st = c->stream;
+  uint32_t n = 0;
+  for (auto* i : cg) {
+    if (!i->h_stat) {
+      HIPCHK(c, hipHostMalloc((void**)&i->h_stat, 2 * efl::kScalars * sizeof(double), hipHostMallocDefault));
+      for (hipEvent_t& e : i->stat_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    int rc = resolve_stats(i, false);
+    if (rc) return rc;
+    const int k = (int)(i->stat_next & 1);
+    if (i->stat_pend[k]) {  // the solve before last is still running: frames in flight
+      ++c->host_waits;
+      rc = resolve_slot(i, k, true);
+      if (rc) return rc;
+    }
+    uint32_t base = i->cg_last ? i->cg_last : 4u;
+    for (uint32_t q = 0; q < std::min(i->n_recent, 4u); ++q) base = std::max(base, i->recent[q]);
+    n = std::max(n, std::min(base + c->async_margin, i->h.cg_maxc));
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    if (iter) HIPCHK(c, hipGraphLaunch(iter, st));
+    else cg_iteration(st, cg);
+  }
+  efl::launch_cg_tally(st, cg_batch(cg));
+  for (auto* i : cg) {
+    const int k = (int)(i->stat_next & 1);
+    HIPCHK(c, hipMemcpyAsync(i->h_stat + (size_t)k * efl::kScalars, i->h.scal, efl::kScalars * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipEventRecord(i->stat_ev[k], st));
+    i->stat_pend[k] = true;
+    i->stat_seq[k] = i->stat_next++;
+    i->stat_pending = true;
+    i->rpar = i->fax = 0;
+  }
+  return EFL_OK;
+}
+
+int cg_run(efl_ctx* c, const IV& cg, hipGraphExec_t iter, bool sync = false) {
+  if (c->async_margin && !sync) return cg_run_async(c, cg, iter);
   hipStream_t st = c->stream;
   uint32_t n = 0;
   for (auto* i : cg) n = std::max(n, std::min(i->cg_last ? i->cg_last : 4u, i->h.cg_maxc));
@@ -806,7 +914,7 @@ int proj_learned(efl_ctx* c, efl_instance* i) {
   reset_fused(v);
   precondition(st, v);
   const uint32_t keep = i->cg_last;
-  rc = cg_run(c, v, nullptr);
+  rc = cg_run(c, v, nullptr, true);
   i->nn_in_fallback = false;
   if (rc) return rc;
   rc = upload(i);
@@ -835,7 +943,8 @@ int repair_sealed(efl_ctx* c, efl_instance* i, int* changed) {
   HIPCHK(c, hipMemcpyAsync(ls.data(), i->h.ls0, n * 4, hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipMemcpyAsync(div.data(), i->h.div, n * 4, hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipMemcpyAsync(af.data(), i->h.afs, n * 16, hipMemcpyDeviceToHost, st));
-  HIPCHK(c, hipStreamSynchronize(st));
+  int rc = host_wait(c);
+  if (rc) return rc;
   auto active = [&](size_t k) {
     return ls[k] < 0.0f && !(af[4 * k] == 0.0f && af[4 * k + 1] == 0.0f && af[4 * k + 2] == 0.0f && af[4 * k + 3] == 0.0f);
   };
@@ -898,7 +1007,7 @@ int resolve_sealed(efl_ctx* c, const IV& cg) {
   }
   if (again.empty()) return EFL_OK;
   cg_begin(c->stream, again);
-  int rc = cg_run(c, again, nullptr);
+  int rc = cg_run(c, again, nullptr, true);
   if (rc) return rc;
   for (size_t k = 0; k < again.size(); ++k) again[k]->stats.cycles += first[k];
   return EFL_OK;
@@ -919,8 +1028,10 @@ int stage_projection(efl_ctx* c, const IV& v) {
     cg_begin(st, cg);
     int rc = cg_run(c, cg, nullptr);
     if (rc) return rc;
-    rc = resolve_sealed(c, cg);
-    if (rc) return rc;
+    if (!c->async_margin) {  // the repair needs the outcome on the host
+      rc = resolve_sealed(c, cg);
+      if (rc) return rc;
+    }
   }
   for (auto* i : v)
     if (is_learned(i)) {
@@ -987,7 +1098,8 @@ int measure_divergence(efl_ctx* c, const IV& v) {
   std::vector<float> host(4 * d.size());
   for (size_t j = 0; j < d.size(); ++j)
     HIPCHK(c, hipMemcpyAsync(&host[4 * j], d[j]->h.diag, 4 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc = host_wait(c);
+  if (rc) return rc;
   for (size_t j = 0; j < d.size(); ++j) d[j]->stats.max_norm_divergence = d[j]->h.dt * host[4 * j];
   return EFL_OK;
 }
@@ -1002,10 +1114,12 @@ int run_init(efl_ctx* c, const IV& v) {
       size_t n = (size_t)i->s.width * i->s.height;
       std::vector<float> cur(n);
       HIPCHK(c, hipMemcpyAsync(cur.data(), i->h.ls1, n * 4, hipMemcpyDeviceToHost, st));
-      HIPCHK(c, hipStreamSynchronize(st));
+      int rc = host_wait(c);
+      if (rc) return rc;
       for (size_t k = 0; k < n; ++k) cur[k] = std::fmin(cur[k], i->custom_ls[k]);
       HIPCHK(c, hipMemcpyAsync(i->h.ls1, cur.data(), n * 4, hipMemcpyHostToDevice, st));
-      HIPCHK(c, hipStreamSynchronize(st));
+      rc = host_wait(c);
+      if (rc) return rc;
     }
   }
   stage_reinit(st, v);
@@ -1046,6 +1160,9 @@ void phase_post(hipStream_t st, const IV& v) {
 void step_done(const IV& v) {
   for (auto* i : v) {
     i->stats.steps += 1;
+    // a synchronous solve's outcome is known here (asynchronous ones are
+    // counted on the device and added when their statistics resolve)
+    if (!i->stat_pending && !i->stats.converged) i->stats.unconverged_solves += 1;
     if (i->h.nforces) {  // apply_forces_and_clear (apply_forces.rs:157-167)
       i->h.nforces = 0;
       i->dirty = true;
@@ -1071,6 +1188,7 @@ int run_step(efl_ctx* c, const IV& v) {
   if (timed) HIPCHK(c, hipEventRecord(c->ev[3], st));
   step_done(v);
   if (timed) {
+    ++c->host_waits;
     HIPCHK(c, hipEventSynchronize(c->ev[3]));
     float a = 0, b = 0;
     hipEventElapsedTime(&a, c->ev[1], c->ev[2]);
@@ -1222,8 +1340,10 @@ int run_step_graph(efl_ctx* c, const IV& v, float dt, bool* used) {
   if (!cg.empty()) {
     int rc = cg_run(c, cg, g->iter);
     if (rc) return rc;
-    rc = resolve_sealed(c, cg);  // eager launches on this rare path
-    if (rc) return rc;
+    if (!c->async_margin) {
+      rc = resolve_sealed(c, cg);  // eager launches on this rare path
+      if (rc) return rc;
+    }
   }
   HIPCHK(c, hipGraphLaunch(g->post, st));
   step_done(v);
@@ -1339,6 +1459,23 @@ int efl_sync(efl_ctx* c) {
   if (!c) return EFL_E_INVALID;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto* i : c->insts) {
+    int rc = resolve_stats(i, true);
+    if (rc) return rc;
+  }
+  return EFL_OK;
+}
+
+int efl_set_async(efl_ctx* c, uint32_t margin) {
+  if (!c) return EFL_E_INVALID;
+  if (margin > 1000) return fail(c, EFL_E_RANGE, "margin %u > 1000", margin);
+  c->async_margin = margin;
+  return EFL_OK;
+}
+
+int efl_host_waits(const efl_ctx* c, uint64_t* out) {
+  if (!c || !out) return EFL_E_INVALID;
+  *out = c->host_waits;
   return EFL_OK;
 }
 
@@ -1574,6 +1711,9 @@ int efl_instance_destroy(efl_instance* in) {
   if (in->d_obs_own) hipFree(in->d_obs_own);
   if (in->d_bodies_own) hipFree(in->d_bodies_own);
   if (in->mem) hipFree(in->mem);
+  if (in->h_stat) hipHostFree(in->h_stat);
+  for (hipEvent_t e : in->stat_ev)
+    if (e) hipEventDestroy(e);
   nn_free_activations(in->nn_act);
   c->insts.erase(std::remove(c->insts.begin(), c->insts.end(), in), c->insts.end());
   drop_graphs(c);  // their launches name this instance
@@ -2011,6 +2151,8 @@ int efl_measure(efl_instance* in, float dt, float* rel, float* mdiv) {
 
 int efl_get_stats(efl_instance* in, efl_stats* out) {
   if (!in || !out) return EFL_E_INVALID;
+  int rc = resolve_stats(in, true);
+  if (rc) return rc;
   *out = in->stats;
   return EFL_OK;
 }
@@ -2038,6 +2180,14 @@ int efl_autotune(efl_instance* in, float dt, int apply, efl_tuning* res) {
   if (rc) return rc;
   hipStream_t st = c->stream;
   IV v{in};
+  rc = resolve_stats(in, true);
+  if (rc) return rc;
+  struct KeepMargin {  // the trials read each solve's outcome: synchronous solves meanwhile
+    efl_ctx* c;
+    uint32_t m;
+    ~KeepMargin() { c->async_margin = m; }
+  } keep_margin{c, c->async_margin};
+  c->async_margin = 0;
   const size_t n = (size_t)in->s.width * in->s.height;
   float* saved = nullptr;
   if (hipMalloc(&saved, n * 4) != hipSuccess) return fail(c, EFL_E_OOM, "autotune");

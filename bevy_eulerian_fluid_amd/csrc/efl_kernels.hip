@@ -1339,6 +1339,16 @@ __global__ __launch_bounds__(NT) void k_cg_check(Batch b, int init) {
   if (threadIdx.x == 0) cg_stop(I, t[0], true);
 }
 
+// After an asynchronous solve's iteration budget (one block per instance):
+// count the solve in scal[SC_MISSED] when it has not met its tolerance, i.e.
+// it has not stopped (the budget ran out) or stopped without converging.
+__global__ void k_cg_tally(Batch b) {
+  const DevInst* I = b.inst[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  double* sc = I->scal;
+  if (!(sc[SC_DONE] != 0.0 && sc[SC_CONV] != 0.0)) sc[SC_MISSED] = sc[SC_MISSED] + 1.0;
+}
+
 // One block per instance with DevInst::prefold folds the partials its CG
 // consumers would otherwise each fold for themselves (the same fold_partials,
 // so the scalar is the same either way): kind 0, after a preconditioner:
@@ -1753,6 +1763,9 @@ void launch_pcg_apply(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_apply, 
 void launch_pcg_axpy(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_axpy, b, st); }
 void launch_cg_check(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_cg_check, b, st, 0); }
 void launch_prefold(hipStream_t st, const Batch& b, int kind) { EFL_LAUNCH_PER_INST(k_prefold, b, st, kind); }
+void launch_cg_tally(hipStream_t st, const Batch& b) {
+  if (b.n > 0) hipLaunchKernelGGL(k_cg_tally, dim3(b.n), dim3(64), 0, st, b);
+}
 void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_mask_p, b, st); }
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
   if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);

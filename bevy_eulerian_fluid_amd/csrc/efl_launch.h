@@ -62,6 +62,7 @@ void launch_cg_check(hipStream_t st, const Batch& b);
 // the CG tiles of the instances with DevInst::prefold
 void launch_prefold(hipStream_t st, const Batch& b, int kind);
 void launch_mask_p(hipStream_t st, const Batch& b);
+void launch_cg_tally(hipStream_t st, const Batch& b);  // one block per instance
 void launch_max_divergence(hipStream_t st, const Batch& b);
 void launch_fluid_diagnostics(hipStream_t st, const FluidDiagArgs& a);
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a);

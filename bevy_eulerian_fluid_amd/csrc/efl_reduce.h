@@ -33,6 +33,7 @@ constexpr int SC_KCUR = 14;          // the iteration k_pcg_axpy just ran
 constexpr int SC_STALL = 15;         // flexible CG: relative residual at the last stall checkpoint
 constexpr int SC_RZF = 16;           // k_prefold: r . z (and z . q_prev) of the last preconditioner
 constexpr int SC_PQF = 18;           // k_prefold: p . q of the last k_pcg_apply
+constexpr int SC_MISSED = 20;        // asynchronous solves that ended without meeting the tolerance (k_cg_tally)
 constexpr int kScalars = 24;         // scal[] length (the host reads the first 16)
 
 // Partial counts up to this are folded by every block that needs the

@@ -27,6 +27,7 @@ EXPORTS = [
     "efl_nn_set_params", "efl_nn_get_params", "efl_nn_dataset_reserve", "efl_nn_add_sample",
     "efl_nn_sample_count", "efl_nn_get_sample", "efl_nn_train", "efl_nn_eval", "efl_set_learned",
     "efl_nn_last_solve", "efl_autotune", "efl_get_fluid_diagnostics", "efl_fluid_diagnostics_noalloc",
+    "efl_set_async", "efl_host_waits",
 ]
 
 
@@ -104,6 +105,8 @@ def load():
         "efl_nn_last_solve": [P, P, C.POINTER(C.c_float)],
         "efl_autotune": [P, C.c_float, C.c_int, C.POINTER(abi.efl_tuning)],
         "efl_fluid_diagnostics_noalloc": [P, P, P, P, C.c_uint32, C.c_uint32, P],
+        "efl_set_async": [P, C.c_uint32],
+        "efl_host_waits": [P, C.POINTER(C.c_uint64)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -157,6 +160,16 @@ class Context:
     def set_graphs(self, on):
         """Replay steps as HIP graphs, or launch every kernel eagerly (default)."""
         self._chk(self.L.efl_set_graphs(self.h, int(bool(on))), "efl_set_graphs")
+
+    def set_async(self, margin):
+        """efl_set_async: 0 = MGPCG steps read their stopping state back;
+        margin > 0 = they queue the last iteration count + margin and return."""
+        self._chk(self.L.efl_set_async(self.h, int(margin)), "efl_set_async")
+
+    def host_waits(self):
+        n = C.c_uint64()
+        self._chk(self.L.efl_host_waits(self.h, C.byref(n)), "efl_host_waits")
+        return n.value
 
     def set_obstacles(self, obstacles):
         arr, n = abi.struct_array(abi.efl_obstacle, list(obstacles))

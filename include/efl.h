@@ -18,8 +18,11 @@
  *  - The library owns all device memory.  Callers own host arrays and pass
  *    them by pointer and length for the duration of one call only.
  *  - All GPU work of a context is issued in order on one HIP stream owned by
- *    the context.  efl_step() is asynchronous; getters and efl_sync()
- *    synchronise.
+ *    the context; getters and efl_sync() synchronise.  efl_step() returns
+ *    without waiting for the device for Gauss-Seidel, Jacobi and fixed-cycle
+ *    multigrid projections; an MGPCG step waits for its solve's stopping state
+ *    unless the context is asynchronous (efl_set_async), and learned and
+ *    residual-target multigrid steps always do.
  *  - Grids are row-major [y][x], x fastest, y = 0 is the TOP row (world y is
  *    up).  u is (W+1) x H, v is W x (H+1), every other field is W x H.
  *    Velocities are in cells per second.
@@ -170,6 +173,9 @@ typedef struct {
   float max_norm_divergence;  /* dt * max|D| over fluid, non-solid cells (if measured) */
   float projection_ms;        /* device time of the projection stage (if timing on) */
   float step_ms;              /* device time of the whole step (if timing on) */
+  uint32_t unconverged_solves;  /* projections with a residual target that ended without meeting it,
+                                   since the instance was created (asynchronous ones counted on the device) */
+  uint32_t reserved;
 } efl_stats;
 
 /* Fluid diagnostics (diagnostics/component.rs: FluidVolume,
@@ -263,6 +269,29 @@ int efl_set_timing(efl_ctx* ctx, int enabled);
  * diagnostics, a multigrid residual target or a learned projection always
  * run eagerly. */
 int efl_set_graphs(efl_ctx* ctx, int enabled);
+/* No counterpart in the reference.  By default (margin 0) an MGPCG or learned
+ * step reads the solve's stopping state back to the host (one wait per queued
+ * batch of iterations), so efl_step returns once the solve has finished on
+ * the device and the iteration count is exactly what the tolerance needs.
+ * With margin > 0 an MGPCG step never waits for its own solve: it queues the
+ * largest of the last four known iteration counts plus `margin` (at most
+ * max_cycles) and returns.  Iterations after the solve stopped return at once
+ * on the device, so when the budget covers the solve the fields are the
+ * synchronous ones bit for bit; a solve that needs more stops at the budget,
+ * unconverged (efl_stats.unconverged_solves counts it).  Each instance keeps
+ * at most two such solves in flight: a step waits for the statistics of the
+ * step before last if the device has not finished it (frames in flight), so
+ * the counts it budgets from are at most two steps old.  Statistics resolve
+ * lazily (efl_get_stats waits for them, efl_sync resolves all).  Learned and
+ * residual-target multigrid projections, diagnostics (efl_set_diagnostics)
+ * and timing (efl_set_timing) still read back; the repair of sealed regions
+ * after a failed solve is skipped. */
+int efl_set_async(efl_ctx* ctx, uint32_t margin);
+/* No counterpart in the reference: the number of times the step machinery of
+ * this context has waited for the device (tests of efl_set_async): reads of a
+ * synchronous solve's outcome, diagnostics, timing, and the two-in-flight
+ * limit of asynchronous steps. */
+int efl_host_waits(const efl_ctx* ctx, uint64_t* out);
 
 /* construct_rigid_body_buffer_for_gpu (obstacle.rs:89-160): the obstacle
  * list is global to the context, as SolidObstaclesBuffer is global to the
@@ -333,7 +362,9 @@ int efl_initialize(efl_instance* inst);
 
 /* One fluid step for each of n instances (render_node.rs:205-217 loops over
  * fluid entities; here they are batched).  Uninitialised instances are
- * initialised instead, as the reference does.  dt = FluidTimeStep. */
+ * initialised instead, as the reference does.  dt = FluidTimeStep.  Which
+ * projections make it wait for the device: see the conventions above and
+ * efl_set_async. */
 int efl_step(efl_ctx* ctx, efl_instance* const* insts, uint32_t n, float dt);
 
 /* Run one stage of the step on one instance (parity tests, profiling). */
