@@ -77,6 +77,9 @@ struct efl_ctx {
   // this margin and reads nothing back (statistics resolve lazily)
   uint32_t async_margin = 0;
   uint64_t host_waits = 0;  // blocking waits for the device inside the step machinery (efl_host_waits)
+  // the coarse levels of a V-cycle as one persistent launch (k_mg_coarse):
+  // EFL_COARSE_FUSE=1 at context creation; off by default (measured slower)
+  bool coarse_fuse = false;
 };
 
 struct efl_instance {
@@ -488,10 +491,89 @@ int bottom_level(const efl_instance* i) {
   return -1;
 }
 
+// 64 x 16 tiles of level l
+int mg_tiles(const efl_instance* i, int l) {
+  const efl::Level& L = i->h.lev[l];
+  return ((L.w + efl::mg_tile_w() - 1) / efl::mg_tile_w()) * ((L.h + efl::mg_tile_h() - 1) / efl::mg_tile_h());
+}
+int bottom_cells_of(const efl_instance* i, int lb) {
+  int c = 0;
+  for (int q = lb; q < i->h.nlev; ++q) c += i->h.lev[q].w * i->h.lev[q].h;
+  return c;
+}
+
+// First level of the fused coarse cycle (k_mg_coarse): the first level
+// l >= 1 above the bottom with at most kCoarseTiles tiles (the levels below
+// it have fewer), when every tiled level takes one launch each way (2 pre-
+// and 2 post-smoothing sweeps) and the bottom fits the launch's LDS; -1 if
+// the cycle has none or the context does not fuse (then every level is a
+// launch of its own).
+int coarse_level(const efl_instance* i) {
+  const int lb = bottom_level(i);
+  const efl_projection& p = i->s.projection;
+  if (!i->ctx->coarse_fuse || lb < 2 || p.mg_pre_iterations != 2 || p.mg_post_iterations != 2) return -1;
+  if (efl::bottom_lds_bytes(bottom_cells_of(i, lb)) > 40 * 1024) return -1;
+  for (int l = 1; l < lb; ++l)
+    if (mg_tiles(i, l) <= efl::kCoarseTiles) return lb - l <= efl::kCoarseMaxLevels ? l : -1;
+  return -1;
+}
+
+// The fused coarse cycle of the instances v from level l0: the Batch::sel bits
+// of every phase, and the buffer-pair flags left behind, are those the
+// separate down / bottom / up launches of v_cycle would use and leave.
+void coarse_cycle(hipStream_t st, const IV& v, int l0, int pcg, bool symmetric) {
+  efl::CoarseBatch cb;
+  std::memset(&cb, 0, sizeof(cb));
+  int start = 0, cells = 0;
+  for (size_t k = 0; k < v.size(); ++k) {
+    efl_instance* i = v[k];
+    const int lb = bottom_level(i);
+    cb.start[k] = start;
+    start += mg_tiles(i, l0);
+    cb.inst[k] = i->d;
+    cb.l0[k] = (int8_t)l0;
+    cb.lb[k] = (int8_t)lb;
+    cb.coarse[k] = (int8_t)std::min<uint32_t>(i->s.projection.mg_coarsest_iterations, 127);
+    int ph = 0;
+    for (int q = l0; q < lb; ++q) {  // down: as mg_batch + mg_flip + the coarse x starting at zero
+      int sel = i->xsel[q] | (i->xzero[q] << 2);
+      if (pcg && i->fax == q + 1) {  // level 1 carries the iteration's stopping test
+        sel |= efl::SEL_AXPY;
+        i->fax = 0;
+      }
+      cb.sel[k][ph++] = (uint8_t)sel;
+      i->xsel[q] ^= 1;
+      i->xzero[q] = 0;
+      i->xsel[q + 1] = 0;
+      i->xzero[q + 1] = 1;
+    }
+    cb.sel[k][ph++] = (uint8_t)(i->xsel[lb] | (i->xzero[lb] << 2));  // bottom
+    i->xzero[lb] = 0;
+    for (int q = lb - 1; q >= l0; --q) {  // up
+      cb.sel[k][ph++] = (uint8_t)(i->xsel[q] | (i->xsel[q + 1] << 1) | (i->xzero[q] << 2));
+      i->xsel[q] ^= 1;
+      i->xzero[q] = 0;
+    }
+    cells = std::max(cells, bottom_cells_of(i, lb));
+  }
+  cb.n = (int)v.size();
+  cb.start[cb.n] = start;
+  efl::launch_mg_coarse(st, cb, pcg, symmetric ? 1 : 0, cells);
+}
+
 // v_cycle, multi_grid.rs:390-475 (symmetric: post-smoothing black-red and
 // the coarsest level also run back in reverse colour order, see the oracle)
-void v_cycle(hipStream_t st, const IV& v, int l, int pcg, bool symmetric) {
-  if (v.empty()) return;
+void v_cycle(hipStream_t st, const IV& v_in, int l, int pcg, bool symmetric) {
+  if (v_in.empty()) return;
+  IV v = v_in;
+  if (l >= 1) {  // instances whose remaining cycle is one fused launch
+    IV fused = filter(v, [l](efl_instance* i) { return coarse_level(i) == l; });
+    if (!fused.empty()) {
+      coarse_cycle(st, fused, l, pcg, symmetric);
+      v = filter(v, [l](efl_instance* i) { return coarse_level(i) != l; });
+      if (v.empty()) return;
+    }
+  }
   IV bot = filter(v, [l](efl_instance* i) { return bottom_level(i) == l; });
   IV coarse = filter(v, [l](efl_instance* i) { return bottom_level(i) < 0 && l == i->h.nlev - 1; });
   IV rest = filter(v, [l](efl_instance* i) {
@@ -1432,6 +1514,7 @@ int efl_ctx_create(int device, efl_ctx** out) {
   }
   for (auto& e : c->ev) hipEventCreate(&e);
   c->bottom_cells = efl::bottom_cells_for_device(device);
+  if (const char* e = getenv("EFL_COARSE_FUSE")) c->coarse_fuse = atoi(e) == 1;
   *out = c;
   return EFL_OK;
 }
@@ -1462,6 +1545,9 @@ int efl_sync(efl_ctx* c) {
   for (auto* i : c->insts) {
     int rc = resolve_stats(i, true);
     if (rc) return rc;
+    uint32_t failed = 0;  // a group barrier of the fused coarse cycle gave up (its fields are not valid)
+    HIPCHK(c, hipMemcpy(&failed, i->h.counter + 2, sizeof(failed), hipMemcpyDeviceToHost));
+    if (failed) return fail(c, EFL_E_HIP, "a fused coarse-cycle barrier timed out on an instance");
   }
   return EFL_OK;
 }

@@ -79,7 +79,7 @@ struct DevInst {
   double* red;         // reduction partials: 4 x red_stride doubles
   int red_stride;      // >= blocks of one full-grid launch
   double* scal;        // scalars, slots SC_* (efl_reduce.h)
-  uint32_t* counter;   // last-block counter (efl_reduce.h)
+  uint32_t* counter;   // k_mg_coarse's group barrier: [0] arrivals, [1] generation, [2] a barrier timed out
   // CG stopping rule, evaluated on the device (projection settings)
   float cg_tol;
   uint32_t cg_maxc, cg_every;
@@ -142,6 +142,22 @@ __device__ __forceinline__ int xcd_tile(int blk, int nb) {
 // k_mg_down after it: it runs the stopping test.  SEL_R2: level-0 b (the
 // residual) is in pcg_r2.
 constexpr int SEL_AXPY = 8, SEL_R2 = 16;
+
+// Launch of k_mg_coarse (efl_mg.hip): for each instance a group of
+// workgroups, one per tile of its level l0 (the first level of at most
+// kCoarseTiles 64 x 16 tiles), runs the V-cycle from l0 down through the
+// bottom and back in one launch, the phases separated by a barrier of the
+// group.  sel holds the Batch::sel bits of each phase in order: the down
+// launches of l0 .. lb - 1, the bottom, the up launches of lb - 1 .. l0.
+constexpr int kCoarseTiles = 16;
+constexpr int kCoarseMaxLevels = 6;  // tiled levels of one fused cycle (lb - l0)
+struct CoarseBatch {
+  int n;
+  int start[kMaxBatch + 1];  // first workgroup of each instance's group
+  const DevInst* inst[kMaxBatch];
+  int8_t l0[kMaxBatch], lb[kMaxBatch], coarse[kMaxBatch];
+  uint8_t sel[kMaxBatch][2 * kCoarseMaxLevels + 1];
+};
 
 struct Batch {
   int n;

@@ -83,4 +83,6 @@ void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int fi
                     int cg_check = 0);
 void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz);
 void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells);
+// max_cells: the most cells any instance's bottom levels hold
+void launch_mg_coarse(hipStream_t st, const CoarseBatch& b, int pcg, int symmetric, int max_cells);
 }  // namespace efl

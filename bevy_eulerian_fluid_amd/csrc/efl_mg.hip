@@ -598,10 +598,11 @@ struct BotLds {
   int* fl;
 };
 
-// G threads (the workgroup, or wavefront 0) take part
+// G threads take part: a whole workgroup (G > 64: a workgroup barrier) or
+// wavefront 0 alone (G = 64: an LDS fence and a wavefront barrier)
 template <int G>
 __device__ __forceinline__ void b_sync() {
-  if (G == BOT_NT) {
+  if (G > 64) {
     __syncthreads();
   } else {  // one wavefront: its LDS accesses complete before any later one
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -609,10 +610,11 @@ __device__ __forceinline__ void b_sync() {
   }
 }
 
-// G threads walk a level as a TX x TY grid (no index divisions)
+// G threads walk a level as a TX x TY grid (no index divisions).  Within a
+// colour the update order does not matter, so every grid gives the same bits.
 template <int G>
 struct BGrid {
-  static constexpr int TX = G == BOT_NT ? 32 : 8, TY = G / TX;
+  static constexpr int TX = G >= 1024 ? 32 : (G >= 256 ? 16 : 8), TY = G / TX;
   static __device__ __forceinline__ int tx() { return (int)threadIdx.x % TX; }
   static __device__ __forceinline__ int ty() { return (int)threadIdx.x / TX; }
 };
@@ -825,13 +827,16 @@ __device__ void b_tail16(const BotCfg& g, const BotLds& m, int ls) {
   m.b[cq] = cb;
 }
 
-__global__ __launch_bounds__(BOT_NT) void k_bottom(Batch b, int pcg, int symmetric) {
-  extern __shared__ float4 smem_f4[];
+// The bottom of the V-cycle from level l0 on, by the NT threads of one
+// workgroup, in LDS (smem: bottom_lds_bytes of the levels' cells).  sel: bit
+// 0 x of l0 is in Level::xs, bit 2 it is zero (not read); sweeps: pre, post,
+// coarsest.  x of l0 goes back to the buffer it came from.  done (CG): the
+// value of the done flag read at the kernel's start, tested once the staging
+// loads are in flight (the two latencies overlap): if set, nothing happens.
+template <int NT>
+__device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symmetric, int pre, int post, int coarse,
+                           float4* smem, double done = 0.0) {
   __shared__ int lw[kMaxLevels + 1], lh[kMaxLevels + 1], loff[kMaxLevels + 1];
-  const int slot = blockIdx.x;
-  const DevInst* I = b.inst[slot];
-  const double done_flag = I->scal[SC_DONE];  // tested once the staging loads are in flight
-  const int l0 = b.level[slot];
   const int nl = I->nlev;
   if (threadIdx.x == 0) {
     int total = 0;
@@ -847,41 +852,40 @@ __global__ __launch_bounds__(BOT_NT) void k_bottom(Batch b, int pcg, int symmetr
   __syncthreads();
   const int total = loff[nl];
   BotLds m;
-  m.c = smem_f4;
+  m.c = smem;
   m.x = (float*)(m.c + total);
   m.b = m.x + total;
   m.fl = (int*)(m.b + total);
-  const bool xzero = (b.sel[slot] & 4) != 0;
+  const bool xzero = (sel & 4) != 0;
   for (int l = l0; l < nl; ++l) {
     const Level L = mg_level(I, l, pcg);
     const int off = loff[l];
-    for (int t = threadIdx.x; t < L.w * L.h; t += BOT_NT) {
+    for (int t = threadIdx.x; t < L.w * L.h; t += NT) {
       m.c[off + t] = ldg(L.cf, t);
       m.fl[off + t] = ldg(L.fl, t);
       if (l == l0) {
-        m.x[t] = xzero ? 0.0f : ldg((b.sel[slot] & 1) ? L.xs : L.x, t);
+        m.x[t] = xzero ? 0.0f : ldg((sel & 1) ? L.xs : L.x, t);
         m.b[t] = ldg(L.b, t);
       }
     }
   }
-  if (pcg && done_flag != 0.0) return;
+  if (pcg && done != 0.0) return;  // uniform for the workgroup
   __syncthreads();
-  // sweep counts ride in the batch: ex = pre, ey = post | coarsest << 16
   BotCfg g;
   g.I = I;
   g.lw = lw;
   g.lh = lh;
   g.loff = loff;
   g.nl = nl;
-  g.pre = b.ex[slot];
-  g.post = b.ey[slot] & 0xffff;
-  g.coarse = b.ey[slot] >> 16;
+  g.pre = pre;
+  g.post = post;
+  g.coarse = coarse;
   g.symmetric = symmetric;
   int ls = l0;  // first level small enough for wavefront 0
   while (ls < nl && lw[ls] * lh[ls] > BOT_SMALL) ++ls;
-  for (int l = l0; l < ls && l < nl - 1; ++l) b_down<BOT_NT>(g, m, l);
+  for (int l = l0; l < ls && l < nl - 1; ++l) b_down<NT>(g, m, l);
   if (ls >= nl) {
-    b_coarsest<BOT_NT>(g, m);  // the coarsest level is itself large
+    b_coarsest<NT>(g, m);  // the coarsest level is itself large
   } else {
     const bool tail16 = nl == ls + 2 && lw[ls] == 16 && lh[ls] == 16 && lw[ls + 1] == 8 && lh[ls + 1] == 8;
     if (threadIdx.x < 64) {
@@ -895,10 +899,169 @@ __global__ __launch_bounds__(BOT_NT) void k_bottom(Batch b, int pcg, int symmetr
     }
     __syncthreads();
   }
-  for (int l = min(ls, nl - 1) - 1; l >= l0; --l) b_up<BOT_NT>(g, m, l);
+  for (int l = min(ls, nl - 1) - 1; l >= l0; --l) b_up<NT>(g, m, l);
   const Level L = mg_level(I, l0, pcg);
-  float* xb = (b.sel[slot] & 1) ? L.xs : L.x;
-  for (int t = threadIdx.x; t < L.w * L.h; t += BOT_NT) stg(xb, t, m.x[t]);
+  float* xb = (sel & 1) ? L.xs : L.x;
+  for (int t = threadIdx.x; t < L.w * L.h; t += NT) stg(xb, t, m.x[t]);
+}
+
+__global__ __launch_bounds__(BOT_NT) void k_bottom(Batch b, int pcg, int symmetric) {
+  extern __shared__ float4 smem_f4[];
+  const int slot = blockIdx.x;
+  const DevInst* I = b.inst[slot];
+  const double done = I->scal[SC_DONE];
+  // sweep counts ride in the batch: ex = pre, ey = post | coarsest << 16
+  bottom_run<BOT_NT>(I, b.level[slot], b.sel[slot], pcg, symmetric, b.ex[slot], b.ey[slot] & 0xffff, b.ey[slot] >> 16,
+                     smem_f4, done);
+}
+
+// ---------------------------------------------------------------------------
+// fused coarse V-cycle: the levels of at most kCoarseTiles tiles and the
+// bottom in one launch (k_mg_coarse).  Each phase is the work of one
+// k_mg_down / k_bottom / k_mg_up launch of the separate-launch cycle, with the
+// same expressions in the same order (so the same bits); a barrier of the
+// instance's workgroup group stands in for each kernel boundary.
+// ---------------------------------------------------------------------------
+// Barrier of the G workgroups of one group: arrivals in cnt[0], the
+// generation in cnt[1].  The release fence before arriving and the acquire
+// after leaving are agent-scope, so tiles written by any workgroup before it
+// are read fresh after it.  A waiter that polls kBarrierSpin times without
+// the generation moving records a failure in cnt[2] and leaves (false); a
+// launch that finds cnt[2] set does nothing, so no wait is unbounded.
+constexpr int kBarrierSpin = 1 << 14;
+__device__ bool group_barrier(uint32_t* cnt, uint32_t G) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int good = 1;
+    __threadfence();
+    const uint32_t gen = __hip_atomic_load(cnt + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (atomicAdd(cnt, 1u) == G - 1) {
+      __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(cnt + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      for (int k = 0; __hip_atomic_load(cnt + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen; ++k) {
+        if (k >= kBarrierSpin) {
+          __hip_atomic_store(cnt + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          good = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __threadfence();
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// origin of tile `tile` of a level, or one far outside it when the level has
+// fewer tiles (such a workgroup loads, updates and stores nothing)
+__device__ __forceinline__ void coarse_tile(const Level& L, int tile, int& tx0, int& ty0) {
+  const int tiles_x = (L.w + MT_W - 1) / MT_W, ntiles = tiles_x * ((L.h + MT_H - 1) / MT_H);
+  if (tile < ntiles) {
+    tx0 = (tile % tiles_x) * MT_W;
+    ty0 = (tile / tiles_x) * MT_H;
+  } else {
+    tx0 = ty0 = 1 << 20;
+  }
+}
+
+// k_mg_down's work on level lvl >= 1: NS sweeps, residual, restriction of b
+template <int NS, int HALO>
+__device__ void coarse_down(const DevInst* I, int lvl, int sel, int tile, float* sx, float* sr) {
+  using T = Tile<HALO>;
+  const Level L = I->lev[lvl];
+  int tx0, ty0;
+  coarse_tile(L, tile, tx0, ty0);
+  const float* xin = (sel & 1) ? L.xs : L.x;
+  float* xout = (sel & 1) ? L.x : L.xs;
+  const int ox = tx0 - HALO, oy = ty0 - HALO;
+  T t;
+  t.place(ox, oy, 1, L.w, L.h);
+  t.load(L, xin, (sel & 4) != 0, nullptr, nullptr, ox, oy, sx);
+  __syncthreads();
+  const float scale = (float)(1 << lvl);
+  t.template sweep<NS>(ox, oy, L.w, L.h, L.w, L.h, 1.0f, gs_c0(I, scale), sx);
+  write_tile<HALO>(L, xout, tx0, ty0, sx);
+  const float factor = resid_factor(I, scale);
+#pragma unroll
+  for (int k = 0; k < T::NSL; ++k) {
+    if (!t.in_tile(k)) continue;
+    const int q = t.ly[k] * T::RW + t.lx[k];
+    const float r = t.inl[k] ? res_upd(t.b[k], t.c[k], t.fl[k], t.x[k], sx[q - 1], sx[q + 1], sx[q - T::RW],
+                                       sx[q + T::RW], factor)
+                             : 0.0f;
+    sr[(t.ly[k] - HALO) * MT_W + t.lx[k] - HALO] = r;
+  }
+  __syncthreads();
+  const Level Cc = I->lev[lvl + 1];
+  const int ux = (int)threadIdx.x % 32, uy = (int)threadIdx.x / 32;
+  const int cx = tx0 / 2 + ux, cy = ty0 / 2 + uy;
+  if (cx < Cc.w && cy < Cc.h) {
+    const int q00 = 2 * uy * MT_W + 2 * ux;
+    const float br = 0.25f * (((sr[q00] + sr[q00 + MT_W]) + sr[q00 + 1]) + sr[q00 + MT_W + 1]);
+    stg(Cc.b, (size_t)cy * Cc.w + cx, br);
+  }
+}
+
+// k_mg_up's work on level lvl >= 1: prolongation while staging, NS sweeps
+template <int NS, int HALO>
+__device__ void coarse_up(const DevInst* I, int lvl, int sel, int tile, int first, float* sx) {
+  using T = Tile<HALO>;
+  const Level L = I->lev[lvl];
+  const Level C = I->lev[lvl + 1];
+  int tx0, ty0;
+  coarse_tile(L, tile, tx0, ty0);
+  const float* xin = (sel & 1) ? L.xs : L.x;
+  float* xout = (sel & 1) ? L.x : L.xs;
+  const int ox = tx0 - HALO, oy = ty0 - HALO;
+  T t;
+  t.place(ox, oy, first, L.w, L.h);
+  t.load(L, xin, (sel & 4) != 0, &C, (sel & 2) ? C.xs : C.x, ox, oy, sx);
+  __syncthreads();
+  t.template sweep<NS>(ox, oy, L.w, L.h, L.w, L.h, 1.0f, gs_c0(I, (float)(1 << lvl)), sx);
+  write_tile<HALO>(L, xout, tx0, ty0, sx);
+}
+
+// One group per instance (workgroups start[i] .. start[i+1]-1).  With pcg
+// the group returns at once when the solve has stopped, and the first down
+// phase runs the CG stopping test when it carries it (SEL_AXPY: level 1 of
+// an iteration whose update rode on level 0), as k_mg_down does; every
+// workgroup evaluates it from the same partials and so leaves together.
+__global__ __launch_bounds__(MNT) void k_mg_coarse(CoarseBatch b, int pcg, int symmetric) {
+  extern __shared__ float4 smem_f4[];  // the bottom's LDS (used by workgroup 0 of a group)
+  __shared__ float sx[Tile<5>::RW * Tile<5>::RH];
+  __shared__ float sr[MT_W * MT_H];
+  int i = 0;
+  while (i + 1 < b.n && (int)blockIdx.x >= b.start[i + 1]) ++i;
+  const int w = (int)blockIdx.x - b.start[i];
+  const uint32_t G = (uint32_t)(b.start[i + 1] - b.start[i]);
+  const DevInst* I = b.inst[i];
+  uint32_t* cnt = I->counter;
+  if (__hip_atomic_load(cnt + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+  if (pcg && I->scal[SC_DONE] != 0.0) return;
+  const int l0 = b.l0[i], lb = b.lb[i];
+  const uint8_t* sel = b.sel[i];
+  int ph = 0;
+  for (int l = l0; l < lb; ++l, ++ph) {
+    if (l == l0 && pcg && (sel[ph] & SEL_AXPY)) {  // the stopping test of the iteration that just ran
+      double rr[1];
+      fold_partials<1>(I, RED_RR, I->cg_tiles, false, rr);
+      if (cg_stop(I, rr[0], w == 0 && threadIdx.x == 0)) return;
+    }
+    coarse_down<2, 5>(I, l, sel[ph], w, sx, sr);
+    if (!group_barrier(cnt, G)) return;
+  }
+  if (w == 0) bottom_run<MNT>(I, lb, sel[ph], pcg, symmetric, 2, 2, b.coarse[i], smem_f4);
+  ++ph;
+  if (!group_barrier(cnt, G)) return;
+  const int first = symmetric ? 0 : 1;  // post-smoothing black-red in the symmetric cycle
+  for (int l = lb - 1; l >= l0; --l, ++ph) {
+    coarse_up<2, 4>(I, l, sel[ph], w, first, sx);
+    if (l > l0 && !group_barrier(cnt, G)) return;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -945,6 +1108,12 @@ void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int firs
     case 2: MG_LAUNCH((k_mg_up<2, 4>), nb, st, b, pcg, first, weight, rz); break;
     default: MG_LAUNCH((k_mg_up<3, 6>), nb, st, b, pcg, first, weight, rz); break;
   }
+}
+
+void launch_mg_coarse(hipStream_t st, const CoarseBatch& b, int pcg, int symmetric, int max_cells) {
+  const int nb = b.start[b.n];
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_mg_coarse, dim3(nb), dim3(MNT), bottom_lds_bytes(max_cells), st, b, pcg, symmetric);
 }
 
 // per cell: stencil (16) + x, b (8) + flags (4)

@@ -319,3 +319,50 @@ def test_mgpcg_sealed_region_repair(gpu_ctx):
     finally:
         g.close()
         o.close()
+
+
+@pytest.fixture
+def fused_ctx():
+    """A context whose V-cycles run their coarse levels and bottom as one
+    persistent launch (k_mg_coarse, EFL_COARSE_FUSE=1 at creation)."""
+    import os
+    from bevy_eulerian_fluid_amd import native
+    old = os.environ.get("EFL_COARSE_FUSE")
+    os.environ["EFL_COARSE_FUSE"] = "1"
+    try:
+        ctx = native.Context(0)
+    finally:
+        if old is None:
+            del os.environ["EFL_COARSE_FUSE"]
+        else:
+            os.environ["EFL_COARSE_FUSE"] = old
+    yield ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("method", ["mg_vcycle", "multigrid", "mgpcg"])
+@pytest.mark.parametrize("size", [(100, 70), (200, 136), (1024, 1024), (1000, 700)],
+                         ids=lambda s: f"{s[0]}x{s[1]}")
+def test_projection_parity_fused_coarse_cycle(fused_ctx, size, method):
+    """The persistent coarse-cycle launch does each phase's work with the same
+    expressions as the separate launches: bit-exact with the oracle (MGPCG:
+    its usual tolerance)."""
+    _projection_case(fused_ctx, size, method)
+
+
+@pytest.mark.parametrize("size", [(96, 80), (200, 136)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_multistep_bitexact_fused_coarse_cycle(fused_ctx, size):
+    """Initialisation + 12 multigrid steps through the fused coarse cycle."""
+    _say(f"multistep fused coarse cycle {size}")
+    p = scenes.projection("multigrid", max_cycles=1)
+    sc = small_scene(*size, proj=p, forces=False)
+    g, o = make_pair(fused_ctx, sc, OracleInstance)
+    try:
+        for k in range(13):
+            o.step(DT)
+            g.step(DT)
+            errs = compare_all(g, o, NO_R0)
+            assert not errs, f"step {k}: " + "\n".join(errs)
+    finally:
+        g.close()
+        o.close()
