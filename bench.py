@@ -13,7 +13,8 @@ update, extrapolation, level-set advection, sources, redistancing, gradient,
 fluid-to-solid forces), inputs resident in HBM.
 
 Prints one JSON line.  `value` is steps/s over the timed window (simulation
-steps warmup+1 .. warmup+steps, named in `timed_steps`); `pressure_solve_ms`
+steps warmup+1 .. warmup+steps, three later with --autotune, named in
+`timed_steps`); `pressure_solve_ms`
 and `max_norm_divergence` (dt * max|div u| after the projection) come with it.
 `roofline` prices the kernel that dominates the timed step: k_mg_down<2,5,true>,
 the level-0 launch of every MGPCG iteration, which finishes the iteration's
@@ -178,7 +179,7 @@ def gpu_run(args):
                 pressure_solve_ms=sum(proj_ms) / len(proj_ms), device_step_ms=sum(step_ms) / len(step_ms),
                 max_norm_divergence=max(divs), rel_residual=max(rels),
                 cg_ms=cg_ms, cg_bytes=cg_bytes, sweep_ms=sweep_ms, sweep_bytes=sweep_bytes,
-                traffic=traffic, tuning=tuning)
+                traffic=traffic, tuning=tuning, first_timed=args.warmup + 1 + (3 if tuning else 0))
 
 
 def run_config(args):
@@ -396,7 +397,8 @@ def main():
                     help="time the steps on an asynchronous context (efl_set_async): MGPCG budgets of the last "
                          "iteration count + this margin, no read-back inside efl_step")
     ap.add_argument("--autotune", action="store_true",
-                    help="pick sweep counts and the bottom-solve size with efl_autotune before timing")
+                    help="pick sweep counts, the bottom-solve size and the smoother tile height "
+                         "with efl_autotune before timing")
     ap.add_argument("--config", type=int, default=2,
                     help="workload: 2 = coupled tank 1024^2 (the headline line); 1, 3, 4 (learned), 5 = the other suite configs")
     ap.add_argument("--roofline-only", action="store_true",
@@ -431,7 +433,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "timed_steps": [args.warmup + 1, args.warmup + args.steps],
+        "timed_steps": [g["first_timed"], g["first_timed"] + args.steps - 1],
         "ms_per_step": round(g["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -444,7 +446,7 @@ def main():
                    "grid": [args.size, args.size], "bodies": 16, "projection": "mgpcg",
                    "rel_tol": args.rel_tol, "reinit": "fast_iterative(10)",
                    "autotuned": None if g["tuning"] is None else
-                   {k: g["tuning"][k] for k in ("pre", "post", "coarsest", "bottom_cells")}},
+                   {k: g["tuning"][k] for k in ("pre", "post", "coarsest", "bottom_cells", "tile_h")}},
         "pressure_solve_ms": round(g["pressure_solve_ms"], 4),
         "device_step_ms": round(g["device_step_ms"], 4),
         "cg_iterations_mean": round(g["cycles_mean"], 2),

@@ -154,6 +154,7 @@ class efl_tuning(C.Structure):
         ("coarsest", C.c_uint32),
         ("bottom_cells", C.c_uint32),
         ("nn_strips", C.c_uint32),
+        ("tile_h", C.c_uint32),
         ("trials", C.c_uint32),
         ("best_ms", C.c_float),
         ("default_ms", C.c_float),
@@ -163,7 +164,7 @@ class efl_tuning(C.Structure):
 STRUCT_SIZES = {
     "efl_projection": 32, "efl_reinit": 8, "efl_settings": 128, "efl_source": 32,
     "efl_force": 16, "efl_obstacle": 176, "efl_body": 16, "efl_solid_force": 16, "efl_stats": 40,
-    "efl_tuning": 32, "efl_fluid_diagnostics": 16,
+    "efl_tuning": 36, "efl_fluid_diagnostics": 16,
 }
 
 

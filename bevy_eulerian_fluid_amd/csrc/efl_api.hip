@@ -33,7 +33,7 @@ static_assert(sizeof(efl_force) == 16, "efl_force layout");
 static_assert(sizeof(efl_obstacle) == 176, "efl_obstacle layout (std430 SolidObstacle)");
 static_assert(sizeof(efl_solid_force) == 16, "efl_solid_force layout");
 static_assert(sizeof(efl_stats) == 40, "efl_stats layout");
-static_assert(sizeof(efl_tuning) == 32, "efl_tuning layout");
+static_assert(sizeof(efl_tuning) == 36, "efl_tuning layout");
 
 // Scratch of efl_fluid_diagnostics_noalloc: the per-16x16-block partials
 // (3 floats per block).  A file-scope device array, so that entry point
@@ -124,6 +124,7 @@ struct efl_instance {
   bool nn_in_fallback = false;             // MGPCG is finishing a learned solve
   float nn_rel = 0.0f;                     // ||r|| / ||b|| when the network's part ended
   int bottom_cells = 1024;                 // the bottom solve takes levels of at most this many cells
+  int mg_th = 0;  // smoother tile height (16 or 8; 0: by level size) where no reduction partials are left
   // asynchronous MGPCG steps: each solve's scalars are copied to one of two
   // pinned slots (h_stat, kScalars doubles each) behind that slot's event
   // and turned into stats when asked for or when the slot is needed again
@@ -362,8 +363,21 @@ enum SweepMode { PLAIN, DOWN, UP };
 // CG bits (SEL_R2, SEL_AXPY) go only to launches of the preconditioner
 // (pcg): a pressure launch's b is the divergence whatever state a finished
 // CG solve left behind.
+// The tile height of an instance's smoother launch: 64 x 16 for level 0 of
+// the CG preconditioner (its launches leave reduction partials per 64 x 16
+// tile), else the instance's tuning; by default 64 x 8 on levels of fewer
+// than 2^20 cells (under four 64 x 16 tiles per CU of a 256-CU device, where
+// twice the workgroups hide more latency: 5.9 against 7.2 us for two sweeps
+// at 256^2 on MI355X) and 64 x 16 from there (12.9 against 14.3 us at
+// 1024^2, where the taller tile's smaller halo share wins).
+int mg_th_of(const efl_instance* i, int lvl, int pcg) {
+  if (pcg && lvl == 0) return efl::mg_tile_h();
+  if (i->mg_th) return i->mg_th;
+  return (size_t)i->h.lev[lvl].w * i->h.lev[lvl].h >= ((size_t)1 << 20) ? efl::mg_tile_h() : 8;
+}
+
 Batch mg_batch(const IV& v, Extent ext, int lvl, bool with_coarse, int pcg) {
-  Batch b = make_batch(v, EX_LEVEL, lvl, nullptr, efl::mg_tile_w(), efl::mg_tile_h());
+  Batch b = make_batch(v, EX_LEVEL, lvl, nullptr, efl::mg_tile_w(), mg_th_of(v[0], lvl, pcg));
   for (int n = 0; n < b.n; ++n) {
     extent_of(v[n], ext, lvl, b.ex[n], b.ey[n]);
     efl_instance* i = v[n];
@@ -415,6 +429,16 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
                bool reverse_colour, Extent ext = EX_LEVEL, SweepMode mode = PLAIN, bool rz = false,
                IV* rz_done = nullptr, Batch* rz_batch = nullptr, bool cg_check = false) {
   const int first = reverse_colour ? 0 : 1;
+  const int th = mg_th_of(v[0], lvl, pcg);
+  for (auto* i : v)
+    if (mg_th_of(i, lvl, pcg) != th) {  // instances of two tile heights: one pass for each
+      for (int h : {efl::mg_tile_h(), 8}) {
+        IV part = filter(v, [&](efl_instance* j) { return mg_th_of(j, lvl, pcg) == h; });
+        if (!part.empty())
+          gs_sweeps(st, part, lvl, count, weight, pcg, reverse_colour, ext, mode, rz, rz_done, rz_batch, cg_check);
+      }
+      return;
+    }
   auto chunks = [&](efl_instance* i) {  // number of launches for this instance
     uint32_t c = count(i);
     uint32_t n = (c + 2) / 3;
@@ -436,13 +460,13 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
         IV rest = filter(part, [&](efl_instance* i) { return !(rz && chunks(i) == 1); });
         if (!fuse.empty()) {
           Batch bf = mg_batch(fuse, ext, lvl, true, pcg);
-          efl::launch_mg_up(st, bf, pcg, (int)n, first, weight, 1);
+          efl::launch_mg_up(st, bf, pcg, (int)n, first, weight, 1, th);
           mg_flip(fuse, lvl);
           if (rz_done) *rz_done = fuse;
           if (rz_batch) *rz_batch = bf;
         }
         if (!rest.empty()) {
-          efl::launch_mg_up(st, mg_batch(rest, ext, lvl, true, pcg), pcg, (int)n, first, weight, 0);
+          efl::launch_mg_up(st, mg_batch(rest, ext, lvl, true, pcg), pcg, (int)n, first, weight, 0, th);
           mg_flip(rest, lvl);
         }
       } else {
@@ -458,7 +482,7 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
           for (const IV* grp : {&fin_ax, &fin_pl})
             if (!grp->empty())
               efl::launch_mg_down(st, mg_batch(*grp, ext, lvl, false, pcg), pcg, (int)n, first, weight, 1,
-                                  cg_check && k == 0);
+                                  cg_check && k == 0, th);
           mg_flip(fin, lvl);
           for (auto* i : fin) {  // the coarse x starts at zero (not stored)
             i->xsel[lvl + 1] = 0;
@@ -466,11 +490,12 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
           }
           if (!more.empty()) {
             efl::launch_mg_down(st, mg_batch(more, ext, lvl, false, pcg), pcg, (int)n, first, weight, 0,
-                                cg_check && k == 0);
+                                cg_check && k == 0, th);
             mg_flip(more, lvl);
           }
         } else {
-          efl::launch_mg_down(st, mg_batch(part, ext, lvl, false, pcg), pcg, (int)n, first, weight, 0, cg_check && k == 0);
+          efl::launch_mg_down(st, mg_batch(part, ext, lvl, false, pcg), pcg, (int)n, first, weight, 0, cg_check && k == 0,
+                              th);
           mg_flip(part, lvl);
         }
       }
@@ -1982,14 +2007,14 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
     case EFL_STAGE_FLUID_TO_SOLID: stage_fluid_to_solid(st, v); break;
     case EFL_STAGE_GS_SWEEP: {  // one red + black sweep, weight 1.9, gauss_seidel.rs:142-147
       mg_setup(st, v, false);
-      efl::launch_mg_down(st, mg_batch(v, EX_CENTER, 0, false, 0), 0, 1, 1, 1.9f, 0);
+      efl::launch_mg_down(st, mg_batch(v, EX_CENTER, 0, false, 0), 0, 1, 1, 1.9f, 0, 0, mg_th_of(in, 0, 0));
       mg_flip(v, 0);
       mg_settle(st, v, 0);
       break;
     }
     case EFL_STAGE_MG_SMOOTH: {  // two red-black sweeps at weight 1 on level 0 in one tiled launch
       mg_setup(st, v, false);
-      efl::launch_mg_down(st, mg_batch(v, EX_LEVEL, 0, false, 0), 0, 2, 1, 1.0f, 0);
+      efl::launch_mg_down(st, mg_batch(v, EX_LEVEL, 0, false, 0), 0, 2, 1, 1.0f, 0, 0, mg_th_of(in, 0, 0));
       mg_flip(v, 0);
       mg_settle(st, v, 0);
       break;
@@ -2099,7 +2124,7 @@ int efl_time_stage(efl_instance* in, int stage, float dt, uint32_t reps, float* 
     HIPCHK(c, hipEventRecord(c->ev[0], st));
     for (uint32_t k = 0; k < reps; ++k) {
       b.sel[0] = in->xsel[0];
-      efl::launch_mg_down(st, b, 0, gs ? 1 : 2, 1, gs ? 1.9f : 1.0f, 0);
+      efl::launch_mg_down(st, b, 0, gs ? 1 : 2, 1, gs ? 1.9f : 1.0f, 0, 0, mg_th_of(in, 0, 0));
       mg_flip(v, 0);
     }
     HIPCHK(c, hipEventRecord(c->ev[1], st));
@@ -2249,7 +2274,7 @@ int efl_autotune(efl_instance* in, float dt, int apply, efl_tuning* res) {
   if (!in) return EFL_E_INVALID;
   efl_ctx* c = in->ctx;
   const efl_projection orig = in->s.projection;
-  const int orig_bottom = in->bottom_cells;
+  const int orig_bottom = in->bottom_cells, orig_th = in->mg_th;
   const bool learned = orig.kind == EFL_PROJECTION_LEARNED;
   const bool tunable = orig.kind == EFL_PROJECTION_MGPCG || learned ||
                        (orig.kind == EFL_PROJECTION_MULTIGRID && orig.rel_tol > 0.0f);
@@ -2303,18 +2328,19 @@ int efl_autotune(efl_instance* in, float dt, int apply, efl_tuning* res) {
     *ms = best;
     return EFL_OK;
   };
-  auto set = [&](uint32_t pre, uint32_t post, uint32_t coarse, int bottom) {
+  auto set = [&](uint32_t pre, uint32_t post, uint32_t coarse, int bottom, int th) {
     in->s.projection.mg_pre_iterations = pre;
     in->s.projection.mg_post_iterations = post;
     in->s.projection.mg_coarsest_iterations = coarse;
     in->bottom_cells = bottom;
+    in->mg_th = th;
     c->gversion++;
   };
   float def_ms = 0.0f;
   rc = solve_ms(&def_ms);
   if (rc) return rc;
   uint32_t best[3] = {orig.mg_pre_iterations, orig.mg_post_iterations, orig.mg_coarsest_iterations};
-  int best_bottom = in->bottom_cells;
+  int best_bottom = in->bottom_cells, best_th = in->mg_th;
   float best_ms = def_ms;
   static const uint32_t kSweeps[][3] = {{2, 2, 3}, {1, 1, 2}, {1, 1, 3}, {2, 2, 2}, {2, 2, 4},
                                         {3, 3, 3}, {1, 2, 3}, {2, 1, 3}, {3, 3, 4}};
@@ -2322,7 +2348,7 @@ int efl_autotune(efl_instance* in, float dt, int apply, efl_tuning* res) {
   const int bottoms[2] = {1024, big};
   for (int bi = 0; bi < (big > 1024 ? 2 : 1); ++bi)
     for (const auto& sw : kSweeps) {
-      set(sw[0], sw[1], sw[2], bottoms[bi]);
+      set(sw[0], sw[1], sw[2], bottoms[bi], best_th);
       float ms = 0.0f;
       rc = solve_ms(&ms);
       if (rc) return rc;
@@ -2334,7 +2360,19 @@ int efl_autotune(efl_instance* in, float dt, int apply, efl_tuning* res) {
         best_bottom = bottoms[bi];
       }
     }
-  set(best[0], best[1], best[2], best_bottom);
+  // the smoother tile height, with the chosen sweeps
+  for (int th : {0, efl::mg_tile_h(), 8}) {
+    if (th == best_th) continue;
+    set(best[0], best[1], best[2], best_bottom, th);
+    float ms = 0.0f;
+    rc = solve_ms(&ms);
+    if (rc) return rc;
+    if (ms < 0.97f * best_ms) {
+      best_ms = ms;
+      best_th = th;
+    }
+  }
+  set(best[0], best[1], best[2], best_bottom, best_th);
   // the network's convolution strips, one level at a time
   int strips_before[efl::nn::kLevels];
   uint32_t strip_bits = 0;
@@ -2359,7 +2397,7 @@ int efl_autotune(efl_instance* in, float dt, int apply, efl_tuning* res) {
   }
   HIPCHK(c, hipMemcpyAsync(in->h.p0, saved, n * 4, hipMemcpyDeviceToDevice, st));
   if (!apply) {
-    set(orig.mg_pre_iterations, orig.mg_post_iterations, orig.mg_coarsest_iterations, orig_bottom);
+    set(orig.mg_pre_iterations, orig.mg_post_iterations, orig.mg_coarsest_iterations, orig_bottom, orig_th);
     if (learned)
       for (int l = 0; l < efl::nn::kLevels; ++l) in->nn_act.strip[l] = strips_before[l];
   }
@@ -2370,10 +2408,35 @@ int efl_autotune(efl_instance* in, float dt, int apply, efl_tuning* res) {
     res->coarsest = best[2];
     res->bottom_cells = (uint32_t)best_bottom;
     res->nn_strips = strip_bits;
+    res->tile_h = (uint32_t)best_th;
     res->trials = trials;
     res->best_ms = best_ms;
     res->default_ms = def_ms;
   }
+  return EFL_OK;
+}
+
+int efl_apply_tuning(efl_instance* in, const efl_tuning* t) {
+  if (!in || !t) return EFL_E_INVALID;
+  efl_ctx* c = in->ctx;
+  for (uint32_t k : {t->pre, t->post, t->coarsest})
+    if (k < 1 || k > 8) return fail(c, EFL_E_INVALID, "sweep counts must lie in 1..8");
+  if (t->tile_h != 0 && t->tile_h != 16 && t->tile_h != 8)
+    return fail(c, EFL_E_INVALID, "tile_h must be 0, 16 or 8");
+  if (t->bottom_cells != 1024 && t->bottom_cells != 4096)
+    return fail(c, EFL_E_INVALID, "bottom_cells must be 1024 or 4096");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (t->bottom_cells == 4096 && efl::bottom_cells_enable(c->device, 4096) != 4096)
+    return fail(c, EFL_E_INVALID, "the device does not grant the 4096-cell bottom solve");
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // queued steps keep the settings they were queued with
+  in->s.projection.mg_pre_iterations = t->pre;
+  in->s.projection.mg_post_iterations = t->post;
+  in->s.projection.mg_coarsest_iterations = t->coarsest;
+  in->bottom_cells = (int)t->bottom_cells;
+  in->mg_th = (int)t->tile_h;
+  if (in->nn)
+    for (int l = 0; l < efl::nn::kLevels; ++l) in->nn_act.strip[l] = (t->nn_strips >> l) & 1u ? 64 : 16;
+  c->gversion++;
   return EFL_OK;
 }
 

@@ -79,9 +79,10 @@ int mg_setup_tile_w();
 int mg_setup_tile_h();
 void launch_mg_setup(hipStream_t st, const Batch& b);
 size_t bottom_lds_bytes(int cells);
+// th: tile height, 16 (64 x 16 tiles; the only one with CG reductions) or 8
 void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_,
-                    int cg_check = 0);
-void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz);
+                    int cg_check = 0, int th = 16);
+void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz, int th = 16);
 void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells);
 // max_cells: the most cells any instance's bottom levels hold
 void launch_mg_coarse(hipStream_t st, const CoarseBatch& b, int pcg, int symmetric, int max_cells);

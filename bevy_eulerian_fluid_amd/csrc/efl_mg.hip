@@ -49,7 +49,11 @@
 namespace efl {
 
 constexpr int MT_W = 64;   // tile width (one wavefront per tile row)
-constexpr int MT_H = 16;   // tile height
+constexpr int MT_H = 16;   // tile height of the launches that leave reduction partials
+// Launches without reductions (every level of a plain multigrid or Gauss-
+// Seidel solve, and the CG preconditioner below level 0) take the tile
+// height of the instance's tuning: MT_H or MT_H8.
+constexpr int MT_H8 = 8;
 constexpr int MNT = 256;   // threads per block
 static_assert(MNT == RED_NT, "the r . z reduction uses RED_NT threads per block");
 static_assert(MT_W == kRedTileW && MT_H == kRedTileH,
@@ -240,6 +244,7 @@ struct MSite {
 // Slot 0's fields are read together with the slot count, unconditionally, so
 // that a launch of one instance (the usual case) has its tile origin and
 // instance after one round of scalar loads; other slots are looked up after.
+template <int TH = MT_H>
 __device__ __forceinline__ MSite mg_locate(const Batch& b) {
   const int n = b.n, nb = b.nblocks;
   int st = b.start[0], tx = b.tiles_x[0];
@@ -268,7 +273,7 @@ __device__ __forceinline__ MSite mg_locate(const Batch& b) {
   const int lb = blk - st;
   s.blk = blk;
   s.tx0 = (lb % tx) * MT_W;
-  s.ty0 = (lb / tx) * MT_H;
+  s.ty0 = (lb / tx) * TH;
   return s;
 }
 
@@ -277,9 +282,9 @@ __device__ __forceinline__ MSite mg_locate(const Batch& b) {
 // colour is the thread's k-th cell in a row-major walk over that colour's
 // cells.  Stencil, b and x of its cells live in registers; x is mirrored in
 // LDS for the neighbours.
-template <int HALO>
+template <int HALO, int TH = MT_H>
 struct Tile {
-  static constexpr int RW = MT_W + 2 * HALO, RH = MT_H + 2 * HALO, HW = RW / 2;
+  static constexpr int RW = MT_W + 2 * HALO, RH = TH + 2 * HALO, HW = RW / 2;
   static constexpr int NC = HW * RH;                 // cells of one colour
   static constexpr int NIT = (NC + MNT - 1) / MNT;   // slots per colour
   static constexpr int NSL = 2 * NIT;
@@ -384,17 +389,18 @@ struct Tile {
   }
 
   __device__ __forceinline__ bool in_tile(int s) const {
-    return valid[s] && lx[s] >= HALO && lx[s] < HALO + MT_W && ly[s] >= HALO && ly[s] < HALO + MT_H;
+    return valid[s] && lx[s] >= HALO && lx[s] < HALO + MT_W && ly[s] >= HALO && ly[s] < HALO + TH;
   }
 };
 
 // Every in-level cell of the tile goes to xout, updated or not, so xout is a
 // complete copy of x after the launch (row-major from LDS: coalesced).
-template <int HALO>
+template <int HALO, int TH = MT_H>
 __device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, int y0, const float* sx) {
   constexpr int RW = MT_W + 2 * HALO;
+  static_assert((MT_W * TH) % MNT == 0, "whole rows per thread round");
 #pragma unroll
-  for (int k = 0; k < (MT_W * MT_H) / MNT; ++k) {
+  for (int k = 0; k < (MT_W * TH) / MNT; ++k) {
     int t = threadIdx.x + k * MNT;
     int lx = t % MT_W, ly = t / MT_W;
     int gx = x0 + lx, gy = y0 + ly;
@@ -415,14 +421,15 @@ __device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, 
 // and leaves the partial of ||r||^2; the stopping test then runs in the
 // level-1 launch that follows (SEL_AXPY there).  A neighbour's halo reads the
 // old r, which no block of the launch writes, so the tiles need no order.
-template <int NS, int HALO, bool AX = false>
+template <int NS, int HALO, bool AX = false, int TH = MT_H>
 __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, float weight, int restrict_,
                                                  int cg_check) {
-  using T = Tile<HALO>;
+  static_assert(!AX || TH == MT_H, "the fused CG update leaves one partial per 64 x 16 tile");
+  using T = Tile<HALO, TH>;
   __shared__ float sx[T::RW * T::RH];
-  __shared__ float sr[MT_W * MT_H];
+  __shared__ float sr[MT_W * TH];
   __shared__ double wsum[MNT / 64];
-  const MSite s = mg_locate(b);
+  const MSite s = mg_locate<TH>(b);
   const DevInst* I = s.I;
   // The done flag (written by an earlier launch, usually not in this XCD's
   // L2) is read first but tested only after the tile loads are in flight, so
@@ -437,7 +444,7 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
   T t;
   float qv[T::NSL];
-  constexpr int XU = (MT_W * MT_H) / MNT;  // tile cells per thread, row-major over the tile
+  constexpr int XU = (MT_W * TH) / MNT;  // tile cells per thread, row-major over the tile
   float pv[XU], xv[XU];
   t.place(ox, oy, first, L.w, L.h);
   t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, sx, axpy ? I->pcg_q : nullptr, qv);
@@ -505,7 +512,7 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
   t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(I, scale), sx);
-  write_tile<HALO>(L, xout, s.tx0, s.ty0, sx);
+  write_tile<HALO, TH>(L, xout, s.tx0, s.ty0, sx);
   if (!restrict_) return;
   const float factor = resid_factor(I, scale);
 #pragma unroll
@@ -518,11 +525,11 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
     sr[(t.ly[k] - HALO) * MT_W + t.lx[k] - HALO] = r;
   }
   __syncthreads();
-  // one coarse cell per thread: 32 x 8 coarse cells under the 64 x 16 tile
+  // one coarse cell per thread: 32 x TH/2 coarse cells under the tile
   const Level Cc = I->lev[s.lvl + 1];
   const int ux = (int)threadIdx.x % 32, uy = (int)threadIdx.x / 32;
   const int cx = s.tx0 / 2 + ux, cy = s.ty0 / 2 + uy;
-  if (cx >= Cc.w || cy >= Cc.h) return;
+  if (uy >= TH / 2 || cx >= Cc.w || cy >= Cc.h) return;
   const int q00 = 2 * uy * MT_W + 2 * ux;
   // (0,0), (0,1), (1,0), (1,1) as restriction.wgsl sums them
   const float br = 0.25f * (((sr[q00] + sr[q00 + MT_W]) + sr[q00 + 1]) + sr[q00 + MT_W + 1]);
@@ -534,12 +541,12 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
 // of the CG preconditioner) z is masked to the pressure unknowns and the block
 // partial of r . z goes to red[RED_RZ][tile].
 // ---------------------------------------------------------------------------
-template <int NS, int HALO>
+template <int NS, int HALO, int TH = MT_H>
 __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, float weight, int rz) {
-  using T = Tile<HALO>;
+  using T = Tile<HALO, TH>;
   __shared__ float sx[T::RW * T::RH];
   __shared__ double wsum[MNT / 64];
-  const MSite s = mg_locate(b);
+  const MSite s = mg_locate<TH>(b);
   const DevInst* I = s.I;
   const double done_flag = I->scal[SC_DONE];  // tested once the tile loads are in flight
   Level L = mg_level(I, s.lvl, pcg);
@@ -555,8 +562,8 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
   t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(I, scale), sx);
-  if (!rz) {
-    write_tile<HALO>(L, xout, s.tx0, s.ty0, sx);
+  if (TH != MT_H || !rz) {  // r . z partials come from 64 x 16 tiles only
+    write_tile<HALO, TH>(L, xout, s.tx0, s.ty0, sx);
     return;
   }
   double acc = 0.0;
@@ -1078,15 +1085,28 @@ void launch_mg_setup(hipStream_t st, const Batch& b) {
 // nsweeps 0..3 pre-smoothing sweeps (+ residual/restriction)
 // with cg_check, the launch first runs the CG stopping test (every block)
 void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_,
-                    int cg_check) {
+                    int cg_check, int th) {
   bool axpy = false;  // the fused CG update needs the AX instantiation (2 sweeps only)
   for (int n = 0; n < b.n; ++n) axpy = axpy || ((b.sel[n] & SEL_AXPY) && b.level[n] == 0);
-  if (axpy && nsweeps != 2) {  // the host marks only 2-sweep instances; anything else is a bug
-    set_launch_error("launch_mg_down: fused CG update on a launch without 2 sweeps");
+  if (axpy && (nsweeps != 2 || th != MT_H)) {  // the host marks only 2-sweep 64 x 16 launches; anything else is a bug
+    set_launch_error("launch_mg_down: fused CG update on a launch without 2 sweeps on 64 x 16 tiles");
+    return;
+  }
+  if (th != MT_H && th != MT_H8) {
+    set_launch_error("launch_mg_down: tile height neither 16 nor 8");
     return;
   }
   int nb = b.start[b.n];
   if (nb <= 0) return;
+  if (th == MT_H8) {
+    switch (nsweeps) {
+      case 0: MG_LAUNCH((k_mg_down<0, 1, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+      case 1: MG_LAUNCH((k_mg_down<1, 3, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+      case 2: MG_LAUNCH((k_mg_down<2, 5, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+      default: MG_LAUNCH((k_mg_down<3, 7, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+    }
+    return;
+  }
   switch (nsweeps) {
     case 0: MG_LAUNCH((k_mg_down<0, 1>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
     case 1: MG_LAUNCH((k_mg_down<1, 3>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
@@ -1098,10 +1118,23 @@ void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int fi
   }
 }
 
-// prolongation + nsweeps 0..3 post-smoothing sweeps (+ r . z partials)
-void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz) {
+// prolongation + nsweeps 0..3 post-smoothing sweeps (+ r . z partials, 64 x 16 tiles only)
+void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz, int th) {
+  if ((th != MT_H && th != MT_H8) || (rz && th != MT_H)) {
+    set_launch_error("launch_mg_up: tile height neither 16 nor 8, or r . z partials on 64 x 8 tiles");
+    return;
+  }
   int nb = b.start[b.n];
   if (nb <= 0) return;
+  if (th == MT_H8) {
+    switch (nsweeps) {
+      case 0: MG_LAUNCH((k_mg_up<0, 0, MT_H8>), nb, st, b, pcg, first, weight, rz); break;
+      case 1: MG_LAUNCH((k_mg_up<1, 2, MT_H8>), nb, st, b, pcg, first, weight, rz); break;
+      case 2: MG_LAUNCH((k_mg_up<2, 4, MT_H8>), nb, st, b, pcg, first, weight, rz); break;
+      default: MG_LAUNCH((k_mg_up<3, 6, MT_H8>), nb, st, b, pcg, first, weight, rz); break;
+    }
+    return;
+  }
   switch (nsweeps) {
     case 0: MG_LAUNCH((k_mg_up<0, 0>), nb, st, b, pcg, first, weight, rz); break;
     case 1: MG_LAUNCH((k_mg_up<1, 2>), nb, st, b, pcg, first, weight, rz); break;

@@ -26,7 +26,7 @@ EXPORTS = [
     "efl_nn_param_count", "efl_nn_flops_per_pixel", "efl_nn_create", "efl_nn_destroy", "efl_nn_init_params",
     "efl_nn_set_params", "efl_nn_get_params", "efl_nn_dataset_reserve", "efl_nn_add_sample",
     "efl_nn_sample_count", "efl_nn_get_sample", "efl_nn_train", "efl_nn_eval", "efl_set_learned",
-    "efl_nn_last_solve", "efl_autotune", "efl_get_fluid_diagnostics", "efl_fluid_diagnostics_noalloc",
+    "efl_nn_last_solve", "efl_autotune", "efl_apply_tuning", "efl_get_fluid_diagnostics", "efl_fluid_diagnostics_noalloc",
     "efl_set_async", "efl_host_waits",
 ]
 
@@ -104,6 +104,7 @@ def load():
         "efl_set_learned": [P, P],
         "efl_nn_last_solve": [P, P, C.POINTER(C.c_float)],
         "efl_autotune": [P, C.c_float, C.c_int, C.POINTER(abi.efl_tuning)],
+        "efl_apply_tuning": [P, C.POINTER(abi.efl_tuning)],
         "efl_fluid_diagnostics_noalloc": [P, P, P, P, C.c_uint32, C.c_uint32, P],
         "efl_set_async": [P, C.c_uint32],
         "efl_host_waits": [P, C.POINTER(C.c_uint64)],
@@ -348,6 +349,15 @@ class Instance:
         t = abi.efl_tuning()
         self._chk(self.L.efl_autotune(self.h, C.c_float(dt), int(bool(apply)), C.byref(t)), "efl_autotune")
         return {f: getattr(t, f) for f, _ in abi.efl_tuning._fields_}
+
+    def apply_tuning(self, **kw):
+        """efl_apply_tuning with the fields of an autotune() result; fields not
+        given take the defaults (2, 2, 3, 1024, tile_h 0 = by level size, no
+        64-pixel strips)."""
+        t = abi.efl_tuning(pre=2, post=2, coarsest=3, bottom_cells=1024, tile_h=0)
+        for k, val in kw.items():
+            setattr(t, k, val)
+        self._chk(self.L.efl_apply_tuning(self.h, C.byref(t)), "efl_apply_tuning")
 
     def set_learned(self, net):
         self._chk(self.L.efl_set_learned(self.h, net.h if net is not None else None), "efl_set_learned")

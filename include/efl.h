@@ -428,8 +428,11 @@ int efl_fluid_diagnostics_noalloc(efl_ctx* ctx, const float* levelset_air, const
  * efl_autotune times this instance's pressure solve on its own current
  * system for each candidate and keeps the fastest: multigrid sweep counts
  * (pre, post, coarsest) from a fixed list, the size of the bottom solve
- * (levels of at most 1024 or 4096 cells in one workgroup) and, for a learned
- * projection, the convolution strip width of each network level.  It runs
+ * (levels of at most 1024 or 4096 cells in one workgroup), the tile height of
+ * the smoother launches (64 x 16 or 64 x 8 cells per workgroup, or by level
+ * size: 64 x 8 below 2^20 cells, the default; level 0 of an MGPCG
+ * preconditioner stays at 64 x 16) and, for a learned projection, the
+ * convolution strip width of each network level.  It runs
  * the first stages of a step (solids to divergence) to form a fresh system,
  * restores the initial guess before every trial, and leaves velocities,
  * level set and pressure as they were; every candidate meets the same
@@ -441,11 +444,19 @@ typedef struct {
   uint32_t pre, post, coarsest;   /* chosen sweep counts */
   uint32_t bottom_cells;          /* 1024 or 4096 */
   uint32_t nn_strips;             /* bit l: 64-pixel strips on network level l (else 16) */
+  uint32_t tile_h;                /* smoother tile height: 16, 8 or 0 (by level size) */
   uint32_t trials;                /* timed solves */
   float best_ms;                  /* projection time of the chosen settings */
   float default_ms;               /* projection time of the settings on entry */
 } efl_tuning;
 int efl_autotune(efl_instance* inst, float dt, int apply, efl_tuning* result);
+/* Apply settings that efl_autotune reported (a host can keep them and skip
+ * the trials next time): pre, post, coarsest (1..8 each), bottom_cells (1024,
+ * or 4096 where the device grants the bottom solve's LDS), tile_h (16, 8 or 0)
+ * and, for an instance with a network, nn_strips; trials and the times are
+ * ignored.  Waits for the instance's queued work first.  Values out of range
+ * return EFL_E_INVALID and change nothing. */
+int efl_apply_tuning(efl_instance* inst, const efl_tuning* tuning);
 
 /* ---- learned preconditioner (no counterpart in the reference) -----------
  *

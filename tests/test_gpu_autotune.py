@@ -39,6 +39,7 @@ def test_autotune_mgpcg():
         assert r["best_ms"] <= r["default_ms"]
         assert r["trials"] >= 2 * 10
         assert r["bottom_cells"] in (1024, 4096)
+        assert r["tile_h"] in (0, 8, 16)
         r = i.autotune(DT, apply=True)
         _say(f"  applied: {r}")
         for _ in range(3):
@@ -46,6 +47,35 @@ def test_autotune_mgpcg():
             st = i.stats()
             assert st.converged == 1 and st.rel_residual <= 1e-5
         i.close()
+    finally:
+        ctx.close()
+
+
+def test_autotune_reports_what_it_applies():
+    """The reported tuning, reapplied with efl_apply_tuning to a fresh
+    instance of the same scene, reproduces the tuned instance's steps bit for
+    bit (same sweeps, bottom size and tile height), and both tile heights
+    give the same fields as well (the smoother's result does not depend on
+    its tiling)."""
+    _say("autotune: reapply the reported tuning, 256x256, and compare tile heights")
+    ctx = native.Context(0)
+    try:
+        proj = scenes.projection("mgpcg", max_cycles=100, rel_tol=1e-5)
+        a, b, c = _tank(ctx, 256, proj), _tank(ctx, 256, proj), _tank(ctx, 256, proj)
+        r = a.autotune(DT, apply=True)
+        _say(f"  tuned: {r}")
+        keep = {k: r[k] for k in ("pre", "post", "coarsest", "bottom_cells", "nn_strips", "tile_h")}
+        b.apply_tuning(**keep)
+        c.apply_tuning(**dict(keep, tile_h=8 if keep["tile_h"] == 16 else 16))  # another tile height
+        for _ in range(4):
+            for i in (a, b, c):
+                i.step(DT)
+        for f in STATE:
+            ga = a.get(f)
+            assert np.array_equal(ga.view(np.uint32), b.get(f).view(np.uint32)), f
+            assert np.array_equal(ga.view(np.uint32), c.get(f).view(np.uint32)), f
+        for i in (a, b, c):
+            i.close()
     finally:
         ctx.close()
 

@@ -9,7 +9,7 @@ relative) or reduces in a different order (MGPCG: 1e-5 relative on p0).
 import numpy as np
 import pytest
 
-from bevy_eulerian_fluid_amd import abi, scenes
+from bevy_eulerian_fluid_amd import abi, native, scenes
 from oracle.oracle import OracleInstance
 from tests.parity_util import ALL_FIELDS, compare_all, copy_state, make_pair, small_scene
 
@@ -113,8 +113,8 @@ def test_projection_parity_multiwave(gpu_ctx, size, method):
     _projection_case(gpu_ctx, size, method)
 
 
-def _projection_case(gpu_ctx, size, method):
-    _say(f"projection parity {method} {size}")
+def _projection_case(gpu_ctx, size, method, tile_h=None):
+    _say(f"projection parity {method} {size} tile_h {tile_h}")
     stage = "PROJECTION"
     proj = scenes.projection("multigrid", max_cycles=1)
     if method in ("gauss_seidel", "jacobi"):
@@ -135,6 +135,9 @@ def _projection_case(gpu_ctx, size, method):
         proj = scenes.projection("multigrid", pre=3, post=1, coarsest=4, max_cycles=3)
     sc = small_scene(*size, proj=proj)
     g, o = make_pair(gpu_ctx, sc, OracleInstance)
+    if tile_h is not None:  # else the default: tile height by level size
+        g.apply_tuning(pre=proj.mg_pre_iterations, post=proj.mg_post_iterations,
+                       coarsest=proj.mg_coarsest_iterations, tile_h=tile_h)
     try:
         _warm_oracle(o, sc)
         for st in STEP_ORDER[:STEP_ORDER.index("PROJECTION")]:
@@ -348,6 +351,58 @@ def test_projection_parity_fused_coarse_cycle(fused_ctx, size, method):
     expressions as the separate launches: bit-exact with the oracle (MGPCG:
     its usual tolerance)."""
     _projection_case(fused_ctx, size, method)
+
+
+@pytest.mark.parametrize("tile_h", [16, 8])
+@pytest.mark.parametrize("method", ["gs_sweep", "mg_smooth", "mg_vcycle", "multigrid", "multigrid_3cycles", "mgpcg"])
+@pytest.mark.parametrize("size", [(100, 70), (200, 136), (1000, 700), (1024, 1024)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_projection_parity_tile_h(gpu_ctx, size, method, tile_h):
+    """Smoother launches of 64 x 16 or 64 x 8 cells per workgroup on every
+    level (efl_apply_tuning tile_h; the default picks by level size): every
+    cell's update reads the same values either way, so the result is
+    bit-exact with the oracle (MGPCG: its usual tolerance; its level 0 keeps
+    64 x 16)."""
+    _projection_case(gpu_ctx, size, method, tile_h=tile_h)
+
+
+def test_mixed_tile_heights_in_one_batch(gpu_ctx):
+    """Two instances of one scene, tile heights 16 and 8, stepped together by
+    one efl_step (the scheduler splits the smoother launches by tile height):
+    both match the oracle bit for bit over 6 multigrid steps."""
+    _say("mixed tile heights in one batched step, 200x136, 6 steps")
+    p = scenes.projection("multigrid", max_cycles=1)
+    sc = small_scene(200, 136, proj=p, forces=False)
+    g, o = make_pair(gpu_ctx, sc, OracleInstance)
+    h = gpu_ctx.instance(sc["settings"])
+    h.set_sources(sc.get("init_sources", []), True)
+    h.set_sources(sc.get("sources", []), False)
+    g.apply_tuning(tile_h=16)
+    h.apply_tuning(tile_h=8)
+    try:
+        for k in range(7):
+            o.step(DT)
+            gpu_ctx.step([g, h], DT)
+            for inst, name in ((g, "tile_h 16"), (h, "tile_h 8")):
+                errs = compare_all(inst, o, NO_R0)
+                assert not errs, f"step {k} {name}: " + "\n".join(errs)
+    finally:
+        g.close()
+        h.close()
+        o.close()
+
+
+def test_apply_tuning_rejects_out_of_range(gpu_ctx):
+    sc = small_scene(96, 80, proj=scenes.projection("multigrid", max_cycles=1))
+    g = gpu_ctx.instance(sc["settings"])
+    try:
+        for bad in ({"tile_h": 4}, {"tile_h": 32}, {"pre": 0}, {"post": 9}, {"bottom_cells": 2048}):
+            with pytest.raises(native.EflError) as e:
+                g.apply_tuning(**bad)
+            assert e.value.code == abi.E_INVALID, bad
+        for ok in (0, 8, 16):
+            g.apply_tuning(tile_h=ok)
+    finally:
+        g.close()
 
 
 @pytest.mark.parametrize("size", [(96, 80), (200, 136)], ids=lambda s: f"{s[0]}x{s[1]}")
