@@ -93,6 +93,13 @@ struct DevInst {
   float* fdr;
 };
 
+// Kernels never write a DevInst (the host uploads it between launches):
+// read through the constant address space, its fields come in scalar loads,
+// which keeps the descriptor reads out of the waits for the data they point
+// to.  Values kernels do write (scal[], red[]) are read through the pointers.
+using CDevInst = const __attribute__((address_space(4))) DevInst;
+__device__ __forceinline__ CDevInst* cdev(const DevInst* I) { return (CDevInst*)I; }
+
 // Fluid diagnostics (volume, min / max cell-centred speed) of one grid:
 // the fields, the per-16x16-block partials (3 (W/16)(H/16) floats) and the
 // 3 results, passed by value so that an instance's buffers or a caller's
@@ -124,6 +131,18 @@ constexpr int MG_GSZ = 1 << 8;      // the GS update gives 0 (ls >= 0, closed, |
 constexpr int MG_RZ = 1 << 9;       // the residual is 0 (ls >= 0 or closed)
 constexpr int MG_LSNEG = 1 << 10;   // ls < 0 (prolongation target, z mask)
 constexpr int MG_CLOSED = 1 << 11;  // all four area fractions are 0
+constexpr int MG_UNIT = 1 << 12;    // the stencil is (1, 1, 1, 1)
+
+// Whether a cell's stencil has to be read: an MG_UNIT stencil is known, and
+// one whose update and residual are both forced to 0 (MG_GSZ and MG_RZ) is
+// never used; mg_cf_implied stands in for it then (same results).
+__host__ __device__ __forceinline__ bool mg_cf_needed(int fl) {
+  return !(fl & MG_UNIT) && (fl & (MG_GSZ | MG_RZ)) != (MG_GSZ | MG_RZ);
+}
+__host__ __device__ __forceinline__ float4 mg_cf_implied(int fl) {
+  const float v = (fl & MG_UNIT) ? 1.0f : 0.0f;
+  return make_float4(v, v, v, v);
+}
 
 // Workgroups go to the 8 XCDs round robin (block i to XCD i mod 8), and
 // each XCD has its own L2.  Renumbering gives XCD k a contiguous band of

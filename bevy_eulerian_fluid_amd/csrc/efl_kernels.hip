@@ -1403,10 +1403,52 @@ __global__ __launch_bounds__(NT) void k_pcg_rz(Batch b) {
 // q = A p (the operator of residual.wgsl with b = 0, negated) and partials
 // of p . q.
 __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
+  // p = z + beta p_old over the tile and a one-cell ring, formed once per
+  // cell; the operator reads its neighbours here
+  constexpr int RW = TX + 2, RH = CG_TH + 2, RN = RW * RH, RIT = (RN + NT - 1) / NT;
+  __shared__ float sp[RN];
   const CSite s = locate_cg(b);
   const DevInst* I = s.I;
-  if (cg_done(I)) return;
+  // Every load this block needs is issued before the r . z fold and the
+  // done test, so their latencies overlap: the done flag and the iteration
+  // (scalar), the flags of the thread's cells, z and p_old over the region,
+  // then the stencils of the active cells (which the flags select).
+  const double done_flag = I->scal[SC_DONE];
   const int iter = (int)I->scal[SC_ITER] + 1;
+  const Level& L = I->lev[0];
+  // the first iteration takes p = z without reading the previous direction,
+  // which a solve that diverged before may have left non-finite (0 * inf)
+  const float* pold = iter > 1 ? I->pcg_p[(iter - 1) & 1] : nullptr;
+  float* pnew = I->pcg_p[iter & 1];
+  const float* z = I->pcg_z;
+  const int lx = (int)threadIdx.x % TX, ly0 = (int)threadIdx.x / TX;
+  const int tx0 = s.x - lx, ty0 = s.y0 - ly0;
+  int fl[CG_ROWS];
+  size_t id[CG_ROWS];
+  bool in[CG_ROWS];
+#pragma unroll
+  for (int r = 0; r < CG_ROWS; ++r) {
+    const int y = s.y0 + TY * r;
+    in[r] = s.x < s.ex && y < s.ey;
+    id[r] = in[r] ? (size_t)y * L.w + s.x : 0;
+    fl[r] = in[r] ? L.fl[id[r]] : 0;
+  }
+  float zr[RIT], pr[RIT];
+#pragma unroll
+  for (int k = 0; k < RIT; ++k) {
+    const int u = (int)threadIdx.x + k * NT;
+    const int gx = tx0 - 1 + u % RW, gy = ty0 - 1 + u / RW;
+    const bool inside = u < RN && gx >= 0 && gy >= 0 && gx < L.w && gy < L.h;
+    const size_t g = inside ? (size_t)gy * L.w + gx : 0;
+    zr[k] = inside ? z[g] : 0.0f;
+    pr[k] = inside && pold ? pold[g] : 0.0f;
+  }
+  float4 cf[CG_ROWS];
+#pragma unroll
+  for (int r = 0; r < CG_ROWS; ++r) {
+    const bool act = (fl[r] & MG_LSNEG) && !(fl[r] & MG_CLOSED);  // active_at()
+    cf[r] = act && !(fl[r] & MG_UNIT) ? L.cf[id[r]] : mg_cf_implied(fl[r]);
+  }
   double rz[2] = {0.0, 0.0};
   double beta = 0.0;
   if (I->prefold) {
@@ -1419,6 +1461,7 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
     fold_partials<1>(I, RED_RZ, I->cg_tiles, false, t);
     rz[0] = t[0];
   }
+  if (done_flag != 0.0) return;  // uniform for the block
   if (I->cg_flex) {  // flexible CG: beta = -(z . q_prev) / (p . q)_prev
     if (iter > 1 && I->scal[SC_PQ] != 0.0) beta = -rz[1] / I->scal[SC_PQ];
   } else {
@@ -1429,42 +1472,39 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
     I->scal[SC_BETA] = beta;
   }
   const float fb = (float)beta;
-  // the first iteration takes p = z without reading the previous direction,
-  // which a solve that diverged before may have left non-finite (0 * inf)
-  const float* pold = iter > 1 ? I->pcg_p[(iter - 1) & 1] : nullptr;
-  float* pnew = I->pcg_p[iter & 1];
-  const float* z = I->pcg_z;
-  const Level& L = I->lev[0];
+#pragma unroll
+  for (int k = 0; k < RIT; ++k) {
+    const int u = (int)threadIdx.x + k * NT;
+    if (u < RN) sp[u] = pold ? zr[k] + fb * pr[k] : zr[k];
+  }
+  __syncthreads();
   const float factor = I->dt / I->rho / I->dx / I->dx;
   double v[1] = {0.0};
+#pragma unroll
   for (int r = 0; r < CG_ROWS; ++r) {
-    const int y = s.y0 + TY * r;
-    if (s.x >= s.ex || y >= s.ey) continue;
-    const size_t id = (size_t)y * L.w + s.x;
-    const float pc = pold ? z[id] + fb * pold[id] : z[id];
-    pnew[id] = pc;
+    if (!in[r]) continue;
+    const int q0 = (ly0 + TY * r + 1) * RW + lx + 1;
+    const float pc = sp[q0];
+    pnew[id[r]] = pc;
     // residual.wgsl's operator from the precomputed stencil (k_mg_setup):
     // fluid neighbours c_k (p - p_k), air neighbours c_k p, times factor
-    const int fl = L.fl[id];
     float q = 0.0f;
-    if ((fl & MG_LSNEG) && !(fl & MG_CLOSED)) {  // active_at()
-      const float4 c = L.cf[id];
-      const float cv[4] = {c.x, c.y, c.z, c.w};
-      const long off[4] = {-1, 1, -(long)L.w, (long)L.w};
+    if ((fl[r] & MG_LSNEG) && !(fl[r] & MG_CLOSED)) {
+      const float cv[4] = {cf[r].x, cf[r].y, cf[r].z, cf[r].w};
+      const int off[4] = {-1, 1, -RW, RW};
       float res = 0.0f;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        if (!((fl >> k) & 1)) continue;
-        if ((fl >> (4 + k)) & 1) {
-          const size_t jd = (size_t)((long)id + off[k]);
-          res = res - cv[k] * (pc - (pold ? z[jd] + fb * pold[jd] : z[jd])) * factor;
+        if (!((fl[r] >> k) & 1)) continue;
+        if ((fl[r] >> (4 + k)) & 1) {
+          res = res - cv[k] * (pc - sp[q0 + off[k]]) * factor;
         } else {
           res = res - cv[k] * pc * factor;
         }
       }
       q = -res;
     }
-    I->pcg_q[id] = q;
+    I->pcg_q[id[r]] = q;
     v[0] += (double)pc * (double)q;
   }
   block_partials<1>(I, RED_PQ, s.lb, v);

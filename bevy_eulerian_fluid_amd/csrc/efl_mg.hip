@@ -81,6 +81,19 @@ __device__ __forceinline__ float4 ldg(const float4* p, size_t i) {
   const f4v v = ((const __attribute__((address_space(1))) f4v*)p)[i];
   return make_float4(v.x, v.y, v.z, v.w);
 }
+// Loads at a 32-bit element index from a uniform base (global_load with an
+// SGPR base and a 32-bit VGPR offset: one VGPR per address instead of two).
+// Every level array is far below 4 GiB.
+template <class T>
+__device__ __forceinline__ T ld32(const T* p, uint32_t i) {
+  using G = const __attribute__((address_space(1))) char;
+  return *(const __attribute__((address_space(1))) T*)((G*)p + i * (uint32_t)sizeof(T));
+}
+__device__ __forceinline__ float4 ld32(const float4* p, uint32_t i) {
+  using G = const __attribute__((address_space(1))) char;
+  const f4v v = *(const __attribute__((address_space(1))) f4v*)((G*)p + i * 16u);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ void stg(float* p, size_t i, float v) {
   ((__attribute__((address_space(1))) float*)p)[i] = v;
 }
@@ -95,6 +108,24 @@ __device__ __forceinline__ void stg(float4* p, size_t i, float4 v) {
 __device__ __forceinline__ const Level& mg_level(const DevInst* I, int l, int pcg) {
   return (pcg && l == 0) ? I->lev_pcg0 : I->lev[l];
 }
+// a level descriptor out of the constant address space, field by field
+__device__ __forceinline__ Level lev_copy(const __attribute__((address_space(4))) Level& c) {
+  Level r;
+  r.w = c.w;
+  r.h = c.h;
+  r.x = c.x;
+  r.b = c.b;
+  r.ls = c.ls;
+  r.af = c.af;
+  r.r = c.r;
+  r.xs = c.xs;
+  r.cf = c.cf;
+  r.fl = c.fl;
+  return r;
+}
+__device__ __forceinline__ Level mg_level(CDevInst* I, int l, int pcg) {
+  return (pcg && l == 0) ? lev_copy(I->lev_pcg0) : lev_copy(I->lev[l]);
+}
 
 // dx * dx * rho / dt and dt / rho / dx / dx, evaluated left to right as the
 // WGSL does
@@ -103,6 +134,14 @@ __device__ __forceinline__ float gs_c0(const DevInst* I, float scale) {
   return dx * dx * I->rho / I->dt;
 }
 __device__ __forceinline__ float resid_factor(const DevInst* I, float scale) {
+  float dx = I->dx * scale;
+  return I->dt / I->rho / dx / dx;
+}
+__device__ __forceinline__ float gs_c0(CDevInst* I, float scale) {
+  float dx = scale * I->dx;
+  return dx * dx * I->rho / I->dt;
+}
+__device__ __forceinline__ float resid_factor(CDevInst* I, float scale) {
   float dx = I->dx * scale;
   return I->dt / I->rho / dx / dx;
 }
@@ -198,6 +237,7 @@ __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
   if (lc >= 0.0f || closed) fl |= F_RZ;
   if (lc < 0.0f) fl |= F_LSNEG;
   if (closed) fl |= F_CLOSED;
+  if (cv[0] == 1.0f && cv[1] == 1.0f && cv[2] == 1.0f && cv[3] == 1.0f) fl |= MG_UNIT;
   stg(L.cf, id, make_float4(cv[0], cv[1], cv[2], cv[3]));
   stg(L.fl, id, fl);
   // restriction.wgsl:10-60 of the level set and the area fractions, by the
@@ -231,6 +271,11 @@ __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
 // ---------------------------------------------------------------------------
 // tiled smoothers
 // ---------------------------------------------------------------------------
+
+// A level of at least 2^20 cells: its smoother launches are bound by the
+// bytes they stream from HBM (smaller ones by latency), so they read only the
+// stencils they need (Tile::load's lazy_cf).
+__device__ __forceinline__ bool mg_streams(const Level& L) { return (size_t)L.w * L.h >= ((size_t)1 << 20); }
 struct MSite {
   const DevInst* I;
   int blk;       // tile index in the launch
@@ -288,7 +333,7 @@ struct Tile {
   static constexpr int NC = HW * RH;                 // cells of one colour
   static constexpr int NIT = (NC + MNT - 1) / MNT;   // slots per colour
   static constexpr int NSL = 2 * NIT;
-  float b[NSL], x[NSL];
+  float b[NSL], x[NSL], dn[NSL];  // dn: the Gauss-Seidel denominator of the cell's stencil
   float4 c[NSL];
   int fl[NSL];
   int lx[NSL], ly[NSL];
@@ -308,55 +353,78 @@ struct Tile {
     }
   }
 
-  // stencils, b and x from the level (x = 0 when xzero); with C, the
-  // prolongation adds the coarse x (cxin) on fluid cells that have a parent;
-  // with qv, qo[s] = qv at the slot's cell (0 outside the level).
-  // Loads are issued in the order they are consumed (x, the coarse x and
-  // the flags for the LDS copy, then b and the coefficients of the colour
-  // swept first, then those of the other colour), so the LDS copy and the
-  // first half-sweep wait only for their own data.
+  // Stencils, b and x from the level (x = 0 when xzero); with C, the
+  // coarse x (cxin) the prolongation adds; with qv, qo[s] = qv at the slot's
+  // cell.  The loads are straight-line (a cell outside the level reads index
+  // 0) and their values are masked only in stage(), so no load waits for an
+  // earlier one and every wait covers just the loads it needs.  They are
+  // issued in the order they are consumed: the flags, x and the coarse x
+  // (the LDS copy), then b, q and the coefficients.  With lazy_cf the
+  // coefficients are left for load_cf.
+  float xc[NSL];
+  bool par[NSL];
   __device__ __forceinline__ void load(const Level& L, const float* xin, bool xzero, const Level* C,
-                                       const float* cxin, int ox, int oy, float* sx, const float* qv = nullptr,
-                                       float* qo = nullptr) {
-    float xc[NSL];
-    bool par[NSL];
+                                       const float* cxin, int ox, int oy, const float* qv = nullptr,
+                                       float* qo = nullptr, bool lazy_cf = false) {
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
-      x[s] = xc[s] = 0.0f;
-      fl[s] = 0;
+      const int gx = ox + lx[s], gy = oy + ly[s];
+      const uint32_t id = inl[s] ? (uint32_t)(gy * L.w + gx) : 0u;
+      fl[s] = ld32(L.fl, id);
+      x[s] = xzero ? 0.0f : ld32(xin, id);
       par[s] = false;
-      if (inl[s]) {
-        const int gx = ox + lx[s], gy = oy + ly[s];
-        const size_t id = (size_t)gy * L.w + gx;
-        if (!xzero) x[s] = ldg(xin, id);
-        fl[s] = ldg(L.fl, id);
-        if (C) {
-          const int cx = gx >> 1, cy = gy >> 1;
-          if (cx < C->w && cy < C->h) {
-            par[s] = true;
-            xc[s] = ldg(cxin, (size_t)cy * C->w + cx);
-          }
-        }
+      xc[s] = 0.0f;
+      if (C) {
+        const int cx = gx >> 1, cy = gy >> 1;
+        par[s] = inl[s] && cx < C->w && cy < C->h;
+        xc[s] = ld32(cxin, par[s] ? (uint32_t)(cy * C->w + cx) : 0u);
       }
     }
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
-      b[s] = 0.0f;
-      c[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (qv) qo[s] = 0.0f;
-      if (inl[s]) {
-        const size_t id = (size_t)(oy + ly[s]) * L.w + ox + lx[s];
-        b[s] = ldg(L.b, id);
-        if (qv) qo[s] = ldg(qv, id);
-        c[s] = ldg(L.cf, id);
-      }
+      const uint32_t id = inl[s] ? (uint32_t)((oy + ly[s]) * L.w + ox + lx[s]) : 0u;
+      b[s] = ld32(L.b, id);
+      if (qv) qo[s] = ld32(qv, id);
+      if (!lazy_cf) c[s] = ld32(L.cf, id);
     }
+  }
+  // The coefficients where the flags need them (mg_cf_needed; stage() puts
+  // mg_cf_implied in the others): on a level that streams from HBM most
+  // cells (open fluid, air) need none, a third of the bytes.  A cell that
+  // needs none reads index 0, which the wavefront's other such lanes share.
+  // These loads wait for the flags, so they go after every other load.
+  __device__ __forceinline__ void load_cf(const Level& L, int ox, int oy) {
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
+      const bool need = inl[s] && mg_cf_needed(fl[s]);
+      c[s] = ld32(L.cf, need ? (uint32_t)((oy + ly[s]) * L.w + ox + lx[s]) : 0u);
+    }
+  }
+  // Cells outside the level take flags, x, b, q and coefficients 0 (after
+  // load_cf, lazy_cf: the implied coefficients where none were read); then
+  // the prolongation (x += coarse x on fluid cells with a parent), the LDS
+  // copy of x and the denominators.
+  __device__ __forceinline__ void stage(const Level* C, float* sx, float* qo = nullptr, bool lazy_cf = false) {
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      if (lazy_cf && inl[s] && !mg_cf_needed(fl[s])) c[s] = mg_cf_implied(fl[s]);
+      if (!inl[s]) {
+        fl[s] = 0;
+        x[s] = b[s] = 0.0f;
+        c[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (qo) qo[s] = 0.0f;
+      }
       if (C && par[s] && (fl[s] & F_LSNEG)) x[s] = xc[s] + x[s];
       if (valid[s]) sx[ly[s] * RW + lx[s]] = x[s];
     }
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) dn[s] = b_denominator(c[s], fl[s]);
   }
+
+  // The LDS index of slot s, or of an interior cell (read, never used) when
+  // the slot is inactive, so that every slot's neighbour loads can be issued
+  // unconditionally and all at once.
+  __device__ __forceinline__ int lds_at(int s, bool act) const { return act ? ly[s] * RW + lx[s] : RW + 1; }
 
   // 2 * NS half-sweeps (first colour = the one placed in slots 0..NIT-1).
   // Half-sweep hs updates cells at least hs + 1 from the region border (their
@@ -371,17 +439,28 @@ struct Tile {
       const int lo_x = max(m, bx0), hi_x = min(RW - m, bx1), lo_y = max(m, by0), hi_y = min(RH - m, by1);
       const int s0 = (hs & 1) ? NIT : 0;
       // The cells a half-sweep updates and the neighbours they read are of
-      // different colours, so each slot updates and stores in turn; a
-      // wavefront with no active cell in a slot skips it.
+      // different colours: the neighbours of every slot are loaded first
+      // (the loads overlap), then each slot updates and stores.
+      bool act[NIT];
+      int q[NIT];
+      float nb[NIT][4];
 #pragma unroll
       for (int k = 0; k < NIT; ++k) {
         const int s = s0 + k;
-        if (valid[s] && lx[s] >= lo_x && lx[s] < hi_x && ly[s] >= lo_y && ly[s] < hi_y) {
-          const int q = ly[s] * RW + lx[s];
-          const float nv = gs_upd(b[s], c[s], b_denominator(c[s], fl[s]), fl[s], x[s], sx[q - 1], sx[q + 1],
-                                  sx[q - RW], sx[q + RW], weight, c0);
+        act[k] = valid[s] && lx[s] >= lo_x && lx[s] < hi_x && ly[s] >= lo_y && ly[s] < hi_y;
+        q[k] = lds_at(s, act[k]);
+        nb[k][0] = sx[q[k] - 1];
+        nb[k][1] = sx[q[k] + 1];
+        nb[k][2] = sx[q[k] - RW];
+        nb[k][3] = sx[q[k] + RW];
+      }
+#pragma unroll
+      for (int k = 0; k < NIT; ++k) {
+        const int s = s0 + k;
+        if (act[k]) {
+          const float nv = gs_upd(b[s], c[s], dn[s], fl[s], x[s], nb[k][0], nb[k][1], nb[k][2], nb[k][3], weight, c0);
           x[s] = nv;
-          sx[q] = nv;
+          sx[q[k]] = nv;
         }
       }
       __syncthreads();
@@ -390,6 +469,29 @@ struct Tile {
 
   __device__ __forceinline__ bool in_tile(int s) const {
     return valid[s] && lx[s] >= HALO && lx[s] < HALO + MT_W && ly[s] >= HALO && ly[s] < HALO + TH;
+  }
+
+  // The residual of every tile cell (0 outside the level) into sr, row-major
+  // over the tile; neighbour loads first, as in sweep.
+  __device__ __forceinline__ void residual(float factor, const float* sx, float* sr) const {
+    bool act[NSL];
+    int q[NSL];
+    float nb[NSL][4];
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      act[s] = in_tile(s);
+      q[s] = lds_at(s, act[s]);
+      nb[s][0] = sx[q[s] - 1];
+      nb[s][1] = sx[q[s] + 1];
+      nb[s][2] = sx[q[s] - RW];
+      nb[s][3] = sx[q[s] + RW];
+    }
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      if (!act[s]) continue;
+      const float r = inl[s] ? res_upd(b[s], c[s], fl[s], x[s], nb[s][0], nb[s][1], nb[s][2], nb[s][3], factor) : 0.0f;
+      sr[(ly[s] - HALO) * MT_W + lx[s] - HALO] = r;
+    }
   }
 };
 
@@ -431,12 +533,14 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   __shared__ double wsum[MNT / 64];
   const MSite s = mg_locate<TH>(b);
   const DevInst* I = s.I;
+  CDevInst* CI = cdev(I);
+  double* const scal = CI->scal;
   // The done flag (written by an earlier launch, usually not in this XCD's
   // L2) is read first but tested only after the tile loads are in flight, so
   // the two latencies overlap; returning then is uniform for the block.
-  const double done_flag = I->scal[SC_DONE];
-  Level L = mg_level(I, s.lvl, pcg);
-  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = I->pcg_r2;  // CG bits count in preconditioner launches only
+  const double done_flag = scal[SC_DONE];
+  Level L = mg_level(CI, s.lvl, pcg);
+  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = CI->pcg_r2;  // CG bits count in preconditioner launches only
   const bool fax = pcg && (s.sel & SEL_AXPY) != 0;
   const bool axpy = AX && fax && s.lvl == 0;  // AX: the instantiation that carries the update
   const float* xin = (s.sel & 1) ? L.xs : L.x;
@@ -447,33 +551,37 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   constexpr int XU = (MT_W * TH) / MNT;  // tile cells per thread, row-major over the tile
   float pv[XU], xv[XU];
   t.place(ox, oy, first, L.w, L.h);
-  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, sx, axpy ? I->pcg_q : nullptr, qv);
+  const bool lazy = mg_streams(L);
+  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, axpy ? CI->pcg_q : nullptr, qv, lazy);
   int iter = 0;
+  float* const p0 = CI->p0;
   if (axpy) {  // p and x of the tile's cells (coalesced)
-    iter = (int)I->scal[SC_ITER] + 1;
-    const float* p = I->pcg_p[iter & 1];
+    iter = (int)scal[SC_ITER] + 1;
+    const float* p = (iter & 1) ? CI->pcg_p[1] : CI->pcg_p[0];
 #pragma unroll
     for (int k = 0; k < XU; ++k) {
       const int u = (int)threadIdx.x + k * MNT, gx = s.tx0 + u % MT_W, gy = s.ty0 + u / MT_W;
       pv[k] = xv[k] = 0.0f;
       if (gx < L.w && gy < L.h) {
         pv[k] = ldg(p, (size_t)gy * L.w + gx);
-        xv[k] = ldg(I->p0, (size_t)gy * L.w + gx);
+        xv[k] = ldg(p0, (size_t)gy * L.w + gx);
       }
     }
   }
+  if (lazy) t.load_cf(L, ox, oy);
+  t.stage(nullptr, sx, axpy ? qv : nullptr, lazy);
   if (pcg && done_flag != 0.0) return;
   if (axpy) {
     const int tile = s.blk - b.start[s.slot];
     double pq[1];
-    fold_partials<1>(I, RED_PQ, I->cg_tiles, false, pq);
+    fold_partials<1>(I, RED_PQ, CI->cg_tiles, false, pq);
     if (!(pq[0] > 0.0)) {  // breakdown: stop before the update (as k_pcg_axpy)
-      if (tile == 0 && threadIdx.x == 0) I->scal[SC_DONE] = 1.0;
+      if (tile == 0 && threadIdx.x == 0) scal[SC_DONE] = 1.0;
       return;
     }
-    const double alpha = I->scal[SC_RZ0 + ((iter - 1) & 1)] / pq[0];
+    const double alpha = scal[SC_RZ0 + ((iter - 1) & 1)] / pq[0];
     const float fa = (float)alpha;
-    float* rout = (s.sel & SEL_R2) ? I->pcg_r : I->pcg_r2;
+    float* rout = (s.sel & SEL_R2) ? CI->pcg_r : CI->pcg_r2;
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < T::NSL; ++k) {
@@ -487,7 +595,7 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
 #pragma unroll
     for (int k = 0; k < XU; ++k) {
       const int u = (int)threadIdx.x + k * MNT, gx = s.tx0 + u % MT_W, gy = s.ty0 + u / MT_W;
-      if (gx < L.w && gy < L.h) stg(I->p0, (size_t)gy * L.w + gx, xv[k] + fa * pv[k]);
+      if (gx < L.w && gy < L.h) stg(p0, (size_t)gy * L.w + gx, xv[k] + fa * pv[k]);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
@@ -495,38 +603,30 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
     if (threadIdx.x == 0) {
       double tot = wsum[0];
       for (int w = 1; w < MNT / 64; ++w) tot += wsum[w];
-      I->red[(size_t)RED_RR * I->red_stride + tile] = tot;
+      CI->red[(size_t)RED_RR * CI->red_stride + tile] = tot;
       if (tile == 0) {
-        I->scal[SC_PQ] = pq[0];
-        I->scal[SC_ALPHA] = alpha;
-        I->scal[SC_KCUR] = (double)iter;
-        I->scal[SC_NRR] = (double)(b.start[s.slot + 1] - b.start[s.slot]);
+        scal[SC_PQ] = pq[0];
+        scal[SC_ALPHA] = alpha;
+        scal[SC_KCUR] = (double)iter;
+        scal[SC_NRR] = (double)(b.start[s.slot + 1] - b.start[s.slot]);
       }
     }
   }
-  if (fax ? s.lvl == 1 : (cg_check && !I->prefold)) {  // the CG stopping test of the iteration that just ran
+  if (fax ? s.lvl == 1 : (cg_check && !CI->prefold)) {  // the CG stopping test of the iteration that just ran
     double rr[1];  // one partial per 64 x 16 tile (k_pcg_axpy's, or this kernel's at level 0)
-    fold_partials<1>(I, RED_RR, I->cg_tiles, false, rr);
+    fold_partials<1>(I, RED_RR, CI->cg_tiles, false, rr);
     if (cg_stop(I, rr[0], s.blk == b.start[s.slot] && threadIdx.x == 0)) return;
   }
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
-  t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(I, scale), sx);
+  t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(CI, scale), sx);
   write_tile<HALO, TH>(L, xout, s.tx0, s.ty0, sx);
   if (!restrict_) return;
-  const float factor = resid_factor(I, scale);
-#pragma unroll
-  for (int k = 0; k < T::NSL; ++k) {
-    if (!t.in_tile(k)) continue;
-    const int q = t.ly[k] * T::RW + t.lx[k];
-    const float r = t.inl[k] ? res_upd(t.b[k], t.c[k], t.fl[k], t.x[k], sx[q - 1], sx[q + 1], sx[q - T::RW],
-                                       sx[q + T::RW], factor)
-                             : 0.0f;
-    sr[(t.ly[k] - HALO) * MT_W + t.lx[k] - HALO] = r;
-  }
+  const float factor = resid_factor(CI, scale);
+  t.residual(factor, sx, sr);
   __syncthreads();
   // one coarse cell per thread: 32 x TH/2 coarse cells under the tile
-  const Level Cc = I->lev[s.lvl + 1];
+  const Level Cc = lev_copy(CI->lev[s.lvl + 1]);
   const int ux = (int)threadIdx.x % 32, uy = (int)threadIdx.x / 32;
   const int cx = s.tx0 / 2 + ux, cy = s.ty0 / 2 + uy;
   if (uy >= TH / 2 || cx >= Cc.w || cy >= Cc.h) return;
@@ -547,21 +647,25 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   __shared__ float sx[T::RW * T::RH];
   __shared__ double wsum[MNT / 64];
   const MSite s = mg_locate<TH>(b);
-  const DevInst* I = s.I;
-  const double done_flag = I->scal[SC_DONE];  // tested once the tile loads are in flight
-  Level L = mg_level(I, s.lvl, pcg);
-  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = I->pcg_r2;
-  const Level C = I->lev[s.lvl + 1];
+  CDevInst* CI = cdev(s.I);
+  double* const scal = CI->scal;
+  const double done_flag = scal[SC_DONE];  // tested once the tile loads are in flight
+  Level L = mg_level(CI, s.lvl, pcg);
+  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = CI->pcg_r2;
+  const Level C = lev_copy(CI->lev[s.lvl + 1]);
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
   const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
   T t;
   t.place(ox, oy, first, L.w, L.h);
-  t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, sx);
+  const bool lazy = mg_streams(L);
+  t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, nullptr, nullptr, lazy);
+  if (lazy) t.load_cf(L, ox, oy);
+  t.stage(&C, sx, nullptr, lazy);
   if (pcg && done_flag != 0.0) return;
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
-  t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(I, scale), sx);
+  t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(CI, scale), sx);
   if (TH != MT_H || !rz) {  // r . z partials come from 64 x 16 tiles only
     write_tile<HALO, TH>(L, xout, s.tx0, s.ty0, sx);
     return;
@@ -582,8 +686,8 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
     double tot = wsum[0];
     for (int w = 1; w < MNT / 64; ++w) tot += wsum[w];
     const int tile = s.blk - b.start[s.slot];
-    I->red[(size_t)RED_RZ * I->red_stride + tile] = tot;
-    if (tile == 0) I->scal[SC_NRZ] = (double)(b.start[s.slot + 1] - b.start[s.slot]);
+    CI->red[(size_t)RED_RZ * CI->red_stride + tile] = tot;
+    if (tile == 0) scal[SC_NRZ] = (double)(b.start[s.slot + 1] - b.start[s.slot]);
   }
 }
 
@@ -619,28 +723,40 @@ __device__ __forceinline__ void b_sync() {
 
 // G threads walk a level as a TX x TY grid (no index divisions).  Within a
 // colour the update order does not matter, so every grid gives the same bits.
-template <int G>
+template <int G, int TX_ = (G >= 1024 ? 32 : (G >= 256 ? 16 : 8))>
 struct BGrid {
-  static constexpr int TX = G >= 1024 ? 32 : (G >= 256 ? 16 : 8), TY = G / TX;
+  static constexpr int TX = TX_, TY = G / TX;
   static __device__ __forceinline__ int tx() { return (int)threadIdx.x % TX; }
   static __device__ __forceinline__ int ty() { return (int)threadIdx.x / TX; }
 };
 
+// Run f<TX>() with the narrowest row width TX in {32, 16, 8} (at most G) that
+// covers n columns, so that the active threads fill whole wavefronts and the
+// others skip the loop: the VALU time of a phase follows the cells, not G.
+template <int G, typename F>
+__device__ __forceinline__ void b_cols(int n, F&& f) {
+  if (G >= 256 && n > 16) f(BGrid<G, 32>{});
+  else if (G >= 128 && n > 8) f(BGrid<G, 16>{});
+  else f(BGrid<G, 8>{});
+}
+
 template <int G>
 __device__ __forceinline__ void b_sweep(int w, int h, int off, const BotLds& m, int parity, float c0) {
-  using T = BGrid<G>;
   const int hw = (w + 1) / 2;  // cells of one colour in a row, at most
-  for (int gy = T::ty(); gy < h; gy += T::TY)
-    for (int i = T::tx(); i < hw; i += T::TX) {
-      const int gx = 2 * i + ((parity + gy) & 1);
-      if (gx >= w) continue;
-      const int q = off + gy * w + gx;
-      const int j0 = gx > 0 ? q - 1 : q, j1 = gx + 1 < w ? q + 1 : q;
-      const int j2 = gy > 0 ? q - w : q, j3 = gy + 1 < h ? q + w : q;
-      const float4 c = m.c[q];
-      const int fl = m.fl[q];
-      m.x[q] = gs_upd(m.b[q], c, b_denominator(c, fl), fl, m.x[q], m.x[j0], m.x[j1], m.x[j2], m.x[j3], 1.0f, c0);
-    }
+  b_cols<G>(hw, [&](auto grid) {
+    using T = decltype(grid);
+    for (int gy = T::ty(); gy < h; gy += T::TY)
+      for (int i = T::tx(); i < hw; i += T::TX) {
+        const int gx = 2 * i + ((parity + gy) & 1);
+        if (gx >= w) continue;
+        const int q = off + gy * w + gx;
+        const int j0 = gx > 0 ? q - 1 : q, j1 = gx + 1 < w ? q + 1 : q;
+        const int j2 = gy > 0 ? q - w : q, j3 = gy + 1 < h ? q + w : q;
+        const float4 c = m.c[q];
+        const int fl = m.fl[q];
+        m.x[q] = gs_upd(m.b[q], c, b_denominator(c, fl), fl, m.x[q], m.x[j0], m.x[j1], m.x[j2], m.x[j3], 1.0f, c0);
+      }
+  });
   b_sync<G>();
 }
 
@@ -649,7 +765,8 @@ __device__ __forceinline__ void b_sweep(int w, int h, int off, const BotLds& m, 
 template <int G>
 __device__ __forceinline__ void b_restrict(int w, int h, int off, int cw, int ch, int coff, const BotLds& m,
                                            float factor) {
-  using T = BGrid<G>;
+  b_cols<G>(cw, [&](auto grid) {
+  using T = decltype(grid);
   for (int cy = T::ty(); cy < ch; cy += T::TY)
     for (int cx = T::tx(); cx < cw; cx += T::TX) {
       const int dxs[4] = {0, 0, 1, 1}, dys[4] = {0, 1, 0, 1};  // (0,0) (0,1) (1,0) (1,1)
@@ -668,18 +785,21 @@ __device__ __forceinline__ void b_restrict(int w, int h, int off, int cw, int ch
       m.b[q] = 0.25f * (((r4[0] + r4[1]) + r4[2]) + r4[3]);
       m.x[q] = 0.0f;
     }
+  });
   b_sync<G>();
 }
 
 template <int G>
 __device__ __forceinline__ void b_prolong(int w, int h, int off, int cw, int ch, int coff, const BotLds& m) {
-  using T = BGrid<G>;
-  for (int gy = T::ty(); gy < h; gy += T::TY)
-    for (int gx = T::tx(); gx < w; gx += T::TX) {
-      const int cx = gx >> 1, cy = gy >> 1;
-      const int q = off + gy * w + gx;
-      if (cx < cw && cy < ch && (m.fl[q] & F_LSNEG)) m.x[q] = m.x[coff + cy * cw + cx] + m.x[q];
-    }
+  b_cols<G>(w, [&](auto grid) {
+    using T = decltype(grid);
+    for (int gy = T::ty(); gy < h; gy += T::TY)
+      for (int gx = T::tx(); gx < w; gx += T::TX) {
+        const int cx = gx >> 1, cy = gy >> 1;
+        const int q = off + gy * w + gx;
+        if (cx < cw && cy < ch && (m.fl[q] & F_LSNEG)) m.x[q] = m.x[coff + cy * cw + cx] + m.x[q];
+      }
+  });
   b_sync<G>();
 }
 
@@ -844,15 +964,25 @@ template <int NT>
 __device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symmetric, int pre, int post, int coarse,
                            float4* smem, double done = 0.0) {
   __shared__ int lw[kMaxLevels + 1], lh[kMaxLevels + 1], loff[kMaxLevels + 1];
+  __shared__ const float4* lcf[kMaxLevels];
+  __shared__ const int32_t* lfl[kMaxLevels];
   const int nl = I->nlev;
+  // the level table: one thread per level loads its descriptor (the loads
+  // overlap), then thread 0 forms the offsets
+  if ((int)threadIdx.x < nl - l0) {
+    const int l = l0 + (int)threadIdx.x;
+    const Level& L = mg_level(I, l, pcg);
+    lw[l] = L.w;
+    lh[l] = L.h;
+    lcf[l] = L.cf;
+    lfl[l] = L.fl;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     int total = 0;
     for (int l = l0; l < nl; ++l) {
-      const Level& L = mg_level(I, l, pcg);
-      lw[l] = L.w;
-      lh[l] = L.h;
       loff[l] = total;
-      total += L.w * L.h;
+      total += lw[l] * lh[l];
     }
     loff[nl] = total;
   }
@@ -864,15 +994,42 @@ __device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symme
   m.b = m.x + total;
   m.fl = (int*)(m.b + total);
   const bool xzero = (sel & 4) != 0;
-  for (int l = l0; l < nl; ++l) {
-    const Level L = mg_level(I, l, pcg);
-    const int off = loff[l];
-    for (int t = threadIdx.x; t < L.w * L.h; t += NT) {
-      m.c[off + t] = ldg(L.cf, t);
-      m.fl[off + t] = ldg(L.fl, t);
-      if (l == l0) {
-        m.x[t] = xzero ? 0.0f : ldg((sel & 1) ? L.xs : L.x, t);
-        m.b[t] = ldg(L.b, t);
+  // stencils of every level as one flat range (cell t of the range belongs to
+  // the level whose offsets bracket it), two cells per thread per round so
+  // that their loads are in flight together
+  {
+    const Level& L0 = mg_level(I, l0, pcg);
+    const float* xsrc = (sel & 1) ? L0.xs : L0.x;
+    const float* bsrc = L0.b;
+    const int n0 = lw[l0] * lh[l0];
+    for (int t0 = threadIdx.x; t0 < total; t0 += 2 * NT) {
+      float4 c[2];
+      int f[2], at[2];
+      float xv[2], bv[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int t = t0 + u * NT;
+        at[u] = t;
+        if (t >= total) continue;
+        int l = l0;
+        while (t >= loff[l + 1]) ++l;
+        c[u] = ldg(lcf[l], t - loff[l]);
+        f[u] = ldg(lfl[l], t - loff[l]);
+        if (t < n0) {
+          xv[u] = xzero ? 0.0f : ldg(xsrc, t);
+          bv[u] = ldg(bsrc, t);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int t = at[u];
+        if (t >= total) continue;
+        m.c[t] = c[u];
+        m.fl[t] = f[u];
+        if (t < n0) {
+          m.x[t] = xv[u];
+          m.b[t] = bv[u];
+        }
       }
     }
   }
@@ -987,21 +1144,14 @@ __device__ void coarse_down(const DevInst* I, int lvl, int sel, int tile, float*
   const int ox = tx0 - HALO, oy = ty0 - HALO;
   T t;
   t.place(ox, oy, 1, L.w, L.h);
-  t.load(L, xin, (sel & 4) != 0, nullptr, nullptr, ox, oy, sx);
+  t.load(L, xin, (sel & 4) != 0, nullptr, nullptr, ox, oy);
+  t.stage(nullptr, sx);
   __syncthreads();
   const float scale = (float)(1 << lvl);
   t.template sweep<NS>(ox, oy, L.w, L.h, L.w, L.h, 1.0f, gs_c0(I, scale), sx);
   write_tile<HALO>(L, xout, tx0, ty0, sx);
   const float factor = resid_factor(I, scale);
-#pragma unroll
-  for (int k = 0; k < T::NSL; ++k) {
-    if (!t.in_tile(k)) continue;
-    const int q = t.ly[k] * T::RW + t.lx[k];
-    const float r = t.inl[k] ? res_upd(t.b[k], t.c[k], t.fl[k], t.x[k], sx[q - 1], sx[q + 1], sx[q - T::RW],
-                                       sx[q + T::RW], factor)
-                             : 0.0f;
-    sr[(t.ly[k] - HALO) * MT_W + t.lx[k] - HALO] = r;
-  }
+  t.residual(factor, sx, sr);
   __syncthreads();
   const Level Cc = I->lev[lvl + 1];
   const int ux = (int)threadIdx.x % 32, uy = (int)threadIdx.x / 32;
@@ -1026,7 +1176,8 @@ __device__ void coarse_up(const DevInst* I, int lvl, int sel, int tile, int firs
   const int ox = tx0 - HALO, oy = ty0 - HALO;
   T t;
   t.place(ox, oy, first, L.w, L.h);
-  t.load(L, xin, (sel & 4) != 0, &C, (sel & 2) ? C.xs : C.x, ox, oy, sx);
+  t.load(L, xin, (sel & 4) != 0, &C, (sel & 2) ? C.xs : C.x, ox, oy);
+  t.stage(&C, sx);
   __syncthreads();
   t.template sweep<NS>(ox, oy, L.w, L.h, L.w, L.h, 1.0f, gs_c0(I, (float)(1 << lvl)), sx);
   write_tile<HALO>(L, xout, tx0, ty0, sx);
