@@ -382,6 +382,7 @@ Batch mg_batch(const IV& v, Extent ext, int lvl, bool with_coarse, int pcg) {
     extent_of(v[n], ext, lvl, b.ex[n], b.ey[n]);
     efl_instance* i = v[n];
     int sel = i->xsel[lvl] | (with_coarse ? i->xsel[lvl + 1] << 1 : 0) | (i->xzero[lvl] << 2);
+    if ((size_t)i->h.lev[lvl].w * i->h.lev[lvl].h >= ((size_t)1 << 20)) sel |= efl::SEL_LAZY;
     if (pcg && lvl == 0 && i->rpar) sel |= efl::SEL_R2;
     if (pcg && !with_coarse && i->fax == lvl + 1) {  // a down launch that carries the CG update
       sel |= efl::SEL_AXPY;

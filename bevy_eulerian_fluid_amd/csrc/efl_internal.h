@@ -99,6 +99,10 @@ struct DevInst {
 // to.  Values kernels do write (scal[], red[]) are read through the pointers.
 using CDevInst = const __attribute__((address_space(4))) DevInst;
 __device__ __forceinline__ CDevInst* cdev(const DevInst* I) { return (CDevInst*)I; }
+// A scal[] value an earlier launch wrote, in a scalar load: only for slots
+// the reading launch does not write before it reads them (the scalar cache
+// does not see the launch's own vector stores).
+__device__ __forceinline__ double sld(const double* p) { return *(const __attribute__((address_space(4))) double*)p; }
 
 // Fluid diagnostics (volume, min / max cell-centred speed) of one grid:
 // the fields, the per-16x16-block partials (3 (W/16)(H/16) floats) and the
@@ -159,8 +163,11 @@ __device__ __forceinline__ int xcd_tile(int blk, int nb) {
 // level-0 k_mg_down of an iteration: it also does k_pcg_axpy's update (r
 // read from the buffer SEL_R2 names, written to the other); on the level-1
 // k_mg_down after it: it runs the stopping test.  SEL_R2: level-0 b (the
-// residual) is in pcg_r2.
-constexpr int SEL_AXPY = 8, SEL_R2 = 16;
+// residual) is in pcg_r2.  SEL_LAZY (any tiled multigrid launch): the level
+// has 2^20 cells or more, so its launches are bound by the bytes they stream
+// from HBM (smaller ones by latency); a launch whose instances all carry it
+// reads only the stencils the flags need (mg_cf_needed).
+constexpr int SEL_AXPY = 8, SEL_R2 = 16, SEL_LAZY = 32;
 
 // Launch of k_mg_coarse (efl_mg.hip): for each instance a group of
 // workgroups, one per tile of its level l0 (the first level of at most

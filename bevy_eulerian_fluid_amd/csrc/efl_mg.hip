@@ -272,10 +272,6 @@ __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
 // tiled smoothers
 // ---------------------------------------------------------------------------
 
-// A level of at least 2^20 cells: its smoother launches are bound by the
-// bytes they stream from HBM (smaller ones by latency), so they read only the
-// stencils they need (Tile::load's lazy_cf).
-__device__ __forceinline__ bool mg_streams(const Level& L) { return (size_t)L.w * L.h >= ((size_t)1 << 20); }
 struct MSite {
   const DevInst* I;
   int blk;       // tile index in the launch
@@ -334,6 +330,7 @@ struct Tile {
   static constexpr int NIT = (NC + MNT - 1) / MNT;   // slots per colour
   static constexpr int NSL = 2 * NIT;
   float b[NSL], x[NSL], dn[NSL];  // dn: the Gauss-Seidel denominator of the cell's stencil
+  float q[NSL];                    // the CG update's A p at the cell (with_q)
   float4 c[NSL];
   int fl[NSL];
   int lx[NSL], ly[NSL];
@@ -354,7 +351,7 @@ struct Tile {
   }
 
   // Stencils, b and x from the level (x = 0 when xzero); with C, the
-  // coarse x (cxin) the prolongation adds; with qv, qo[s] = qv at the slot's
+  // coarse x (cxin) the prolongation adds; with with_q, q[s] = qv at the slot's
   // cell.  The loads are straight-line (a cell outside the level reads index
   // 0) and their values are masked only in stage(), so no load waits for an
   // earlier one and every wait covers just the loads it needs.  They are
@@ -365,7 +362,7 @@ struct Tile {
   bool par[NSL];
   __device__ __forceinline__ void load(const Level& L, const float* xin, bool xzero, const Level* C,
                                        const float* cxin, int ox, int oy, const float* qv = nullptr,
-                                       float* qo = nullptr, bool lazy_cf = false) {
+                                       bool with_q = false, bool lazy_cf = false) {
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
       const int gx = ox + lx[s], gy = oy + ly[s];
@@ -384,7 +381,7 @@ struct Tile {
     for (int s = 0; s < NSL; ++s) {
       const uint32_t id = inl[s] ? (uint32_t)((oy + ly[s]) * L.w + ox + lx[s]) : 0u;
       b[s] = ld32(L.b, id);
-      if (qv) qo[s] = ld32(qv, id);
+      if (with_q) q[s] = ld32(qv, id);
       if (!lazy_cf) c[s] = ld32(L.cf, id);
     }
   }
@@ -404,7 +401,7 @@ struct Tile {
   // load_cf, lazy_cf: the implied coefficients where none were read); then
   // the prolongation (x += coarse x on fluid cells with a parent), the LDS
   // copy of x and the denominators.
-  __device__ __forceinline__ void stage(const Level* C, float* sx, float* qo = nullptr, bool lazy_cf = false) {
+  __device__ __forceinline__ void stage(const Level* C, float* sx, bool with_q = false, bool lazy_cf = false) {
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
       if (lazy_cf && inl[s] && !mg_cf_needed(fl[s])) c[s] = mg_cf_implied(fl[s]);
@@ -412,7 +409,7 @@ struct Tile {
         fl[s] = 0;
         x[s] = b[s] = 0.0f;
         c[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (qo) qo[s] = 0.0f;
+        if (with_q) q[s] = 0.0f;
       }
       if (C && par[s] && (fl[s] & F_LSNEG)) x[s] = xc[s] + x[s];
       if (valid[s]) sx[ly[s] * RW + lx[s]] = x[s];
@@ -523,7 +520,7 @@ __device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, 
 // and leaves the partial of ||r||^2; the stopping test then runs in the
 // level-1 launch that follows (SEL_AXPY there).  A neighbour's halo reads the
 // old r, which no block of the launch writes, so the tiles need no order.
-template <int NS, int HALO, bool AX = false, int TH = MT_H>
+template <int NS, int HALO, bool AX = false, int TH = MT_H, bool LZ = false>
 __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, float weight, int restrict_,
                                                  int cg_check) {
   static_assert(!AX || TH == MT_H, "the fused CG update leaves one partial per 64 x 16 tile");
@@ -537,39 +534,37 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   double* const scal = CI->scal;
   // The done flag (written by an earlier launch, usually not in this XCD's
   // L2) is read first but tested only after the tile loads are in flight, so
-  // the two latencies overlap; returning then is uniform for the block.
-  const double done_flag = scal[SC_DONE];
+  // the two latencies overlap; returning then is uniform for the block.  The
+  // iteration and r . z of the CG update come from earlier launches too.
+  const double done_flag = sld(scal + SC_DONE);
   Level L = mg_level(CI, s.lvl, pcg);
   if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = CI->pcg_r2;  // CG bits count in preconditioner launches only
   const bool fax = pcg && (s.sel & SEL_AXPY) != 0;
   const bool axpy = AX && fax && s.lvl == 0;  // AX: the instantiation that carries the update
+  const int iter = axpy ? (int)sld(scal + SC_ITER) + 1 : 0;
+  const double rz_prev = axpy ? sld(scal + SC_RZ0 + ((iter - 1) & 1)) : 0.0;
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
   const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
   T t;
-  float qv[T::NSL];
   constexpr int XU = (MT_W * TH) / MNT;  // tile cells per thread, row-major over the tile
   float pv[XU], xv[XU];
   t.place(ox, oy, first, L.w, L.h);
-  const bool lazy = mg_streams(L);
-  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, axpy ? CI->pcg_q : nullptr, qv, lazy);
-  int iter = 0;
+  constexpr bool lazy = LZ;  // SEL_LAZY on every instance of the launch
+  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, CI->pcg_q, axpy, lazy);
   float* const p0 = CI->p0;
-  if (axpy) {  // p and x of the tile's cells (coalesced)
-    iter = (int)scal[SC_ITER] + 1;
+  if (axpy) {  // p and x of the tile's cells (coalesced; a cell outside the level reads index 0, unused)
     const float* p = (iter & 1) ? CI->pcg_p[1] : CI->pcg_p[0];
 #pragma unroll
     for (int k = 0; k < XU; ++k) {
       const int u = (int)threadIdx.x + k * MNT, gx = s.tx0 + u % MT_W, gy = s.ty0 + u / MT_W;
-      pv[k] = xv[k] = 0.0f;
-      if (gx < L.w && gy < L.h) {
-        pv[k] = ldg(p, (size_t)gy * L.w + gx);
-        xv[k] = ldg(p0, (size_t)gy * L.w + gx);
-      }
+      const uint32_t id = gx < L.w && gy < L.h ? (uint32_t)(gy * L.w + gx) : 0u;
+      pv[k] = ld32(p, id);
+      xv[k] = ld32(p0, id);
     }
   }
   if (lazy) t.load_cf(L, ox, oy);
-  t.stage(nullptr, sx, axpy ? qv : nullptr, lazy);
+  t.stage(nullptr, sx, axpy, lazy);
   if (pcg && done_flag != 0.0) return;
   if (axpy) {
     const int tile = s.blk - b.start[s.slot];
@@ -579,14 +574,14 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
       if (tile == 0 && threadIdx.x == 0) scal[SC_DONE] = 1.0;
       return;
     }
-    const double alpha = scal[SC_RZ0 + ((iter - 1) & 1)] / pq[0];
+    const double alpha = rz_prev / pq[0];
     const float fa = (float)alpha;
     float* rout = (s.sel & SEL_R2) ? CI->pcg_r : CI->pcg_r2;
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < T::NSL; ++k) {
       if (!t.inl[k]) continue;
-      const float rv = t.b[k] - fa * qv[k];
+      const float rv = t.b[k] - fa * t.q[k];
       t.b[k] = rv;
       if (!t.in_tile(k)) continue;
       stg(rout, (size_t)(oy + t.ly[k]) * L.w + ox + t.lx[k], rv);
@@ -641,7 +636,7 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
 // of the CG preconditioner) z is masked to the pressure unknowns and the block
 // partial of r . z goes to red[RED_RZ][tile].
 // ---------------------------------------------------------------------------
-template <int NS, int HALO, int TH = MT_H>
+template <int NS, int HALO, int TH = MT_H, bool LZ = false>
 __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, float weight, int rz) {
   using T = Tile<HALO, TH>;
   __shared__ float sx[T::RW * T::RH];
@@ -649,7 +644,7 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   const MSite s = mg_locate<TH>(b);
   CDevInst* CI = cdev(s.I);
   double* const scal = CI->scal;
-  const double done_flag = scal[SC_DONE];  // tested once the tile loads are in flight
+  const double done_flag = sld(scal + SC_DONE);  // tested once the tile loads are in flight
   Level L = mg_level(CI, s.lvl, pcg);
   if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = CI->pcg_r2;
   const Level C = lev_copy(CI->lev[s.lvl + 1]);
@@ -658,10 +653,10 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
   T t;
   t.place(ox, oy, first, L.w, L.h);
-  const bool lazy = mg_streams(L);
-  t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, nullptr, nullptr, lazy);
+  constexpr bool lazy = LZ;  // SEL_LAZY on every instance of the launch
+  t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, nullptr, false, lazy);
   if (lazy) t.load_cf(L, ox, oy);
-  t.stage(&C, sx, nullptr, lazy);
+  t.stage(&C, sx, false, lazy);
   if (pcg && done_flag != 0.0) return;
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
@@ -1249,6 +1244,8 @@ void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int fi
   }
   int nb = b.start[b.n];
   if (nb <= 0) return;
+  bool lz = true;  // lazy stencil loads when every level of the launch streams from HBM
+  for (int n = 0; n < b.n; ++n) lz = lz && (b.sel[n] & SEL_LAZY);
   if (th == MT_H8) {
     switch (nsweeps) {
       case 0: MG_LAUNCH((k_mg_down<0, 1, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
@@ -1258,15 +1255,21 @@ void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int fi
     }
     return;
   }
+#define MG_DOWN16(NS, HALO, AX)                                                                            \
+  do {                                                                                                   \
+    if (lz) MG_LAUNCH((k_mg_down<NS, HALO, AX, MT_H, true>), nb, st, b, pcg, first, weight, restrict_, cg_check); \
+    else MG_LAUNCH((k_mg_down<NS, HALO, AX, MT_H, false>), nb, st, b, pcg, first, weight, restrict_, cg_check);  \
+  } while (0)
   switch (nsweeps) {
-    case 0: MG_LAUNCH((k_mg_down<0, 1>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
-    case 1: MG_LAUNCH((k_mg_down<1, 3>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+    case 0: MG_DOWN16(0, 1, false); break;
+    case 1: MG_DOWN16(1, 3, false); break;
     case 2:
-      if (axpy) MG_LAUNCH((k_mg_down<2, 5, true>), nb, st, b, pcg, first, weight, restrict_, cg_check);
-      else MG_LAUNCH((k_mg_down<2, 5>), nb, st, b, pcg, first, weight, restrict_, cg_check);
+      if (axpy) MG_DOWN16(2, 5, true);
+      else MG_DOWN16(2, 5, false);
       break;
-    default: MG_LAUNCH((k_mg_down<3, 7>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+    default: MG_DOWN16(3, 7, false); break;
   }
+#undef MG_DOWN16
 }
 
 // prolongation + nsweeps 0..3 post-smoothing sweeps (+ r . z partials, 64 x 16 tiles only)
@@ -1286,12 +1289,20 @@ void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int firs
     }
     return;
   }
+  bool lz = true;  // lazy stencil loads when every level of the launch streams from HBM
+  for (int n = 0; n < b.n; ++n) lz = lz && (b.sel[n] & SEL_LAZY);
+#define MG_UP16(NS, HALO)                                                                 \
+  do {                                                                                    \
+    if (lz) MG_LAUNCH((k_mg_up<NS, HALO, MT_H, true>), nb, st, b, pcg, first, weight, rz);  \
+    else MG_LAUNCH((k_mg_up<NS, HALO, MT_H, false>), nb, st, b, pcg, first, weight, rz);    \
+  } while (0)
   switch (nsweeps) {
-    case 0: MG_LAUNCH((k_mg_up<0, 0>), nb, st, b, pcg, first, weight, rz); break;
-    case 1: MG_LAUNCH((k_mg_up<1, 2>), nb, st, b, pcg, first, weight, rz); break;
-    case 2: MG_LAUNCH((k_mg_up<2, 4>), nb, st, b, pcg, first, weight, rz); break;
-    default: MG_LAUNCH((k_mg_up<3, 6>), nb, st, b, pcg, first, weight, rz); break;
+    case 0: MG_UP16(0, 0); break;
+    case 1: MG_UP16(1, 2); break;
+    case 2: MG_UP16(2, 4); break;
+    default: MG_UP16(3, 6); break;
   }
+#undef MG_UP16
 }
 
 void launch_mg_coarse(hipStream_t st, const CoarseBatch& b, int pcg, int symmetric, int max_cells) {
