@@ -99,6 +99,21 @@ struct DevInst {
 // to.  Values kernels do write (scal[], red[]) are read through the pointers.
 using CDevInst = const __attribute__((address_space(4))) DevInst;
 __device__ __forceinline__ CDevInst* cdev(const DevInst* I) { return (CDevInst*)I; }
+// a level descriptor out of the constant address space, field by field
+__device__ __forceinline__ Level lev_copy(const __attribute__((address_space(4))) Level& c) {
+  Level r;
+  r.w = c.w;
+  r.h = c.h;
+  r.x = c.x;
+  r.b = c.b;
+  r.ls = c.ls;
+  r.af = c.af;
+  r.r = c.r;
+  r.xs = c.xs;
+  r.cf = c.cf;
+  r.fl = c.fl;
+  return r;
+}
 // A scal[] value an earlier launch wrote, in a scalar load: only for slots
 // the reading launch does not write before it reads them (the scalar cache
 // does not see the launch's own vector stores).

@@ -1409,18 +1409,20 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
   __shared__ float sp[RN];
   const CSite s = locate_cg(b);
   const DevInst* I = s.I;
+  CDevInst* CI = cdev(I);  // descriptor fields in scalar loads, read once
+  double* const scal = CI->scal;
   // Every load this block needs is issued before the r . z fold and the
   // done test, so their latencies overlap: the done flag and the iteration
   // (scalar), the flags of the thread's cells, z and p_old over the region,
   // then the stencils of the active cells (which the flags select).
-  const double done_flag = I->scal[SC_DONE];
-  const int iter = (int)I->scal[SC_ITER] + 1;
-  const Level& L = I->lev[0];
+  const double done_flag = sld(scal + SC_DONE);
+  const int iter = (int)sld(scal + SC_ITER) + 1;
+  const Level L = lev_copy(CI->lev[0]);
   // the first iteration takes p = z without reading the previous direction,
   // which a solve that diverged before may have left non-finite (0 * inf)
-  const float* pold = iter > 1 ? I->pcg_p[(iter - 1) & 1] : nullptr;
-  float* pnew = I->pcg_p[iter & 1];
-  const float* z = I->pcg_z;
+  const float* pold = iter > 1 ? ((iter - 1) & 1 ? CI->pcg_p[1] : CI->pcg_p[0]) : nullptr;
+  float* pnew = iter & 1 ? CI->pcg_p[1] : CI->pcg_p[0];
+  const float* z = CI->pcg_z;
   const int lx = (int)threadIdx.x % TX, ly0 = (int)threadIdx.x / TX;
   const int tx0 = s.x - lx, ty0 = s.y0 - ly0;
   int fl[CG_ROWS];
@@ -1449,27 +1451,31 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
     const bool act = (fl[r] & MG_LSNEG) && !(fl[r] & MG_CLOSED);  // active_at()
     cf[r] = act && !(fl[r] & MG_UNIT) ? L.cf[id[r]] : mg_cf_implied(fl[r]);
   }
+  // earlier launches' scalars this launch does not write
+  const bool flex = CI->cg_flex != 0;
+  const double pq_prev = sld(scal + SC_PQ);
+  const double rz_prev = sld(scal + SC_RZ0 + ((iter - 2) & 1));
   double rz[2] = {0.0, 0.0};
   double beta = 0.0;
-  if (I->prefold) {
-    rz[0] = I->scal[SC_RZF];
-    rz[1] = I->scal[SC_RZF + 1];
-  } else if (I->cg_flex) {
-    fold_partials<2>(I, RED_RZ, (int)I->scal[SC_NRZ], false, rz);
+  if (CI->prefold) {
+    rz[0] = sld(scal + SC_RZF);
+    rz[1] = sld(scal + SC_RZF + 1);
+  } else if (flex) {
+    fold_partials<2>(I, RED_RZ, (int)sld(scal + SC_NRZ), false, rz);
   } else {  // r . z of k_mg_up or k_pcg_rz: one partial per 64 x 16 tile
     double t[1];
-    fold_partials<1>(I, RED_RZ, I->cg_tiles, false, t);
+    fold_partials<1>(I, RED_RZ, CI->cg_tiles, false, t);
     rz[0] = t[0];
   }
   if (done_flag != 0.0) return;  // uniform for the block
-  if (I->cg_flex) {  // flexible CG: beta = -(z . q_prev) / (p . q)_prev
-    if (iter > 1 && I->scal[SC_PQ] != 0.0) beta = -rz[1] / I->scal[SC_PQ];
+  if (flex) {  // flexible CG: beta = -(z . q_prev) / (p . q)_prev
+    if (iter > 1 && pq_prev != 0.0) beta = -rz[1] / pq_prev;
   } else {
-    if (iter > 1) beta = rz[0] / I->scal[SC_RZ0 + ((iter - 2) & 1)];
+    if (iter > 1) beta = rz[0] / rz_prev;
   }
   if (s.lb == 0 && threadIdx.x == 0) {
-    I->scal[SC_RZ0 + ((iter - 1) & 1)] = rz[0];
-    I->scal[SC_BETA] = beta;
+    scal[SC_RZ0 + ((iter - 1) & 1)] = rz[0];
+    scal[SC_BETA] = beta;
   }
   const float fb = (float)beta;
 #pragma unroll
@@ -1478,7 +1484,8 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
     if (u < RN) sp[u] = pold ? zr[k] + fb * pr[k] : zr[k];
   }
   __syncthreads();
-  const float factor = I->dt / I->rho / I->dx / I->dx;
+  const float factor = CI->dt / CI->rho / CI->dx / CI->dx;
+  float* const qout = CI->pcg_q;
   double v[1] = {0.0};
 #pragma unroll
   for (int r = 0; r < CG_ROWS; ++r) {
@@ -1504,7 +1511,7 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
       }
       q = -res;
     }
-    I->pcg_q[id[r]] = q;
+    qout[id[r]] = q;
     v[0] += (double)pc * (double)q;
   }
   block_partials<1>(I, RED_PQ, s.lb, v);

@@ -108,21 +108,6 @@ __device__ __forceinline__ void stg(float4* p, size_t i, float4 v) {
 __device__ __forceinline__ const Level& mg_level(const DevInst* I, int l, int pcg) {
   return (pcg && l == 0) ? I->lev_pcg0 : I->lev[l];
 }
-// a level descriptor out of the constant address space, field by field
-__device__ __forceinline__ Level lev_copy(const __attribute__((address_space(4))) Level& c) {
-  Level r;
-  r.w = c.w;
-  r.h = c.h;
-  r.x = c.x;
-  r.b = c.b;
-  r.ls = c.ls;
-  r.af = c.af;
-  r.r = c.r;
-  r.xs = c.xs;
-  r.cf = c.cf;
-  r.fl = c.fl;
-  return r;
-}
 __device__ __forceinline__ Level mg_level(CDevInst* I, int l, int pcg) {
   return (pcg && l == 0) ? lev_copy(I->lev_pcg0) : lev_copy(I->lev[l]);
 }
