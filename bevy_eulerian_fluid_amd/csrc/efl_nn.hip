@@ -179,6 +179,107 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs g) {
   }
 }
 
+// The same convolution with the block's input staged in LDS once: rows
+// y0 - 1 .. y0 + 4 and pixels x0 - 1 .. x0 + 16 MBW of all CIN channels (A
+// upsampled while staging, zero outside the image), so each input element
+// comes from L2 about once instead of once per tap.  Every wavefront then
+// runs k_conv3x3's MFMA sequence with the same operands in the same order
+// (fragments from LDS), so the results are the same bits.  A pixel takes
+// CIN + 8 elements of LDS: 16-byte stores and fewer bank conflicts on the
+// 8-byte fragment reads.
+template <int CIN, int MBW>
+constexpr size_t conv_lds_bytes() { return (size_t)6 * (16 * MBW + 2) * (CIN + 8) * 2; }
+
+template <int CA, int CB, int COUT, bool UPA, bool RELU, int MBW>
+__global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs g) {
+  constexpr int CIN = CA + CB, NB = COUT / 16, TW = 16 * MBW, RW = TW + 2, PS = CIN + 8;
+  constexpr int CH = CIN / 8, NCH = 6 * RW * CH, U = (NCH + 255) / 256 < 8 ? (NCH + 255) / 256 : 8;
+  static_assert(CA % 8 == 0 && CB % 8 == 0, "16-byte channel chunks");
+  extern __shared__ uint4 conv_smem[];
+  uint16_t* sm = (uint16_t*)conv_smem;
+  if (stopped(g.stop)) return;  // uniform for the block
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int n = blockIdx.z;
+  const int y0 = blockIdx.y * 4, x0 = blockIdx.x * TW;
+  const uint16_t* A = g.a + n * g.sa;
+  const uint16_t* B = CB ? g.b + n * g.sb : nullptr;
+  const int Wa = UPA ? g.W >> 1 : g.W;
+  // staging, U chunks of 16 bytes per thread in flight at a time (a chunk
+  // outside the image reads A's first chunk and stores zeros)
+  for (int u0 = threadIdx.x; u0 < NCH; u0 += U * 256) {
+    uint4 v[U];
+    int at[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int u = u0 + k * 256;
+      const int c8 = u % CH, p = u / CH, px = p % RW, py = p / RW;
+      const int xx = x0 - 1 + px, yy = y0 - 1 + py, c = c8 * 8;
+      const bool in = u < NCH && xx >= 0 && xx < g.W && yy >= 0 && yy < g.H;
+      const uint16_t* src = A;
+      if (in) {
+        if (c < CA) {
+          const int xa = UPA ? xx >> 1 : xx, ya = UPA ? yy >> 1 : yy;
+          src = A + ((size_t)ya * Wa + xa) * CA + c;
+        } else {
+          src = B + ((size_t)yy * g.W + xx) * CB + (c - CA);
+        }
+      }
+      v[k] = *(const uint4*)src;
+      if (!in) v[k] = uint4{0u, 0u, 0u, 0u};
+      at[k] = u < NCH ? p * PS + c : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (at[k] >= 0) *(uint4*)(sm + at[k]) = v[k];
+  }
+  __syncthreads();
+  const int y = y0 + wave;
+  if (y >= g.H) return;  // no barriers below: a whole wavefront leaves
+  f32x4 acc[MBW][NB];
+#pragma unroll
+  for (int mb = 0; mb < MBW; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  // taps unrolled: the weights of the next taps load while these multiply
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const uint16_t* row = sm + (size_t)(wave + 1 + dy) * RW * PS;
+#pragma unroll
+    for (int c0 = 0; c0 < CIN; c0 += 16) {
+      bf16x4 bw[NB];
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        bw[nb] = *(const bf16x4*)(g.w + ((size_t)(tap * COUT + nb * 16 + lr) * CIN + c0 + 4 * lq));
+#pragma unroll
+      for (int mb = 0; mb < MBW; ++mb) {
+        const bf16x4 av = *(const bf16x4*)(row + (mb * 16 + lr + dx + 1) * PS + c0 + 4 * lq);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[nb], acc[mb][nb], 0, 0, 0);
+      }
+    }
+  }
+  // lane holds D[pixel 4*lq + i][channel lr] of every (M, N) block
+  uint16_t* Y = g.y + n * g.sy + (size_t)y * g.W * COUT;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int co = nb * 16 + lr;
+    const float bv = g.bias ? g.bias[co] : 0.0f;
+#pragma unroll
+    for (int mb = 0; mb < MBW; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = x0 + mb * 16 + 4 * lq + i;
+        if (x >= g.W) continue;
+        float v = acc[mb][nb][i] + bv;
+        if (RELU) v = fmaxf(v, 0.0f);
+        Y[(size_t)x * COUT + co] = f2bf(v);
+      }
+  }
+}
+
 // ---------------------------------------------------------------- pooling
 
 // out (W x H, C) = mean of the 2 x 2 children of in (2W x 2H, C)
@@ -559,15 +660,28 @@ inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 // 64-pixel strips while that gives >= 256 workgroups (one per CU), else
 // 16-pixel strips (measured at 1024^2: 16-pixel strips cost 1.7x on the
 // 256^2 level and save 2x on the 128^2 level)
+template <int CA, int CB, int COUT, bool UPA, bool RELU, int MBW>
+void conv_lds_launch(hipStream_t st, const ConvArgs& g, int n, unsigned rows) {
+  constexpr size_t bytes = conv_lds_bytes<CA + CB, MBW>();
+  static bool granted = bytes <= 65536 ||
+                        hipFuncSetAttribute(reinterpret_cast<const void*>(k_conv3x3_lds<CA, CB, COUT, UPA, RELU, MBW>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess;
+  if (granted) {
+    hipLaunchKernelGGL((k_conv3x3_lds<CA, CB, COUT, UPA, RELU, MBW>), dim3((g.W + 16 * MBW - 1) / (16 * MBW), rows, n),
+                       dim3(256), bytes, st, g);
+  } else {  // the device refused the LDS: the unstaged kernel (same results)
+    (void)hipGetLastError();
+    hipLaunchKernelGGL((k_conv3x3<CA, CB, COUT, UPA, RELU, MBW>), dim3((g.W + 16 * MBW - 1) / (16 * MBW), rows, n),
+                       dim3(256), 0, st, g);
+  }
+}
+
 template <int CA, int CB, int COUT, bool UPA, bool RELU>
 void conv_t(hipStream_t st, const ConvArgs& g, int n) {
   const unsigned rows = (unsigned)(g.H + 3) / 4;
   const bool wide = g.strip ? g.strip == 64 : (unsigned)((g.W + 63) / 64) * rows * (unsigned)n >= 256;
-  if (wide) {
-    hipLaunchKernelGGL((k_conv3x3<CA, CB, COUT, UPA, RELU, 4>), dim3((g.W + 63) / 64, rows, n), dim3(256), 0, st, g);
-  } else {
-    hipLaunchKernelGGL((k_conv3x3<CA, CB, COUT, UPA, RELU, 1>), dim3((g.W + 15) / 16, rows, n), dim3(256), 0, st, g);
-  }
+  if (wide) conv_lds_launch<CA, CB, COUT, UPA, RELU, 4>(st, g, n, rows);
+  else conv_lds_launch<CA, CB, COUT, UPA, RELU, 1>(st, g, n, rows);
 }
 
 // every (ca, cb, cout, upa, relu) the network and its input gradient use
