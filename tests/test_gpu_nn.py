@@ -218,3 +218,35 @@ def test_trained_network_gain_over_untrained():
         b.close()
     finally:
         ctx.close()
+
+
+def test_learned_vs_mgpcg_wall_time_1024(record_property):
+    """The trained network against the method without it, in wall time: on
+    the same 1024^2 coupled-tank systems (learned.evaluate: the learned
+    instance takes the MGPCG instance's state before each step), flexible CG
+    with the shipped network (MGPCG fallback when it stalls) and plain MGPCG
+    both solve to ||r|| / ||b|| <= 1e-5, each projection timed with hipEvents.
+    The ratio learned / MGPCG is printed and recorded as a test property.
+    It is above 1: the network cuts the residual for about 12 iterations and
+    then stalls, so MGPCG finishes every solve, and each network application
+    costs about four MGPCG iterations."""
+    from bevy_eulerian_fluid_amd import learned
+    w = learned.load_weights()
+    if w is None:
+        pytest.skip("no shipped weights (bevy_eulerian_fluid_amd/data/learned_preconditioner.f32)")
+    _say("nn: learned vs MGPCG wall time, 1024x1024 coupled tank, 4 steps")
+    ctx = native.Context(0)
+    try:
+        res = learned.evaluate(ctx, w, size=1024, steps=4, warmup=4, rel_tol=1e-5)
+    finally:
+        ctx.close()
+    ratio = res["learned_ms"] / res["mgpcg_ms"]
+    _say(f"  MGPCG {res['mgpcg_iters']:.1f} iterations {res['mgpcg_ms']:.3f} ms; learned "
+         f"{res['learned_iters']:.1f} flexible + {res['fallback_iters']:.1f} fallback iterations "
+         f"{res['learned_ms']:.3f} ms; wall-time ratio learned / MGPCG = {ratio:.2f}")
+    record_property("learned_over_mgpcg_wall_time_1024", round(ratio, 3))
+    assert res["converged"] == res["steps"], res["rows"]
+    for r in res["rows"]:
+        assert r["learned_rel"] <= 1e-5, r
+        assert r["learned_iters"] >= 1
+    assert np.isfinite(ratio) and ratio > 0.0
