@@ -43,19 +43,39 @@ struct Site {
   bool ok;
 };
 
-__device__ __forceinline__ Site locate(const Batch& b) {
-  int blk = blockIdx.x;
+// The batch slot of this block.  Slot 0's entries are read first, in one
+// round of scalar loads with b.n, and replaced only when the batch holds
+// more than one instance (the search over Batch::start is a chain of
+// dependent loads).
+__device__ __forceinline__ int batch_slot(const Batch& b, int blk) {
   int i = 0;
-  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
-  int lb = blk - b.start[i];
-  int tx = lb % b.tiles_x[i], ty = lb / b.tiles_x[i];
+  if (b.n > 1)
+    while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  return i;
+}
+
+__device__ __forceinline__ Site locate(const Batch& b) {
+  const int blk = blockIdx.x;
+  int st = b.start[0], tw = b.tiles_x[0], ex = b.ex[0], ey = b.ey[0], lvl = b.level[0];
+  const DevInst* I = b.inst[0];
+  const int i = batch_slot(b, blk);
+  if (i > 0) {
+    st = b.start[i];
+    tw = b.tiles_x[i];
+    ex = b.ex[i];
+    ey = b.ey[i];
+    lvl = b.level[i];
+    I = b.inst[i];
+  }
+  const int lb = blk - st;
+  const int tx = lb % tw, ty = lb / tw;
   Site s;
-  s.I = b.inst[i];
+  s.I = I;
   s.x = tx * TX + (int)(threadIdx.x % TX);
   s.y = ty * TY + (int)(threadIdx.x / TX);
-  s.ex = b.ex[i];
-  s.ey = b.ey[i];
-  s.lvl = b.level[i];
+  s.ex = ex;
+  s.ey = ey;
+  s.lvl = lvl;
   s.slot = i;
   s.ok = s.x < s.ex && s.y < s.ey;
   return s;
@@ -91,7 +111,8 @@ __device__ __forceinline__ int to_i32(float v) {
   return (int)v;
 }
 // mat4 (column-major) * (x, y, 0, 1)
-__device__ __forceinline__ float2 xform(const float* m, float x, float y) {
+template <class P>
+__device__ __forceinline__ float2 xform(P m, float x, float y) {
   float ox = ((m[0] * x + m[4] * y) + m[8] * 0.0f) + m[12] * 1.0f;
   float oy = ((m[1] * x + m[5] * y) + m[9] * 0.0f) + m[13] * 1.0f;
   return make_float2(ox, oy);
@@ -126,7 +147,8 @@ __device__ __forceinline__ float sample_v(const float* v, int w, int h, float px
   float b = lerp_w(ldz(v, w, h, i, j + 1), ldz(v, w, h, i + 1, j + 1), fi);
   return lerp_w(a, b, fj);
 }
-__device__ __forceinline__ float2 backtrace(const DevInst* I, float px, float py, float dt) {
+template <class DI>
+__device__ __forceinline__ float2 backtrace(DI I, float px, float py, float dt) {
   int W = I->W, H = I->H;
   float vx = sample_u(I->u0, W + 1, H, px, py), vy = sample_v(I->v0, W, H + 1, px, py);
   float hdt = 0.5f * dt;
@@ -141,7 +163,7 @@ __device__ __forceinline__ float2 backtrace(const DevInst* I, float px, float py
 __global__ __launch_bounds__(NT) void k_init_center(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   size_t id = (size_t)s.y * I->W + s.x;
   float level = (float)(uint32_t)(I->H - s.y);
   I->ls0[id] = level;
@@ -153,7 +175,7 @@ __global__ __launch_bounds__(NT) void k_init_center(Batch b) {
 __global__ __launch_bounds__(NT) void k_init_edge(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H;
   stz(I->u0, W + 1, H, s.x, s.y, 0.0f);
   stz(I->u1, W + 1, H, s.x, s.y, 0.0f);
@@ -174,7 +196,8 @@ __device__ __forceinline__ float box_level(float hx, float hy, float cx, float c
   return dy < 0.0f ? dx : hyp(dx, dy);
 }
 
-__device__ SrcOut eval_sources(const DevInst* I, const efl_source* src, uint32_t n, int x, int y, float level_in) {
+template <class DI>
+__device__ SrcOut eval_sources(DI I, const efl_source* src, uint32_t n, int x, int y, float level_in) {
   float dimx = (float)I->W, dimy = (float)I->H;
   float ux = (float)x / dimx, uy = (float)y / dimy;
   float nx = ux * 2.0f + (-1.0f), ny = uy * (-2.0f) + 1.0f;
@@ -205,7 +228,7 @@ __device__ SrcOut eval_sources(const DevInst* I, const efl_source* src, uint32_t
 __global__ __launch_bounds__(NT) void k_fluid_source(Batch b, int oneshot) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   uint32_t n = oneshot ? I->nsrc_init : I->nsrc;
   const efl_source* src = oneshot ? I->src_init : I->src;
   int W = I->W;
@@ -339,7 +362,7 @@ __device__ float4 obstacle_bound(const efl_obstacle& o) {
 __global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
   __shared__ float4 bnd[EFL_MAX_SOLIDS];
   Site s = locate(b);
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   for (uint32_t k = threadIdx.x; k < I->nobs; k += NT) bnd[k] = obstacle_bound(I->obs[k]);
   __syncthreads();
   if (!s.ok) return;
@@ -420,7 +443,7 @@ __global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
 __global__ __launch_bounds__(NT) void k_area_fraction(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   I->afs[(size_t)s.y * I->W + s.x] = fractions_at(I->lss, I->W, I->H, s.x, s.y);
 }
 
@@ -430,7 +453,7 @@ __global__ __launch_bounds__(NT) void k_area_fraction(Batch b) {
 __global__ __launch_bounds__(NT) void k_advect_u(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H;
   float2 q = backtrace(I, (float)s.x + (-0.5f), (float)s.y + 0.0f, I->dt);
   bool inside = q.x >= 0.0f && q.y >= 0.0f && q.x <= (float)(W + 1) - 1.0f && q.y <= (float)H - 1.0f;
@@ -440,7 +463,7 @@ __global__ __launch_bounds__(NT) void k_advect_u(Batch b) {
 __global__ __launch_bounds__(NT) void k_advect_v(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H;
   float2 q = backtrace(I, (float)s.x + 0.0f, (float)s.y + (-0.5f), I->dt);
   bool inside = q.x >= 0.0f && q.y >= 0.0f && q.x <= (float)W - 1.0f && q.y <= (float)(H + 1) - 1.0f;
@@ -451,7 +474,8 @@ __global__ __launch_bounds__(NT) void k_advect_v(Batch b) {
 // ---------------------------------------------------------------------------
 // K8 / K9 apply_forces_u, apply_forces_v (apply_forces.wgsl:17-76)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float local_force(const DevInst* I, int comp, int x, int y) {
+template <class DI>
+__device__ __forceinline__ float local_force(DI I, int comp, int x, int y) {
   const float b = -1.0f / ((2.0f * 10.0f) * 10.0f);
   float net = comp == 0 ? I->gx : I->gy;
   for (int n = (int)I->nforces - 1; n >= 0; --n) {
@@ -464,7 +488,7 @@ __device__ __forceinline__ float local_force(const DevInst* I, int comp, int x, 
 __global__ __launch_bounds__(NT) void k_forces_u(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H;
   size_t id = (size_t)s.y * (W + 1) + s.x;
   float4 fa = fractions_at(I->ls0, W, H, s.x, s.y);
@@ -476,7 +500,7 @@ __global__ __launch_bounds__(NT) void k_forces_u(Batch b) {
 __global__ __launch_bounds__(NT) void k_forces_v(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H;
   size_t id = (size_t)s.y * W + s.x;
   float4 fa = fractions_at(I->ls0, W, H, s.x, s.y);
@@ -492,7 +516,7 @@ __global__ __launch_bounds__(NT) void k_forces_v(Batch b) {
 __global__ __launch_bounds__(NT) void k_divergence(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   float4 f = fractions_at(I->lss, W, H, x, y);
   float du_f = f.y * ldz(I->u1, W + 1, H, x + 1, y) - f.x * ldz(I->u1, W + 1, H, x, y);
@@ -508,8 +532,12 @@ __global__ __launch_bounds__(NT) void k_divergence(Batch b) {
 __device__ __forceinline__ const Level& level_of(const DevInst* I, int l, int pcg) {
   return (pcg && l == 0) ? I->lev_pcg0 : I->lev[l];
 }
+__device__ __forceinline__ Level level_of(CDevInst* I, int l, int pcg) {
+  return (pcg && l == 0) ? lev_copy(I->lev_pcg0) : lev_copy(I->lev[l]);
+}
 
-__device__ __forceinline__ float gs_value(const DevInst* I, const Level& L, int x, int y, float weight, float scale) {
+template <class DI>
+__device__ __forceinline__ float gs_value(DI I, const Level& L, int x, int y, float weight, float scale) {
   size_t id = (size_t)y * L.w + x;
   float lc = L.ls[id];
   if (lc >= 0.0f) return 0.0f;
@@ -544,7 +572,7 @@ __device__ __forceinline__ float gs_value(const DevInst* I, const Level& L, int 
 __global__ __launch_bounds__(NT) void k_gs_color(Batch b, int parity, float weight, int pcg) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   const Level& L = level_of(I, s.lvl, pcg);
   if (s.x >= L.w || s.y >= L.h) return;
   if (((s.x + s.y) & 1) != parity) return;
@@ -552,7 +580,8 @@ __global__ __launch_bounds__(NT) void k_gs_color(Batch b, int parity, float weig
   L.x[(size_t)s.y * L.w + s.x] = v;
 }
 
-__device__ __forceinline__ float residual_value(const DevInst* I, const Level& L, int x, int y, float scale) {
+template <class DI>
+__device__ __forceinline__ float residual_value(DI I, const Level& L, int x, int y, float scale) {
   size_t id = (size_t)y * L.w + x;
   float phi = L.ls[id];
   if (phi >= 0.0f) return 0.0f;
@@ -581,7 +610,7 @@ __device__ __forceinline__ float residual_value(const DevInst* I, const Level& L
 __global__ __launch_bounds__(NT) void k_residual(Batch b, int pcg) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   const Level& L = level_of(I, s.lvl, pcg);
   L.r[(size_t)s.y * L.w + s.x] = residual_value(I, L, s.x, s.y, (float)(1 << s.lvl));
 }
@@ -590,9 +619,9 @@ __global__ __launch_bounds__(NT) void k_residual(Batch b, int pcg) {
 __global__ __launch_bounds__(NT) void k_restrict(Batch b, int pcg) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   const Level& F = level_of(I, s.lvl, pcg);
-  const Level& Cc = I->lev[s.lvl + 1];
+  const Level Cc = lev_copy(I->lev[s.lvl + 1]);
   int x = s.x, y = s.y, fx = 2 * x, fy = 2 * y;
   float br = 0.25f * (((ldz(F.r, F.w, F.h, fx, fy) + ldz(F.r, F.w, F.h, fx, fy + 1)) + ldz(F.r, F.w, F.h, fx + 1, fy)) +
                       ldz(F.r, F.w, F.h, fx + 1, fy + 1));
@@ -615,9 +644,9 @@ __global__ __launch_bounds__(NT) void k_restrict(Batch b, int pcg) {
 __global__ __launch_bounds__(NT) void k_prolong(Batch b, int pcg) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   const Level& F = level_of(I, s.lvl, pcg);
-  const Level& Cc = I->lev[s.lvl + 1];
+  const Level Cc = lev_copy(I->lev[s.lvl + 1]);
   float corr = Cc.x[(size_t)s.y * Cc.w + s.x];
   const int ox[4] = {0, 1, 0, 1}, oy[4] = {0, 0, 1, 1};
 #pragma unroll
@@ -634,7 +663,7 @@ __global__ __launch_bounds__(NT) void k_prolong(Batch b, int pcg) {
 __global__ __launch_bounds__(NT) void k_jacobi(Batch b, int reverse) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   const float* src = reverse ? I->p1 : I->p0;
   float* dst = reverse ? I->p0 : I->p1;
@@ -664,7 +693,7 @@ __global__ __launch_bounds__(NT) void k_jacobi(Batch b, int reverse) {
 __global__ __launch_bounds__(NT) void k_solve_u(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   size_t id = (size_t)y * (W + 1) + x;
   float factor = I->dt / (I->dx * I->rho);
@@ -679,7 +708,7 @@ __global__ __launch_bounds__(NT) void k_solve_u(Batch b) {
 __global__ __launch_bounds__(NT) void k_solve_v(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   size_t id = (size_t)y * W + x;
   float factor = I->dt / (I->dx * I->rho);
@@ -858,7 +887,7 @@ __device__ __forceinline__ float catmull(float y0, float y1, float y2, float y3,
 __global__ __launch_bounds__(NT) void k_advect_levelset(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H;
   float2 q = backtrace(I, (float)s.x, (float)s.y, I->dt);
   float nx = fminf(fmaxf(q.x, 0.0f), (float)W - 1.0f);
@@ -882,7 +911,7 @@ __global__ __launch_bounds__(NT) void k_advect_levelset(Batch b) {
 __global__ __launch_bounds__(NT) void k_jfa_init(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   float level = I->ls1[(size_t)y * W + x];
   float md = 10.0f;
@@ -907,7 +936,7 @@ __device__ __forceinline__ bool seed_ok(float2 q) { return q.x != -1.0f && q.y !
 __global__ __launch_bounds__(NT) void k_jfa_step(Batch b, int step) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   const int src_idx = s.lvl;  // per-instance ping-pong index
   const float2* si = src_idx ? I->seed1 : I->seed0;
@@ -931,7 +960,7 @@ __global__ __launch_bounds__(NT) void k_jfa_step(Batch b, int step) {
 __global__ __launch_bounds__(NT) void k_jfa_sdf(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   size_t id = (size_t)s.y * I->W + s.x;
   float2 q = (s.lvl ? I->seed1 : I->seed0)[id];
   float sdf = hyp(q.x - (float)s.x, q.y - (float)s.y);
@@ -953,7 +982,7 @@ constexpr float FIM_FAR = 1e30f;
 __global__ __launch_bounds__(NT) void k_fim_init(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   size_t id = (size_t)y * W + x;
   float level = I->ls1[id];
@@ -970,7 +999,7 @@ __global__ __launch_bounds__(NT) void k_fim_init(Batch b) {
 __global__ __launch_bounds__(NT) void k_fim_active(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   size_t id = (size_t)y * W + x;
   if (I->lab0[id] == L_SOURCE) { I->lab[id] = L_SOURCE; return; }
@@ -1113,7 +1142,7 @@ __global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
 __global__ __launch_bounds__(NT) void k_ls_gradient(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   if (x == 0 || y == 0) return;
   float gx = 0.5f * (ldz(I->ls0, W, H, x + 1, y) - ldz(I->ls0, W, H, x - 1, y));
@@ -1135,7 +1164,7 @@ __global__ void k_clear_bins(Batch b) {
 __device__ __forceinline__ int fixed_point(float v) { return to_i32(v * (1.0f / 0.1f)); }
 __global__ __launch_bounds__(NT) void k_sample_forces(Batch b) {
   Site s = locate(b);
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   int sid = -1;
   int q[4] = {0, 0, 0, 0};
   if (s.ok) {
@@ -1211,14 +1240,15 @@ __global__ void k_accumulate_forces(Batch b) {
 // Reductions (deterministic: a fixed tree per block, partials folded in
 // index order by fold_partials, efl_reduce.h).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool active_at(const DevInst* I, size_t id) {
+template <class DI>
+__device__ __forceinline__ bool active_at(DI I, size_t id) {
   return I->ls0[id] < 0.0f && !all_closed(I->afs[id]);
 }
 
 // Block partial sums: wavefront shuffles, then the 4 wave results in a
 // fixed order; thread 0 stores component k at red[(c0 + k) * red_stride + lb].
-template <int N>
-__device__ __forceinline__ void block_partials(const DevInst* I, int c0, int lb, double (&v)[N], bool is_max = false) {
+template <int N, class DI>
+__device__ __forceinline__ void block_partials(DI I, int c0, int lb, double (&v)[N], bool is_max = false) {
   __shared__ double sh[N][NT / 64];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
@@ -1252,16 +1282,24 @@ struct CSite {
   int x, y0, ex, ey, slot, lb;
 };
 __device__ __forceinline__ CSite locate_cg(const Batch& b) {
-  int blk = blockIdx.x;
-  int i = 0;
-  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  const int blk = blockIdx.x;
+  int st = b.start[0], tw = b.tiles_x[0], ex = b.ex[0], ey = b.ey[0];
+  const DevInst* I = b.inst[0];
+  const int i = batch_slot(b, blk);
+  if (i > 0) {
+    st = b.start[i];
+    tw = b.tiles_x[i];
+    ex = b.ex[i];
+    ey = b.ey[i];
+    I = b.inst[i];
+  }
   CSite s;
-  s.lb = blk - b.start[i];
-  s.I = b.inst[i];
-  s.x = (s.lb % b.tiles_x[i]) * TX + (int)(threadIdx.x % TX);
-  s.y0 = (s.lb / b.tiles_x[i]) * CG_TH + (int)(threadIdx.x / TX);
-  s.ex = b.ex[i];
-  s.ey = b.ey[i];
+  s.lb = blk - st;
+  s.I = I;
+  s.x = (s.lb % tw) * TX + (int)(threadIdx.x % TX);
+  s.y0 = (s.lb / tw) * CG_TH + (int)(threadIdx.x / TX);
+  s.ex = ex;
+  s.ey = ey;
   s.slot = i;
   return s;
 }
@@ -1271,9 +1309,9 @@ __device__ __forceinline__ CSite locate_cg(const Batch& b) {
 // also goes to pcg_r.
 __global__ __launch_bounds__(NT) void k_residual_norm(Batch b, int pcg_init) {
   const CSite s = locate_cg(b);
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   double v[2] = {0.0, 0.0};
-  const Level& L = I->lev[0];
+  const Level L = lev_copy(I->lev[0]);
   for (int r = 0; r < CG_ROWS; ++r) {
     const int y = s.y0 + TY * r;
     if (s.x >= s.ex || y >= s.ey) continue;
@@ -1381,7 +1419,7 @@ __global__ __launch_bounds__(NT) void k_prefold(Batch b, int kind) {
 // z masked to active cells, partials of r . z (RED_RZ)
 __global__ __launch_bounds__(NT) void k_pcg_rz(Batch b) {
   const CSite s = locate_cg(b);
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   if (cg_done(I)) return;
   double v[1] = {0.0};
   for (int r = 0; r < CG_ROWS; ++r) {
@@ -1524,7 +1562,7 @@ __global__ __launch_bounds__(NT) void k_pcg_apply(Batch b) {
 // block sees the same p . q, so all of them return.
 __global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
   const CSite s = locate_cg(b);
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   if (cg_done(I)) return;
   const int iter = (int)I->scal[SC_ITER] + 1;
   double t[1];
@@ -1561,7 +1599,7 @@ __global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
 // the CG initial guess (the previous pressure) restricted to the unknowns
 __global__ __launch_bounds__(NT) void k_mask_p(Batch b) {
   const CSite s = locate_cg(b);
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   for (int r = 0; r < CG_ROWS; ++r) {
     const int y = s.y0 + TY * r;
     if (s.x >= s.ex || y >= s.ey) continue;
@@ -1573,7 +1611,7 @@ __global__ __launch_bounds__(NT) void k_mask_p(Batch b) {
 // partial max over active cells of |D| (the K10 numerator) on u0/v0
 __global__ __launch_bounds__(NT) void k_max_divergence(Batch b) {
   Site s = locate(b);
-  const DevInst* I = s.I;
+  CDevInst* I = cdev(s.I);
   double v[1] = {0.0};
   if (s.ok) {
     int W = I->W, H = I->H, x = s.x, y = s.y;

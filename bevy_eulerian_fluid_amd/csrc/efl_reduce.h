@@ -49,8 +49,8 @@ constexpr int kInlineFold = 2048;
 // or a max of non-negative values unchanged: the accumulator starts at +0
 // and never becomes -0), so one round trip covers up to FOLD_U * 256.
 constexpr int FOLD_U = 8;
-template <int N>
-__device__ __forceinline__ void fold_partials(const DevInst* I, int c0, int nblk, bool is_max, double (&out)[N]) {
+template <int N, class DI>
+__device__ __forceinline__ void fold_partials(DI I, int c0, int nblk, bool is_max, double (&out)[N]) {
   __shared__ double sh[N][RED_NT / 64];
   const int t = threadIdx.x;
 #pragma unroll
@@ -81,14 +81,16 @@ __device__ __forceinline__ void fold_partials(const DevInst* I, int c0, int nblk
 
 // True once the CG solve of this instance has stopped: every kernel of an
 // iteration returns at once (the host may queue iterations ahead).
-__device__ __forceinline__ bool cg_done(const DevInst* I) { return I->scal[SC_DONE] != 0.0; }
+template <class DI>
+__device__ __forceinline__ bool cg_done(DI I) { return I->scal[SC_DONE] != 0.0; }
 
 // The stopping test of iteration k = scal[SC_KCUR] with ||r||^2 = rr: stop
 // when ||r|| / ||b|| <= rel_tol at a checked iteration (every check_every,
 // and the last) or when max_cycles iterations have run.  Every block that
 // evaluates it gets the same answer (it reads only values of earlier
 // launches); the one with `record` stores the outcome.
-__device__ __forceinline__ bool cg_stop(const DevInst* I, double rr, bool record) {
+template <class DI>
+__device__ __forceinline__ bool cg_stop(DI I, double rr, bool record) {
   double* sc = I->scal;
   const uint32_t k = (uint32_t)sc[SC_KCUR];
   bool stop = k >= I->cg_maxc;
