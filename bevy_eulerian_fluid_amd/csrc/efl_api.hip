@@ -1541,6 +1541,7 @@ int efl_ctx_create(int device, efl_ctx** out) {
   for (auto& e : c->ev) hipEventCreate(&e);
   c->bottom_cells = efl::bottom_cells_for_device(device);
   if (const char* e = getenv("EFL_COARSE_FUSE")) c->coarse_fuse = atoi(e) == 1;
+  if (c->coarse_fuse) c->bottom_cells = 1024;  // the fused coarse cycle holds its bottom in a 256-thread launch
   *out = c;
   return EFL_OK;
 }

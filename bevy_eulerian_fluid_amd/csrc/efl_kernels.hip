@@ -1311,18 +1311,57 @@ __global__ __launch_bounds__(NT) void k_residual_norm(Batch b, int pcg_init) {
   const CSite s = locate_cg(b);
   CDevInst* I = cdev(s.I);
   double v[2] = {0.0, 0.0};
+  // Level 0 is x = p0, b = div, ls = ls0, af = afs.  Every load of the
+  // thread's rows (the cell, its four neighbours' level set and x) is issued
+  // straight-line before any is used (a cell outside reads index 0); then
+  // residual_value's expression, in its order, on the loaded values.
   const Level L = lev_copy(I->lev[0]);
+  float* const rout = I->pcg_r;
+  const float dx = I->dx * 1.0f;
+  const float factor = I->dt / I->rho / dx / dx;
+  bool in[CG_ROWS], nin[CG_ROWS][4];
+  uint32_t id[CG_ROWS];
+  float phi[CG_ROWS], xc[CG_ROWS], bv[CG_ROWS], pn[CG_ROWS][4], xn[CG_ROWS][4];
+  float4 af[CG_ROWS];
+  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+#pragma unroll
   for (int r = 0; r < CG_ROWS; ++r) {
     const int y = s.y0 + TY * r;
-    if (s.x >= s.ex || y >= s.ey) continue;
-    size_t id = (size_t)y * L.w + s.x;
-    bool act = active_at(I, id);
-    float rv = residual_value(I, L, s.x, y, 1.0f);
-    if (!act) rv = 0.0f;
-    if (pcg_init) I->pcg_r[id] = rv;
-    float bv = act ? L.b[id] : 0.0f;
+    in[r] = s.x < s.ex && y < s.ey;
+    id[r] = in[r] ? (uint32_t)(y * L.w + s.x) : 0u;
+    phi[r] = L.ls[id[r]];
+    af[r] = L.af[id[r]];
+    xc[r] = L.x[id[r]];
+    bv[r] = L.b[id[r]];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int jx = s.x + ox[k], jy = y + oy[k];
+      nin[r][k] = in[r] && inb(jx, jy, L.w, L.h);
+      const uint32_t jd = nin[r][k] ? (uint32_t)(jy * L.w + jx) : 0u;
+      pn[r][k] = L.ls[jd];
+      xn[r][k] = L.x[jd];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < CG_ROWS; ++r) {
+    if (!in[r]) continue;
+    const bool act = phi[r] < 0.0f && !all_closed(af[r]);  // active_at()
+    float rv = 0.0f;
+    if (act) {  // residual_value()
+      float res = bv[r];
+      const float fv[4] = {af[r].x, af[r].y, af[r].z, af[r].w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (!nin[r][k]) continue;
+        if (pn[r][k] < 0.0f) res -= fv[k] * (xc[r] - xn[r][k]) * factor;
+        else res -= fv[k] * (1.0f - pn[r][k] / phi[r]) * xc[r] * factor;
+      }
+      rv = res;
+    }
+    if (pcg_init) rout[id[r]] = rv;
+    const float b2 = act ? bv[r] : 0.0f;
     v[0] += (double)rv * (double)rv;
-    v[1] += (double)bv * (double)bv;
+    v[1] += (double)b2 * (double)b2;
   }
   block_partials<2>(I, RED_RR, s.lb, v);
 }

@@ -934,6 +934,147 @@ __device__ void b_tail16(const BotCfg& g, const BotLds& m, int ls) {
   m.b[cq] = cb;
 }
 
+// Register blocks: the levels of the bottom solve that the whole workgroup
+// sweeps (more than BOT_SMALL cells) with thread t owning the 2 x 2 block t
+// (row-major over ceil(w / 2) x ceil(h / 2) blocks): cell k = 2 dy + dx at
+// (2 bx + dx, 2 by + dy), its x, b and flags in registers (x also in LDS for
+// the neighbours in other blocks), its stencil in LDS.  A block holds both
+// colours, so a half-sweep updates two of its cells, reading the other two
+// from registers and the rest from LDS; the four children of a coarse cell
+// are one block, so the restriction needs no other thread's residuals.
+// Every update, residual, restriction and prolongation is b_sweep's /
+// b_restrict's / b_prolong's expression with the same operands in the same
+// order (a neighbour outside the level is the cell itself), so the results
+// equal the LDS loops' bit for bit.
+struct RbLevel {
+  float x[4], b[4];
+  int fl[4];
+  bool in[4];
+  bool own;
+  int bx, by, w, h, off;  // block, level size, offset of the level in the LDS arrays
+};
+
+__device__ __forceinline__ int rb_cell(const RbLevel& L, int k) {
+  return (2 * L.by + (k >> 1)) * L.w + 2 * L.bx + (k & 1);
+}
+// the block of this thread on a level, x, b and flags from the LDS copy
+__device__ __forceinline__ void rb_load(RbLevel& L, const BotLds& m, int w, int h, int off) {
+  const int bw = (w + 1) / 2, bh = (h + 1) / 2, t = (int)threadIdx.x;
+  L.w = w;
+  L.h = h;
+  L.off = off;
+  L.own = t < bw * bh;
+  L.bx = L.own ? t % bw : 0;
+  L.by = L.own ? t / bw : 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    L.in[k] = L.own && 2 * L.bx + (k & 1) < w && 2 * L.by + (k >> 1) < h;
+    const int q = off + (L.in[k] ? rb_cell(L, k) : 0);
+    L.x[k] = L.in[k] ? m.x[q] : 0.0f;
+    L.b[k] = L.in[k] ? m.b[q] : 0.0f;
+    L.fl[k] = L.in[k] ? m.fl[q] : 0;
+  }
+}
+// the four neighbours (-x, +x, -y, +y) of cell k: those in the block from
+// registers, the others from LDS (the cell itself when outside the level; a
+// cell outside the level, whose results are dropped, reads around cell 0)
+__device__ __forceinline__ int rb_nbrs(const RbLevel& L, const float* sx, int k, float (&n)[4]) {
+  const int dx = k & 1, dy = k >> 1, gx = L.in[k] ? 2 * L.bx + dx : 0, gy = L.in[k] ? 2 * L.by + dy : 0;
+  const int q = L.off + gy * L.w + gx;
+  if (dx == 0) {
+    n[0] = sx[gx > 0 ? q - 1 : q];
+    n[1] = L.x[k + 1];
+  } else {
+    n[0] = L.x[k - 1];
+    n[1] = sx[gx + 1 < L.w ? q + 1 : q];
+  }
+  if (dy == 0) {
+    n[2] = sx[gy > 0 ? q - L.w : q];
+    n[3] = L.x[k + 2];
+  } else {
+    n[2] = L.x[k - 2];
+    n[3] = sx[gy + 1 < L.h ? q + L.w : q];
+  }
+  return q;
+}
+// one half-sweep: parity 0 = cells 0 and 3, parity 1 = cells 1 and 2 (a
+// compile-time choice, so that the block's registers are indexed statically)
+template <int G, int P>
+__device__ __forceinline__ void rb_half(RbLevel& L, const BotLds& m, float c0) {
+  constexpr int k0 = P ? 1 : 0, k1 = P ? 2 : 3;
+  if (L.own) {
+    float n0[4], n1[4];
+    const int q0 = rb_nbrs(L, m.x, k0, n0), q1 = rb_nbrs(L, m.x, k1, n1);
+    const float4 c0v = m.c[q0], c1v = m.c[q1];
+    const float v0 = gs_upd(L.b[k0], c0v, b_denominator(c0v, L.fl[k0]), L.fl[k0], L.x[k0], n0[0], n0[1], n0[2],
+                            n0[3], 1.0f, c0);
+    const float v1 = gs_upd(L.b[k1], c1v, b_denominator(c1v, L.fl[k1]), L.fl[k1], L.x[k1], n1[0], n1[1], n1[2],
+                            n1[3], 1.0f, c0);
+    if (L.in[k0]) {
+      L.x[k0] = v0;
+      m.x[q0] = v0;
+    }
+    if (L.in[k1]) {
+      L.x[k1] = v1;
+      m.x[q1] = v1;
+    }
+  }
+  b_sync<G>();
+}
+// residuals of the block's cells restricted into the coarse level at coff
+// (its b, and its x set to 0), as b_restrict
+template <int G>
+__device__ __forceinline__ void rb_restrict(const RbLevel& L, const BotLds& m, int coff, int cw, int ch,
+                                            float factor) {
+  if (L.own && L.bx < cw && L.by < ch) {
+    float r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float n[4];
+      const int q = rb_nbrs(L, m.x, k, n);
+      const float v = res_upd(L.b[k], m.c[q], L.fl[k], L.x[k], n[0], n[1], n[2], n[3], factor);
+      r[k] = L.in[k] ? v : 0.0f;
+    }
+    const int q = coff + L.by * cw + L.bx;
+    m.b[q] = 0.25f * (((r[0] + r[2]) + r[1]) + r[3]);  // children (0,0) (0,1) (1,0) (1,1)
+    m.x[q] = 0.0f;
+  }
+  b_sync<G>();
+}
+// the coarse x added to the block's fluid cells that have a parent, as b_prolong
+template <int G>
+__device__ __forceinline__ void rb_prolong(RbLevel& L, const BotLds& m, int coff, int cw, int ch) {
+  if (L.own && L.bx < cw && L.by < ch) {
+    const float cx = m.x[coff + L.by * cw + L.bx];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (L.in[k] && (L.fl[k] & F_LSNEG)) {
+        L.x[k] = cx + L.x[k];
+        m.x[L.off + rb_cell(L, k)] = L.x[k];
+      }
+  }
+  b_sync<G>();
+}
+template <int G>
+__device__ __forceinline__ void rb_pre(RbLevel& L, const BotLds& m, int pre, float c0) {
+  for (int k = 0; k < pre; ++k) {
+    rb_half<G, 1>(L, m, c0);
+    rb_half<G, 0>(L, m, c0);
+  }
+}
+template <int G>
+__device__ __forceinline__ void rb_post(RbLevel& L, const BotLds& m, int post, int symmetric, float c0) {
+  for (int k = 0; k < post; ++k) {
+    if (symmetric) {
+      rb_half<G, 0>(L, m, c0);
+      rb_half<G, 1>(L, m, c0);
+    } else {
+      rb_half<G, 1>(L, m, c0);
+      rb_half<G, 0>(L, m, c0);
+    }
+  }
+}
+
 // The bottom of the V-cycle from level l0 on, by the NT threads of one
 // workgroup, in LDS (smem: bottom_lds_bytes of the levels' cells).  sel: bit
 // 0 x of l0 is in Level::xs, bit 2 it is zero (not read); sweeps: pre, post,
@@ -946,6 +1087,7 @@ __device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symme
   __shared__ int lw[kMaxLevels + 1], lh[kMaxLevels + 1], loff[kMaxLevels + 1];
   __shared__ const float4* lcf[kMaxLevels];
   __shared__ const int32_t* lfl[kMaxLevels];
+  __shared__ int sh_ls;
   const int nl = I->nlev;
   // the level table: one thread per level loads its descriptor (the loads
   // overlap), then thread 0 forms the offsets
@@ -965,6 +1107,9 @@ __device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symme
       total += lw[l] * lh[l];
     }
     loff[nl] = total;
+    int ls = l0;  // first level small enough for wavefront 0
+    while (ls < nl && lw[ls] * lh[ls] > BOT_SMALL) ++ls;
+    sh_ls = ls;
   }
   __syncthreads();
   const int total = loff[nl];
@@ -1025,9 +1170,24 @@ __device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symme
   g.post = post;
   g.coarse = coarse;
   g.symmetric = symmetric;
-  int ls = l0;  // first level small enough for wavefront 0
-  while (ls < nl && lw[ls] * lh[ls] > BOT_SMALL) ++ls;
-  for (int l = l0; l < ls && l < nl - 1; ++l) b_down<NT>(g, m, l);
+  const int ls = sh_ls;  // first level small enough for wavefront 0
+  // one or two levels for the whole workgroup, each of at most NT 2 x 2
+  // blocks, then a tail for wavefront 0: those levels in register blocks
+  bool rb = NT == 1024 && ls < nl && (ls - l0 == 1 || ls - l0 == 2);
+  for (int l = l0; l < ls; ++l) rb = rb && ((lw[l] + 1) / 2) * ((lh[l] + 1) / 2) <= NT;
+  RbLevel A, B;
+  if (rb) {
+    rb_load(A, m, lw[l0], lh[l0], loff[l0]);
+    rb_pre<NT>(A, m, pre, gs_c0(I, (float)(1 << l0)));
+    rb_restrict<NT>(A, m, loff[l0 + 1], lw[l0 + 1], lh[l0 + 1], resid_factor(I, (float)(1 << l0)));
+    if (ls - l0 == 2) {
+      rb_load(B, m, lw[l0 + 1], lh[l0 + 1], loff[l0 + 1]);
+      rb_pre<NT>(B, m, pre, gs_c0(I, (float)(1 << (l0 + 1))));
+      rb_restrict<NT>(B, m, loff[ls], lw[ls], lh[ls], resid_factor(I, (float)(1 << (l0 + 1))));
+    }
+  } else {
+    for (int l = l0; l < ls && l < nl - 1; ++l) b_down<NT>(g, m, l);
+  }
   if (ls >= nl) {
     b_coarsest<NT>(g, m);  // the coarsest level is itself large
   } else {
@@ -1043,9 +1203,21 @@ __device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symme
     }
     __syncthreads();
   }
-  for (int l = min(ls, nl - 1) - 1; l >= l0; --l) b_up<NT>(g, m, l);
   const Level L = mg_level(I, l0, pcg);
   float* xb = (sel & 1) ? L.xs : L.x;
+  if (rb) {
+    if (ls - l0 == 2) {
+      rb_prolong<NT>(B, m, loff[ls], lw[ls], lh[ls]);
+      rb_post<NT>(B, m, post, symmetric, gs_c0(I, (float)(1 << (l0 + 1))));
+    }
+    rb_prolong<NT>(A, m, loff[l0 + 1], lw[l0 + 1], lh[l0 + 1]);
+    rb_post<NT>(A, m, post, symmetric, gs_c0(I, (float)(1 << l0)));
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (A.in[k]) stg(xb, rb_cell(A, k), A.x[k]);
+    return;
+  }
+  for (int l = min(ls, nl - 1) - 1; l >= l0; --l) b_up<NT>(g, m, l);
   for (int t = threadIdx.x; t < L.w * L.h; t += NT) stg(xb, t, m.x[t]);
 }
 
@@ -1299,23 +1471,15 @@ void launch_mg_coarse(hipStream_t st, const CoarseBatch& b, int pcg, int symmetr
 // per cell: stencil (16) + x, b (8) + flags (4)
 size_t bottom_lds_bytes(int cells) { return (size_t)cells * 28; }
 
-// The bottom takes every level of at most 1024 cells (a 32 x 32 level and
-// the ones below it).  EFL_BOTTOM_CELLS=4096 also gives it the 64 x 64 level
-// where a workgroup may hold the ~150 KB of LDS that needs; at 1024^2 the two
-// cost the same (one CU sweeps 4096 cells about as fast as two tiled launches
-// with their gaps), so the smaller footprint is the default.
+// The bottom takes every level of at most 4096 cells (a 64 x 64 level and
+// the ones below it) where a workgroup may hold the ~150 KB of LDS that needs,
+// else of at most 1024 cells (EFL_BOTTOM_CELLS=1024 asks for the smaller one).
+// With the register-block levels, at 1024^2 the larger bottom replaces two
+// tiled launches per V-cycle: 654.7 against 644.7 steps/s on MI355X.
 int bottom_cells_for_device(int device) {
   const char* e = getenv("EFL_BOTTOM_CELLS");
-  if (!e || atoi(e) != 4096) return 1024;
-  int lds = 0;
-  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 0;
-  const size_t need = bottom_lds_bytes(4096 * 4 / 3 + 64) + 1024;
-  if ((size_t)lds >= need &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(k_bottom), hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)bottom_lds_bytes(4096 * 4 / 3 + 64)) == hipSuccess)
-    return 4096;
-  (void)hipGetLastError();
-  return 1024;
+  if (e && atoi(e) == 1024) return 1024;
+  return bottom_cells_enable(device, 4096);
 }
 
 // Let the bottom solve take levels of up to `cells` cells on this device

@@ -113,8 +113,8 @@ def test_projection_parity_multiwave(gpu_ctx, size, method):
     _projection_case(gpu_ctx, size, method)
 
 
-def _projection_case(gpu_ctx, size, method, tile_h=None):
-    _say(f"projection parity {method} {size} tile_h {tile_h}")
+def _projection_case(gpu_ctx, size, method, tile_h=None, bottom_cells=1024):
+    _say(f"projection parity {method} {size} tile_h {tile_h} bottom {bottom_cells}")
     stage = "PROJECTION"
     proj = scenes.projection("multigrid", max_cycles=1)
     if method in ("gauss_seidel", "jacobi"):
@@ -135,9 +135,9 @@ def _projection_case(gpu_ctx, size, method, tile_h=None):
         proj = scenes.projection("multigrid", pre=3, post=1, coarsest=4, max_cycles=3)
     sc = small_scene(*size, proj=proj)
     g, o = make_pair(gpu_ctx, sc, OracleInstance)
-    if tile_h is not None:  # else the default: tile height by level size
+    if tile_h is not None or bottom_cells != 1024:  # else the defaults
         g.apply_tuning(pre=proj.mg_pre_iterations, post=proj.mg_post_iterations,
-                       coarsest=proj.mg_coarsest_iterations, tile_h=tile_h)
+                       coarsest=proj.mg_coarsest_iterations, tile_h=tile_h or 0, bottom_cells=bottom_cells)
     try:
         _warm_oracle(o, sc)
         for st in STEP_ORDER[:STEP_ORDER.index("PROJECTION")]:
@@ -363,6 +363,17 @@ def test_projection_parity_tile_h(gpu_ctx, size, method, tile_h):
     bit-exact with the oracle (MGPCG: its usual tolerance; its level 0 keeps
     64 x 16)."""
     _projection_case(gpu_ctx, size, method, tile_h=tile_h)
+
+
+@pytest.mark.parametrize("method", ["mg_vcycle", "multigrid", "multigrid_3cycles", "mgpcg"])
+@pytest.mark.parametrize("size", [(100, 70), (200, 136), (1000, 700), (1024, 1024)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_projection_parity_bottom_4096(gpu_ctx, size, method):
+    """The bottom solve from the first level of at most 4096 cells (64 x 64
+    at 1024^2; 50 x 35 and 50 x 34 at the small sizes): two levels for the
+    whole workgroup in register blocks, including partial blocks at odd
+    sizes, then the wavefront-0 tail: bit-exact with the oracle (MGPCG:
+    its usual tolerance)."""
+    _projection_case(gpu_ctx, size, method, bottom_cells=4096)
 
 
 def test_mixed_tile_heights_in_one_batch(gpu_ctx):
