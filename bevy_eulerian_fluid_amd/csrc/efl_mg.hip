@@ -947,7 +947,7 @@ __device__ void b_tail16(const BotCfg& g, const BotLds& m, int ls) {
 // order (a neighbour outside the level is the cell itself), so the results
 // equal the LDS loops' bit for bit.
 struct RbLevel {
-  float x[4], b[4];
+  float x[4], b[4], dn[4];  // dn: the Gauss-Seidel denominator of the cell's stencil
   int fl[4];
   bool in[4];
   bool own;
@@ -973,6 +973,7 @@ __device__ __forceinline__ void rb_load(RbLevel& L, const BotLds& m, int w, int 
     L.x[k] = L.in[k] ? m.x[q] : 0.0f;
     L.b[k] = L.in[k] ? m.b[q] : 0.0f;
     L.fl[k] = L.in[k] ? m.fl[q] : 0;
+    L.dn[k] = b_denominator(m.c[q], L.fl[k]);
   }
 }
 // the four neighbours (-x, +x, -y, +y) of cell k: those in the block from
@@ -1006,10 +1007,8 @@ __device__ __forceinline__ void rb_half(RbLevel& L, const BotLds& m, float c0) {
     float n0[4], n1[4];
     const int q0 = rb_nbrs(L, m.x, k0, n0), q1 = rb_nbrs(L, m.x, k1, n1);
     const float4 c0v = m.c[q0], c1v = m.c[q1];
-    const float v0 = gs_upd(L.b[k0], c0v, b_denominator(c0v, L.fl[k0]), L.fl[k0], L.x[k0], n0[0], n0[1], n0[2],
-                            n0[3], 1.0f, c0);
-    const float v1 = gs_upd(L.b[k1], c1v, b_denominator(c1v, L.fl[k1]), L.fl[k1], L.x[k1], n1[0], n1[1], n1[2],
-                            n1[3], 1.0f, c0);
+    const float v0 = gs_upd(L.b[k0], c0v, L.dn[k0], L.fl[k0], L.x[k0], n0[0], n0[1], n0[2], n0[3], 1.0f, c0);
+    const float v1 = gs_upd(L.b[k1], c1v, L.dn[k1], L.fl[k1], L.x[k1], n1[0], n1[1], n1[2], n1[3], 1.0f, c0);
     if (L.in[k0]) {
       L.x[k0] = v0;
       m.x[q0] = v0;
@@ -1120,19 +1119,20 @@ __device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symme
   m.fl = (int*)(m.b + total);
   const bool xzero = (sel & 4) != 0;
   // stencils of every level as one flat range (cell t of the range belongs to
-  // the level whose offsets bracket it), two cells per thread per round so
+  // the level whose offsets bracket it), four cells per thread per round so
   // that their loads are in flight together
   {
     const Level& L0 = mg_level(I, l0, pcg);
     const float* xsrc = (sel & 1) ? L0.xs : L0.x;
     const float* bsrc = L0.b;
     const int n0 = lw[l0] * lh[l0];
-    for (int t0 = threadIdx.x; t0 < total; t0 += 2 * NT) {
-      float4 c[2];
-      int f[2], at[2];
-      float xv[2], bv[2];
+    constexpr int SU = 4;
+    for (int t0 = threadIdx.x; t0 < total; t0 += SU * NT) {
+      float4 c[SU];
+      int f[SU], at[SU];
+      float xv[SU], bv[SU];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < SU; ++u) {
         const int t = t0 + u * NT;
         at[u] = t;
         if (t >= total) continue;
@@ -1146,7 +1146,7 @@ __device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symme
         }
       }
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < SU; ++u) {
         const int t = at[u];
         if (t >= total) continue;
         m.c[t] = c[u];
