@@ -49,7 +49,9 @@ struct StepGraph {
   std::vector<efl_instance*> insts;
   uint32_t dt_bits = 0;
   uint64_t version = 0;
-  hipGraphExec_t pre = nullptr, iter = nullptr, post = nullptr;
+  // iter[0] / iter[1]: the odd / even CG iterations (a fused iteration's
+  // residual buffers alternate)
+  hipGraphExec_t pre = nullptr, iter[2] = {nullptr, nullptr}, post = nullptr;
 };
 
 struct efl_ctx {
@@ -80,6 +82,11 @@ struct efl_ctx {
   // the coarse levels of a V-cycle as one persistent launch (k_mg_coarse):
   // EFL_COARSE_FUSE=1 at context creation; off by default (measured slower)
   bool coarse_fuse = false;
+  // MGPCG iterations fused into the preconditioner's launches wherever an
+  // instance allows it (fused_axpy); EFL_CG_FUSE=0 at context creation runs
+  // every iteration as k_pcg_apply + k_pcg_axpy (the textbook form of CG,
+  // whose rounding differs)
+  bool cg_fuse = true;
 };
 
 struct efl_instance {
@@ -111,10 +118,11 @@ struct efl_instance {
   // per multigrid level: 1 while x is known to be zero (not read)
   uint8_t xzero[efl::kMaxLevels + 1] = {};
   uint32_t cg_last = 0;  // CG iterations of the previous solve
-  // fused CG update (k_mg_down's SEL_AXPY): the residual buffer that holds
-  // the current r (0: pcg_r, 1: pcg_r2), and the launch that carries the
-  // update next (1: the level-0 k_mg_down, 2: the level-1 one, which runs
-  // the stopping test; 0: none)
+  // fused CG iterations (k_mg_down's SEL_AXPY): the residual buffer that
+  // holds the current r (0: pcg_r, 1: pcg_r2; in a fused solve 1 before the
+  // even iterations), and the launch that carries the iteration next (1: the
+  // level-0 k_mg_down, 2: the level-1 one, which runs the stopping test; 0:
+  // none)
   uint8_t rpar = 0, fax = 0;
   // learned preconditioner (efl_set_learned): the network and this
   // instance's activations (padded to multiples of 8)
@@ -384,7 +392,7 @@ Batch mg_batch(const IV& v, Extent ext, int lvl, bool with_coarse, int pcg) {
     int sel = i->xsel[lvl] | (with_coarse ? i->xsel[lvl + 1] << 1 : 0) | (i->xzero[lvl] << 2);
     if ((size_t)i->h.lev[lvl].w * i->h.lev[lvl].h >= ((size_t)1 << 20)) sel |= efl::SEL_LAZY;
     if (pcg && lvl == 0 && i->rpar) sel |= efl::SEL_R2;
-    if (pcg && !with_coarse && i->fax == lvl + 1) {  // a down launch that carries the CG update
+    if (pcg && !with_coarse && i->fax == lvl + 1) {  // a down launch that carries the CG iteration
       sel |= efl::SEL_AXPY;
       if (lvl == 0) i->rpar ^= 1;  // r is written to the other buffer
       i->fax = lvl == 0 ? 2 : 0;
@@ -801,19 +809,23 @@ void cg_begin(hipStream_t st, const IV& cg) {
   reset_fused(cg);
   precondition(st, cg);
 }
-// With fuse (eager launches: the host tracks the residual buffer of each
-// iteration, which a replayed graph could not), instances whose first two
-// down launches are tiled 2-sweep launches take k_pcg_axpy's update in the
-// level-0 one and the stopping test in the level-1 one.
-void cg_iteration(hipStream_t st, const IV& cg, bool fuse = true) {
-  Batch b = cg_batch(cg);
-  efl::launch_pcg_apply(st, b);  // p = z + beta p, q = A p
-  prefold(st, cg, 1);
+// With fuse, instances whose first two down launches are tiled 2-sweep
+// launches run the whole iteration (k_pcg_apply's and k_pcg_axpy's work, in
+// the Chronopoulos-Gear form) in the level-0 one and the stopping test in the
+// level-1 one.  Such an instance's residual changes buffer every iteration
+// (the host tracks it), so a replayed graph captures an odd and an even
+// iteration.
+void cg_iteration(hipStream_t st, const IV& cg, bool fuse) {
   IV fx = fuse ? filter(cg, fused_axpy) : IV{};
   if (fx.size() < cg.size()) {
     IV plain = filter(cg, [&](efl_instance* i) { return std::find(fx.begin(), fx.end(), i) == fx.end(); });
-    efl::launch_pcg_axpy(st, fx.empty() ? b : cg_batch(plain));  // x, r, ||r||^2
+    Batch b = cg_batch(plain);
+    efl::launch_pcg_apply(st, b);  // p = z + beta p, q = A p
+    prefold(st, plain, 1);
+    efl::launch_pcg_axpy(st, b);  // x, r, ||r||^2
   }
+  // the others run the whole iteration (p, A p, x, r) in the level-0 launch
+  // of the preconditioner
   for (auto* i : fx) i->fax = 1;
   IV sep = filter(cg, [](efl_instance* i) { return !check_in_smoother(i); });
   if (!sep.empty()) efl::launch_cg_check(st, cg_batch(sep));
@@ -867,7 +879,8 @@ int resolve_stats(efl_instance* i, bool block) {
 // slot this solve reuses holds the solve before last, whose statistics are
 // taken first (a wait only if the device has not finished that step), which
 // keeps the estimate at most two steps old.
-int cg_run_async(efl_ctx* c, const IV& cg, hipGraphExec_t iter) {
+// iter: null (eager launches) or the graphs of the odd and even iterations.
+int cg_run_async(efl_ctx* c, const IV& cg, const hipGraphExec_t* iter) {
   hipStream_t st = c->stream;
   uint32_t n = 0;
   for (auto* i : cg) {
@@ -888,8 +901,8 @@ int cg_run_async(efl_ctx* c, const IV& cg, hipGraphExec_t iter) {
     n = std::max(n, std::min(base + c->async_margin, i->h.cg_maxc));
   }
   for (uint32_t k = 0; k < n; ++k) {
-    if (iter) HIPCHK(c, hipGraphLaunch(iter, st));
-    else cg_iteration(st, cg);
+    if (iter) HIPCHK(c, hipGraphLaunch(iter[k & 1], st));
+    else cg_iteration(st, cg, c->cg_fuse);
   }
   efl::launch_cg_tally(st, cg_batch(cg));
   for (auto* i : cg) {
@@ -905,16 +918,16 @@ int cg_run_async(efl_ctx* c, const IV& cg, hipGraphExec_t iter) {
   return EFL_OK;
 }
 
-int cg_run(efl_ctx* c, const IV& cg, hipGraphExec_t iter, bool sync = false) {
+int cg_run(efl_ctx* c, const IV& cg, const hipGraphExec_t* iter, bool sync = false) {
   if (c->async_margin && !sync) return cg_run_async(c, cg, iter);
   hipStream_t st = c->stream;
-  uint32_t n = 0;
+  uint32_t n = 0, queued = 0;
   for (auto* i : cg) n = std::max(n, std::min(i->cg_last ? i->cg_last : 4u, i->h.cg_maxc));
   IV act = cg;
   while (!act.empty()) {
-    for (uint32_t k = 0; k < n; ++k) {
-      if (iter) HIPCHK(c, hipGraphLaunch(iter, st));
-      else cg_iteration(st, act);
+    for (uint32_t k = 0; k < n; ++k, ++queued) {
+      if (iter) HIPCHK(c, hipGraphLaunch(iter[queued & 1], st));
+      else cg_iteration(st, act, c->cg_fuse);
     }
     n = 1;
     int rc = read_scalars(c, act);
@@ -1357,7 +1370,7 @@ void integrate_bodies(efl_ctx* c, const IV& stepped, float dt) {
 // ---- replayed steps -------------------------------------------------------------
 void drop_graphs(efl_ctx* c) {
   for (auto& g : c->graphs)
-    for (hipGraphExec_t e : {g.pre, g.iter, g.post})
+    for (hipGraphExec_t e : {g.pre, g.iter[0], g.iter[1], g.post})
       if (e) hipGraphExecDestroy(e);
   c->graphs.clear();
 }
@@ -1411,20 +1424,21 @@ int run_step_graph(efl_ctx* c, const IV& v, float dt, bool* used) {
       proj_fixed(st, fixed);
       if (!cg.empty()) cg_begin(st, cg);
     });
-    if (!cg.empty()) n.iter = capture(c, [&] { cg_iteration(st, cg, false); });
+    for (int k = 0; k < 2 && !cg.empty(); ++k) n.iter[k] = capture(c, [&] { cg_iteration(st, cg, c->cg_fuse); });
     n.post = capture(c, [&] {
       phase_velocity(st, v);
       phase_post(st, v);
       integrate_bodies(c, v, dt);
     });
-    bool ok = n.pre && n.post && (cg.empty() || n.iter);
+    bool ok = n.pre && n.post && (cg.empty() || (n.iter[0] && n.iter[1]));
+    for (auto* i : cg) ok = ok && i->rpar == 0;  // the odd iteration's buffers again
     for (size_t k = 0; k < v.size() && ok; ++k) {
       std::vector<uint8_t> now(v[k]->xsel, v[k]->xsel + efl::kMaxLevels + 1);
       now.insert(now.end(), v[k]->xzero, v[k]->xzero + efl::kMaxLevels + 1);
       ok = now == snap[k];
     }
     if (!ok) {  // run this configuration eagerly from now on
-      for (hipGraphExec_t e : {n.pre, n.iter, n.post})
+      for (hipGraphExec_t e : {n.pre, n.iter[0], n.iter[1], n.post})
         if (e) hipGraphExecDestroy(e);
       (void)hipGetLastError();
       c->graphs_on = false;
@@ -1435,7 +1449,7 @@ int run_step_graph(efl_ctx* c, const IV& v, float dt, bool* used) {
       return EFL_OK;
     }
     if (c->graphs.size() >= 8) {
-      for (hipGraphExec_t e : {c->graphs[0].pre, c->graphs[0].iter, c->graphs[0].post})
+      for (hipGraphExec_t e : {c->graphs[0].pre, c->graphs[0].iter[0], c->graphs[0].iter[1], c->graphs[0].post})
         if (e) hipGraphExecDestroy(e);
       c->graphs.erase(c->graphs.begin());
     }
@@ -1446,7 +1460,7 @@ int run_step_graph(efl_ctx* c, const IV& v, float dt, bool* used) {
   proj_fixed_stats(v);
   HIPCHK(c, hipGraphLaunch(g->pre, st));
   if (!cg.empty()) {
-    int rc = cg_run(c, cg, g->iter);
+    int rc = cg_run(c, cg, g->iter);  // the odd and even iterations in turn
     if (rc) return rc;
     if (!c->async_margin) {
       rc = resolve_sealed(c, cg);  // eager launches on this rare path
@@ -1541,6 +1555,7 @@ int efl_ctx_create(int device, efl_ctx** out) {
   for (auto& e : c->ev) hipEventCreate(&e);
   c->bottom_cells = efl::bottom_cells_for_device(device);
   if (const char* e = getenv("EFL_COARSE_FUSE")) c->coarse_fuse = atoi(e) == 1;
+  if (const char* e = getenv("EFL_CG_FUSE")) c->cg_fuse = atoi(e) != 0;
   if (c->coarse_fuse) c->bottom_cells = 1024;  // the fused coarse cycle holds its bottom in a 256-thread launch
   *out = c;
   return EFL_OK;
@@ -2041,11 +2056,15 @@ int efl_run_stage(efl_instance* in, int stage, float dt) {
 namespace {
 // efl_time_stage(CG_SMOOTH): k_mg_down<2,5,true> on level 0 as an MGPCG
 // iteration launches it.  The solve starts from x = 0 (so the initial test
-// does not stop it and p . q > 0), k_pcg_apply forms p, q and the p . q
-// partials, then the same launch (r read from pcg_r, written to pcg_r2,
-// x += alpha p, z known to be zero, residual and restriction of b) runs reps
-// times between two events.  The pressure and the buffer-pair flags are put
-// back afterwards.
+// does not stop it), its first preconditioner leaves the r . z and z . A z
+// partials; the launch then runs as a later iteration (the count set to 1,
+// so beta is formed) with an odd iteration's buffers: p_prev = 0 in
+// pcg_p[0], the previous r . z (slot 1) set to 1e300 and alpha to 1, so
+// beta ~ 0 and p . A p ~ z . A z > 0; p = z + beta p_prev, q = A p, r read
+// from pcg_r and written to pcg_r2, x += alpha p, z known to be zero,
+// residual and restriction of b, reps times between two events (the
+// scalars it reads do not change).  The pressure and the buffer-pair flags
+// are put back afterwards.
 int time_cg_smooth(efl_instance* in, float dt, uint32_t reps, float* ms) {
   efl_ctx* c = in->ctx;
   if (!is_cg(in) || !fused_axpy(in))
@@ -2077,11 +2096,16 @@ int time_cg_smooth(efl_instance* in, float dt, uint32_t reps, float* ms) {
       hipMemsetAsync(in->h.p0, 0, bytes, st) != hipSuccess)
     return done(fail(c, EFL_E_HIP, "CG_SMOOTH: saving the pressure"));
   cg_begin(st, v);
-  efl::launch_pcg_apply(st, cg_batch(v));
-  prefold(st, v, 1);
+  const double one = 1.0, big = 1e300;
+  if (hipMemsetAsync(in->h.pcg_p[0], 0, bytes, st) != hipSuccess ||
+      hipMemcpyAsync(in->h.scal + efl::SC_ITER, &one, sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(in->h.scal + efl::SC_RZ0 + 1, &big, sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(in->h.scal + efl::SC_AL0 + 1, &one, sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return done(fail(c, EFL_E_HIP, "CG_SMOOTH: setting up the iteration"));
   in->xsel[0] = 0;  // the preconditioner's z starts at zero: not read
   in->xzero[0] = 1;
-  in->rpar = 0;
+  in->rpar = 0;  // r read from pcg_r, written to pcg_r2 (an odd iteration's buffers)
   in->fax = 1;
   const Batch b = mg_batch(v, EX_LEVEL, 0, false, 1);  // SEL_AXPY: the update rides on this launch
   efl::launch_mg_down(st, b, 1, 2, 1, 1.0f, 1, 1);       // warm

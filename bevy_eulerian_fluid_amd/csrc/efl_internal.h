@@ -74,7 +74,7 @@ struct DevInst {
   Level lev_pcg0;
   // PCG vectors (W x H) and scalars
   float *pcg_r, *pcg_z, *pcg_q;
-  float* pcg_r2;    // second residual buffer of the fused x / r update (Batch::sel SEL_R2)
+  float* pcg_r2;    // second residual buffer of the fused CG iteration (Batch::sel SEL_R2)
   float* pcg_p[2];  // search direction of iteration k in pcg_p[k & 1]
   double* red;         // reduction partials: 4 x red_stride doubles
   int red_stride;      // >= blocks of one full-grid launch
@@ -175,10 +175,14 @@ __device__ __forceinline__ int xcd_tile(int blk, int nb) {
 // per-instance ranges; inside a range, blocks tile the instance's extent.
 // Batch::sel bits of the level-0 / level-1 CG preconditioner launches (bits
 // 0..2 are the fused multigrid's buffer selection): SEL_AXPY on the first
-// level-0 k_mg_down of an iteration: it also does k_pcg_axpy's update (r
-// read from the buffer SEL_R2 names, written to the other); on the level-1
-// k_mg_down after it: it runs the stopping test.  SEL_R2: level-0 b (the
-// residual) is in pcg_r2.  SEL_LAZY (any tiled multigrid launch): the level
+// level-0 k_mg_down of an iteration: it also runs the iteration itself (p,
+// A p, x and r; r read from the buffer SEL_R2 names, written to the other);
+// on the level-1 k_mg_down after it: it runs the stopping test.  SEL_R2:
+// level-0 b (the residual) is in pcg_r2; a fused solve's residual changes
+// buffer every iteration (the first reads pcg_r), so on its SEL_AXPY launch
+// SEL_R2 also marks an even iteration, which names the buffers of p (p of
+// iteration k in pcg_p[k & 1]) and the scalar slots the launch reads and
+// writes.  SEL_LAZY (any tiled multigrid launch): the level
 // has 2^20 cells or more, so its launches are bound by the bytes they stream
 // from HBM (smaller ones by latency); a launch whose instances all carry it
 // reads only the stencils the flags need (mg_cf_needed).

@@ -1615,6 +1615,7 @@ __global__ __launch_bounds__(NT) void k_pcg_axpy(Batch b) {
   if (s.lb == 0 && threadIdx.x == 0) {
     I->scal[SC_PQ] = t[0];
     I->scal[SC_ALPHA] = alpha;
+    I->scal[SC_AL0 + ((iter - 1) & 1)] = alpha;  // what the fused form of the next iteration reads
   }
   if (s.lb == 0 && threadIdx.x == 0) {
     I->scal[SC_KCUR] = (double)iter;

@@ -170,14 +170,16 @@ __device__ __forceinline__ float gs_upd(float b, float4 c, float dn, int fl, flo
 }
 
 // residual.wgsl:12-57 with the stencil precomputed: fluid neighbours
-// subtract c_k (x - x_k) factor, air neighbours c_k x factor.
+// subtract c_k (x - x_k) factor, air neighbours c_k x factor.  The operand
+// (x - x_k or x) is selected first, so each face costs one product chain;
+// the rounding is that of the two separate expressions.
 __device__ __forceinline__ float res_upd(float b, float4 c, int fl, float xc, float x0, float x1, float x2, float x3,
                                          float factor) {
   float res = b;
-  res = fsel<F_IN0 + 0>(fl, fsel<F_FL0 + 0>(fl, res - c.x * (xc - x0) * factor, res - c.x * xc * factor), res);
-  res = fsel<F_IN0 + 1>(fl, fsel<F_FL0 + 1>(fl, res - c.y * (xc - x1) * factor, res - c.y * xc * factor), res);
-  res = fsel<F_IN0 + 2>(fl, fsel<F_FL0 + 2>(fl, res - c.z * (xc - x2) * factor, res - c.z * xc * factor), res);
-  res = fsel<F_IN0 + 3>(fl, fsel<F_FL0 + 3>(fl, res - c.w * (xc - x3) * factor, res - c.w * xc * factor), res);
+  res = fsel<F_IN0 + 0>(fl, res - c.x * fsel<F_FL0 + 0>(fl, xc - x0, xc) * factor, res);
+  res = fsel<F_IN0 + 1>(fl, res - c.y * fsel<F_FL0 + 1>(fl, xc - x1, xc) * factor, res);
+  res = fsel<F_IN0 + 2>(fl, res - c.z * fsel<F_FL0 + 2>(fl, xc - x2, xc) * factor, res);
+  res = fsel<F_IN0 + 3>(fl, res - c.w * fsel<F_FL0 + 3>(fl, xc - x3, xc) * factor, res);
   return fsel<9>(fl, 0.0f, res);  // F_RZ
 }
 
@@ -315,7 +317,6 @@ struct Tile {
   static constexpr int NIT = (NC + MNT - 1) / MNT;   // slots per colour
   static constexpr int NSL = 2 * NIT;
   float b[NSL], x[NSL], dn[NSL];  // dn: the Gauss-Seidel denominator of the cell's stencil
-  float q[NSL];                    // the CG update's A p at the cell (with_q)
   float4 c[NSL];
   int fl[NSL];
   int lx[NSL], ly[NSL];
@@ -336,18 +337,16 @@ struct Tile {
   }
 
   // Stencils, b and x from the level (x = 0 when xzero); with C, the
-  // coarse x (cxin) the prolongation adds; with with_q, q[s] = qv at the slot's
-  // cell.  The loads are straight-line (a cell outside the level reads index
+  // coarse x (cxin) the prolongation adds.  The loads are straight-line (a cell outside the level reads index
   // 0) and their values are masked only in stage(), so no load waits for an
   // earlier one and every wait covers just the loads it needs.  They are
   // issued in the order they are consumed: the flags, x and the coarse x
-  // (the LDS copy), then b, q and the coefficients.  With lazy_cf the
+  // (the LDS copy), then b and the coefficients.  With lazy_cf the
   // coefficients are left for load_cf.
   float xc[NSL];
   bool par[NSL];
   __device__ __forceinline__ void load(const Level& L, const float* xin, bool xzero, const Level* C,
-                                       const float* cxin, int ox, int oy, const float* qv = nullptr,
-                                       bool with_q = false, bool lazy_cf = false) {
+                                       const float* cxin, int ox, int oy, bool lazy_cf = false) {
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
       const int gx = ox + lx[s], gy = oy + ly[s];
@@ -366,7 +365,6 @@ struct Tile {
     for (int s = 0; s < NSL; ++s) {
       const uint32_t id = inl[s] ? (uint32_t)((oy + ly[s]) * L.w + ox + lx[s]) : 0u;
       b[s] = ld32(L.b, id);
-      if (with_q) q[s] = ld32(qv, id);
       if (!lazy_cf) c[s] = ld32(L.cf, id);
     }
   }
@@ -382,11 +380,11 @@ struct Tile {
       c[s] = ld32(L.cf, need ? (uint32_t)((oy + ly[s]) * L.w + ox + lx[s]) : 0u);
     }
   }
-  // Cells outside the level take flags, x, b, q and coefficients 0 (after
+  // Cells outside the level take flags, x, b and coefficients 0 (after
   // load_cf, lazy_cf: the implied coefficients where none were read); then
   // the prolongation (x += coarse x on fluid cells with a parent), the LDS
   // copy of x and the denominators.
-  __device__ __forceinline__ void stage(const Level* C, float* sx, bool with_q = false, bool lazy_cf = false) {
+  __device__ __forceinline__ void stage(const Level* C, float* sx, bool lazy_cf = false) {
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
       if (lazy_cf && inl[s] && !mg_cf_needed(fl[s])) c[s] = mg_cf_implied(fl[s]);
@@ -394,7 +392,6 @@ struct Tile {
         fl[s] = 0;
         x[s] = b[s] = 0.0f;
         c[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (with_q) q[s] = 0.0f;
       }
       if (C && par[s] && (fl[s] & F_LSNEG)) x[s] = xc[s] + x[s];
       if (valid[s]) sx[ly[s] * RW + lx[s]] = x[s];
@@ -498,20 +495,34 @@ __device__ __forceinline__ void write_tile(const Level& L, float* xout, int x0, 
 // cell sees exact neighbours.
 // ---------------------------------------------------------------------------
 //
-// With SEL_AXPY at level 0 (the AX instantiation) the launch first finishes the CG iteration that
-// k_pcg_apply began (k_pcg_axpy's work): every block folds p . q, alpha =
-// rz / pq, stages b = r - alpha q (r from the buffer SEL_R2 names) over its
-// region, writes the new r of its tile to the other buffer and x += alpha p,
-// and leaves the partial of ||r||^2; the stopping test then runs in the
-// level-1 launch that follows (SEL_AXPY there).  A neighbour's halo reads the
-// old r, which no block of the launch writes, so the tiles need no order.
+// With SEL_AXPY at level 0 (the AX instantiation) the launch first runs a
+// whole CG iteration in the Chronopoulos-Gear form (the preconditioner's
+// r . z = gamma and z . A z = delta, partials of the k_mg_up launch that
+// ended it): every block folds gamma and delta, beta = gamma / gamma_prev,
+// alpha = gamma / (delta - beta gamma / alpha_prev) (= gamma / p . A p in
+// exact arithmetic; the first iteration takes p = z and gamma / delta),
+// forms p = z + beta p_prev over its region and a one-cell ring in LDS and
+// q = A p over the region (the operator of residual.wgsl with b = 0,
+// negated), stages b = r - alpha q (r from the buffer SEL_R2 names), writes
+// the new r of its tile to the other buffer, p of its tile to
+// pcg_p[iter & 1] and x += alpha p, and leaves the partial of ||r||^2; the
+// stopping test then runs in the level-1 launch that follows (SEL_AXPY
+// there).  A neighbour's halo reads the old r and p, which no block of the
+// launch writes, so the tiles need no order.  One launch thus replaces
+// k_pcg_apply + k_pcg_axpy: the only global sums an iteration needs are
+// those the preconditioner's last launch already leaves.
 template <int NS, int HALO, bool AX = false, int TH = MT_H, bool LZ = false>
 __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, float weight, int restrict_,
                                                  int cg_check) {
   static_assert(!AX || TH == MT_H, "the fused CG update leaves one partial per 64 x 16 tile");
   using T = Tile<HALO, TH>;
+  // AX: z and p_prev over the region and a one-cell ring (PW x PH), then p
+  // in place of z; the residual of the restriction reuses the first array
+  constexpr int PW = T::RW + 2, PH = T::RH + 2, PN = PW * PH, PIT = (PN + MNT - 1) / MNT;
+  constexpr int SRN = AX ? (PN > MT_W * TH ? PN : MT_W * TH) : MT_W * TH;
   __shared__ float sx[T::RW * T::RH];
-  __shared__ float sr[MT_W * TH];
+  __shared__ float sr[SRN];
+  __shared__ float spo[AX ? PN : 1];
   __shared__ double wsum[MNT / 64];
   const MSite s = mg_locate<TH>(b);
   const DevInst* I = s.I;
@@ -523,59 +534,128 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   // iteration and r . z of the CG update come from earlier launches too.
   const double done_flag = sld(scal + SC_DONE);
   Level L = mg_level(CI, s.lvl, pcg);
-  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = CI->pcg_r2;  // CG bits count in preconditioner launches only
+  // CG bits count in preconditioner launches only.  SEL_R2: the residual is
+  // in pcg_r2; in a fused solve that marks the even iterations, so the
+  // iteration's buffers and scalar slots follow from it without waiting for
+  // the iteration count (which only the fold needs).
+  const bool rsel = pcg && s.lvl == 0 && (s.sel & SEL_R2);
+  if (rsel) L.b = CI->pcg_r2;
   const bool fax = pcg && (s.sel & SEL_AXPY) != 0;
   const bool axpy = AX && fax && s.lvl == 0;  // AX: the instantiation that carries the update
+  const int even = rsel ? 1 : 0;
   const int iter = axpy ? (int)sld(scal + SC_ITER) + 1 : 0;
-  const double rz_prev = axpy ? sld(scal + SC_RZ0 + ((iter - 1) & 1)) : 0.0;
+  // gamma and alpha of iteration k - 1 (slot (k - 2) & 1; this launch
+  // writes slot (k - 1) & 1); unused in the first
+  const double rz_prev = axpy ? sld(scal + SC_RZ0 + 1 - even) : 0.0;
+  const double al_prev = axpy ? sld(scal + SC_AL0 + 1 - even) : 0.0;
   const float* xin = (s.sel & 1) ? L.xs : L.x;
   float* xout = (s.sel & 1) ? L.x : L.xs;
   const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
   T t;
   constexpr int XU = (MT_W * TH) / MNT;  // tile cells per thread, row-major over the tile
-  float pv[XU], xv[XU];
+  float xv[XU];
   t.place(ox, oy, first, L.w, L.h);
   constexpr bool lazy = LZ;  // SEL_LAZY on every instance of the launch
-  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, CI->pcg_q, axpy, lazy);
   float* const p0 = CI->p0;
-  if (axpy) {  // p and x of the tile's cells (coalesced; a cell outside the level reads index 0, unused)
-    const float* p = (iter & 1) ? CI->pcg_p[1] : CI->pcg_p[0];
+  // AX: z and p_prev over the region and its ring go to LDS (cells outside
+  // the level: 0).  Their loads are issued before the tile's, and stored as
+  // soon as they land while the tile's are still in flight, so that they do
+  // not hold registers beside the tile's.  The first iteration reads p_prev
+  // too (any value: it is not used), so the loads stay straight-line.
+  float zv[PIT], pv[PIT];
+  if (axpy) {
+    const float* z = CI->pcg_z;
+    const float* pold = even ? CI->pcg_p[1] : CI->pcg_p[0];  // p of iteration k - 1 in pcg_p[(k - 1) & 1]
 #pragma unroll
-    for (int k = 0; k < XU; ++k) {
+    for (int k = 0; k < PIT; ++k) {
+      const int u = (int)threadIdx.x + k * MNT;
+      const int gx = ox - 1 + u % PW, gy = oy - 1 + u / PW;
+      const uint32_t id = u < PN && mg_inb(gx, gy, L.w, L.h) ? (uint32_t)(gy * L.w + gx) : 0u;
+      zv[k] = ld32(z, id);
+      pv[k] = ld32(pold, id);
+    }
+  }
+  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, lazy);
+  if (axpy) {
+#pragma unroll
+    for (int k = 0; k < PIT; ++k) {
+      const int u = (int)threadIdx.x + k * MNT;
+      const int gx = ox - 1 + u % PW, gy = oy - 1 + u / PW;
+      const bool inside = u < PN && mg_inb(gx, gy, L.w, L.h);
+      if (u < PN) {
+        sr[u] = inside ? zv[k] : 0.0f;
+        spo[u] = inside ? pv[k] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < XU; ++k) {  // x of the tile's cells (coalesced; outside the level: index 0, unused)
       const int u = (int)threadIdx.x + k * MNT, gx = s.tx0 + u % MT_W, gy = s.ty0 + u / MT_W;
-      const uint32_t id = gx < L.w && gy < L.h ? (uint32_t)(gy * L.w + gx) : 0u;
-      pv[k] = ld32(p, id);
-      xv[k] = ld32(p0, id);
+      xv[k] = ld32(p0, gx < L.w && gy < L.h ? (uint32_t)(gy * L.w + gx) : 0u);
     }
   }
   if (lazy) t.load_cf(L, ox, oy);
-  t.stage(nullptr, sx, axpy, lazy);
+  t.stage(nullptr, sx, lazy);
   if (pcg && done_flag != 0.0) return;
   if (axpy) {
     const int tile = s.blk - b.start[s.slot];
-    double pq[1];
-    fold_partials<1>(I, RED_PQ, CI->cg_tiles, false, pq);
-    if (!(pq[0] > 0.0)) {  // breakdown: stop before the update (as k_pcg_axpy)
+    double gd[2];  // gamma = r . z, delta = z . A z
+    fold_partials<2>(I, RED_RZ, CI->cg_tiles, false, gd);
+    const double beta = iter > 1 ? gd[0] / rz_prev : 0.0;
+    const double den = iter > 1 ? gd[1] - beta * gd[0] / al_prev : gd[1];  // p . A p
+    if (!(den > 0.0)) {  // breakdown (e.g. a non-finite state): stop before the update
       if (tile == 0 && threadIdx.x == 0) scal[SC_DONE] = 1.0;
       return;
     }
-    const double alpha = rz_prev / pq[0];
-    const float fa = (float)alpha;
-    float* rout = (s.sel & SEL_R2) ? CI->pcg_r : CI->pcg_r2;
-    double acc = 0.0;
+    const double alpha = gd[0] / den;
+    const float fa = (float)alpha, fb = (float)beta;
+    // p = z + beta p_prev (the first iteration: p = z), in place of z
 #pragma unroll
-    for (int k = 0; k < T::NSL; ++k) {
-      if (!t.inl[k]) continue;
-      const float rv = t.b[k] - fa * t.q[k];
-      t.b[k] = rv;
-      if (!t.in_tile(k)) continue;
-      stg(rout, (size_t)(oy + t.ly[k]) * L.w + ox + t.lx[k], rv);
-      acc += (double)rv * (double)rv;
+    for (int k = 0; k < PIT; ++k) {
+      const int u = (int)threadIdx.x + k * MNT;
+      if (u < PN && iter > 1) sr[u] = sr[u] + fb * spo[u];
     }
+    __syncthreads();
+    const float factor = resid_factor(CI, 1.0f);
+    float* rout = rsel ? CI->pcg_r : CI->pcg_r2;
+    double acc = 0.0;
+    // q = A p at every region cell in the level, two slots at a time
+    // (neighbour loads first; more at once would cost the occupancy that
+    // keeps all of a 1024^2 level's tiles resident)
+    static_assert(T::NSL % 2 == 0, "slots in pairs");
+#pragma unroll
+    for (int h = 0; h < T::NSL / 2; ++h) {
+      float nb[2][5];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int k = h * 2 + j;
+        const int q = t.inl[k] ? (t.ly[k] + 1) * PW + t.lx[k] + 1 : PW + 1;
+        nb[j][0] = sr[q];
+        nb[j][1] = sr[q - 1];
+        nb[j][2] = sr[q + 1];
+        nb[j][3] = sr[q - PW];
+        nb[j][4] = sr[q + PW];
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int k = h * 2 + j;
+        if (!t.inl[k]) continue;
+        const float q = 0.0f - res_upd(0.0f, t.c[k], t.fl[k], nb[j][0], nb[j][1], nb[j][2], nb[j][3], nb[j][4], factor);
+        const float rv = t.b[k] - fa * q;
+        t.b[k] = rv;
+        if (!t.in_tile(k)) continue;
+        stg(rout, (size_t)(oy + t.ly[k]) * L.w + ox + t.lx[k], rv);
+        acc += (double)rv * (double)rv;
+      }
+    }
+    float* pnew = even ? CI->pcg_p[0] : CI->pcg_p[1];
 #pragma unroll
     for (int k = 0; k < XU; ++k) {
-      const int u = (int)threadIdx.x + k * MNT, gx = s.tx0 + u % MT_W, gy = s.ty0 + u / MT_W;
-      if (gx < L.w && gy < L.h) stg(p0, (size_t)gy * L.w + gx, xv[k] + fa * pv[k]);
+      const int u = (int)threadIdx.x + k * MNT, lx = u % MT_W, ly = u / MT_W, gx = s.tx0 + lx, gy = s.ty0 + ly;
+      const float pc = sr[(ly + HALO + 1) * PW + lx + HALO + 1];
+      if (gx < L.w && gy < L.h) {
+        stg(p0, (size_t)gy * L.w + gx, xv[k] + fa * pc);
+        stg(pnew, (size_t)gy * L.w + gx, pc);
+      }
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
@@ -585,7 +665,10 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
       for (int w = 1; w < MNT / 64; ++w) tot += wsum[w];
       CI->red[(size_t)RED_RR * CI->red_stride + tile] = tot;
       if (tile == 0) {
-        scal[SC_PQ] = pq[0];
+        scal[SC_RZ0 + even] = gd[0];
+        scal[SC_AL0 + even] = alpha;
+        scal[SC_BETA] = beta;
+        scal[SC_PQ] = den;
         scal[SC_ALPHA] = alpha;
         scal[SC_KCUR] = (double)iter;
         scal[SC_NRR] = (double)(b.start[s.slot + 1] - b.start[s.slot]);
@@ -619,13 +702,17 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
 // ---------------------------------------------------------------------------
 // up: prolongation while staging, NS post-smoothing sweeps.  With rz (level 0
 // of the CG preconditioner) z is masked to the pressure unknowns and the block
-// partial of r . z goes to red[RED_RZ][tile].
+// partials of r . z and z . A z go to red[RED_RZ][tile] and red[RED_ZAZ][tile]
+// (z . A z: the halo is one cell wider than the sweeps need, so z is exact on
+// the tile and a one-cell ring; the AX launch of the next CG iteration takes
+// its step length from both sums).
 // ---------------------------------------------------------------------------
 template <int NS, int HALO, int TH = MT_H, bool LZ = false>
 __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, float weight, int rz) {
   using T = Tile<HALO, TH>;
+  constexpr bool zaz = HALO > 2 * NS;  // the r . z instantiation
   __shared__ float sx[T::RW * T::RH];
-  __shared__ double wsum[MNT / 64];
+  __shared__ double wsum[2][MNT / 64];
   const MSite s = mg_locate<TH>(b);
   CDevInst* CI = cdev(s.I);
   double* const scal = CI->scal;
@@ -639,34 +726,62 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
   T t;
   t.place(ox, oy, first, L.w, L.h);
   constexpr bool lazy = LZ;  // SEL_LAZY on every instance of the launch
-  t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, nullptr, false, lazy);
+  t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, lazy);
   if (lazy) t.load_cf(L, ox, oy);
-  t.stage(&C, sx, false, lazy);
+  t.stage(&C, sx, lazy);
   if (pcg && done_flag != 0.0) return;
   __syncthreads();
   const float scale = (float)(1 << s.lvl);
   t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(CI, scale), sx);
-  if (TH != MT_H || !rz) {  // r . z partials come from 64 x 16 tiles only
+  if (TH != MT_H || !zaz || !rz) {  // r . z partials come from 64 x 16 tiles only
     write_tile<HALO, TH>(L, xout, s.tx0, s.ty0, sx);
     return;
   }
-  double acc = 0.0;
+  // z masked to the unknowns over the region, in registers and LDS
+#pragma unroll
+  for (int k = 0; k < T::NSL; ++k) {
+    if (!(t.fl[k] & F_LSNEG) || (t.fl[k] & F_CLOSED)) t.x[k] = 0.0f;
+    if (t.valid[k]) sx[t.ly[k] * T::RW + t.lx[k]] = t.x[k];
+  }
+  __syncthreads();
+  const float factor = resid_factor(CI, 1.0f);
+  float nb[T::NSL][4];
+#pragma unroll
+  for (int k = 0; k < T::NSL; ++k) {
+    const int q = t.lds_at(k, t.in_tile(k));
+    nb[k][0] = sx[q - 1];
+    nb[k][1] = sx[q + 1];
+    nb[k][2] = sx[q - T::RW];
+    nb[k][3] = sx[q + T::RW];
+  }
+  double acc = 0.0, acd = 0.0;
 #pragma unroll
   for (int k = 0; k < T::NSL; ++k) {
     if (!t.in_tile(k) || !t.inl[k]) continue;
-    float z = t.x[k];
-    if (!(t.fl[k] & F_LSNEG) || (t.fl[k] & F_CLOSED)) z = 0.0f;
+    const float z = t.x[k];
     stg(xout, (size_t)(oy + t.ly[k]) * L.w + ox + t.lx[k], z);
     acc += (double)t.b[k] * (double)z;
+    const float az = 0.0f - res_upd(0.0f, t.c[k], t.fl[k], z, nb[k][0], nb[k][1], nb[k][2], nb[k][3], factor);
+    acd += (double)z * (double)az;
   }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+  for (int off = 32; off > 0; off >>= 1) {
+    acc += __shfl_xor(acc, off, 64);
+    acd += __shfl_xor(acd, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wsum[0][threadIdx.x >> 6] = acc;
+    wsum[1][threadIdx.x >> 6] = acd;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double tot = wsum[0];
-    for (int w = 1; w < MNT / 64; ++w) tot += wsum[w];
+    double tot = wsum[0][0], tod = wsum[1][0];
+    for (int w = 1; w < MNT / 64; ++w) {
+      tot += wsum[0][w];
+      tod += wsum[1][w];
+    }
     const int tile = s.blk - b.start[s.slot];
     CI->red[(size_t)RED_RZ * CI->red_stride + tile] = tot;
+    CI->red[(size_t)RED_ZAZ * CI->red_stride + tile] = tod;
     if (tile == 0) scal[SC_NRZ] = (double)(b.start[s.slot + 1] - b.start[s.slot]);
   }
 }
@@ -1453,11 +1568,24 @@ void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int firs
     if (lz) MG_LAUNCH((k_mg_up<NS, HALO, MT_H, true>), nb, st, b, pcg, first, weight, rz);  \
     else MG_LAUNCH((k_mg_up<NS, HALO, MT_H, false>), nb, st, b, pcg, first, weight, rz);    \
   } while (0)
+  // with rz a halo one cell wider than the sweeps need (z . A z on the tile)
   switch (nsweeps) {
-    case 0: MG_UP16(0, 0); break;
-    case 1: MG_UP16(1, 2); break;
-    case 2: MG_UP16(2, 4); break;
-    default: MG_UP16(3, 6); break;
+    case 0:
+      if (rz) MG_UP16(0, 1);
+      else MG_UP16(0, 0);
+      break;
+    case 1:
+      if (rz) MG_UP16(1, 3);
+      else MG_UP16(1, 2);
+      break;
+    case 2:
+      if (rz) MG_UP16(2, 5);
+      else MG_UP16(2, 4);
+      break;
+    default:
+      if (rz) MG_UP16(3, 7);
+      else MG_UP16(3, 6);
+      break;
   }
 #undef MG_UP16
 }

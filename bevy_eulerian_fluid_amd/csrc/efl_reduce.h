@@ -22,6 +22,7 @@ constexpr int RED_NT = 256;  // threads of every block that takes part
 // partial components (red[])
 constexpr int RED_PQ = 0, RED_RR = 1, RED_BB = 2, RED_RZ = 2;  // bb is folded before any r . z exists
 constexpr int RED_ZQ = 3;  // z . q_prev of the flexible (learned) preconditioner
+constexpr int RED_ZAZ = 3;  // z . A z of the multigrid preconditioner's last launch (never flexible)
 
 // scal[] slots
 constexpr int SC_PQ = 1, SC_RR = 2, SC_BB = 3, SC_ALPHA = 4, SC_BETA = 5, SC_DONE = 6;
@@ -34,6 +35,7 @@ constexpr int SC_STALL = 15;         // flexible CG: relative residual at the la
 constexpr int SC_RZF = 16;           // k_prefold: r . z (and z . q_prev) of the last preconditioner
 constexpr int SC_PQF = 18;           // k_prefold: p . q of the last k_pcg_apply
 constexpr int SC_MISSED = 20;        // asynchronous solves that ended without meeting the tolerance (k_cg_tally)
+constexpr int SC_AL0 = 21;           // alpha of iteration k at SC_AL0 + (k & 1)
 constexpr int kScalars = 24;         // scal[] length (the host reads the first 16)
 
 // Partial counts up to this are folded by every block that needs the

@@ -7,7 +7,12 @@ use the same fixed-order fold, so a forced instance must step bit for bit as
 an unforced one: every field and the solver statistics, for MGPCG on a batch
 of instances of different sizes (tiled, bottom-only and coarse-only
 preconditioners) and for the flexible CG of the learned preconditioner.
+A forced instance runs its CG iterations as k_pcg_apply + k_pcg_axpy, so
+the MGPCG comparison runs both contexts that way (EFL_CG_FUSE=0); the fused
+iteration (Chronopoulos-Gear form, other rounding) is held to it within the
+solver tolerance here too.
 """
+import numpy as np
 import pytest
 
 from bevy_eulerian_fluid_amd import native, scenes
@@ -33,11 +38,15 @@ def _make(ctx, sc, monkeypatch, forced):
 
 
 def test_prefold_matches_inline_fold_mgpcg(monkeypatch):
-    _say("one-block folds vs inline folds: MGPCG batch")
+    _say("one-block folds vs inline folds: MGPCG batch, unfused iterations")
     p = scenes.projection("mgpcg", max_cycles=60, rel_tol=1e-5)
     scs = [scenes.batch_instance(scenes.SplitMix64(scenes.SEED_BASE + 3, k), proj=p) for k in range(3)]
     scs += [small_scene(200, 136, proj=p), small_scene(32, 24, proj=p)]
-    ca, cb = native.Context(0), native.Context(0)
+    monkeypatch.setenv("EFL_CG_FUSE", "0")
+    try:
+        ca, cb = native.Context(0), native.Context(0)
+    finally:
+        monkeypatch.delenv("EFL_CG_FUSE", raising=False)
     try:
         ia, ib = [], []
         for sc in scs:
@@ -79,3 +88,47 @@ def test_prefold_matches_inline_fold_learned(monkeypatch):
         b.close()
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("size", [(96, 80), (200, 136), (1024, 1024)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_fused_cg_matches_unfused(monkeypatch, size):
+    """The fused iteration (one launch: p = z + beta p, q = A p, x, r, with
+    alpha = r.z / (z.Az - beta r.z / alpha_prev)) against k_pcg_apply +
+    k_pcg_axpy (alpha = r.z / p.q) on the same states: the same iterates in
+    exact arithmetic, so both converge, their iteration counts differ by at
+    most one and the pressures agree within 1e-5 of max |p| (the MGPCG
+    parity bound)."""
+    _say(f"fused vs unfused CG iterations {size}")
+    p = scenes.projection("mgpcg", max_cycles=100, rel_tol=1e-5)
+    if size[0] >= 1024:
+        sc = scenes.coupled_tank(size[0], proj=p, n_each=1)
+    else:
+        sc = small_scene(*size, proj=p)
+    monkeypatch.setenv("EFL_CG_FUSE", "0")
+    try:
+        cu = native.Context(0)
+    finally:
+        monkeypatch.delenv("EFL_CG_FUSE", raising=False)
+    cf = native.Context(0)
+    try:
+        f, u = _setup(cf, sc), _setup(cu, sc)
+        for c in (cf, cu):
+            c.set_obstacles(sc["obstacles"])
+        f.step(DT)  # initialisation
+        u.step(DT)
+        for fld in ("U0", "V0", "P0", "LS_AIR0"):
+            f.set(fld, u.get(fld))
+        for k in range(4):
+            f.step(DT)
+            u.step(DT)
+            sf, su = f.stats(), u.stats()
+            assert sf.converged == 1 and su.converged == 1, k
+            assert abs(int(sf.cycles) - int(su.cycles)) <= 1, (k, sf.cycles, su.cycles)
+            pf, pu = f.get("P0"), u.get("P0")
+            scale = np.abs(pu).max()
+            assert np.abs(pf - pu).max() <= 1e-5 * scale, (k, np.abs(pf - pu).max(), scale)
+            for fld in ("U0", "V0", "P0", "LS_AIR0"):  # the next step starts from the same state
+                f.set(fld, u.get(fld))
+    finally:
+        cf.close()
+        cu.close()

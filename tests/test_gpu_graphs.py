@@ -6,10 +6,12 @@ with host reads and synchronisation in between - every field and the solver
 statistics must agree bit for bit.  Covers the MG-preconditioned CG (a
 graph for the CG iteration replayed a data-dependent number of times), a
 fixed Gauss-Seidel solve, two-way coupled bodies, and a batch of instances
-of different sizes.  The replayed CG iteration keeps k_pcg_axpy, while eager
-launches fuse its update into the first level-0 smoother launch wherever an
-instance's first two levels are tiled (128x96, 96x80 and 128x64 here), so
-these tests also hold the fused update to the unfused one bit for bit.
+of different sizes.  Wherever an instance's first two levels are tiled
+(128x96, 96x80 and 128x64 here) the CG iteration runs fused into the
+preconditioner's level-0 launch, in the replayed graphs (an odd and an even
+iteration, whose residual buffers differ) as in eager launches, so the
+replays are held to the eager iterations bit for bit; smaller instances of
+the batch run k_pcg_apply + k_pcg_axpy.
 """
 import numpy as np
 import pytest
