@@ -87,6 +87,11 @@ struct efl_ctx {
   // every iteration as k_pcg_apply + k_pcg_axpy (the textbook form of CG,
   // whose rounding differs)
   bool cg_fuse = true;
+  // the up pass takes levels in pairs (k_mg_up2: level l + 1's post-
+  // smoothing inside level l's launch, l even) wherever both are tiled
+  // 2-sweep levels; the same bits, one launch fewer per pair.
+  // EFL_UP_MERGE=0 at context creation launches every level on its own.
+  bool up_merge = true;
 };
 
 struct efl_instance {
@@ -595,6 +600,35 @@ void coarse_cycle(hipStream_t st, const IV& v, int l0, int pcg, bool symmetric) 
   efl::launch_mg_coarse(st, cb, pcg, symmetric ? 1 : 0, cells);
 }
 
+// Level l of instance i takes level l + 1's up pass into its own launch
+// (k_mg_up2): l even, both levels tiled (above the bottom, the coarsest and
+// a fused coarse cycle), one 2-sweep post-smoothing launch on each.
+bool up_pairs(const efl_instance* i, int l) {
+  if (!i->ctx->up_merge || (l & 1) || cnt_post(const_cast<efl_instance*>(i)) != 2) return false;
+  const int b = bottom_level(i), c = coarse_level(i);
+  const auto tiled = [&](int q) { return (b < 0 || q < b) && q < i->h.nlev - 1 && (c < 0 || q < c); };
+  return tiled(l) && tiled(l + 1);
+}
+
+// The up pass of levels l and l + 1 of the instances v (up_pairs) in one
+// launch per tile height; the buffer-pair state moves as the two separate
+// launches would move it.  rz (level 0 of the CG preconditioner): z masked,
+// r . z and z . A z partials; the instances are appended to *rz_done.
+void up_pair(hipStream_t st, const IV& v, int l, int pcg, bool symmetric, bool rz, IV* rz_done) {
+  for (int h : {efl::mg_tile_h(), 8}) {
+    IV part = filter(v, [&](efl_instance* i) { return mg_th_of(i, l, pcg) == h; });
+    if (part.empty()) continue;
+    std::vector<int> selc;
+    for (auto* i : part) selc.push_back(i->xsel[l + 1] | (i->xsel[l + 2] << 1) | (i->xzero[l + 1] << 2));
+    Batch b = mg_batch(part, EX_LEVEL, l, true, pcg);
+    for (int n = 0; n < b.n; ++n) b.sel[n] |= selc[n] << 8;
+    efl::launch_mg_up2(st, b, pcg, symmetric ? 0 : 1, 1.0f, rz ? 1 : 0, h);
+    mg_flip(part, l + 1);
+    mg_flip(part, l);
+    if (rz && rz_done) rz_done->insert(rz_done->end(), part.begin(), part.end());
+  }
+}
+
 // v_cycle, multi_grid.rs:390-475 (symmetric: post-smoothing black-red and
 // the coarsest level also run back in reverse colour order, see the oracle)
 void v_cycle(hipStream_t st, const IV& v_in, int l, int pcg, bool symmetric) {
@@ -640,7 +674,12 @@ void v_cycle(hipStream_t st, const IV& v_in, int l, int pcg, bool symmetric) {
   if (!rest.empty()) {
     gs_sweeps(st, rest, l, cnt_pre, 1.0f, pcg, false, EX_LEVEL, DOWN);
     v_cycle(st, rest, l + 1, pcg, symmetric);
-    gs_sweeps(st, rest, l, cnt_post, 1.0f, pcg, symmetric, EX_LEVEL, UP);
+    // level l's up: with level l + 1's (up_pairs), on its own, or in level
+    // l - 1's launch
+    IV pair = filter(rest, [l](efl_instance* i) { return up_pairs(i, l); });
+    IV own = filter(rest, [l](efl_instance* i) { return !up_pairs(i, l) && !(l >= 1 && up_pairs(i, l - 1)); });
+    if (!pair.empty()) up_pair(st, pair, l, pcg, symmetric, false, nullptr);
+    if (!own.empty()) gs_sweeps(st, own, l, cnt_post, 1.0f, pcg, symmetric, EX_LEVEL, UP);
   }
   if (l == 0) mg_settle(st, v, pcg);
 }
@@ -689,8 +728,15 @@ void precondition(hipStream_t st, const IV& v, bool cg_check = false) {
     }
     if (!rest.empty()) {
       gs_sweeps(st, rest, 0, cnt_pre, 1.0f, 1, false, EX_LEVEL, DOWN, false, nullptr, nullptr, cg_check);
-      v_cycle(st, rest, 1, 1, true);
-      gs_sweeps(st, rest, 0, cnt_post, 1.0f, 1, true, EX_LEVEL, UP, true, &fused);
+      v_cycle(st, rest, 1, 1, true);  // leaves level 1's up to level 0 where they pair
+      IV pair = filter(rest, [](efl_instance* i) { return up_pairs(i, 0); });
+      IV own = filter(rest, [](efl_instance* i) { return !up_pairs(i, 0); });
+      if (!pair.empty()) up_pair(st, pair, 0, 1, true, true, &fused);
+      if (!own.empty()) {
+        IV f;
+        gs_sweeps(st, own, 0, cnt_post, 1.0f, 1, true, EX_LEVEL, UP, true, &f);
+        fused.insert(fused.end(), f.begin(), f.end());
+      }
     }
   }
   mg_settle(st, v, 1);
@@ -1556,6 +1602,7 @@ int efl_ctx_create(int device, efl_ctx** out) {
   c->bottom_cells = efl::bottom_cells_for_device(device);
   if (const char* e = getenv("EFL_COARSE_FUSE")) c->coarse_fuse = atoi(e) == 1;
   if (const char* e = getenv("EFL_CG_FUSE")) c->cg_fuse = atoi(e) != 0;
+  if (const char* e = getenv("EFL_UP_MERGE")) c->up_merge = atoi(e) != 0;
   if (c->coarse_fuse) c->bottom_cells = 1024;  // the fused coarse cycle holds its bottom in a 256-thread launch
   *out = c;
   return EFL_OK;

@@ -83,6 +83,9 @@ size_t bottom_lds_bytes(int cells);
 void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_,
                     int cg_check = 0, int th = 16);
 void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int rz, int th = 16);
+// levels lvl and lvl + 1 of the up pass in one launch (k_mg_up2: 2 sweeps on
+// each; Batch::sel bits 8..10 carry level lvl + 1's buffer state)
+void launch_mg_up2(hipStream_t st, const Batch& b, int pcg, int first, float weight, int rz, int th);
 void launch_bottom(hipStream_t st, const Batch& b, int pcg, int symmetric, int max_cells);
 // max_cells: the most cells any instance's bottom levels hold
 void launch_mg_coarse(hipStream_t st, const CoarseBatch& b, int pcg, int symmetric, int max_cells);

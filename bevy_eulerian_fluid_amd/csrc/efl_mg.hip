@@ -309,10 +309,11 @@ __device__ __forceinline__ MSite mg_locate(const Batch& b) {
 // cells of the colour swept first, NIT..2NIT-1 the other colour; slot k of a
 // colour is the thread's k-th cell in a row-major walk over that colour's
 // cells.  Stencil, b and x of its cells live in registers; x is mirrored in
-// LDS for the neighbours.
-template <int HALO, int TH = MT_H>
+// LDS for the neighbours.  The tile is TW x TH cells (TW + 2 HALO even).
+template <int HALO, int TH = MT_H, int TW = MT_W>
 struct Tile {
-  static constexpr int RW = MT_W + 2 * HALO, RH = TH + 2 * HALO, HW = RW / 2;
+  static constexpr int RW = TW + 2 * HALO, RH = TH + 2 * HALO, HW = RW / 2;
+  static_assert(RW % 2 == 0, "rows of whole colour pairs");
   static constexpr int NC = HW * RH;                 // cells of one colour
   static constexpr int NIT = (NC + MNT - 1) / MNT;   // slots per colour
   static constexpr int NSL = 2 * NIT;
@@ -367,6 +368,11 @@ struct Tile {
       b[s] = ld32(L.b, id);
       if (!lazy_cf) c[s] = ld32(L.cf, id);
     }
+  }
+  // Every in-level cell's coefficients (load with lazy_cf left them out).
+  __device__ __forceinline__ void load_c(const Level& L, int ox, int oy) {
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) c[s] = ld32(L.cf, inl[s] ? (uint32_t)((oy + ly[s]) * L.w + ox + lx[s]) : 0u);
   }
   // The coefficients where the flags need them (mg_cf_needed; stage() puts
   // mg_cf_implied in the others): on a level that streams from HBM most
@@ -447,7 +453,7 @@ struct Tile {
   }
 
   __device__ __forceinline__ bool in_tile(int s) const {
-    return valid[s] && lx[s] >= HALO && lx[s] < HALO + MT_W && ly[s] >= HALO && ly[s] < HALO + TH;
+    return valid[s] && lx[s] >= HALO && lx[s] < HALO + TW && ly[s] >= HALO && ly[s] < HALO + TH;
   }
 
   // The residual of every tile cell (0 outside the level) into sr, row-major
@@ -469,7 +475,7 @@ struct Tile {
     for (int s = 0; s < NSL; ++s) {
       if (!act[s]) continue;
       const float r = inl[s] ? res_upd(b[s], c[s], fl[s], x[s], nb[s][0], nb[s][1], nb[s][2], nb[s][3], factor) : 0.0f;
-      sr[(ly[s] - HALO) * MT_W + lx[s] - HALO] = r;
+      sr[(ly[s] - HALO) * TW + lx[s] - HALO] = r;
     }
   }
 };
@@ -699,44 +705,21 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   stg(Cc.b, (size_t)cy * Cc.w + cx, br);
 }
 
-// ---------------------------------------------------------------------------
-// up: prolongation while staging, NS post-smoothing sweeps.  With rz (level 0
-// of the CG preconditioner) z is masked to the pressure unknowns and the block
-// partials of r . z and z . A z go to red[RED_RZ][tile] and red[RED_ZAZ][tile]
-// (z . A z: the halo is one cell wider than the sweeps need, so z is exact on
-// the tile and a one-cell ring; the AX launch of the next CG iteration takes
-// its step length from both sums).
-// ---------------------------------------------------------------------------
-template <int NS, int HALO, int TH = MT_H, bool LZ = false>
-__global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, float weight, int rz) {
+// The end of an up launch, after its sweeps: x of the tile to xout; with rz
+// (level 0 of the CG preconditioner: 64 x 16 tiles, a halo one cell wider
+// than the sweeps need) z masked to the pressure unknowns instead, and the
+// block partials of r . z and z . A z.
+template <int NS, int HALO, int TH>
+__device__ __forceinline__ void up_finish(Tile<HALO, TH>& t, const Batch& b, const MSite& s, CDevInst* CI,
+                                          const Level& L, float* xout, int rz, float* sx, double (*wsum)[MNT / 64]) {
   using T = Tile<HALO, TH>;
   constexpr bool zaz = HALO > 2 * NS;  // the r . z instantiation
-  __shared__ float sx[T::RW * T::RH];
-  __shared__ double wsum[2][MNT / 64];
-  const MSite s = mg_locate<TH>(b);
-  CDevInst* CI = cdev(s.I);
-  double* const scal = CI->scal;
-  const double done_flag = sld(scal + SC_DONE);  // tested once the tile loads are in flight
-  Level L = mg_level(CI, s.lvl, pcg);
-  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = CI->pcg_r2;
-  const Level C = lev_copy(CI->lev[s.lvl + 1]);
-  const float* xin = (s.sel & 1) ? L.xs : L.x;
-  float* xout = (s.sel & 1) ? L.x : L.xs;
-  const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
-  T t;
-  t.place(ox, oy, first, L.w, L.h);
-  constexpr bool lazy = LZ;  // SEL_LAZY on every instance of the launch
-  t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, lazy);
-  if (lazy) t.load_cf(L, ox, oy);
-  t.stage(&C, sx, lazy);
-  if (pcg && done_flag != 0.0) return;
-  __syncthreads();
-  const float scale = (float)(1 << s.lvl);
-  t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(CI, scale), sx);
   if (TH != MT_H || !zaz || !rz) {  // r . z partials come from 64 x 16 tiles only
     write_tile<HALO, TH>(L, xout, s.tx0, s.ty0, sx);
     return;
   }
+  double* const scal = CI->scal;
+  const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
   // z masked to the unknowns over the region, in registers and LDS
 #pragma unroll
   for (int k = 0; k < T::NSL; ++k) {
@@ -784,6 +767,101 @@ __global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, floa
     CI->red[(size_t)RED_ZAZ * CI->red_stride + tile] = tod;
     if (tile == 0) scal[SC_NRZ] = (double)(b.start[s.slot + 1] - b.start[s.slot]);
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// up: prolongation while staging, NS post-smoothing sweeps.  With rz (level 0
+// of the CG preconditioner) z is masked to the pressure unknowns and the block
+// partials of r . z and z . A z go to red[RED_RZ][tile] and red[RED_ZAZ][tile]
+// (z . A z: the halo is one cell wider than the sweeps need, so z is exact on
+// the tile and a one-cell ring; the AX launch of the next CG iteration takes
+// its step length from both sums).
+// ---------------------------------------------------------------------------
+template <int NS, int HALO, int TH = MT_H, bool LZ = false>
+__global__ __launch_bounds__(MNT) void k_mg_up(Batch b, int pcg, int first, float weight, int rz) {
+  using T = Tile<HALO, TH>;
+  __shared__ float sx[T::RW * T::RH];
+  __shared__ double wsum[2][MNT / 64];
+  const MSite s = mg_locate<TH>(b);
+  CDevInst* CI = cdev(s.I);
+  double* const scal = CI->scal;
+  const double done_flag = sld(scal + SC_DONE);  // tested once the tile loads are in flight
+  Level L = mg_level(CI, s.lvl, pcg);
+  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = CI->pcg_r2;
+  const Level C = lev_copy(CI->lev[s.lvl + 1]);
+  const float* xin = (s.sel & 1) ? L.xs : L.x;
+  float* xout = (s.sel & 1) ? L.x : L.xs;
+  const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
+  T t;
+  t.place(ox, oy, first, L.w, L.h);
+  constexpr bool lazy = LZ;  // SEL_LAZY on every instance of the launch
+  t.load(L, xin, (s.sel & 4) != 0, &C, (s.sel & 2) ? C.xs : C.x, ox, oy, lazy);
+  if (lazy) t.load_cf(L, ox, oy);
+  t.stage(&C, sx, lazy);
+  if (pcg && done_flag != 0.0) return;
+  __syncthreads();
+  const float scale = (float)(1 << s.lvl);
+  t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(CI, scale), sx);
+  up_finish<NS, HALO, TH>(t, b, s, CI, L, xout, rz, sx, wsum);
+}
+
+// Up of two levels in one launch: level lvl + 1's prolongation (from
+// lvl + 2) and two post-smoothing sweeps over the parents of this tile's
+// region and a 4-cell halo, in LDS and never stored (only the prolongation
+// to lvl reads them), then lvl's work as k_mg_up does it.  Every parent lies
+// 4 or more cells inside the coarse region, so its value is the one k_mg_up
+// on level lvl + 1 would have stored, and the results are bit for bit those
+// of the two separate launches, one kernel boundary fewer.  Batch::sel bits
+// 8..10 hold level lvl + 1's buffer state as bits 0..2 hold lvl's.
+template <int NS, int HALO, int TH = MT_H, bool LZ = false>
+__global__ __launch_bounds__(MNT) void k_mg_up2(Batch b, int pcg, int first, float weight, int rz) {
+  using T = Tile<HALO, TH>;
+  constexpr int HC = 4;         // two coarse sweeps
+  constexpr int OD = HALO & 1;  // tiles start at even cells: an odd halo starts the region at an odd cell
+  constexpr int TWC = (T::RW + OD) / 2 + OD, THC = (T::RH + OD) / 2 + OD;  // the parents of the region
+  using TC = Tile<HC, THC, TWC>;
+  __shared__ float sx[T::RW * T::RH];
+  __shared__ float sxc[TC::RW * TC::RH];
+  __shared__ double wsum[2][MNT / 64];
+  const MSite s = mg_locate<TH>(b);
+  CDevInst* CI = cdev(s.I);
+  const double done_flag = sld(CI->scal + SC_DONE);  // tested once the tile loads are in flight
+  Level L = mg_level(CI, s.lvl, pcg);
+  if (pcg && s.lvl == 0 && (s.sel & SEL_R2)) L.b = CI->pcg_r2;
+  const Level C = lev_copy(CI->lev[s.lvl + 1]);
+  const Level C2 = lev_copy(CI->lev[s.lvl + 2]);
+  const int selc = s.sel >> 8;
+  const float* xin = (s.sel & 1) ? L.xs : L.x;
+  float* xout = (s.sel & 1) ? L.x : L.xs;
+  const int ox = s.tx0 - HALO, oy = s.ty0 - HALO;
+  const int cox = (ox >> 1) - HC, coy = (oy >> 1) - HC;  // LDS cell (0, 0) of the coarse region
+  TC tc;
+  tc.place(cox, coy, first, C.w, C.h);
+  tc.load(C, (selc & 1) ? C.xs : C.x, (selc & 4) != 0, &C2, (selc & 2) ? C2.xs : C2.x, cox, coy);
+  T t;
+  t.place(ox, oy, first, L.w, L.h);
+  constexpr bool lazy = LZ;  // SEL_LAZY on every instance of the launch (fine level)
+  t.load(L, xin, (s.sel & 4) != 0, nullptr, nullptr, ox, oy, true);  // the coefficients below
+  tc.stage(&C2, sxc);
+  if (pcg && done_flag != 0.0) return;
+  __syncthreads();
+  tc.template sweep<2>(cox, coy, C.w, C.h, C.w, C.h, weight, gs_c0(CI, (float)(2 << s.lvl)), sxc);
+  // the fine coefficients once the coarse tile's registers are free
+  if (lazy) t.load_cf(L, ox, oy);
+  else t.load_c(L, ox, oy);
+  // the parents' values (the sweep ended with a barrier), then the staging
+  // with the prolongation as k_mg_up does it
+#pragma unroll
+  for (int k = 0; k < T::NSL; ++k) {
+    const int cx = (ox + t.lx[k]) >> 1, cy = (oy + t.ly[k]) >> 1;
+    t.par[k] = t.inl[k] && cx < C.w && cy < C.h;
+    t.xc[k] = sxc[t.valid[k] ? (cy - coy) * TC::RW + cx - cox : 0];
+  }
+  t.stage(&C, sx, lazy);
+  __syncthreads();
+  t.template sweep<NS>(ox, oy, L.w, L.h, s.ex, s.ey, weight, gs_c0(CI, (float)(1 << s.lvl)), sx);
+  up_finish<NS, HALO, TH>(t, b, s, CI, L, xout, rz, sx, wsum);
 }
 
 // ---------------------------------------------------------------------------
@@ -1588,6 +1666,28 @@ void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int firs
       break;
   }
 #undef MG_UP16
+}
+
+// levels lvl and lvl + 1 of an up pass in one launch (2 post-smoothing
+// sweeps on each, + r . z and z . A z partials with rz: 64 x 16 tiles only)
+void launch_mg_up2(hipStream_t st, const Batch& b, int pcg, int first, float weight, int rz, int th) {
+  if ((th != MT_H && th != MT_H8) || (rz && th != MT_H)) {
+    set_launch_error("launch_mg_up2: tile height neither 16 nor 8, or r . z partials on 64 x 8 tiles");
+    return;
+  }
+  int nb = b.start[b.n];
+  if (nb <= 0) return;
+  bool lz = true;
+  for (int n = 0; n < b.n; ++n) lz = lz && (b.sel[n] & SEL_LAZY);
+  if (th == MT_H8) {
+    MG_LAUNCH((k_mg_up2<2, 4, MT_H8>), nb, st, b, pcg, first, weight, rz);
+  } else if (rz) {
+    if (lz) MG_LAUNCH((k_mg_up2<2, 5, MT_H, true>), nb, st, b, pcg, first, weight, rz);
+    else MG_LAUNCH((k_mg_up2<2, 5, MT_H, false>), nb, st, b, pcg, first, weight, rz);
+  } else {
+    if (lz) MG_LAUNCH((k_mg_up2<2, 4, MT_H, true>), nb, st, b, pcg, first, weight, rz);
+    else MG_LAUNCH((k_mg_up2<2, 4, MT_H, false>), nb, st, b, pcg, first, weight, rz);
+  }
 }
 
 void launch_mg_coarse(hipStream_t st, const CoarseBatch& b, int pcg, int symmetric, int max_cells) {
