@@ -17,16 +17,17 @@ steps warmup+1 .. warmup+steps, three later with --autotune, named in
 `timed_steps`); `pressure_solve_ms`
 and `max_norm_divergence` (dt * max|div u| after the projection) come with it.
 `roofline` prices the kernel that dominates the timed step: k_mg_down<2,5,true>,
-the level-0 launch of every MGPCG iteration, which finishes the iteration's
-update (x += alpha p, r -= alpha q, ||r||^2 partials), runs the two red-black
-pre-smoothing sweeps of the preconditioner's V-cycle, forms the residual and
-restricts it.  Its algorithmic bytes follow the per-unit model of the
-reference dataflow (each kernel reads each input once and writes each output
-once, 4 B per word, no credit for fusing): the CG update 6 words per cell
-(x, p, r, q read; x, r written), 2 sweeps x 8 (x, b, air level set and 4 area
-fractions read, x written), the residual pass 8 (the same 7 inputs, r
-written) and the restriction of the residual 1.25 (r read, one coarse b per 4
-cells written) = 31.25 words per cell; it is timed with hipEvents around back-
+the level-0 launch of every MGPCG iteration, which runs the iteration itself
+(p = z + beta p, q = A p, x += alpha p, r -= alpha q, ||r||^2 partials), the
+two red-black pre-smoothing sweeps of the preconditioner's V-cycle, the
+residual and its restriction.  Its algorithmic bytes follow the per-unit
+model of the reference dataflow (each kernel reads each input once and
+writes each output once, 4 B per word, no credit for fusing): the search
+direction 9 words per cell (z, p, air level set and 4 area fractions read;
+p, q written), the update 6 (x, p, r, q read; x, r written), 2 sweeps x 8
+(x, b, air level set and 4 area fractions read, x written), the residual pass
+8 (the same 7 inputs, r written) and the restriction of the residual 1.25 (r
+read, one coarse b per 4 cells written) = 40.25 words per cell; it is timed with hipEvents around back-
 to-back launches of exactly that launch on the library's stream
 (efl_time_stage CG_SMOOTH); `traffic` is its measured HBM bytes per launch from
 the committed FETCH_SIZE / WRITE_SIZE profile.  `step_frac` prices the whole
@@ -48,10 +49,11 @@ sys.path.insert(0, ROOT)
 METRIC = "sim steps/s + pressure-solve ms at 1024² to ‖∇·u‖∞<1e-4; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
 SMOOTH_SWEEPS = 2  # red-black sweeps of one MG_SMOOTH launch
-# words per cell of the level-0 CG launch: update 6 + 2 sweeps x 8 + residual 8 + restriction 1.25
-CG_SMOOTH_WORDS = 6 + SMOOTH_SWEEPS * 8 + 8 + 1.25
-CG_SMOOTH_KERNEL = ("k_mg_down<2,5,true>: level-0 launch of each MGPCG iteration (x/r update, 2 red-black "
-                    "sweeps, residual, restriction)")
+# words per cell of the level-0 CG launch: search direction 9 + update 6 + 2 sweeps x 8 + residual 8
+# + restriction 1.25
+CG_SMOOTH_WORDS = 9 + 6 + SMOOTH_SWEEPS * 8 + 8 + 1.25
+CG_SMOOTH_KERNEL = ("k_mg_down<2,5,true>: level-0 launch of each MGPCG iteration (p = z + beta p, q = A p, "
+                    "x/r update, 2 red-black sweeps, residual, restriction)")
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_mg_down_cg_level0_1024.json")
 DT = 1.0 / 60.0
 

@@ -133,3 +133,31 @@ def test_batched_step_matches_single_steps():
         cb.close()
         for c in cs:
             c.close()
+
+
+@pytest.mark.parametrize("proj", ["mgpcg", "multigrid"])
+@pytest.mark.parametrize("size", [(200, 136), (1024, 1024)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_up_pairs_match_separate_launches(monkeypatch, size, proj):
+    """The up pass in level pairs (k_mg_up2: level l + 1 smoothed inside
+    level l's launch) against one launch per level (EFL_UP_MERGE=0): every
+    field and the solver statistics bit for bit over a few steps."""
+    _say(f"paired vs separate up launches {proj} {size}")
+    p = scenes.projection(proj, max_cycles=60 if proj == "mgpcg" else 2, rel_tol=1e-5 if proj == "mgpcg" else 0.0)
+    sc = small_scene(*size, proj=p, forces=False)
+    monkeypatch.setenv("EFL_UP_MERGE", "0")
+    try:
+        cs = native.Context(0)
+    finally:
+        monkeypatch.delenv("EFL_UP_MERGE", raising=False)
+    cp = native.Context(0)
+    try:
+        a, b = _setup(cp, sc), _setup(cs, sc)
+        for c in (cp, cs):
+            c.set_obstacles(sc["obstacles"])
+        for k in range(4):
+            a.step(DT)
+            b.step(DT)
+            _same(a, b, f"step {k}")
+    finally:
+        cp.close()
+        cs.close()
