@@ -188,10 +188,11 @@ def run_config(args):
     """The other workloads of the benchmark suite, one JSON line each:
       1  dam break 256^2, Gauss-Seidel 100 sweeps (weight 1.9), no bodies
       3  64 independent instances, sides log-uniform in [128, 1024], 0-8
-         two-way coupled bodies each, one multigrid V-cycle per step
+         two-way coupled bodies each, MG-preconditioned CG to --rel-tol
       5  4096^2 pool + 200 droplets (an initial level set), 24 bodies, 64
          local forces resampled every 30 steps, MG-preconditioned CG to
-         1e-5, stepped together with 8 instances of 512^2"""
+         1e-5 (with --learned: the learned preconditioner, shipped weights,
+         MGPCG fallback), stepped together with 8 instances of 512^2 (MGPCG)"""
     from bevy_eulerian_fluid_amd import native, scenes
     if args.config == 4:
         return run_learned(args)
@@ -215,8 +216,19 @@ def run_config(args):
         desc = {"workload": "64 instances, sides log-uniform [128, 1024], 0-8 coupled bodies each, "
                             f"MG-preconditioned CG to {args.rel_tol}", "instances": 64, "cells": cells}
     elif args.config == 5:
-        sc = scenes.combined(4096)
+        if args.learned:
+            from bevy_eulerian_fluid_amd import learned
+            w = learned.load_weights()
+            if w is None:
+                raise SystemExit("--learned: no shipped weights (bevy_eulerian_fluid_amd/data)")
+            sc = scenes.combined(4096, proj=scenes.projection("learned", max_cycles=200, rel_tol=1e-5))
+        else:
+            sc = scenes.combined(4096)
         big = ctx.instance(sc["settings"])
+        if args.learned:
+            net = native.Network(ctx)
+            net.set_params(w)
+            big.set_learned(net)
         big.init_levelset(sc["levelset"])
         ctx.set_obstacles(sc["obstacles"])
         ctx.set_bodies(scenes.bodies_for(sc["obstacles"], sc["densities"]))
@@ -230,8 +242,9 @@ def run_config(args):
             insts.append(i)
         forces = (sc["forces"], sc["rng"])
         desc = {"workload": "4096x4096 pool + 200 droplets, 24 coupled bodies, 64 local forces, "
-                            "MG-preconditioned CG to 1e-5, + 8 instances of 512x512", "instances": 9,
-                "grid": [4096, 4096]}
+                            + ("learned preconditioner (flexible CG, MGPCG fallback)" if args.learned
+                               else "MG-preconditioned CG") + " to 1e-5, + 8 instances of 512x512 (MGPCG)",
+                "instances": 9, "grid": [4096, 4096]}
     else:
         raise SystemExit(f"unknown --config {args.config}")
     cells = sum(i.W * i.H for i in insts)
@@ -403,6 +416,8 @@ def main():
                          "with efl_autotune before timing")
     ap.add_argument("--config", type=int, default=2,
                     help="workload: 2 = coupled tank 1024^2 (the headline line); 1, 3, 4 (learned), 5 = the other suite configs")
+    ap.add_argument("--learned", action="store_true",
+                    help="config 5: the 4096^2 instance solves with the learned preconditioner")
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the roofline kernel (for profiling it alone)")
     ap.add_argument("--roofline-kernel", choices=["cg_smooth", "mg_smooth"], default="cg_smooth")
