@@ -361,12 +361,74 @@ __device__ float4 obstacle_bound(const efl_obstacle& o) {
 // id and the face velocities, are unchanged).
 __global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
   __shared__ float4 bnd[EFL_MAX_SOLIDS];
+  __shared__ int cand[EFL_MAX_SOLIDS];
+  __shared__ float wmin[NT / 64];
+  __shared__ int wcnt[NT / 64];
+  static_assert(EFL_MAX_SOLIDS <= NT, "one thread per obstacle");
   Site s = locate(b);
   CDevInst* I = cdev(s.I);
-  for (uint32_t k = threadIdx.x; k < I->nobs; k += NT) bnd[k] = obstacle_bound(I->obs[k]);
+  const int W = I->W, H = I->H, x = s.x, y = s.y;
+  auto world = [&](float fx, float fy) {
+    float ux = fx / (float)W, uy = fy / (float)H;
+    return xform(I->T, (ux - 0.5f) * I->sizex, (-uy + 0.5f) * I->sizey);
+  };
+  // The tile's candidates: the obstacles that can change any of its cells'
+  // results, in index order.  The cells' sample points (centres and the
+  // faces half a cell below and left) lie in a parallelogram, the world
+  // image of [x0 - 0.5, x0 + TX - 1] x [y0 - 0.5, y0 + TY - 1], inside the
+  // disk (tc, rt) through its corners.  Per obstacle with bounding disk
+  // (c, R) and d = |tc - c|: every cell's bound lb = |pc - c| - R - 1 is at
+  // least L = d - rt - R - 2 (one cell of slack for rounding), and its level
+  // anywhere in the tile at most U = d + rt + R + 1.  The border distance
+  // bounds a cell's level too, so every cell ends with level <= Umin (the
+  // least of those bounds).  An obstacle with L > Umin and L - fd >= 0.5 is
+  // then skipped by the per-cell tests at every cell of the tile (lb >=
+  // level, lb >= 0.5, lb - fd >= 0.5) and is not the cell's smallest bound
+  // either (the smallest bound is at most Umin), so leaving it out of both
+  // loops changes no result.
+  const int x0 = x - (int)(threadIdx.x % TX), y0 = y - (int)(threadIdx.x / TX);
+  const float2 q0 = world((float)x0 - 0.5f, (float)y0 - 0.5f), q3 = world((float)(x0 + TX - 1), (float)(y0 + TY - 1));
+  const float2 q1 = world((float)(x0 + TX - 1), (float)y0 - 0.5f), q2 = world((float)x0 - 0.5f, (float)(y0 + TY - 1));
+  const float tcx = 0.5f * (q0.x + q3.x), tcy = 0.5f * (q0.y + q3.y);
+  const float rt = fmaxf(fmaxf(hyp(q0.x - tcx, q0.y - tcy), hyp(q3.x - tcx, q3.y - tcy)),
+                         fmaxf(hyp(q1.x - tcx, q1.y - tcy), hyp(q2.x - tcx, q2.y - tcy)));
+  const float2 c00 = world((float)x0, (float)y0), u00 = world((float)x0 - 0.5f, (float)y0),
+               v00 = world((float)x0, (float)y0 - 0.5f);
+  const float fdt = fmaxf(hyp(u00.x - c00.x, u00.y - c00.y), hyp(v00.x - c00.x, v00.y - c00.y));
+  // the largest border distance min(tx, ty) over the tile's cells
+  const int xe = min(x0 + TX - 1, W - 1), ye = min(y0 + TY - 1, H - 1);
+  const int xm = min(max((W - 1) / 2, x0), xe), ym = min(max((H - 1) / 2, y0), ye);
+  const float ub = (float)min(min(xm, abs(W - 1 - xm)), min(ym, abs(H - 1 - ym)));
+  const uint32_t nobs = I->nobs;
+  const int k = (int)threadIdx.x;
+  float L = 0.0f, U = 3.0e38f;
+  bool use = false;
+  if ((uint32_t)k < nobs) {
+    const float4 bd = obstacle_bound(I->obs[k]);
+    bnd[k] = bd;
+    use = bd.z >= 0.0f;  // R = -1: level 1e6 everywhere, changes nothing
+    const float d = hyp(tcx - bd.x, tcy - bd.y);
+    L = d - rt - bd.z - 2.0f;
+    if (use) U = d + rt + bd.z + 1.0f;
+  }
+  float um = U;
+  for (int off = 32; off > 0; off >>= 1) um = fminf(um, __shfl_xor(um, off, 64));
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = um;
+  __syncthreads();
+  float umin = ub;
+  for (int w = 0; w < NT / 64; ++w) umin = fminf(umin, wmin[w]);
+  use = use && (L <= umin || L - fdt < 0.5f);
+  const unsigned long long m = __ballot(use);
+  if ((threadIdx.x & 63) == 0) wcnt[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  int base = 0, ncand = 0;
+  for (int w = 0; w < NT / 64; ++w) {
+    base += w < (int)(threadIdx.x >> 6) ? wcnt[w] : 0;
+    ncand += wcnt[w];
+  }
+  if (use) cand[base + __popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull))] = k;
   __syncthreads();
   if (!s.ok) return;
-  int W = I->W, H = I->H, x = s.x, y = s.y;
   size_t id = (size_t)y * W + x;
   if (x == 0 || y == 0 || x == W - 1 || y == H - 1) {
     I->lss[id] = 0.0f;
@@ -375,10 +437,6 @@ __global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
     I->sid[id] = -1;
     return;
   }
-  auto world = [&](float fx, float fy) {
-    float ux = fx / (float)W, uy = fy / (float)H;
-    return xform(I->T, (ux - 0.5f) * I->sizex, (-uy + 0.5f) * I->sizey);
-  };
   float2 pc = world((float)x, (float)y);
   float2 pu = world((float)x - 0.5f, (float)y - 0.0f);
   float2 pv = world((float)x - 0.0f, (float)y - 0.5f);
@@ -395,15 +453,16 @@ __global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
   // the reference's rule that the last body in index order sets the id and
   // the face velocities; a body is evaluated at the centre only if it could
   // lower the level or set the id, at the faces only if it could reach 0.5.
+  // Both passes run over the tile's candidates (index order).
   int kb = -1;
   float lbest = 3.0e38f;
-  for (uint32_t k = 0; k < I->nobs; ++k) {
-    const float4 bd = bnd[k];
-    if (bd.z < 0.0f) continue;  // level 1e6 everywhere: changes nothing
+  for (int j = 0; j < ncand; ++j) {
+    const int kk = __builtin_amdgcn_readfirstlane(cand[j]);
+    const float4 bd = bnd[kk];
     const float lb = hyp(pc.x - bd.x, pc.y - bd.y) - bd.z - 1.0f;
     if (lb < lbest) {
       lbest = lb;
-      kb = (int)k;
+      kb = kk;
     }
   }
   float lc_best = 0.0f;
@@ -411,15 +470,15 @@ __global__ __launch_bounds__(NT) void k_update_solid(Batch b) {
     lc_best = obstacle_level(I->obs[kb], pc.x, pc.y);
     if (lc_best < level) level = lc_best;
   }
-  for (uint32_t k = 0; k < I->nobs; ++k) {
-    const float4 bd = bnd[k];
-    if (bd.z < 0.0f) continue;
+  for (int j = 0; j < ncand; ++j) {
+    const int kk = __builtin_amdgcn_readfirstlane(cand[j]);
+    const float4 bd = bnd[kk];
     const float lb = hyp(pc.x - bd.x, pc.y - bd.y) - bd.z - 1.0f;
-    const bool centre = (int)k == kb || lb < level || lb < 0.5f, faces = lb - fd < 0.5f;
+    const bool centre = kk == kb || lb < level || lb < 0.5f, faces = lb - fd < 0.5f;
     if (!centre && !faces) continue;
-    const efl_obstacle& o = I->obs[k];
+    const efl_obstacle& o = I->obs[kk];
     if (centre) {
-      const float lc = (int)k == kb ? lc_best : obstacle_level(o, pc.x, pc.y);
+      const float lc = kk == kb ? lc_best : obstacle_level(o, pc.x, pc.y);
       if (lc < level) level = lc;
       if (lc < 0.5f) id_s = (int32_t)o.entity_id;
     }
