@@ -2560,7 +2560,7 @@ int efl_nn_destroy(efl_nn* nn) {
     }
   efl::nn::Trainer& t = nn->tr;
   for (void* p : {(void*)t.params, (void*)t.grads, (void*)t.m, (void*)t.v, (void*)t.loss, (void*)t.data,
-                  (void*)t.partial, (void*)t.idx, (void*)t.target, (void*)t.mask, (void*)nn->d_tmp,
+                  (void*)t.partial, (void*)t.idx, (void*)t.target, (void*)t.mask, (void*)t.rsums, (void*)nn->d_tmp,
                   (void*)nn->d_hist})
     if (p) (void)hipFree(p);
   nn_free_activations(t.act);
@@ -2620,7 +2620,8 @@ int efl_nn_dataset_reserve(efl_nn* nn, uint32_t w, uint32_t h, uint32_t capacity
   nn_free_activations(t.act);
   t.w = (int)w;
   t.h = (int)h;
-  if (hipMalloc(&t.data, (size_t)capacity * 5 * w * h * 4) != hipSuccess) return fail(c, EFL_E_OOM, "dataset");
+  if (hipMalloc(&t.data, (size_t)capacity * efl::nn::kSamplePlanes * w * h * 4) != hipSuccess)
+    return fail(c, EFL_E_OOM, "dataset");
   t.capacity = capacity;
   if (nn->tmp_floats < 2ull * w * h) {
     if (nn->d_tmp) (void)hipFree(nn->d_tmp);
@@ -2668,10 +2669,19 @@ int efl_nn_get_sample(const efl_nn* nn, uint32_t index, float* planes) {
   efl_ctx* c = nn->ctx;
   const efl::nn::Trainer& t = nn->tr;
   if (index >= t.count) return fail(c, EFL_E_RANGE, "sample %u of %u", index, t.count);
-  const size_t n = 5ull * t.w * t.h;
+  const size_t n = 5ull * t.w * t.h;  // the feature, mask and target planes; the stencil stays on the device
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipMemcpyAsync(planes, t.data + (size_t)index * n, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(planes, t.data + (size_t)index * efl::nn::kSamplePlanes * t.w * t.h, n * 4,
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return EFL_OK;
+}
+
+int efl_nn_set_loss(efl_nn* nn, int loss) {
+  if (!nn) return EFL_E_INVALID;
+  if (loss != EFL_NN_LOSS_MSE && loss != EFL_NN_LOSS_RESIDUAL)
+    return fail(nn->ctx, EFL_E_INVALID, "unknown loss %d", loss);
+  nn->tr.loss_mode = loss;
   return EFL_OK;
 }
 

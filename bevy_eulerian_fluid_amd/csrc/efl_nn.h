@@ -23,6 +23,13 @@ constexpr int kInCh = 16;      // input tensor channels (3 features, zero padded
 constexpr int kFeatures = 3;
 constexpr int kLevels = 4;     // resolutions 1, 1/2, 1/4, 1/8
 constexpr float kLsScale = 0.125f;  // level-set feature = clamp(phi / 8, -1, 1)
+// A dataset sample: planes r / sigma, solid fraction, clamped level set,
+// mask, target / rms(target) (the five efl_nn_get_sample returns), then the
+// pressure operator's stencil: diagonal and the four off-diagonal weights
+// (-x, +x, -y, +y) of each unknown, so that the residual-norm loss can form
+// A y for that sample.
+constexpr int kSamplePlanes = 10;
+constexpr int kLossMse = 0, kLossResidual = 1;  // efl_nn_set_loss
 
 // Layer k: out = act(conv3x3(cat(up?(A), B)) + bias); A = tensor in_a[k]
 // (upsampled x2 when upa), B = tensor in_b[k] (or none).
@@ -77,7 +84,7 @@ void precondition(hipStream_t st, const Packed& p, Activations& a, const DevInst
 struct Trainer {
   int w = 0, h = 0;        // sample size
   uint32_t capacity = 0, count = 0;
-  float* data = nullptr;   // per sample: 5 planes (r, solid, phi, mask, target), fp32
+  float* data = nullptr;   // per sample: kSamplePlanes planes, fp32
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
   float* partial = nullptr;  // weight-gradient partials
   size_t partial_floats = 0;
@@ -85,6 +92,8 @@ struct Trainer {
   int* idx = nullptr;        // batch sample indices (device)
   float* target = nullptr;   // batch target plane
   float* mask = nullptr;     // batch mask plane
+  double* rsums = nullptr;   // residual loss: r.r, r.Ay, Ay.Ay per batch sample
+  int loss_mode = kLossMse;
   Activations act{};
   int act_n = 0;
   uint64_t adam_t = 0;
@@ -93,9 +102,12 @@ struct Trainer {
 // Device buffers of a training batch of n samples.
 int trainer_reserve_batch(Trainer& t, int n);
 void pack(hipStream_t st, const float* params, Packed& p);
-// Loss (mean over the batch's pixels of mask (o - t)^2) and, with grads,
-// the parameter gradients into t.grads.  Returns the loss in *loss_out after
-// a stream sync.
+// Loss into t.loss and, with grads, the parameter gradients into t.grads.
+// kLossMse: mean over the batch's pixels of mask (o - t)^2.  kLossResidual:
+// mean over the batch's samples of 1 - (r . A y)^2 / (|r|^2 |A y|^2), y the
+// output: the squared relative residual that the best multiple of y leaves
+// (min over a of |r - a A y|^2 / |r|^2), which is what a flexible CG step
+// with preconditioner output y achieves.
 void eval_batch(hipStream_t st, Trainer& t, const Packed& p, const int* host_idx, int n, bool grads);
 void adam_step(hipStream_t st, Trainer& t, float lr);
 // Build one dataset sample on the device from an instance's fields and two

@@ -516,10 +516,16 @@ __global__ __launch_bounds__(256) void k_adam(float* p, const float* g, float* m
 
 // ---------------------------------------------------------------- data
 
-// A dataset sample: planes r / sigma, solid fraction, clamped level set,
-// mask (pressure unknown), target / rms(target).  Flexible CG fixes the
+// A dataset sample (kSamplePlanes planes): r / sigma, solid fraction,
+// clamped level set, mask (pressure unknown), target / rms(target), then the
+// stencil of the pressure operator at that cell.  Flexible CG fixes the
 // length of every step itself, so only the shape of A^-1 r matters and each
-// target is scaled to unit rms.
+// target is scaled to unit rms.  The stencil is residual.wgsl's operator
+// without its positive factor dt / (rho dx^2): (A y)_c = diag_c y_c -
+// sum_k off_k y_k over the neighbours -x, +x, -y, +y inside the grid, with
+// off_k = f_k for a fluid neighbour and f_k (1 - phi_k / phi_c) added to the
+// diagonal for an air one (f_k the area fraction of the shared face); rows
+// of cells that are not unknowns are zero.
 __global__ __launch_bounds__(256) void k_make_sample(float* dst, const DevInst* I, int W, int H, const float* r,
                                                      const float* t, float inv_sigma, float inv_trms) {
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
@@ -528,12 +534,34 @@ __global__ __launch_bounds__(256) void k_make_sample(float* dst, const DevInst* 
   const float4 a = I->afs[p];
   // the pressure unknowns of the current fields (k_mg_setup's MG_LSNEG and
   // not MG_CLOSED)
-  const bool act = I->ls0[p] < 0.0f && !(a.x == 0.0f && a.y == 0.0f && a.z == 0.0f && a.w == 0.0f);
+  const float phi = I->ls0[p];
+  const bool act = phi < 0.0f && !(a.x == 0.0f && a.y == 0.0f && a.z == 0.0f && a.w == 0.0f);
   dst[p] = r[p] * inv_sigma;
   dst[n + p] = 1.0f - 0.25f * (a.x + a.y + a.z + a.w);
-  dst[2 * n + p] = fminf(fmaxf(I->ls0[p] * kLsScale, -1.0f), 1.0f);
+  dst[2 * n + p] = fminf(fmaxf(phi * kLsScale, -1.0f), 1.0f);
   dst[3 * n + p] = act ? 1.0f : 0.0f;
   dst[4 * n + p] = t[p] * inv_trms;
+  const int x = (int)(p % W), y = (int)(p / W);
+  const float f[4] = {a.x, a.y, a.z, a.w};
+  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+  float diag = 0.0f, off[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int jx = x + ox[k], jy = y + oy[k];
+      if (jx < 0 || jy < 0 || jx >= W || jy >= H) continue;
+      const float pn = I->ls0[(long)jy * W + jx];
+      if (pn < 0.0f) {
+        off[k] = f[k];
+        diag += f[k];
+      } else {
+        diag += f[k] * (1.0f - pn / phi);
+      }
+    }
+  }
+  dst[5 * n + p] = diag;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) dst[(6 + k) * n + p] = off[k];
 }
 
 // batch input tensor, target and mask from dataset samples idx[n]
@@ -543,7 +571,7 @@ __global__ __launch_bounds__(256) void k_gather(const float* data, const int* id
   const long npx = (long)W * H;
   if (p >= npx) return;
   const int n = blockIdx.y;
-  const float* s = data + (size_t)idx[n] * 5 * npx;
+  const float* s = data + (size_t)idx[n] * kSamplePlanes * npx;
   uint16_t* o = x + ((size_t)n * npx + p) * kInCh;
   bf16x4 v0 = {(short)f2bf(s[p]), (short)f2bf(s[npx + p]), (short)f2bf(s[2 * npx + p]), 0};
   *(bf16x4*)o = v0;
@@ -575,6 +603,83 @@ __global__ __launch_bounds__(256) void k_loss(const uint16_t* o, const float* ta
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = l;
   __syncthreads();
   if (threadIdx.x == 0) atomicAdd(loss, (sh[0] + sh[1] + sh[2] + sh[3]) * (double)inv_count);
+}
+
+// Residual-norm loss, first pass: A y of each batch sample (y = output
+// channel 0; the stencil's zero rows and columns leave cells that are not
+// unknowns out) into ay, and the sums r.r, r.Ay, Ay.Ay (r = plane 0 on the
+// unknowns) into sums[3 n ..].  Grid (pixel blocks, samples).
+__global__ __launch_bounds__(256) void k_rloss_fwd(const float* data, const int* idx, const uint16_t* o, float* ay,
+                                                   double* sums, int W, int H) {
+  __shared__ double sh[3][4];
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long npx = (long)W * H;
+  const int n = blockIdx.y;
+  const float* s = data + (size_t)idx[n] * kSamplePlanes * npx;
+  const uint16_t* y = o + (size_t)n * npx * kInCh;
+  double rr = 0.0, ra = 0.0, aa = 0.0;
+  if (p < npx) {
+    const int x = (int)(p % W), yy = (int)(p / W);
+    float v = s[5 * npx + p] * bf2f(y[p * kInCh]);
+    if (x > 0) v -= s[6 * npx + p] * bf2f(y[(p - 1) * kInCh]);
+    if (x + 1 < W) v -= s[7 * npx + p] * bf2f(y[(p + 1) * kInCh]);
+    if (yy > 0) v -= s[8 * npx + p] * bf2f(y[(p - W) * kInCh]);
+    if (yy + 1 < H) v -= s[9 * npx + p] * bf2f(y[(p + W) * kInCh]);
+    const float r = s[3 * npx + p] != 0.0f ? s[p] : 0.0f;
+    ay[(size_t)n * npx + p] = v;
+    rr = (double)r * r;
+    ra = (double)r * v;
+    aa = (double)v * v;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    rr += __shfl_xor(rr, off, 64);
+    ra += __shfl_xor(ra, off, 64);
+    aa += __shfl_xor(aa, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = rr;
+    sh[1][threadIdx.x >> 6] = ra;
+    sh[2][threadIdx.x >> 6] = aa;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) atomicAdd(sums + 3 * n + threadIdx.x, sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3]);
+}
+
+// Second pass: loss += (1 - ra^2 / (rr aa)) / N per sample and, with g,
+// d loss / d y = A v (A is symmetric on the unknowns) with v = d loss / d Ay
+// = 2 ra / (rr aa) (alpha Ay - r) / N, alpha = ra / aa, in output channel 0
+// of g (other channels zero).  A sample whose A y is zero adds 1 and no
+// gradient.
+__global__ __launch_bounds__(256) void k_rloss_bwd(const float* data, const int* idx, const float* ay,
+                                                   const double* sums, uint16_t* g, double* loss, int W, int H,
+                                                   float inv_n) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long npx = (long)W * H;
+  const int n = blockIdx.y;
+  const double rr = sums[3 * n], ra = sums[3 * n + 1], aa = sums[3 * n + 2];
+  const bool ok = rr > 0.0 && aa > 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(loss, (ok ? 1.0 - ra * ra / (rr * aa) : 1.0) * (double)inv_n);
+  if (!g || p >= npx) return;
+  const float* s = data + (size_t)idx[n] * kSamplePlanes * npx;
+  const float* a = ay + (size_t)n * npx;
+  const float alpha = ok ? (float)(ra / aa) : 0.0f;
+  const float coef = ok ? (float)(2.0 * ra / (rr * aa)) * inv_n : 0.0f;
+  auto v = [&](long q) { return s[3 * npx + q] != 0.0f ? coef * (alpha * a[q] - s[q]) : 0.0f; };
+  const int x = (int)(p % W), yy = (int)(p / W);
+  float d = 0.0f;
+  if (s[3 * npx + p] != 0.0f) {
+    d = s[5 * npx + p] * v(p);
+    if (x > 0) d -= s[6 * npx + p] * v(p - 1);
+    if (x + 1 < W) d -= s[7 * npx + p] * v(p + 1);
+    if (yy > 0) d -= s[8 * npx + p] * v(p - W);
+    if (yy + 1 < H) d -= s[9 * npx + p] * v(p + W);
+  }
+  uint16_t* o = g + ((size_t)n * npx + p) * kInCh;
+  const bf16x4 zero = {0, 0, 0, 0};
+  bf16x4 v0 = {(short)f2bf(d), 0, 0, 0};
+  *(bf16x4*)o = v0;
+#pragma unroll
+  for (int q = 1; q < kInCh / 4; ++q) *(bf16x4*)(o + 4 * q) = zero;
 }
 
 // network output channel 0 -> fp32 plane
@@ -826,8 +931,12 @@ int trainer_reserve_batch(Trainer& t, int n) {
   if (t.idx) hipFree(t.idx);
   if (t.target) hipFree(t.target);
   if (t.mask) hipFree(t.mask);
+  if (t.rsums) hipFree(t.rsums);
   a.scratch = nullptr;
   t.partial = nullptr;
+  t.idx = nullptr;
+  t.target = t.mask = nullptr;
+  t.rsums = nullptr;
   t.act_n = 0;
   a.s = Shapes{n, t.w, t.h};
   size_t scratch = 0, part = 0;
@@ -848,6 +957,7 @@ int trainer_reserve_batch(Trainer& t, int n) {
   if (hipMalloc(&t.idx, n * sizeof(int)) != hipSuccess) return -1;
   if (hipMalloc(&t.target, (size_t)n * t.w * t.h * 4) != hipSuccess) return -1;
   if (hipMalloc(&t.mask, (size_t)n * t.w * t.h * 4) != hipSuccess) return -1;
+  if (hipMalloc(&t.rsums, (size_t)n * 3 * sizeof(double)) != hipSuccess) return -1;
   t.act_n = n;
   return 0;
 }
@@ -862,9 +972,17 @@ void eval_batch(hipStream_t st, Trainer& t, const Packed& p, const int* host_idx
                      t.w, t.h);
   forward_impl(st, p, a, nullptr);
   hipMemsetAsync(t.loss, 0, sizeof(double), st);
-  const float inv_count = 1.0f / (float)(npx * n);
-  hipLaunchKernelGGL(k_loss, dim3(blocks(npx * n)), dim3(256), 0, st, a.t[14], t.target, t.mask,
-                     grads ? a.g[14] : nullptr, npx * n, inv_count, t.loss);
+  if (t.loss_mode == kLossResidual) {
+    hipMemsetAsync(t.rsums, 0, sizeof(double) * 3 * n, st);
+    hipLaunchKernelGGL(k_rloss_fwd, dim3(blocks(npx), n), dim3(256), 0, st, t.data, t.idx, a.t[14], t.target,
+                       t.rsums, t.w, t.h);
+    hipLaunchKernelGGL(k_rloss_bwd, dim3(blocks(npx), n), dim3(256), 0, st, t.data, t.idx, t.target, t.rsums,
+                       grads ? a.g[14] : nullptr, t.loss, t.w, t.h, 1.0f / (float)n);
+  } else {
+    const float inv_count = 1.0f / (float)(npx * n);
+    hipLaunchKernelGGL(k_loss, dim3(blocks(npx * n)), dim3(256), 0, st, a.t[14], t.target, t.mask,
+                       grads ? a.g[14] : nullptr, npx * n, inv_count, t.loss);
+  }
   if (!grads) return;
   for (int k = kLayers - 1; k >= 0; --k) {
     const LayerDef& L = kNet[k];
@@ -919,7 +1037,7 @@ void adam_step(hipStream_t st, Trainer& t, float lr) {
 
 void make_sample(hipStream_t st, Trainer& t, uint32_t slot, const DevInst* I, int W, int H, const float* dev_r,
                  const float* dev_t, float sigma, float trms) {
-  float* dst = t.data + (size_t)slot * 5 * W * H;
+  float* dst = t.data + (size_t)slot * kSamplePlanes * W * H;
   hipLaunchKernelGGL(k_make_sample, dim3(blocks((long)W * H)), dim3(256), 0, st, dst, I, W, H, dev_r, dev_t,
                      1.0f / sigma, 1.0f / trms);
 }

@@ -107,9 +107,10 @@ def dataset_summary(net):
     return dict(finite=fin, max_abs=[float(x) for x in mx])
 
 
-def generate(ctx, net, size, n_scenes, captures=(4, 12, 30), log=print):
-    """Dataset of n_scenes x len(captures) x 5 samples of size x size."""
-    net.reserve(size, size, n_scenes * len(captures) * 5)
+def generate(ctx, net, size, n_scenes, captures=(4, 12, 30), truncated=(1, 2, 4), log=print):
+    """Dataset of n_scenes x len(captures) x (2 + len(truncated)) samples of
+    size x size."""
+    net.reserve(size, size, n_scenes * len(captures) * (2 + len(truncated)))
     t0 = time.time()
     skipped = []
     for k in range(n_scenes):
@@ -124,10 +125,10 @@ def generate(ctx, net, size, n_scenes, captures=(4, 12, 30), log=print):
             while step < c - 1:
                 inst.step(DT)
                 step += 1
-            capture(net, inst, skipped)
+            capture(net, inst, skipped, truncated)
             step += 1
         inst.close()
-        if (k + 1) % 8 == 0:
+        if (k + 1) % max(8, n_scenes // 50) == 0:
             log(f"[learned] scenes {k + 1}/{n_scenes}, samples {net.count()}, {time.time() - t0:.1f} s")
     if skipped:
         log(f"[learned] skipped {len(skipped)} samples (rhs finite, target finite): {skipped[:8]}")
@@ -181,8 +182,9 @@ def evaluate(ctx, weights, size=1024, steps=10, warmup=10, rel_tol=1e-5, seed=sc
         a.step(DT)
         b.step(DT)
         sa, sb = a.stats(), b.stats()
-        nn_it, fb = b.learned_iterations()
+        nn_it, fb, nn_rel = b.learned_iterations(with_rel=True)
         rows.append(dict(mgpcg_iters=sa.cycles, mgpcg_ms=sa.projection_ms, learned_iters=nn_it, fallback_iters=fb,
+                         network_part_rel=nn_rel,
                          learned_ms=sb.projection_ms, learned_converged=int(sb.converged),
                          learned_rel=sb.rel_residual))
     ctx.set_timing(False)
@@ -207,7 +209,12 @@ def main():
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--loss", choices=("mse", "residual"), default="mse")
+    ap.add_argument("--captures", default="4,12,30", help="steps of each scene at which samples are taken")
+    ap.add_argument("--truncated", default="1,2,4", help="MGPCG iteration counts of the partly converged samples")
+    ap.add_argument("--init", default=None, help="weights file to start from (default: He-normal from --seed)")
     a = ap.parse_args()
+    ints = lambda t: tuple(int(x) for x in t.split(",") if x)
     os.makedirs(a.out, exist_ok=True)
     ctx = native.Context(0)
     if a.evaluate:
@@ -219,8 +226,11 @@ def main():
         return
     net = native.Network(ctx)
     net.init_params(a.seed)
+    if a.init:
+        net.set_params(np.fromfile(a.init, dtype=np.float32))
+    net.set_loss(a.loss)
     t0 = time.time()
-    n = generate(ctx, net, a.size, a.scenes, log=lambda m: print(m, flush=True))
+    n = generate(ctx, net, a.size, a.scenes, ints(a.captures), ints(a.truncated), log=lambda m: print(m, flush=True))
     t_gen = time.time() - t0
     print(f"[learned] {n} samples of {a.size}x{a.size} in {t_gen:.1f} s; {dataset_summary(net)}", flush=True)
     t0 = time.time()
@@ -228,7 +238,8 @@ def main():
     t_train = time.time() - t0
     w = net.get_params()
     w.tofile(os.path.join(a.out, "weights.f32"))
-    rec = dict(params=int(w.size), size=a.size, scenes=a.scenes, samples=n, iterations=a.iterations,
+    rec = dict(params=int(w.size), size=a.size, scenes=a.scenes, samples=n, loss=a.loss, captures=a.captures,
+               truncated=a.truncated, init=os.path.basename(a.init) if a.init else None, iterations=a.iterations,
                batch=a.batch, lr=a.lr, seed=a.seed, seed_base=SEED, generate_s=round(t_gen, 1),
                train_s=round(t_train, 1), loss_first=float(hist[:20].mean()), loss_last=float(hist[-20:].mean()),
                loss_every_50=[float(x) for x in hist[::50]])

@@ -25,7 +25,7 @@ EXPORTS = [
     "efl_set_instance_obstacles", "efl_get_instance_obstacles",
     "efl_nn_param_count", "efl_nn_flops_per_pixel", "efl_nn_create", "efl_nn_destroy", "efl_nn_init_params",
     "efl_nn_set_params", "efl_nn_get_params", "efl_nn_dataset_reserve", "efl_nn_add_sample",
-    "efl_nn_sample_count", "efl_nn_get_sample", "efl_nn_train", "efl_nn_eval", "efl_set_learned",
+    "efl_nn_sample_count", "efl_nn_get_sample", "efl_nn_set_loss", "efl_nn_train", "efl_nn_eval", "efl_set_learned",
     "efl_nn_last_solve", "efl_autotune", "efl_apply_tuning", "efl_get_fluid_diagnostics", "efl_fluid_diagnostics_noalloc",
     "efl_set_async", "efl_host_waits",
 ]
@@ -99,6 +99,7 @@ def load():
         "efl_nn_add_sample": [P, P, P, P],
         "efl_nn_sample_count": [P, C.POINTER(C.c_uint32)],
         "efl_nn_get_sample": [P, C.c_uint32, P],
+        "efl_nn_set_loss": [P, C.c_int],
         "efl_nn_train": [P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, P],
         "efl_nn_eval": [P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float), P, P],
         "efl_set_learned": [P, P],
@@ -442,6 +443,13 @@ class Network:
         a = np.zeros((5, self.h_, self.w), dtype=np.float32)
         self._chk(self.L.efl_nn_get_sample(self.h, index, a.ctypes.data), "efl_nn_get_sample")
         return a
+
+    LOSS = {"mse": 0, "residual": 1}
+
+    def set_loss(self, loss):
+        """"mse" (output against the sample's target) or "residual" (the
+        relative residual the best multiple of the output leaves)."""
+        self._chk(self.L.efl_nn_set_loss(self.h, self.LOSS[loss]), "efl_nn_set_loss")
 
     def train(self, iterations, batch, lr, seed):
         hist = np.zeros(iterations, dtype=np.float32)

@@ -493,6 +493,16 @@ int efl_nn_sample_count(const efl_nn* nn, uint32_t* count);
  * solid fraction, clamped level set, mask, target / rms(target): flexible CG
  * fixes every step length itself, so only the shape of A^-1 r is learned). */
 int efl_nn_get_sample(const efl_nn* nn, uint32_t index, float* planes);
+/* The loss efl_nn_train and efl_nn_eval minimise and report.
+ * EFL_NN_LOSS_MSE (the default): mean over the batch's pixels of
+ * mask (output - target)^2.  EFL_NN_LOSS_RESIDUAL: mean over the batch's
+ * samples of 1 - (r . A y)^2 / (|r|^2 |A y|^2), y the output and A the
+ * sample's own pressure operator (its stencil is stored with the sample):
+ * the squared relative residual that the best multiple of y leaves, which is
+ * what one flexible-CG step with the network achieves; targets are unused.
+ * Other values return EFL_E_INVALID. */
+enum { EFL_NN_LOSS_MSE = 0, EFL_NN_LOSS_RESIDUAL = 1 };
+int efl_nn_set_loss(efl_nn* nn, int loss);
 /* iterations Adam steps on random batches of `batch` samples (seeded);
  * loss_history (may be NULL) receives each step's loss. */
 int efl_nn_train(efl_nn* nn, uint32_t iterations, uint32_t batch, float lr, uint64_t seed, float* loss_history);

@@ -123,20 +123,13 @@ def forward(params, x, round_=True):
     return t, cache
 
 
-def loss_and_grads(params, planes, round_=True):
-    """Loss = sum mask (o - target)^2 / (N H W) and d loss / d params (float64)."""
-    x = input_tensor(planes)
-    t, cache = forward(params, x, round_)
+def _backward(params, t, cache, g14, round_):
+    """Parameter gradients (flat, float64) of a forward pass given the
+    gradient of the output tensor (N, H, W, 16)."""
     R = bf16 if round_ else (lambda a: np.asarray(a, dtype=np.float64))
     layers = split_params(np.asarray(params, dtype=np.float32 if round_ else np.float64))
-    n, h, w = planes.shape[0], planes.shape[2], planes.shape[3]
-    count = n * h * w
-    target, mask = planes[:, 4].astype(np.float64), planes[:, 3]
-    d = np.where(mask != 0, t[14][..., 0] - target, 0.0)
-    loss = float((d * d).sum() / count)
     g = {k: None for k in range(19)}
-    g[14] = np.zeros_like(t[14])
-    g[14][..., 0] = 2.0 * d / count
+    g[14] = g14
     grads = []
     for k in range(len(NET) - 1, -1, -1):
         ca, cb, co, lvl, upa, relu, ia, ib = NET[k]
@@ -166,4 +159,86 @@ def loss_and_grads(params, planes, round_=True):
         flat[off:off + nw] = dw.reshape(-1)
         flat[off + nw:off + nw + co] = db
         off += nw + co
-    return loss, flat, t[14][..., 0]
+    return flat
+
+
+def loss_and_grads(params, planes, round_=True):
+    """Loss = sum mask (o - target)^2 / (N H W) and d loss / d params (float64)."""
+    x = input_tensor(planes)
+    t, cache = forward(params, x, round_)
+    n, h, w = planes.shape[0], planes.shape[2], planes.shape[3]
+    count = n * h * w
+    target, mask = planes[:, 4].astype(np.float64), planes[:, 3]
+    d = np.where(mask != 0, t[14][..., 0] - target, 0.0)
+    loss = float((d * d).sum() / count)
+    g14 = np.zeros_like(t[14])
+    g14[..., 0] = 2.0 * d / count
+    return loss, _backward(params, t, cache, g14, round_), t[14][..., 0]
+
+
+def stencil(ls, afs):
+    """The pressure operator of a grid, up to its positive factor dt / (rho
+    dx^2), as (diag, off) with off[..., k] the weight of neighbour k (-x, +x,
+    -y, +y): (A y)_c = diag_c y_c - sum_k off_k y_k.  residual.wgsl:12-57's
+    terms: a fluid neighbour (phi < 0) couples with the face's area fraction
+    f_k, an air neighbour adds f_k (1 - phi_k / phi_c) to the diagonal, a
+    neighbour outside the grid adds nothing; cells that are not unknowns
+    (phi >= 0, or all four fractions zero) have zero rows.  float64."""
+    ls = np.asarray(ls, dtype=np.float64)
+    afs = np.asarray(afs, dtype=np.float64)
+    h, w = ls.shape
+    act = (ls < 0) & ~(afs == 0).all(axis=-1)
+    diag = np.zeros((h, w))
+    off = np.zeros((h, w, 4))
+    pad = np.pad(ls, 1, constant_values=np.nan)
+    nb = [pad[1:-1, :-2], pad[1:-1, 2:], pad[:-2, 1:-1], pad[2:, 1:-1]]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for k in range(4):
+            inside = ~np.isnan(nb[k])
+            fluid = inside & (nb[k] < 0)
+            air = inside & ~(nb[k] < 0)
+            off[..., k] = np.where(act & fluid, afs[..., k], 0.0)
+            diag += np.where(act & fluid, afs[..., k], 0.0)
+            diag += np.where(act & air, afs[..., k] * (1.0 - nb[k] / ls), 0.0)
+    return diag, off
+
+
+def apply_stencil(diag, off, y):
+    """A y for (diag, off) of stencil(); y (..., H, W)."""
+    out = diag * y
+    out[..., :, 1:] -= off[..., :, 1:, 0] * y[..., :, :-1]
+    out[..., :, :-1] -= off[..., :, :-1, 1] * y[..., :, 1:]
+    out[..., 1:, :] -= off[..., 1:, :, 2] * y[..., :-1, :]
+    out[..., :-1, :] -= off[..., :-1, :, 3] * y[..., 1:, :]
+    return out
+
+
+def residual_loss(y, r, diag, off, mask):
+    """Per sample 1 - (r . A y)^2 / (|r|^2 |A y|^2) (r on the unknowns) and
+    d loss / d y, summed over the samples and divided by their count."""
+    n = y.shape[0]
+    r = np.where(mask != 0, r, 0.0)
+    ay = apply_stencil(diag, off, y)
+    loss, gy = 0.0, np.zeros_like(y)
+    for i in range(n):
+        rr, ra, aa = (r[i] * r[i]).sum(), (r[i] * ay[i]).sum(), (ay[i] * ay[i]).sum()
+        if rr > 0 and aa > 0:
+            loss += (1.0 - ra * ra / (rr * aa)) / n
+            v = np.where(mask[i] != 0, 2.0 * ra / (rr * aa) * (ra / aa * ay[i] - r[i]) / n, 0.0)
+            gy[i] = np.where(mask[i] != 0, apply_stencil(diag[i], off[i], v), 0.0)
+        else:
+            loss += 1.0 / n
+    return loss, gy
+
+
+def residual_loss_and_grads(params, planes, diag, off, round_=True):
+    """The residual-norm loss of the network's outputs on samples planes
+    (N, 5, H, W) with stencils (diag (N, H, W), off (N, H, W, 4)), and
+    d loss / d params (float64)."""
+    x = input_tensor(planes)
+    t, cache = forward(params, x, round_)
+    y = t[14][..., 0]
+    loss, gy = residual_loss(y, planes[:, 0].astype(np.float64), diag, off, planes[:, 3])
+    g14 = np.zeros_like(t[14])
+    g14[..., 0] = gy
+    return loss, _backward(params, t, cache, g14, round_), y

@@ -100,6 +100,59 @@ def test_gradients_match_oracle(setup):
     assert cos >= 0.998, cos
 
 
+def test_residual_loss_matches_oracle(setup):
+    """EFL_NN_LOSS_RESIDUAL: the loss 1 - (r . A y)^2 / (|r|^2 |A y|^2) and its
+    parameter gradients against the oracle's float64 restatement, with the
+    stencil assembled in NumPy from the instance's level set and area
+    fractions (the fields the samples were made from)."""
+    _say("nn: residual-norm loss and gradients 2 x 64x64 vs oracle")
+    ctx, inst, net = setup
+    net.set_loss("residual")
+    try:
+        loss, grads, _ = net.eval(0, 2, grads=True)
+    finally:
+        net.set_loss("mse")
+    planes = np.stack([net.sample(i) for i in range(2)]).astype(np.float64)
+    diag, off = O.stencil(inst.get("LS_AIR0").reshape(64, 64), inst.get("AREA_FRACTION").reshape(64, 64, 4))
+    loss_o, grads_o, _ = O.residual_loss_and_grads(net.get_params(), planes, np.stack([diag] * 2),
+                                                   np.stack([off] * 2))
+    assert 0.0 < loss_o < 1.0
+    assert abs(loss - loss_o) <= 2e-2 * loss_o, (loss, loss_o)
+    off_p = 0
+    for k, (ca, cb, co, *_r) in enumerate(O.NET):
+        n = co * (ca + cb) * 9 + co
+        g, r = grads[off_p:off_p + n], grads_o[off_p:off_p + n]
+        if np.linalg.norm(r) > 0:
+            err = np.linalg.norm(g - r) / np.linalg.norm(r)
+            assert err <= 6e-2, (k, err)
+        off_p += n
+    cos = grads @ grads_o / (np.linalg.norm(grads) * np.linalg.norm(grads_o))
+    assert cos >= 0.998, cos
+
+
+def test_residual_training_lowers_loss():
+    _say("nn: 150 Adam steps of the residual-norm loss on 64x64 solver samples")
+    ctx = native.Context(0)
+    try:
+        inst = _instance(ctx, 64, scenes.projection("mgpcg", max_cycles=80, rel_tol=1e-7), steps=1)
+        net = native.Network(ctx)
+        net.init_params(12)
+        net.set_loss("residual")
+        net.reserve(64, 64, 12)
+        for _ in range(12):
+            inst.step(DT)
+            net.add_sample(inst, inst.get("DIV"), inst.get("P0"))
+        hist = net.train(150, 4, 2e-3, 99)
+        assert np.isfinite(hist).all() and (hist >= 0).all() and (hist <= 1.0 + 1e-6).all()
+        first, last = hist[:10].mean(), hist[-10:].mean()
+        _say(f"nn: residual loss {first:.4f} -> {last:.4f}")
+        assert last < 0.5 * first, (first, last)
+        net.close()
+        inst.close()
+    finally:
+        ctx.close()
+
+
 def test_training_lowers_loss():
     _say("nn: 150 Adam steps on 64x64 solver samples")
     ctx = native.Context(0)
