@@ -70,6 +70,7 @@ struct efl_ctx {
   bool graphs_on = false;  // eager launches measured faster than replay on this runtime
   int bottom_cells = 1024;  // the bottom solve takes the levels of at most this many cells
   double* h_scal = nullptr;  // pinned, 16 doubles per batch slot
+  double* d_scal = nullptr;  // device staging of read_scalars: 16 doubles per batch slot
   std::vector<efl_instance*> insts;
   efl_body* d_bodies = nullptr;  // EFL_MAX_SOLIDS records
   uint32_t nbodies = 0;          // 0 = no integration of the context's list
@@ -745,9 +746,17 @@ void precondition(hipStream_t st, const IV& v, bool cg_check = false) {
   prefold(st, v, 0);
 }
 
+// The first 16 scalars of every instance of v into c->h_scal (16 per
+// instance, in v's order): one gather launch and one copy back, then a wait.
 int read_scalars(efl_ctx* c, const IV& v) {
-  for (size_t i = 0; i < v.size(); ++i)
-    HIPCHK(c, hipMemcpyAsync(c->h_scal + 16 * i, v[i]->h.scal, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (v.size() > (size_t)EFL_MAX_BATCH) return fail(c, EFL_E_RANGE, "%zu instances in one read", v.size());
+  if (v.empty()) return EFL_OK;
+  efl::Batch b;
+  std::memset(&b, 0, sizeof(b));
+  b.n = (int)v.size();
+  for (size_t j = 0; j < v.size(); ++j) b.inst[j] = v[j]->d;
+  efl::launch_gather_scalars(c->stream, b, c->d_scal);
+  HIPCHK(c, hipMemcpyAsync(c->h_scal, c->d_scal, 16 * v.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   return host_wait(c);
 }
 
@@ -964,17 +973,41 @@ int cg_run_async(efl_ctx* c, const IV& cg, const hipGraphExec_t* iter) {
   return EFL_OK;
 }
 
+// Each instance is queued as many iterations as its previous solve took (4
+// the first time); the eager path leaves an instance out of the queued
+// iterations beyond its own count, since a stopped instance's kernels return
+// at once but its workgroups are still dispatched (in a batch whose solves
+// end at different iterations that was most of the tail's time).  Every
+// instance keeps its own iteration sequence (rpar / fax change only in the
+// launches it takes part in), so leaving it out changes none of its results.
+// Then the flags are read and the instances that have not stopped continue
+// one iteration at a time.  A replayed graph advances the whole set.
 int cg_run(efl_ctx* c, const IV& cg, const hipGraphExec_t* iter, bool sync = false) {
   if (c->async_margin && !sync) return cg_run_async(c, cg, iter);
   hipStream_t st = c->stream;
   uint32_t n = 0, queued = 0;
-  for (auto* i : cg) n = std::max(n, std::min(i->cg_last ? i->cg_last : 4u, i->h.cg_maxc));
+  std::vector<uint32_t> want(cg.size());
+  for (size_t j = 0; j < cg.size(); ++j) {
+    efl_instance* i = cg[j];
+    want[j] = std::min(i->cg_last ? i->cg_last : 4u, i->h.cg_maxc);
+    n = std::max(n, want[j]);
+  }
   IV act = cg;
+  bool first = true;
   while (!act.empty()) {
     for (uint32_t k = 0; k < n; ++k, ++queued) {
-      if (iter) HIPCHK(c, hipGraphLaunch(iter[queued & 1], st));
-      else cg_iteration(st, act, c->cg_fuse);
+      if (iter) {
+        HIPCHK(c, hipGraphLaunch(iter[queued & 1], st));
+      } else if (first) {
+        IV sub;
+        for (size_t j = 0; j < cg.size(); ++j)
+          if (want[j] > k) sub.push_back(cg[j]);
+        cg_iteration(st, sub, c->cg_fuse);
+      } else {
+        cg_iteration(st, act, c->cg_fuse);
+      }
     }
+    first = false;
     n = 1;
     int rc = read_scalars(c, act);
     if (rc) return rc;
@@ -1400,17 +1433,30 @@ void integrate_bodies(efl_ctx* c, const IV& stepped, float dt) {
       efl::launch_integrate_bodies(c->stream, a);
     }
   }
+  // instances with their own lists: up to kMaxBodyLists per launch
+  efl::BodyLists L;
+  std::memset(&L, 0, sizeof(L));
+  L.dt = dt;
   for (auto* i : stepped) {
     if (!i->own_obs || !i->own_bodies || i->nobs_own == 0) continue;
-    efl::BodyStep a;
-    std::memset(&a, 0, sizeof(a));
-    a.obs = i->d_obs_own;
-    a.bodies = i->d_bodies_own;
-    a.state = (float4*)(i->d_bodies_own + EFL_MAX_SOLIDS);
-    a.n = (int)i->nobs_own;
-    fill(a, IV{i});
-    efl::launch_integrate_bodies(c->stream, a);
+    efl::BodyList& b = L.list[L.n++];
+    b.obs = i->d_obs_own;
+    b.bodies = i->d_bodies_own;
+    b.state = (float4*)(i->d_bodies_own + EFL_MAX_SOLIDS);
+    b.fts = i->h.fts;
+    b.n = (int)i->nobs_own;
+    b.dx = i->h.dx;
+    b.gwx = i->h.gx;  // grid frame (y down) -> world frame (y up)
+    b.gwy = -i->h.gy;
+    b.half_w = 0.5f * (float)i->s.width;
+    b.half_h = 0.5f * (float)i->s.height;
+    b.rho_fluid = i->h.rho;
+    if (L.n == efl::kMaxBodyLists) {
+      efl::launch_integrate_own_bodies(c->stream, L);
+      L.n = 0;
+    }
   }
+  if (L.n > 0) efl::launch_integrate_own_bodies(c->stream, L);
 }
 
 // ---- replayed steps -------------------------------------------------------------
@@ -1584,8 +1630,10 @@ int efl_ctx_create(int device, efl_ctx** out) {
     return EFL_E_HIP;
   }
   if (hipMalloc(&c->d_obs, sizeof(efl_obstacle) * EFL_MAX_SOLIDS) != hipSuccess ||
+      hipMalloc(&c->d_scal, sizeof(double) * 16 * EFL_MAX_BATCH) != hipSuccess ||
       hipHostMalloc((void**)&c->h_scal, sizeof(double) * 16 * EFL_MAX_BATCH, hipHostMallocDefault) != hipSuccess) {
     if (c->d_obs) hipFree(c->d_obs);
+    if (c->d_scal) hipFree(c->d_scal);
     hipStreamDestroy(c->stream);
     delete c;
     return EFL_E_OOM;
@@ -1593,6 +1641,7 @@ int efl_ctx_create(int device, efl_ctx** out) {
   // the no-alloc entry point's scratch on this device (its own copy per GPU)
   if (hipGetSymbolAddress((void**)&c->scratch_ptr, HIP_SYMBOL(scratch)) != hipSuccess || !c->scratch_ptr) {
     hipFree(c->d_obs);
+    hipFree(c->d_scal);
     hipHostFree(c->h_scal);
     hipStreamDestroy(c->stream);
     delete c;
@@ -1619,6 +1668,7 @@ int efl_ctx_destroy(efl_ctx* c) {
     if (e) hipEventDestroy(e);
   if (c->d_obs) hipFree(c->d_obs);
   if (c->d_bodies) hipFree(c->d_bodies);
+  if (c->d_scal) hipFree(c->d_scal);
   if (c->h_scal) hipHostFree(c->h_scal);
   hipStreamDestroy(c->stream);
   delete c;

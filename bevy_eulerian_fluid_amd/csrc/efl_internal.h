@@ -143,6 +143,23 @@ struct BodyStep {
   float4* state;  // per body: previous (ax, ay, angular acceleration)
 };
 
+// One instance's own body list for k_integrate_own_bodies (the fields of
+// BodyStep for a single instance).
+struct BodyList {
+  efl_obstacle* obs;
+  const efl_body* bodies;
+  float4* state;
+  const float4* fts;
+  int n;
+  float dx, gwx, gwy, half_w, half_h, rho_fluid;
+};
+constexpr int kMaxBodyLists = 32;  // per launch (kernel argument size)
+struct BodyLists {
+  int n;
+  float dt;
+  BodyList list[kMaxBodyLists];
+};
+
 // Level::fl bits, written by k_mg_setup
 constexpr int MG_IN0 = 0;           // bits 0-3: neighbour k (-x, +x, -y, +y) inside the level
 constexpr int MG_FL0 = 4;           // bits 4-7: neighbour k inside the level and fluid (ls < 0)

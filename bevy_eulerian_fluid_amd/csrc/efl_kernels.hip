@@ -1490,6 +1490,13 @@ __global__ void k_cg_tally(Batch b) {
 // so the scalar is the same either way): kind 0, after a preconditioner:
 // r . z (and z . q_prev for flexible CG) -> scal[SC_RZF..];  kind 1, after
 // k_pcg_apply: p . q over the Batch's tiles -> scal[SC_PQF].
+// The host's view of a batch's scalars: the first 16 of each instance,
+// packed, so that one copy brings them all back.
+__global__ void k_gather_scalars(Batch b, double* dst) {
+  const DevInst* I = b.inst[blockIdx.x];
+  if (threadIdx.x < 16) dst[16 * blockIdx.x + threadIdx.x] = I->scal[threadIdx.x];
+}
+
 __global__ __launch_bounds__(NT) void k_prefold(Batch b, int kind) {
   const int slot = blockIdx.x;
   const DevInst* I = b.inst[slot];
@@ -1810,21 +1817,67 @@ __device__ float body_radius(const efl_obstacle& o) {
   }
 }
 
+// One semi-implicit Euler step of body k of a list, given the fluid force
+// (Fx, Fy) and torque Tq on it in grid units.  F = m a with the fluid force
+// in cells (F_true = F dx, m_true = m dx^2) and an added-mass estimate
+// (m_a = rho_fluid * area, previous acceleration) that keeps explicit
+// coupling of light bodies stable.
+__device__ void integrate_body(efl_obstacle* obs, const efl_body& bd, float4* state, int k, float Fx, float Fy,
+                               float Tq, float dt, float gwx, float gwy, float dx0, float half_w, float half_h,
+                               float rho_fluid) {
+  efl_obstacle o = obs[k];
+  float m = bd.density * bd.area;
+  float J = bd.density * bd.inertia;
+  float ma = 4.0f * rho_fluid * bd.area, Ja = 2.0f * rho_fluid * bd.inertia;
+  float4 prev = state[k];
+  float ax = (Fx + ma * prev.x + m * gwx / dx0) / (m + ma);
+  float ay = (Fy + ma * prev.y + m * gwy / dx0) / (m + ma);
+  float al = (J + Ja) > 0.0f ? (Tq + Ja * prev.z) / (J + Ja) : 0.0f;
+  state[k] = make_float4(ax, ay, al, 0.0f);
+  float vx = o.linear_velocity[0] + ax * dt;
+  float vy = o.linear_velocity[1] + ay * dt;
+  float w = o.angular_velocity + al * dt;
+  // a body moves at most one cell per step (translation, and rotation at its
+  // bounding radius): the area fractions and solid face velocities of the
+  // next step assume it, and faster bodies trap and squeeze fluid
+  float R = body_radius(o);
+  const float vmax = 1.0f / dt;
+  const float sp = sqrtf(vx * vx + vy * vy);
+  if (sp > vmax) {
+    const float sc = vmax / sp;
+    vx = vx * sc;
+    vy = vy * sc;
+  }
+  if (R > 0.0f && fabsf(w) * R > vmax) w = copysignf(vmax / R, w);
+  float tx = o.transform[12] + vx * dt;
+  float ty = o.transform[13] + vy * dt;
+  float ang = atan2f(o.transform[1], o.transform[0]) + w * dt;
+  float lx = fmaxf(half_w - 1.0f - R, 0.0f), ly = fmaxf(half_h - 1.0f - R, 0.0f);
+  if (tx > lx) { tx = lx; vx = fminf(vx, 0.0f); }
+  if (tx < -lx) { tx = -lx; vx = fmaxf(vx, 0.0f); }
+  if (ty > ly) { ty = ly; vy = fminf(vy, 0.0f); }
+  if (ty < -ly) { ty = -ly; vy = fmaxf(vy, 0.0f); }
+  float c = cosf(ang), sn = sinf(ang);
+  float T[16] = {c, sn, 0.f, 0.f, -sn, c, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, tx, ty, 0.f, 1.f};
+  float Ti[16] = {c, -sn, 0.f, 0.f, sn, c, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, -(c * tx + sn * ty), -(-sn * tx + c * ty), 0.f, 1.f};
+  for (int q = 0; q < 16; ++q) {
+    obs[k].transform[q] = T[q];
+    obs[k].inverse_transform[q] = Ti[q];
+  }
+  obs[k].linear_velocity[0] = vx;
+  obs[k].linear_velocity[1] = vy;
+  obs[k].angular_velocity = w;
+}
+
+// The context's body list, driven by the forces of every instance that
+// shares it.
 __global__ void k_integrate_bodies(BodyStep a) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
-  efl_obstacle o = a.obs[k];
   const efl_body bd = a.bodies[k];
   if (!bd.dynamic) return;
-  // F = m a with the fluid force in cells (F_true = F dx, m_true = m dx^2)
-  // and an added-mass estimate (m_a = rho_fluid * area, previous
-  // acceleration) that keeps explicit coupling of light bodies stable.
-  float m = bd.density * bd.area;
-  float J = bd.density * bd.inertia;
-  float ma = 4.0f * a.rho_fluid * bd.area, Ja = 2.0f * a.rho_fluid * bd.inertia;
-  float4 prev = a.state[k];
   float Fx = 0.0f, Fy = 0.0f, Tq = 0.0f;
-  uint32_t id = o.entity_id;
+  uint32_t id = a.obs[k].entity_id;
   if (id < (uint32_t)EFL_MAX_SOLIDS) {
     for (int i = 0; i < a.ninst; ++i) {
       float4 f = a.fts[i][id];
@@ -1834,43 +1887,27 @@ __global__ void k_integrate_bodies(BodyStep a) {
       Tq += f.w * s;   // world-frame torque bin
     }
   }
-  float ax = (Fx + ma * prev.x + m * a.gwx / a.dx[0]) / (m + ma);
-  float ay = (Fy + ma * prev.y + m * a.gwy / a.dx[0]) / (m + ma);
-  float al = (J + Ja) > 0.0f ? (Tq + Ja * prev.z) / (J + Ja) : 0.0f;
-  a.state[k] = make_float4(ax, ay, al, 0.0f);
-  float vx = o.linear_velocity[0] + ax * a.dt;
-  float vy = o.linear_velocity[1] + ay * a.dt;
-  float w = o.angular_velocity + al * a.dt;
-  // a body moves at most one cell per step (translation, and rotation at its
-  // bounding radius): the area fractions and solid face velocities of the
-  // next step assume it, and faster bodies trap and squeeze fluid
-  float R = body_radius(o);
-  const float vmax = 1.0f / a.dt;
-  const float sp = sqrtf(vx * vx + vy * vy);
-  if (sp > vmax) {
-    const float sc = vmax / sp;
-    vx = vx * sc;
-    vy = vy * sc;
+  integrate_body(a.obs, bd, a.state, k, Fx, Fy, Tq, a.dt, a.gwx, a.gwy, a.dx[0], a.half_w, a.half_h, a.rho_fluid);
+}
+
+// Instances with body lists of their own, one block per instance: the same
+// step as k_integrate_bodies with a single instance's forces.
+__global__ void k_integrate_own_bodies(BodyLists a) {
+  const BodyList& L = a.list[blockIdx.x];
+  for (int k = threadIdx.x; k < L.n; k += blockDim.x) {
+    const efl_body bd = L.bodies[k];
+    if (!bd.dynamic) continue;
+    float Fx = 0.0f, Fy = 0.0f, Tq = 0.0f;
+    uint32_t id = L.obs[k].entity_id;
+    if (id < (uint32_t)EFL_MAX_SOLIDS) {
+      float4 f = L.fts[id];
+      float s = 1.0f / (L.dx * L.dx);
+      Fx += f.x * s;
+      Fy += -f.y * s;
+      Tq += f.w * s;
+    }
+    integrate_body(L.obs, bd, L.state, k, Fx, Fy, Tq, a.dt, L.gwx, L.gwy, L.dx, L.half_w, L.half_h, L.rho_fluid);
   }
-  if (R > 0.0f && fabsf(w) * R > vmax) w = copysignf(vmax / R, w);
-  float tx = o.transform[12] + vx * a.dt;
-  float ty = o.transform[13] + vy * a.dt;
-  float ang = atan2f(o.transform[1], o.transform[0]) + w * a.dt;
-  float lx = fmaxf(a.half_w - 1.0f - R, 0.0f), ly = fmaxf(a.half_h - 1.0f - R, 0.0f);
-  if (tx > lx) { tx = lx; vx = fminf(vx, 0.0f); }
-  if (tx < -lx) { tx = -lx; vx = fmaxf(vx, 0.0f); }
-  if (ty > ly) { ty = ly; vy = fminf(vy, 0.0f); }
-  if (ty < -ly) { ty = -ly; vy = fmaxf(vy, 0.0f); }
-  float c = cosf(ang), sn = sinf(ang);
-  float T[16] = {c, sn, 0.f, 0.f, -sn, c, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, tx, ty, 0.f, 1.f};
-  float Ti[16] = {c, -sn, 0.f, 0.f, sn, c, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, -(c * tx + sn * ty), -(-sn * tx + c * ty), 0.f, 1.f};
-  for (int q = 0; q < 16; ++q) {
-    a.obs[k].transform[q] = T[q];
-    a.obs[k].inverse_transform[q] = Ti[q];
-  }
-  a.obs[k].linear_velocity[0] = vx;
-  a.obs[k].linear_velocity[1] = vy;
-  a.obs[k].angular_velocity = w;
 }
 
 // ---------------------------------------------------------------------------
@@ -1947,12 +1984,18 @@ void launch_pcg_apply(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_apply, 
 void launch_pcg_axpy(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_pcg_axpy, b, st); }
 void launch_cg_check(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_cg_check, b, st, 0); }
 void launch_prefold(hipStream_t st, const Batch& b, int kind) { EFL_LAUNCH_PER_INST(k_prefold, b, st, kind); }
+void launch_gather_scalars(hipStream_t st, const Batch& b, double* dst) {
+  if (b.n > 0) hipLaunchKernelGGL(k_gather_scalars, dim3(b.n), dim3(64), 0, st, b, dst);
+}
 void launch_cg_tally(hipStream_t st, const Batch& b) {
   if (b.n > 0) hipLaunchKernelGGL(k_cg_tally, dim3(b.n), dim3(64), 0, st, b);
 }
 void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_mask_p, b, st); }
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
   if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);
+}
+void launch_integrate_own_bodies(hipStream_t st, const BodyLists& a) {
+  if (a.n > 0) hipLaunchKernelGGL(k_integrate_own_bodies, dim3(a.n), dim3(64), 0, st, a);
 }
 void launch_fluid_diagnostics(hipStream_t st, const FluidDiagArgs& a) {
   if (a.W >= 16 && a.H >= 16) hipLaunchKernelGGL(k_fluid_diag_blocks, dim3(a.W / 16, a.H / 16), dim3(256), 0, st, a);

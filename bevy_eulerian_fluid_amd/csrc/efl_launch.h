@@ -63,9 +63,12 @@ void launch_cg_check(hipStream_t st, const Batch& b);
 void launch_prefold(hipStream_t st, const Batch& b, int kind);
 void launch_mask_p(hipStream_t st, const Batch& b);
 void launch_cg_tally(hipStream_t st, const Batch& b);  // one block per instance
+// dst[16 k + j] = scal[j] of b.inst[k], j < 16 (one block per instance)
+void launch_gather_scalars(hipStream_t st, const Batch& b, double* dst);
 void launch_max_divergence(hipStream_t st, const Batch& b);
 void launch_fluid_diagnostics(hipStream_t st, const FluidDiagArgs& a);
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a);
+void launch_integrate_own_bodies(hipStream_t st, const BodyLists& a);  // one block per list
 
 }  // namespace efl
 
