@@ -632,7 +632,35 @@ void up_pair(hipStream_t st, const IV& v, int l, int pcg, bool symmetric, bool r
 
 // v_cycle, multi_grid.rs:390-475 (symmetric: post-smoothing black-red and
 // the coarsest level also run back in reverse colour order, see the oracle)
-void v_cycle(hipStream_t st, const IV& v_in, int l, int pcg, bool symmetric) {
+// The bottom solves of one V-cycle over a batch.  Instances reach their
+// bottom level at different depths; each one's bottom solve is queued when
+// the cycle passes its level and all of them run in one k_bottom launch
+// (one workgroup per instance, each at its own level) just before the first
+// up launch, which is the earliest any of them is needed.  No instance's
+// data is touched between its queueing and the launch, so the results are
+// those of one launch per level.
+struct BottomQueue {
+  Batch b;
+  int cells = 0;
+  BottomQueue() { std::memset(&b, 0, sizeof(b)); }
+};
+void queue_bottom(BottomQueue& q, efl_instance* i, int l) {
+  const int n = q.b.n++;
+  q.b.inst[n] = i->d;
+  q.b.level[n] = l;
+  q.b.sel[n] = i->xsel[l] | (i->xzero[l] << 2);
+  i->xzero[l] = 0;
+  q.b.ex[n] = (int)i->s.projection.mg_pre_iterations;
+  q.b.ey[n] = (int)(i->s.projection.mg_post_iterations & 0xffff) | (int)(i->s.projection.mg_coarsest_iterations << 16);
+  int c = 0;
+  for (int k = l; k < i->h.nlev; ++k) c += i->h.lev[k].w * i->h.lev[k].h;
+  q.cells = std::max(q.cells, c);
+}
+void flush_bottoms(hipStream_t st, BottomQueue& q, int pcg, bool symmetric) {
+  if (q.b.n > 0) efl::launch_bottom(st, q.b, pcg, symmetric ? 1 : 0, q.cells);
+  q = BottomQueue();
+}
+void v_cycle_rec(hipStream_t st, const IV& v_in, int l, int pcg, bool symmetric, BottomQueue& q) {
   if (v_in.empty()) return;
   IV v = v_in;
   if (l >= 1) {  // instances whose remaining cycle is one fused launch
@@ -649,32 +677,15 @@ void v_cycle(hipStream_t st, const IV& v_in, int l, int pcg, bool symmetric) {
     int b = bottom_level(i);
     return (b < 0 || l < b) && l < i->h.nlev - 1;
   });
-  if (!bot.empty()) {
-    Batch b;
-    std::memset(&b, 0, sizeof(b));
-    int cells = 0;
-    for (auto* i : bot) {
-      int n = b.n++;
-      b.inst[n] = i->d;
-      b.level[n] = l;
-      b.sel[n] = i->xsel[l] | (i->xzero[l] << 2);
-      i->xzero[l] = 0;
-      b.ex[n] = (int)i->s.projection.mg_pre_iterations;
-      b.ey[n] = (int)(i->s.projection.mg_post_iterations & 0xffff) |
-                (int)(i->s.projection.mg_coarsest_iterations << 16);
-      int c = 0;
-      for (int q = l; q < i->h.nlev; ++q) c += i->h.lev[q].w * i->h.lev[q].h;
-      cells = std::max(cells, c);
-    }
-    efl::launch_bottom(st, b, pcg, symmetric ? 1 : 0, cells);
-  }
+  for (auto* i : bot) queue_bottom(q, i, l);
   if (!coarse.empty()) {
     gs_sweeps(st, coarse, l, cnt_coarse, 1.0f, pcg, false);
     if (symmetric) gs_sweeps(st, coarse, l, cnt_coarse, 1.0f, pcg, true);
   }
   if (!rest.empty()) {
     gs_sweeps(st, rest, l, cnt_pre, 1.0f, pcg, false, EX_LEVEL, DOWN);
-    v_cycle(st, rest, l + 1, pcg, symmetric);
+    v_cycle_rec(st, rest, l + 1, pcg, symmetric, q);
+    flush_bottoms(st, q, pcg, symmetric);  // every bottom below this level is queued by now
     // level l's up: with level l + 1's (up_pairs), on its own, or in level
     // l - 1's launch
     IV pair = filter(rest, [l](efl_instance* i) { return up_pairs(i, l); });
@@ -682,7 +693,13 @@ void v_cycle(hipStream_t st, const IV& v_in, int l, int pcg, bool symmetric) {
     if (!pair.empty()) up_pair(st, pair, l, pcg, symmetric, false, nullptr);
     if (!own.empty()) gs_sweeps(st, own, l, cnt_post, 1.0f, pcg, symmetric, EX_LEVEL, UP);
   }
-  if (l == 0) mg_settle(st, v, pcg);
+}
+void v_cycle(hipStream_t st, const IV& v_in, int l, int pcg, bool symmetric) {
+  if (v_in.empty()) return;
+  BottomQueue q;
+  v_cycle_rec(st, v_in, l, pcg, symmetric, q);
+  flush_bottoms(st, q, pcg, symmetric);
+  if (l == 0) mg_settle(st, v_in, pcg);
 }
 
 // z = M r (one symmetric V-cycle from z = 0), z masked to the pressure
