@@ -401,7 +401,9 @@ Batch mg_batch(const IV& v, Extent ext, int lvl, bool with_coarse, int pcg) {
     if (pcg && !with_coarse && i->fax == lvl + 1) {  // a down launch that carries the CG iteration
       sel |= efl::SEL_AXPY;
       if (lvl == 0) i->rpar ^= 1;  // r is written to the other buffer
-      i->fax = lvl == 0 ? 2 : 0;
+      // the stopping test in the level-1 launch; an instance with
+      // pre-folded partials takes k_cg_check after this launch instead
+      i->fax = lvl == 0 && !i->h.prefold ? 2 : 0;
     }
     b.sel[n] = sel;
   }
@@ -713,9 +715,13 @@ bool prefolds(const efl_instance* i) { return i->h.prefold != 0; }
 // the fused CG update (cg_iteration): level 0 pre-smooths in one 2-sweep
 // launch and level 1 is a tiled down level too (not the bottom or the
 // coarsest), post-smoothing in one launch (so r . z comes from k_mg_up)
+// (with pre-folded partials too: the launch then reads gamma and delta from
+// k_prefold's scalars, and one k_cg_check block after it runs the stopping
+// test, the same folds in the same order as the inline ones)
 bool fused_axpy(efl_instance* i) {
   const int b = bottom_level(i);
-  return check_in_smoother(i) && cnt_pre(i) == 2 && cnt_post(i) <= 3 && (b > 1 || (b < 0 && i->h.nlev > 2));
+  return (check_in_smoother(i) || (i->h.prefold && b != 0 && i->h.nlev > 1)) && cnt_pre(i) == 2 &&
+         cnt_post(i) <= 3 && (b > 1 || (b < 0 && i->h.nlev > 2));
 }
 void reset_fused(const IV& v) {  // r is in pcg_r after k_residual_norm(pcg_init)
   for (auto* i : v) i->rpar = i->fax = 0;
@@ -745,7 +751,11 @@ void precondition(hipStream_t st, const IV& v, bool cg_check = false) {
       gs_sweeps(st, coarse, 0, cnt_coarse, 1.0f, 1, true);
     }
     if (!rest.empty()) {
+      // instances whose level-0 launch carries the CG iteration with
+      // pre-folded partials: their stopping test follows that launch
+      IV pfc = filter(rest, [](efl_instance* i) { return i->fax == 1 && i->h.prefold; });
       gs_sweeps(st, rest, 0, cnt_pre, 1.0f, 1, false, EX_LEVEL, DOWN, false, nullptr, nullptr, cg_check);
+      if (!pfc.empty()) efl::launch_cg_check(st, cg_batch(pfc));
       v_cycle(st, rest, 1, 1, true);  // leaves level 1's up to level 0 where they pair
       IV pair = filter(rest, [](efl_instance* i) { return up_pairs(i, 0); });
       IV own = filter(rest, [](efl_instance* i) { return !up_pairs(i, 0); });
@@ -905,7 +915,9 @@ void cg_iteration(hipStream_t st, const IV& cg, bool fuse) {
   // the others run the whole iteration (p, A p, x, r) in the level-0 launch
   // of the preconditioner
   for (auto* i : fx) i->fax = 1;
-  IV sep = filter(cg, [](efl_instance* i) { return !check_in_smoother(i); });
+  IV sep = filter(cg, [&](efl_instance* i) {
+    return !check_in_smoother(i) && std::find(fx.begin(), fx.end(), i) == fx.end();
+  });
   if (!sep.empty()) efl::launch_cg_check(st, cg_batch(sep));
   precondition(st, cg, true);  // the stopping test, z = M r, r . z
 }

@@ -1502,14 +1502,11 @@ __global__ __launch_bounds__(NT) void k_prefold(Batch b, int kind) {
   const DevInst* I = b.inst[slot];
   if (!I->prefold || cg_done(I)) return;
   if (kind == 0) {
+    // r . z and the next component: z . q_prev of the flexible
+    // preconditioner, or z . A z of the multigrid one (the fused CG
+    // iteration's delta; unused by an unfused solve)
     double t[2] = {0.0, 0.0};
-    if (I->cg_flex) {
-      fold_partials<2>(I, RED_RZ, (int)I->scal[SC_NRZ], false, t);
-    } else {
-      double u[1];
-      fold_partials<1>(I, RED_RZ, (int)I->scal[SC_NRZ], false, u);
-      t[0] = u[0];
-    }
+    fold_partials<2>(I, RED_RZ, (int)I->scal[SC_NRZ], false, t);
     if (threadIdx.x == 0) {
       I->scal[SC_RZF] = t[0];
       I->scal[SC_RZF + 1] = t[1];

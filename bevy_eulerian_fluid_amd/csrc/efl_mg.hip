@@ -37,6 +37,7 @@
 // (after a restriction, or z at the start of the preconditioner): then it is
 // not read at all.
 #include <hip/hip_runtime.h>
+#include <cstring>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -605,7 +606,12 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
   if (axpy) {
     const int tile = s.blk - b.start[s.slot];
     double gd[2];  // gamma = r . z, delta = z . A z
-    fold_partials<2>(I, RED_RZ, CI->cg_tiles, false, gd);
+    if (CI->prefold) {  // folded once by k_prefold (the same fold)
+      gd[0] = sld(scal + SC_RZF);
+      gd[1] = sld(scal + SC_RZF + 1);
+    } else {
+      fold_partials<2>(I, RED_RZ, CI->cg_tiles, false, gd);
+    }
     const double beta = iter > 1 ? gd[0] / rz_prev : 0.0;
     const double den = iter > 1 ? gd[1] - beta * gd[0] / al_prev : gd[1];  // p . A p
     if (!(den > 0.0)) {  // breakdown (e.g. a non-finite state): stop before the update
@@ -681,7 +687,9 @@ __global__ __launch_bounds__(MNT) void k_mg_down(Batch b, int pcg, int first, fl
       }
     }
   }
-  if (fax ? s.lvl == 1 : (cg_check && !CI->prefold)) {  // the CG stopping test of the iteration that just ran
+  // the CG stopping test of the iteration that just ran (pre-folded
+  // instances: k_cg_check)
+  if (!CI->prefold && (fax ? s.lvl == 1 : cg_check != 0)) {
     double rr[1];  // one partial per 64 x 16 tile (k_pcg_axpy's, or this kernel's at level 0)
     fold_partials<1>(I, RED_RR, CI->cg_tiles, false, rr);
     if (cg_stop(I, rr[0], s.blk == b.start[s.slot] && threadIdx.x == 0)) return;
@@ -1580,6 +1588,40 @@ void launch_mg_setup(hipStream_t st, const Batch& b) {
 
 // nsweeps 0..3 pre-smoothing sweeps (+ residual/restriction)
 // with cg_check, the launch first runs the CG stopping test (every block)
+// The slots of b whose sel has `bit` (with) or lacks it, their tile ranges
+// renumbered from 0.
+Batch batch_subset(const Batch& b, int bit, bool with) {
+  Batch o;
+  std::memset(&o, 0, sizeof(o));
+  int start = 0;
+  for (int n = 0; n < b.n; ++n) {
+    if (((b.sel[n] & bit) != 0) != with) continue;
+    const int k = o.n++;
+    o.start[k] = start;
+    o.tiles_x[k] = b.tiles_x[n];
+    o.ex[k] = b.ex[n];
+    o.ey[k] = b.ey[n];
+    o.level[k] = b.level[n];
+    o.sel[k] = b.sel[n];
+    o.inst[k] = b.inst[n];
+    start += b.start[n + 1] - b.start[n];
+  }
+  o.start[o.n] = start;
+  o.nblocks = start;
+  return o;
+}
+// A 64 x 16 launch that mixes levels with lazy stencil loads (SEL_LAZY,
+// 2^20 cells or more) and levels without runs the plain instantiation for
+// all of them, so the large levels read every stencil.  When the lazy ones
+// hold at least a quarter of the tiles (a 4096^2 instance batched with
+// 512^2 ones) the launch is worth splitting in two.
+bool split_lazy(const Batch& b) {
+  int lazy = 0;
+  for (int n = 0; n < b.n; ++n)
+    if (b.sel[n] & SEL_LAZY) lazy += b.start[n + 1] - b.start[n];
+  return lazy > 0 && lazy < b.start[b.n] && 4 * lazy >= b.start[b.n];
+}
+
 void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_,
                     int cg_check, int th) {
   bool axpy = false;  // the fused CG update needs the AX instantiation (2 sweeps only)
@@ -1594,6 +1636,11 @@ void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int fi
   }
   int nb = b.start[b.n];
   if (nb <= 0) return;
+  if (th == MT_H && split_lazy(b)) {
+    launch_mg_down(st, batch_subset(b, SEL_LAZY, true), pcg, nsweeps, first, weight, restrict_, cg_check, th);
+    launch_mg_down(st, batch_subset(b, SEL_LAZY, false), pcg, nsweeps, first, weight, restrict_, cg_check, th);
+    return;
+  }
   bool lz = true;  // lazy stencil loads when every level of the launch streams from HBM
   for (int n = 0; n < b.n; ++n) lz = lz && (b.sel[n] & SEL_LAZY);
   if (th == MT_H8) {
@@ -1639,6 +1686,11 @@ void launch_mg_up(hipStream_t st, const Batch& b, int pcg, int nsweeps, int firs
     }
     return;
   }
+  if (split_lazy(b)) {
+    launch_mg_up(st, batch_subset(b, SEL_LAZY, true), pcg, nsweeps, first, weight, rz, th);
+    launch_mg_up(st, batch_subset(b, SEL_LAZY, false), pcg, nsweeps, first, weight, rz, th);
+    return;
+  }
   bool lz = true;  // lazy stencil loads when every level of the launch streams from HBM
   for (int n = 0; n < b.n; ++n) lz = lz && (b.sel[n] & SEL_LAZY);
 #define MG_UP16(NS, HALO)                                                                 \
@@ -1677,6 +1729,11 @@ void launch_mg_up2(hipStream_t st, const Batch& b, int pcg, int first, float wei
   }
   int nb = b.start[b.n];
   if (nb <= 0) return;
+  if (th == MT_H && split_lazy(b)) {
+    launch_mg_up2(st, batch_subset(b, SEL_LAZY, true), pcg, first, weight, rz, th);
+    launch_mg_up2(st, batch_subset(b, SEL_LAZY, false), pcg, first, weight, rz, th);
+    return;
+  }
   bool lz = true;
   for (int n = 0; n < b.n; ++n) lz = lz && (b.sel[n] & SEL_LAZY);
   if (th == MT_H8) {
