@@ -1371,10 +1371,8 @@ void phase_pre(hipStream_t st, const IV& v) {
   Batch center = make_batch(v, EX_CENTER), bx = make_batch(v, EX_XMAJOR), by = make_batch(v, EX_YMAJOR);
   efl::launch_update_solid(st, center);
   efl::launch_area_fraction(st, center);
-  efl::launch_advect_u(st, bx);
-  efl::launch_advect_v(st, by);
-  efl::launch_forces_u(st, bx);
-  efl::launch_forces_v(st, by);
+  efl::launch_advect_forces_u(st, bx);  // K6 + K8
+  efl::launch_advect_forces_v(st, by);  // K7 + K9
   efl::launch_divergence(st, center);
 }
 // velocity update (render_node.rs:331-350)

@@ -569,6 +569,43 @@ __global__ __launch_bounds__(NT) void k_forces_v(Batch b) {
   I->v1[id] = I->v1[id] + (net * I->dt) / I->dx;
 }
 
+// K6 + K8 (and K7 + K9) in one launch for the step: the advected face value
+// stays in a register and the forces are applied to it, with the same
+// expressions as k_advect_u / k_forces_u (k_advect_v / k_forces_v).  The two
+// dispatches cover the same faces and the forces read only the air level
+// set and the area fractions at the face, so the results are the same bits
+// as the two launches'; the stage-by-stage entry points keep those.
+template <int COMP>
+__global__ __launch_bounds__(NT) void k_advect_forces(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  CDevInst* I = cdev(s.I);
+  const int W = I->W, H = I->H;
+  float val;
+  size_t id;
+  if (COMP == 0) {
+    float2 q = backtrace(I, (float)s.x + (-0.5f), (float)s.y + 0.0f, I->dt);
+    bool inside = q.x >= 0.0f && q.y >= 0.0f && q.x <= (float)(W + 1) - 1.0f && q.y <= (float)H - 1.0f;
+    id = (size_t)s.y * (W + 1) + s.x;
+    val = inside ? sample_u(I->u0, W + 1, H, q.x, q.y) : I->u0[id];
+  } else {
+    float2 q = backtrace(I, (float)s.x + 0.0f, (float)s.y + (-0.5f), I->dt);
+    bool inside = q.x >= 0.0f && q.y >= 0.0f && q.x <= (float)W - 1.0f && q.y <= (float)(H + 1) - 1.0f;
+    id = (size_t)s.y * W + s.x;
+    val = inside ? sample_v(I->v0, W, H + 1, q.x, q.y) : I->v0[id];
+  }
+  const float4 fa = fractions_at(I->ls0, W, H, s.x, s.y);
+  const float4 fs4 = ldz4(I->afs, W, H, s.x, s.y);
+  const float fo = COMP == 0 ? fa.x : fa.z, fs = COMP == 0 ? fs4.x : fs4.z;
+  if (fo == 1.0f || fs == 0.0f) {
+    val = 0.0f;
+  } else {
+    const float net = local_force(I, COMP, s.x, s.y);
+    val = val + (net * I->dt) / I->dx;
+  }
+  (COMP == 0 ? I->u1 : I->v1)[id] = val;
+}
+
 // ---------------------------------------------------------------------------
 // K10 divergence (divergence.wgsl:16-42)
 // ---------------------------------------------------------------------------
@@ -1988,6 +2025,8 @@ void launch_cg_tally(hipStream_t st, const Batch& b) {
   if (b.n > 0) hipLaunchKernelGGL(k_cg_tally, dim3(b.n), dim3(64), 0, st, b);
 }
 void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_mask_p, b, st); }
+void launch_advect_forces_u(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advect_forces<0>, b, st); }
+void launch_advect_forces_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advect_forces<1>, b, st); }
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
   if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);
 }

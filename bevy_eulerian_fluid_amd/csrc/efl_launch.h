@@ -67,6 +67,9 @@ void launch_cg_tally(hipStream_t st, const Batch& b);  // one block per instance
 void launch_gather_scalars(hipStream_t st, const Batch& b, double* dst);
 void launch_max_divergence(hipStream_t st, const Batch& b);
 void launch_fluid_diagnostics(hipStream_t st, const FluidDiagArgs& a);
+// advection then forces of one velocity component, one launch (the step's path)
+void launch_advect_forces_u(hipStream_t st, const Batch& b);
+void launch_advect_forces_v(hipStream_t st, const Batch& b);
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a);
 void launch_integrate_own_bodies(hipStream_t st, const BodyLists& a);  // one block per list
 
