@@ -1279,31 +1279,40 @@ int stage_projection(efl_ctx* c, const IV& v) {
 // render_node.rs:540-608: the validity masks, then 10 passes per component.
 // u and v are independent: one launch covers both (Batch::level = the
 // component), two when the instance list fills more than half a batch.
-void stage_extrapolate(hipStream_t st, const IV& v) {
-  const int tw = efl::extrapolate_tile_w(), th = efl::extrapolate_tile_h();
-  Batch bu = make_batch(v, EX_XMAJOR, 0, nullptr, tw, th), bv = make_batch(v, EX_YMAJOR, 0, nullptr, tw, th);
+// The u-face batch and the v-face batch of v (Batch::level 0 and 1).  When
+// both fit one batch, *uv gets them appended (u slots first) and the result
+// is true; kernels that take both components then run in one launch.
+bool uv_batches(const IV& v, int tw, int th, Batch& bu, Batch& bv, Batch& uv) {
+  bu = make_batch(v, EX_XMAJOR, 0, nullptr, tw, th);
+  bv = make_batch(v, EX_YMAJOR, 0, nullptr, tw, th);
   for (int n = 0; n < bv.n; ++n) bv.level[n] = 1;
-  if (bu.n + bv.n > efl::kMaxBatch) {
-    efl::launch_extrapolate_fused(st, bu);
-    efl::launch_extrapolate_fused(st, bv);
-    return;
-  }
-  Batch b = bu;  // append the v slots
+  if (bu.n + bv.n > efl::kMaxBatch) return false;
+  uv = bu;  // append the v slots
   const int base = bu.start[bu.n];
   for (int n = 0; n < bv.n; ++n) {
     const int k = bu.n + n;
-    b.start[k] = base + bv.start[n];
-    b.tiles_x[k] = bv.tiles_x[n];
-    b.ex[k] = bv.ex[n];
-    b.ey[k] = bv.ey[n];
-    b.level[k] = 1;
-    b.sel[k] = 0;
-    b.inst[k] = bv.inst[n];
+    uv.start[k] = base + bv.start[n];
+    uv.tiles_x[k] = bv.tiles_x[n];
+    uv.ex[k] = bv.ex[n];
+    uv.ey[k] = bv.ey[n];
+    uv.level[k] = 1;
+    uv.sel[k] = 0;
+    uv.inst[k] = bv.inst[n];
   }
-  b.n = bu.n + bv.n;
-  b.start[b.n] = base + bv.start[bv.n];
-  b.nblocks = b.start[b.n];
-  efl::launch_extrapolate_fused(st, b);
+  uv.n = bu.n + bv.n;
+  uv.start[uv.n] = base + bv.start[bv.n];
+  uv.nblocks = uv.start[uv.n];
+  return true;
+}
+
+void stage_extrapolate(hipStream_t st, const IV& v) {
+  Batch bu, bv, b;
+  if (uv_batches(v, efl::extrapolate_tile_w(), efl::extrapolate_tile_h(), bu, bv, b)) {
+    efl::launch_extrapolate_fused(st, b);
+  } else {
+    efl::launch_extrapolate_fused(st, bu);
+    efl::launch_extrapolate_fused(st, bv);
+  }
 }
 
 void stage_fluid_to_solid(hipStream_t st, const IV& v) {  // render_node.rs:610-641
@@ -1368,17 +1377,27 @@ int run_init(efl_ctx* c, const IV& v) {
 // ---- the step (render_node.rs:278-466) ---------------------------------------
 // solids .. divergence (render_node.rs:278-330)
 void phase_pre(hipStream_t st, const IV& v) {
-  Batch center = make_batch(v, EX_CENTER), bx = make_batch(v, EX_XMAJOR), by = make_batch(v, EX_YMAJOR);
+  Batch center = make_batch(v, EX_CENTER), bx, by, uv;
+  const bool one = uv_batches(v, 0, 0, bx, by, uv);
   efl::launch_update_solid(st, center);
   efl::launch_area_fraction(st, center);
-  efl::launch_advect_forces_u(st, bx);  // K6 + K8
-  efl::launch_advect_forces_v(st, by);  // K7 + K9
+  if (one) {
+    efl::launch_advect_forces_uv(st, uv);  // K6 + K8, K7 + K9
+  } else {
+    efl::launch_advect_forces_u(st, bx);  // K6 + K8
+    efl::launch_advect_forces_v(st, by);  // K7 + K9
+  }
   efl::launch_divergence(st, center);
 }
 // velocity update (render_node.rs:331-350)
 void phase_velocity(hipStream_t st, const IV& v) {
-  efl::launch_solve_u(st, make_batch(v, EX_XMAJOR));
-  efl::launch_solve_v(st, make_batch(v, EX_YMAJOR));
+  Batch bx, by, uv;
+  if (uv_batches(v, 0, 0, bx, by, uv)) {
+    efl::launch_solve_uv(st, uv);
+  } else {
+    efl::launch_solve_u(st, bx);
+    efl::launch_solve_v(st, by);
+  }
 }
 // extrapolation .. fluid -> solid (render_node.rs:351-466)
 void phase_post(hipStream_t st, const IV& v) {

@@ -576,9 +576,7 @@ __global__ __launch_bounds__(NT) void k_forces_v(Batch b) {
 // set and the area fractions at the face, so the results are the same bits
 // as the two launches'; the stage-by-stage entry points keep those.
 template <int COMP>
-__global__ __launch_bounds__(NT) void k_advect_forces(Batch b) {
-  Site s = locate(b);
-  if (!s.ok) return;
+__device__ __forceinline__ void advect_forces_at(const Site& s) {
   CDevInst* I = cdev(s.I);
   const int W = I->W, H = I->H;
   float val;
@@ -604,6 +602,18 @@ __global__ __launch_bounds__(NT) void k_advect_forces(Batch b) {
     val = val + (net * I->dt) / I->dx;
   }
   (COMP == 0 ? I->u1 : I->v1)[id] = val;
+}
+template <int COMP>
+__global__ __launch_bounds__(NT) void k_advect_forces(Batch b) {
+  Site s = locate(b);
+  if (s.ok) advect_forces_at<COMP>(s);
+}
+// both components in one launch (Batch::level 0: u faces, 1: v faces)
+__global__ __launch_bounds__(NT) void k_advect_forces_uv(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  if (s.lvl == 0) advect_forces_at<0>(s);
+  else advect_forces_at<1>(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -786,9 +796,7 @@ __global__ __launch_bounds__(NT) void k_jacobi(Batch b, int reverse) {
 // ---------------------------------------------------------------------------
 // K18 / K19 solve_velocity_u, solve_velocity_v
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_solve_u(Batch b) {
-  Site s = locate(b);
-  if (!s.ok) return;
+__device__ __forceinline__ void solve_u_at(const Site& s) {
   CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   size_t id = (size_t)y * (W + 1) + x;
@@ -801,9 +809,7 @@ __global__ __launch_bounds__(NT) void k_solve_u(Batch b) {
   else if (lm >= 0.0f && lp >= 0.0f) { I->u0[id] = 0.0f; return; }
   I->u0[id] = I->u1[id] - factor * (pij - pim);
 }
-__global__ __launch_bounds__(NT) void k_solve_v(Batch b) {
-  Site s = locate(b);
-  if (!s.ok) return;
+__device__ __forceinline__ void solve_v_at(const Site& s) {
   CDevInst* I = cdev(s.I);
   int W = I->W, H = I->H, x = s.x, y = s.y;
   size_t id = (size_t)y * W + x;
@@ -814,6 +820,23 @@ __global__ __launch_bounds__(NT) void k_solve_v(Batch b) {
   if (lm >= 0.0f && lp < 0.0f) pjm = lm / lp * pij;
   else if (lm < 0.0f && lp >= 0.0f) pij = lp / lm * pjm;
   I->v0[id] = I->v1[id] - factor * (pij - pjm);
+}
+
+__global__ __launch_bounds__(NT) void k_solve_u(Batch b) {
+  Site s = locate(b);
+  if (s.ok) solve_u_at(s);
+}
+__global__ __launch_bounds__(NT) void k_solve_v(Batch b) {
+  Site s = locate(b);
+  if (s.ok) solve_v_at(s);
+}
+// both components in one launch: slots with Batch::level 0 are u faces,
+// level 1 v faces (uniform per block)
+__global__ __launch_bounds__(NT) void k_solve_uv(Batch b) {
+  Site s = locate(b);
+  if (!s.ok) return;
+  if (s.lvl == 0) solve_u_at(s);
+  else solve_v_at(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -2027,6 +2050,8 @@ void launch_cg_tally(hipStream_t st, const Batch& b) {
 void launch_mask_p(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_mask_p, b, st); }
 void launch_advect_forces_u(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advect_forces<0>, b, st); }
 void launch_advect_forces_v(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advect_forces<1>, b, st); }
+void launch_advect_forces_uv(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advect_forces_uv, b, st); }
+void launch_solve_uv(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_solve_uv, b, st); }
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a) {
   if (a.n > 0) hipLaunchKernelGGL(k_integrate_bodies, dim3((a.n + 63) / 64), dim3(64), 0, st, a);
 }

@@ -70,6 +70,9 @@ void launch_fluid_diagnostics(hipStream_t st, const FluidDiagArgs& a);
 // advection then forces of one velocity component, one launch (the step's path)
 void launch_advect_forces_u(hipStream_t st, const Batch& b);
 void launch_advect_forces_v(hipStream_t st, const Batch& b);
+// u and v faces in one launch: Batch::level 0 slots are u faces, level 1 v faces
+void launch_advect_forces_uv(hipStream_t st, const Batch& b);
+void launch_solve_uv(hipStream_t st, const Batch& b);
 void launch_integrate_bodies(hipStream_t st, const BodyStep& a);
 void launch_integrate_own_bodies(hipStream_t st, const BodyLists& a);  // one block per list
 
