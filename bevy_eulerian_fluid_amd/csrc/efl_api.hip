@@ -456,19 +456,23 @@ void gs_sweeps(hipStream_t st, const IV& v, int lvl, uint32_t (*count)(efl_insta
       }
       return;
     }
+  // sweeps per launch: 3, or 4 for plain sweeps on 64 x 8 tiles (a fixed
+  // Gauss-Seidel count on a small grid is launch-bound: 25 launches for 100
+  // sweeps instead of 34); the halo grows by two cells per sweep
+  const uint32_t G = (mode == PLAIN && th == 8) ? 4 : 3;
   auto chunks = [&](efl_instance* i) {  // number of launches for this instance
     uint32_t c = count(i);
-    uint32_t n = (c + 2) / 3;
+    uint32_t n = (c + G - 1) / G;
     return mode == PLAIN ? n : std::max<uint32_t>(n, 1);
   };
   uint32_t K = 0;
   for (auto* i : v) K = std::max(K, chunks(i));
   for (uint32_t k = 0; k < K; ++k) {
-    for (uint32_t n = 0; n <= 3; ++n) {
+    for (uint32_t n = 0; n <= G; ++n) {
       IV part = filter(v, [&](efl_instance* i) {
         if (k >= chunks(i)) return false;
         uint32_t c = count(i);
-        uint32_t left = c > 3 * k ? std::min<uint32_t>(3, c - 3 * k) : 0;
+        uint32_t left = c > G * k ? std::min<uint32_t>(G, c - G * k) : 0;
         return left == n;
       });
       if (part.empty()) continue;

@@ -1586,7 +1586,7 @@ void launch_mg_setup(hipStream_t st, const Batch& b) {
   hipLaunchKernelGGL(k_mg_setup, dim3(nb), dim3(SU_W * SU_H), 0, st, b);
 }
 
-// nsweeps 0..3 pre-smoothing sweeps (+ residual/restriction)
+// nsweeps 0..3 pre-smoothing sweeps (+ residual/restriction); 4 on 64 x 8 tiles
 // with cg_check, the launch first runs the CG stopping test (every block)
 // The slots of b whose sel has `bit` (with) or lacks it, their tile ranges
 // renumbered from 0.
@@ -1648,7 +1648,8 @@ void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int fi
       case 0: MG_LAUNCH((k_mg_down<0, 1, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
       case 1: MG_LAUNCH((k_mg_down<1, 3, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
       case 2: MG_LAUNCH((k_mg_down<2, 5, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
-      default: MG_LAUNCH((k_mg_down<3, 7, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+      case 3: MG_LAUNCH((k_mg_down<3, 7, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
+      default: MG_LAUNCH((k_mg_down<4, 9, false, MT_H8>), nb, st, b, pcg, first, weight, restrict_, cg_check); break;
     }
     return;
   }
