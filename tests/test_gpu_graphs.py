@@ -135,6 +135,41 @@ def test_batched_step_matches_single_steps():
             c.close()
 
 
+def test_mixed_batch_matches_single_steps():
+    """A batch holding a 1024^2 instance (lazy stencils on level 0) beside four
+    smaller ones whose V-cycles bottom out at different levels and whose
+    solves end at different iterations, stepped in one efl_step: the level-0
+    launches split into a lazy and a plain launch, every bottom solve runs in
+    one launch, and each instance is queued only its own iterations.  Every
+    field bit for bit what stepping each instance alone gives."""
+    _say("mixed batch (1024^2 + 4 small) vs single instances")
+    p = scenes.projection("mgpcg", max_cycles=60, rel_tol=1e-5)
+    scs = [scenes.batch_instance(scenes.SplitMix64(scenes.SEED_BASE + 3, 7), proj=p, size=1024)]
+    scs += [scenes.batch_instance(scenes.SplitMix64(scenes.SEED_BASE + 3, k), proj=p) for k in range(4)]
+    cb = native.Context(0)
+    cs = [native.Context(0) for _ in scs]
+    try:
+        ib, isg = [], []
+        for sc, c in zip(scs, cs):
+            for ctx, lst in ((cb, ib), (c, isg)):
+                i = _setup(ctx, sc)
+                i.set_obstacles(sc["obstacles"], scenes.bodies_for(sc["obstacles"], sc["densities"]))
+                lst.append(i)
+        for k in range(4):
+            cb.step(ib, DT)
+            for i in isg:
+                i.step(DT)
+            its = [i.stats().cycles for i in isg]
+            for a, b in zip(ib, isg):
+                _same(a, b, f"step {k}")
+                assert a.stats().cycles == b.stats().cycles
+        _say(f"  last step's iterations per instance: {its}")
+    finally:
+        cb.close()
+        for c in cs:
+            c.close()
+
+
 @pytest.mark.parametrize("proj", ["mgpcg", "multigrid"])
 @pytest.mark.parametrize("size", [(200, 136), (1024, 1024)], ids=lambda s: f"{s[0]}x{s[1]}")
 def test_up_pairs_match_separate_launches(monkeypatch, size, proj):
