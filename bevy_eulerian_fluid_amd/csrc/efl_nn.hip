@@ -10,12 +10,14 @@
 //
 // A 3x3 convolution is an implicit GEMM: M = pixels, N = output channels,
 // K = 9 taps x input channels.  One wavefront owns 64 consecutive pixels of a
-// row (four 16-pixel M blocks) and every output channel; per tap and 16-channel
-// K chunk it loads one 4-channel fragment per lane per M block (8 bytes,
-// zero outside the image) and one weight fragment per N block, and issues
-// v_mfma_f32_16x16x16_bf16 (FP32 accumulation).  Concatenated inputs
+// row (four 16-pixel M blocks) and every output channel; per 32-wide K step
+// (32 channels of one tap, or 16 channels of two taps when the input has 16)
+// it loads one 8-channel fragment per lane per M block (16 bytes, zero
+// outside the image) and one weight fragment per N block, and issues gfx950's
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulation).  Concatenated inputs
 // (decoder: upsampled coarse tensor + skip tensor) are read in place: K
-// chunks below CA come from A at (y/2, x/2), the rest from B.
+// chunks below CA come from A at (y/2, x/2), the rest from B.  The weight
+// gradient (K = pixels) runs v_mfma_f32_16x16x16_bf16.
 #include "efl_nn.h"
 
 #include <algorithm>
@@ -32,6 +34,15 @@ namespace nn {
 
 typedef short bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
 typedef float f32x4 __attribute__((__vector_size__(4 * sizeof(float))));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// gfx950's v_mfma_f32_16x16x32_bf16: D (16 x 16) += A (16 x 32) B (32 x 16);
+// lane l supplies A[l % 16][8 (l / 16) .. + 7] and B[8 (l / 16) .. + 7][l % 16]
+// (8 BF16 each, one 16-byte value) and holds D[4 (l / 16) + i][l % 16].
+__device__ __forceinline__ f32x4 mfma32(uint4 a, uint4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0,
+                                                 0);
+}
 
 // ---------------------------------------------------------------- network
 
@@ -131,33 +142,40 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs g) {
   for (int mb = 0; mb < MBW; ++mb)
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int tap = 0; tap < 9; ++tap) {
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int yy = y + dy;
-    const bool rowok = yy >= 0 && yy < g.H;
-    const int ya = UPA ? (yy >> 1) : yy;
+  // K = 32 per MFMA over (tap, channel): with a multiple of 32 channels a
+  // lane supplies channels c + 8 lq of one tap; with 16, channels 8 (lq & 1)
+  // of tap 2 p + (lq >> 1), the ninth tap beside zeros.  k_conv3x3_lds runs
+  // the same sequence.
+  constexpr int KSTEPS = CIN % 32 == 0 ? 9 * (CIN / 32) : 5;
+  static_assert(CIN % 32 == 0 || CIN == 16, "16 channels or a multiple of 32");
 #pragma unroll 2
-    for (int c0 = 0; c0 < CIN; c0 += 16) {
-      bf16x4 bw[NB];
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    const int tap = CIN % 32 == 0 ? ks / (CIN / 32) : 2 * ks + (lq >> 1);
+    const int c0 = CIN % 32 == 0 ? (ks % (CIN / 32)) * 32 + 8 * lq : 8 * (lq & 1);
+    const bool has = tap < 9;
+    const int tt = has ? tap : 8;
+    const int dy = tt / 3 - 1, dx = tt % 3 - 1;
+    const int yy = y + dy;
+    const bool rowok = has && yy >= 0 && yy < g.H;
+    const int ya = UPA ? (yy >> 1) : yy;
+    uint4 bw[NB];
 #pragma unroll
-      for (int nb = 0; nb < NB; ++nb)
-        bw[nb] = *(const bf16x4*)(g.w + ((size_t)(tap * COUT + nb * 16 + lr) * CIN + c0 + 4 * lq));
+    for (int nb = 0; nb < NB; ++nb)
+      bw[nb] = has ? *(const uint4*)(g.w + ((size_t)(tt * COUT + nb * 16 + lr) * CIN + c0)) : uint4{0u, 0u, 0u, 0u};
 #pragma unroll
-      for (int mb = 0; mb < MBW; ++mb) {
-        const int xx = x0 + mb * 16 + lr + dx;
-        bf16x4 av = {0, 0, 0, 0};
-        if (rowok && xx >= 0 && xx < g.W) {
-          if (c0 < CA) {
-            const int xa = UPA ? (xx >> 1) : xx;
-            av = *(const bf16x4*)(A + ((size_t)ya * Wa + xa) * CA + c0 + 4 * lq);
-          } else {
-            av = *(const bf16x4*)(B + ((size_t)yy * g.W + xx) * CB + (c0 - CA) + 4 * lq);
-          }
+    for (int mb = 0; mb < MBW; ++mb) {
+      const int xx = x0 + mb * 16 + lr + dx;
+      uint4 av = {0u, 0u, 0u, 0u};
+      if (rowok && xx >= 0 && xx < g.W) {
+        if (c0 < CA) {
+          const int xa = UPA ? (xx >> 1) : xx;
+          av = *(const uint4*)(A + ((size_t)ya * Wa + xa) * CA + c0);
+        } else {
+          av = *(const uint4*)(B + ((size_t)yy * g.W + xx) * CB + (c0 - CA));
         }
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[nb], acc[mb][nb], 0, 0, 0);
       }
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = mfma32(av, bw[nb], acc[mb][nb]);
     }
   }
   // lane holds D[pixel 4*lq + i][channel lr] of every (M, N) block
@@ -241,24 +259,29 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs g) {
   for (int mb = 0; mb < MBW; ++mb)
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  // taps unrolled: the weights of the next taps load while these multiply
+  // the K = 32 sequence of k_conv3x3 (the same operands in the same order),
+  // fragments from LDS; unrolled, so the weights of the next steps load
+  // while these multiply
+  constexpr int KSTEPS = CIN % 32 == 0 ? 9 * (CIN / 32) : 5;
+  static_assert(CIN % 32 == 0 || CIN == 16, "16 channels or a multiple of 32");
 #pragma unroll
-  for (int tap = 0; tap < 9; ++tap) {
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    const int tap = CIN % 32 == 0 ? ks / (CIN / 32) : 2 * ks + (lq >> 1);
+    const int c0 = CIN % 32 == 0 ? (ks % (CIN / 32)) * 32 + 8 * lq : 8 * (lq & 1);
+    const bool has = tap < 9;
+    const int tt = has ? tap : 8;
+    const int dy = tt / 3 - 1, dx = tt % 3 - 1;
     const uint16_t* row = sm + (size_t)(wave + 1 + dy) * RW * PS;
+    uint4 bw[NB];
 #pragma unroll
-    for (int c0 = 0; c0 < CIN; c0 += 16) {
-      bf16x4 bw[NB];
+    for (int nb = 0; nb < NB; ++nb)
+      bw[nb] = has ? *(const uint4*)(g.w + ((size_t)(tt * COUT + nb * 16 + lr) * CIN + c0)) : uint4{0u, 0u, 0u, 0u};
 #pragma unroll
-      for (int nb = 0; nb < NB; ++nb)
-        bw[nb] = *(const bf16x4*)(g.w + ((size_t)(tap * COUT + nb * 16 + lr) * CIN + c0 + 4 * lq));
+    for (int mb = 0; mb < MBW; ++mb) {
+      uint4 av = *(const uint4*)(row + (mb * 16 + lr + dx + 1) * PS + c0);
+      if (!has) av = uint4{0u, 0u, 0u, 0u};
 #pragma unroll
-      for (int mb = 0; mb < MBW; ++mb) {
-        const bf16x4 av = *(const bf16x4*)(row + (mb * 16 + lr + dx + 1) * PS + c0 + 4 * lq);
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[nb], acc[mb][nb], 0, 0, 0);
-      }
+      for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = mfma32(av, bw[nb], acc[mb][nb]);
     }
   }
   // lane holds D[pixel 4*lq + i][channel lr] of every (M, N) block
