@@ -1147,19 +1147,32 @@ __device__ void b_tail16(const BotCfg& g, const BotLds& m, int ls) {
 // b_restrict's / b_prolong's expression with the same operands in the same
 // order (a neighbour outside the level is the cell itself), so the results
 // equal the LDS loops' bit for bit.
+// RC: the block's four stencils stay in registers too (the first, largest
+// level: its half-sweeps then read only the neighbours' x from LDS, not 32
+// KB of stencils per half-sweep); otherwise they are read from LDS.
+template <bool RC>
 struct RbLevel {
+  static constexpr bool kRegC = RC;
   float x[4], b[4], dn[4];  // dn: the Gauss-Seidel denominator of the cell's stencil
+  float4 c[RC ? 4 : 1];
   int fl[4];
   bool in[4];
   bool own;
   int bx, by, w, h, off;  // block, level size, offset of the level in the LDS arrays
 };
+template <class LV>
+__device__ __forceinline__ float4 rb_stencil(const LV& L, const BotLds& m, int q, int k) {
+  if constexpr (LV::kRegC) return L.c[k];
+  else return m.c[q];
+}
 
-__device__ __forceinline__ int rb_cell(const RbLevel& L, int k) {
+template <class LV>
+__device__ __forceinline__ int rb_cell(const LV& L, int k) {
   return (2 * L.by + (k >> 1)) * L.w + 2 * L.bx + (k & 1);
 }
 // the block of this thread on a level, x, b and flags from the LDS copy
-__device__ __forceinline__ void rb_load(RbLevel& L, const BotLds& m, int w, int h, int off) {
+template <class LV>
+__device__ __forceinline__ void rb_load(LV& L, const BotLds& m, int w, int h, int off) {
   const int bw = (w + 1) / 2, bh = (h + 1) / 2, t = (int)threadIdx.x;
   L.w = w;
   L.h = h;
@@ -1174,13 +1187,16 @@ __device__ __forceinline__ void rb_load(RbLevel& L, const BotLds& m, int w, int 
     L.x[k] = L.in[k] ? m.x[q] : 0.0f;
     L.b[k] = L.in[k] ? m.b[q] : 0.0f;
     L.fl[k] = L.in[k] ? m.fl[q] : 0;
-    L.dn[k] = b_denominator(m.c[q], L.fl[k]);
+    const float4 cq = m.c[q];
+    if constexpr (LV::kRegC) L.c[k] = cq;
+    L.dn[k] = b_denominator(cq, L.fl[k]);
   }
 }
 // the four neighbours (-x, +x, -y, +y) of cell k: those in the block from
 // registers, the others from LDS (the cell itself when outside the level; a
 // cell outside the level, whose results are dropped, reads around cell 0)
-__device__ __forceinline__ int rb_nbrs(const RbLevel& L, const float* sx, int k, float (&n)[4]) {
+template <class LV>
+__device__ __forceinline__ int rb_nbrs(const LV& L, const float* sx, int k, float (&n)[4]) {
   const int dx = k & 1, dy = k >> 1, gx = L.in[k] ? 2 * L.bx + dx : 0, gy = L.in[k] ? 2 * L.by + dy : 0;
   const int q = L.off + gy * L.w + gx;
   if (dx == 0) {
@@ -1201,13 +1217,13 @@ __device__ __forceinline__ int rb_nbrs(const RbLevel& L, const float* sx, int k,
 }
 // one half-sweep: parity 0 = cells 0 and 3, parity 1 = cells 1 and 2 (a
 // compile-time choice, so that the block's registers are indexed statically)
-template <int G, int P>
-__device__ __forceinline__ void rb_half(RbLevel& L, const BotLds& m, float c0) {
+template <int G, int P, class LV>
+__device__ __forceinline__ void rb_half(LV& L, const BotLds& m, float c0) {
   constexpr int k0 = P ? 1 : 0, k1 = P ? 2 : 3;
   if (L.own) {
     float n0[4], n1[4];
     const int q0 = rb_nbrs(L, m.x, k0, n0), q1 = rb_nbrs(L, m.x, k1, n1);
-    const float4 c0v = m.c[q0], c1v = m.c[q1];
+    const float4 c0v = rb_stencil(L, m, q0, k0), c1v = rb_stencil(L, m, q1, k1);
     const float v0 = gs_upd(L.b[k0], c0v, L.dn[k0], L.fl[k0], L.x[k0], n0[0], n0[1], n0[2], n0[3], 1.0f, c0);
     const float v1 = gs_upd(L.b[k1], c1v, L.dn[k1], L.fl[k1], L.x[k1], n1[0], n1[1], n1[2], n1[3], 1.0f, c0);
     if (L.in[k0]) {
@@ -1223,16 +1239,15 @@ __device__ __forceinline__ void rb_half(RbLevel& L, const BotLds& m, float c0) {
 }
 // residuals of the block's cells restricted into the coarse level at coff
 // (its b, and its x set to 0), as b_restrict
-template <int G>
-__device__ __forceinline__ void rb_restrict(const RbLevel& L, const BotLds& m, int coff, int cw, int ch,
-                                            float factor) {
+template <int G, class LV>
+__device__ __forceinline__ void rb_restrict(const LV& L, const BotLds& m, int coff, int cw, int ch, float factor) {
   if (L.own && L.bx < cw && L.by < ch) {
     float r[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       float n[4];
       const int q = rb_nbrs(L, m.x, k, n);
-      const float v = res_upd(L.b[k], m.c[q], L.fl[k], L.x[k], n[0], n[1], n[2], n[3], factor);
+      const float v = res_upd(L.b[k], rb_stencil(L, m, q, k), L.fl[k], L.x[k], n[0], n[1], n[2], n[3], factor);
       r[k] = L.in[k] ? v : 0.0f;
     }
     const int q = coff + L.by * cw + L.bx;
@@ -1242,8 +1257,8 @@ __device__ __forceinline__ void rb_restrict(const RbLevel& L, const BotLds& m, i
   b_sync<G>();
 }
 // the coarse x added to the block's fluid cells that have a parent, as b_prolong
-template <int G>
-__device__ __forceinline__ void rb_prolong(RbLevel& L, const BotLds& m, int coff, int cw, int ch) {
+template <int G, class LV>
+__device__ __forceinline__ void rb_prolong(LV& L, const BotLds& m, int coff, int cw, int ch) {
   if (L.own && L.bx < cw && L.by < ch) {
     const float cx = m.x[coff + L.by * cw + L.bx];
 #pragma unroll
@@ -1255,15 +1270,15 @@ __device__ __forceinline__ void rb_prolong(RbLevel& L, const BotLds& m, int coff
   }
   b_sync<G>();
 }
-template <int G>
-__device__ __forceinline__ void rb_pre(RbLevel& L, const BotLds& m, int pre, float c0) {
+template <int G, class LV>
+__device__ __forceinline__ void rb_pre(LV& L, const BotLds& m, int pre, float c0) {
   for (int k = 0; k < pre; ++k) {
     rb_half<G, 1>(L, m, c0);
     rb_half<G, 0>(L, m, c0);
   }
 }
-template <int G>
-__device__ __forceinline__ void rb_post(RbLevel& L, const BotLds& m, int post, int symmetric, float c0) {
+template <int G, class LV>
+__device__ __forceinline__ void rb_post(LV& L, const BotLds& m, int post, int symmetric, float c0) {
   for (int k = 0; k < post; ++k) {
     if (symmetric) {
       rb_half<G, 0>(L, m, c0);
@@ -1376,7 +1391,8 @@ __device__ void bottom_run(const DevInst* I, int l0, int sel, int pcg, int symme
   // blocks, then a tail for wavefront 0: those levels in register blocks
   bool rb = NT == 1024 && ls < nl && (ls - l0 == 1 || ls - l0 == 2);
   for (int l = l0; l < ls; ++l) rb = rb && ((lw[l] + 1) / 2) * ((lh[l] + 1) / 2) <= NT;
-  RbLevel A, B;
+  RbLevel<true> A;  // the first level: stencils in registers
+  RbLevel<false> B;
   if (rb) {
     rb_load(A, m, lw[l0], lh[l0], loff[l0]);
     rb_pre<NT>(A, m, pre, gs_c0(I, (float)(1 << l0)));
