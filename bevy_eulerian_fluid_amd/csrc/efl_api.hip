@@ -338,8 +338,7 @@ void stage_reinit(hipStream_t st, const IV& v) {  // reinitialize_levelset.rs:16
     efl::launch_jfa_sdf(st, make_batch(jfa, EX_CENTER, 0, &fin));
   }
   if (!fim.empty()) {
-    efl::launch_fim_init(st, make_batch(fim, EX_CENTER));
-    efl::launch_fim_active(st, make_batch(fim, EX_CENTER));
+    efl::launch_fim_init_active(st, make_batch(fim, EX_CENTER));
     uint32_t M = 0;
     for (auto* i : fim) M = std::max(M, i->s.reinit.fim_iterations);
     if (M == 0) return;

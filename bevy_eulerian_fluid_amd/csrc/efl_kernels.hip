@@ -1098,36 +1098,49 @@ __global__ __launch_bounds__(NT) void k_jfa_sdf(Batch b) {
 constexpr uint32_t L_NONE = 0u, L_SOURCE = 1u, L_ACTIVE = 2u, L_CONVERGED = 3u;
 constexpr float FIM_FAR = 1e30f;
 
-__global__ __launch_bounds__(NT) void k_fim_init(Batch b) {
-  Site s = locate(b);
-  if (!s.ok) return;
-  CDevInst* I = cdev(s.I);
-  int W = I->W, H = I->H, x = s.x, y = s.y;
-  size_t id = (size_t)y * W + x;
-  float level = I->ls1[id];
+// K28 initialisation (source cells keep their level, the others start at
+// +-FIM_FAR) and the first active labels (the neighbours of source cells) in
+// one launch.  A cell is a source when an in-grid neighbour's level set has
+// the other sign or is zero.  A covered neighbour's label is its own source
+// test, recomputed here from the level set with the same expression; an
+// uncovered neighbour (outside the dispatch cover 8 floor(W / 8) x
+// 8 floor(H / 8)) keeps the label it had, which no thread of this launch
+// writes.  So the results are those of the initialisation launch followed
+// by the labelling launch.
+__device__ __forceinline__ bool fim_source(const float* ls1, int W, int H, int x, int y, float level) {
   const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
   bool src = false;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     int nx = x + ox[k], ny = y + oy[k];
-    if (!src && inb(nx, ny, W, H) && level * I->ls1[(size_t)ny * W + nx] <= 0.0f) src = true;
+    if (!src && inb(nx, ny, W, H) && level * ls1[(size_t)ny * W + nx] <= 0.0f) src = true;
   }
-  I->ls0[id] = src ? level : sgn(level) * FIM_FAR;
-  I->lab0[id] = src ? L_SOURCE : L_NONE;
+  return src;
 }
-__global__ __launch_bounds__(NT) void k_fim_active(Batch b) {
+__global__ __launch_bounds__(NT) void k_fim_init_active(Batch b) {
   Site s = locate(b);
   if (!s.ok) return;
   CDevInst* I = cdev(s.I);
-  int W = I->W, H = I->H, x = s.x, y = s.y;
-  size_t id = (size_t)y * W + x;
-  if (I->lab0[id] == L_SOURCE) { I->lab[id] = L_SOURCE; return; }
+  const int W = I->W, H = I->H, x = s.x, y = s.y;
+  const size_t id = (size_t)y * W + x;
+  const float level = I->ls1[id];
+  const bool src = fim_source(I->ls1, W, H, x, y, level);
+  I->ls0[id] = src ? level : sgn(level) * FIM_FAR;
+  I->lab0[id] = src ? L_SOURCE : L_NONE;
+  if (src) {
+    I->lab[id] = L_SOURCE;
+    return;
+  }
   const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
   uint32_t out = L_NONE;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    int nx = x + ox[k], ny = y + oy[k];
-    if (inb(nx, ny, W, H) && I->lab0[(size_t)ny * W + nx] == L_SOURCE) out = L_ACTIVE;
+    const int nx = x + ox[k], ny = y + oy[k];
+    if (!inb(nx, ny, W, H)) continue;
+    const size_t nid = (size_t)ny * W + nx;
+    const bool nsrc = (nx < s.ex && ny < s.ey) ? fim_source(I->ls1, W, H, nx, ny, I->ls1[nid])
+                                              : I->lab0[nid] == L_SOURCE;
+    if (nsrc) out = L_ACTIVE;
   }
   I->lab[id] = out;
 }
@@ -2014,8 +2027,7 @@ void launch_advect_levelset(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_advec
 void launch_jfa_init(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_jfa_init, b, st); }
 void launch_jfa_step(hipStream_t st, const Batch& b, int step) { EFL_LAUNCH(k_jfa_step, b, st, step); }
 void launch_jfa_sdf(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_jfa_sdf, b, st); }
-void launch_fim_init(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_init, b, st); }
-void launch_fim_active(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_active, b, st); }
+void launch_fim_init_active(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_fim_init_active, b, st); }
 void launch_fim_fused(hipStream_t st, const Batch& b, int src) {
   const int nb = b.start[b.n];
   if (nb <= 0) return;

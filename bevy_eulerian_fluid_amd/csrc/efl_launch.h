@@ -42,8 +42,7 @@ void launch_advect_levelset(hipStream_t st, const Batch& b);
 void launch_jfa_init(hipStream_t st, const Batch& b);
 void launch_jfa_step(hipStream_t st, const Batch& b, int step);  // Batch.level = source seed buffer
 void launch_jfa_sdf(hipStream_t st, const Batch& b);  // Batch.level = final seed buffer
-void launch_fim_init(hipStream_t st, const Batch& b);
-void launch_fim_active(hipStream_t st, const Batch& b);
+void launch_fim_init_active(hipStream_t st, const Batch& b);  // K28 + the first active labels
 void launch_fim_fused(hipStream_t st, const Batch& b, int src);  // Batch::level = iterations
 int fim_tile_w();
 int fim_tile_h();
