@@ -188,6 +188,20 @@ __device__ __forceinline__ int xcd_tile(int blk, int nb) {
   return blk < 8 * q ? (blk & 7) * q + (blk >> 3) : blk;
 }
 
+// The slot whose block range holds block blk: the last i < n with
+// start[i] <= blk (start[0] = 0, non-decreasing; a slot without blocks has
+// start[i] = start[i + 1] and is passed over).  A binary search: 6 dependent
+// scalar loads for 64 slots where a scan took up to 63.
+__device__ __forceinline__ int slot_of(const int* start, int n, int blk) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Launch map of a batched kernel: the flat grid of blocks is split into
 // per-instance ranges; inside a range, blocks tile the instance's extent.
 // Batch::sel bits of the level-0 / level-1 CG preconditioner launches (bits

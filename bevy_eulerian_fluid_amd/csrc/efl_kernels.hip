@@ -48,10 +48,7 @@ struct Site {
 // more than one instance (the search over Batch::start is a chain of
 // dependent loads).
 __device__ __forceinline__ int batch_slot(const Batch& b, int blk) {
-  int i = 0;
-  if (b.n > 1)
-    while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
-  return i;
+  return b.n > 1 ? slot_of(b.start, b.n, blk) : 0;
 }
 
 __device__ __forceinline__ Site locate(const Batch& b) {
@@ -987,8 +984,7 @@ __global__ __launch_bounds__(XNT) void k_extrapolate_fused(Batch b) {
   __shared__ uint8_t sv[2][XR_N];
   __shared__ float sl[XL_N];
   const int blk = xcd_tile((int)blockIdx.x, b.nblocks);
-  int i = 0;
-  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  const int i = slot_of(b.start, b.n, blk);
   const int lb = blk - b.start[i];
   if (b.level[i] == 0) extrapolate_tile<0>(b, i, lb, sf, sv[0], sv[1], sl);
   else extrapolate_tile<1>(b, i, lb, sf, sv[0], sv[1], sl);
@@ -1176,8 +1172,7 @@ __global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
   __shared__ float sp[2][FR_N];
   __shared__ uint8_t sl[2][FR_N];
   const int blk = xcd_tile((int)blockIdx.x, b.nblocks);
-  int i = 0;
-  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  const int i = slot_of(b.start, b.n, blk);
   const int lb = blk - b.start[i];
   const DevInst* I = b.inst[i];
   const int iters = b.level[i];

@@ -191,8 +191,7 @@ constexpr int SU_W = 64, SU_H = 4;
 
 __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
   int blk = blockIdx.x;
-  int i = 0;
-  while (i + 1 < b.n && blk >= b.start[i + 1]) ++i;
+  const int i = slot_of(b.start, b.n, blk);
   const int lb = blk - b.start[i];
   const DevInst* I = b.inst[i];
   const int l = b.level[i];
@@ -288,8 +287,7 @@ __device__ __forceinline__ MSite mg_locate(const Batch& b) {
   asm volatile("" ::"s"(st), "s"(tx), "s"(s.I), "s"(s.lvl), "s"(s.ex), "s"(s.ey), "s"(s.sel));
   const int blk = xcd_tile((int)blockIdx.x, nb);
   if (n > 1) {
-    int i = 0;
-    while (i + 1 < n && blk >= b.start[i + 1]) ++i;
+    const int i = slot_of(b.start, n, blk);
     st = b.start[i];
     tx = b.tiles_x[i];
     s.I = b.inst[i];
@@ -1561,8 +1559,7 @@ __global__ __launch_bounds__(MNT) void k_mg_coarse(CoarseBatch b, int pcg, int s
   extern __shared__ float4 smem_f4[];  // the bottom's LDS (used by workgroup 0 of a group)
   __shared__ float sx[Tile<5>::RW * Tile<5>::RH];
   __shared__ float sr[MT_W * MT_H];
-  int i = 0;
-  while (i + 1 < b.n && (int)blockIdx.x >= b.start[i + 1]) ++i;
+  const int i = slot_of(b.start, b.n, (int)blockIdx.x);
   const int w = (int)blockIdx.x - b.start[i];
   const uint32_t G = (uint32_t)(b.start[i + 1] - b.start[i]);
   const DevInst* I = b.inst[i];
