@@ -976,6 +976,7 @@ int resolve_stats(efl_instance* i, bool block) {
 int cg_run_async(efl_ctx* c, const IV& cg, const hipGraphExec_t* iter) {
   hipStream_t st = c->stream;
   uint32_t n = 0;
+  std::vector<uint32_t> want;
   for (auto* i : cg) {
     if (!i->h_stat) {
       HIPCHK(c, hipHostMalloc((void**)&i->h_stat, 2 * efl::kScalars * sizeof(double), hipHostMallocDefault));
@@ -991,11 +992,20 @@ int cg_run_async(efl_ctx* c, const IV& cg, const hipGraphExec_t* iter) {
     }
     uint32_t base = i->cg_last ? i->cg_last : 4u;
     for (uint32_t q = 0; q < std::min(i->n_recent, 4u); ++q) base = std::max(base, i->recent[q]);
-    n = std::max(n, std::min(base + c->async_margin, i->h.cg_maxc));
+    want.push_back(std::min(base + c->async_margin, i->h.cg_maxc));
+    n = std::max(n, want.back());
   }
+  // each instance takes part in its own budget of iterations (eager path;
+  // a replayed graph advances the whole set), as in cg_run
   for (uint32_t k = 0; k < n; ++k) {
-    if (iter) HIPCHK(c, hipGraphLaunch(iter[k & 1], st));
-    else cg_iteration(st, cg, c->cg_fuse);
+    if (iter) {
+      HIPCHK(c, hipGraphLaunch(iter[k & 1], st));
+    } else {
+      IV sub;
+      for (size_t j = 0; j < cg.size(); ++j)
+        if (want[j] > k) sub.push_back(cg[j]);
+      cg_iteration(st, sub, c->cg_fuse);
+    }
   }
   efl::launch_cg_tally(st, cg_batch(cg));
   for (auto* i : cg) {
