@@ -867,7 +867,7 @@ constexpr int XR_W = XT_W + 2 * XP, XR_H = XT_H + 2 * XP, XR_N = XR_W * XR_H;
 constexpr int XL_W = XR_W + 2, XL_N = XL_W * (XR_H + 2);
 
 template <int COMP>
-__device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, float* sf, uint8_t* sv0, uint8_t* sv1,
+__device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, float* sf, uint32_t* sv0, uint32_t* sv1,
                                                  float* sl) {
   const DevInst* I = b.inst[i];
   const int W = I->W, H = I->H;
@@ -887,7 +887,7 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
   bool any_valid = false, any_open = false;
   for (int q = threadIdx.x; q < XR_N; q += XNT) {
     const int lx = q % XR_W, ly = q / XR_W, x = ox + lx, y = oy + ly;
-    uint8_t v0 = 0, v1 = 0;
+    uint32_t v0 = 0, v1 = 0;
     if (inb(x, y, w, h)) {
       const size_t id = (size_t)y * w + x;
       if (x < ex && y < ey) {  // the validity kernels cover the same faces
@@ -901,8 +901,8 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
         const float vm = 0.25f * (((c0 + c1) + c2) + c3), vp = 0.25f * (((c2 + c3) + c4) + c5);
         v0 = face_open(vm, vp) > 0.0f ? 0 : 1;
       } else {
-        v0 = (uint8_t)V0[id];
-        v1 = (uint8_t)V1[id];
+        v0 = (uint32_t)V0[id];
+        v1 = (uint32_t)V1[id];
       }
     }
     sv0[q] = v0;
@@ -938,8 +938,8 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
   __syncthreads();
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 16 threads walk the window
   for (int p = 0; p < XP; ++p) {
-    const uint8_t* vin = (p & 1) ? sv1 : sv0;  // pass 0: valid0 -> valid1, then back
-    uint8_t* vout = (p & 1) ? sv0 : sv1;
+    const uint32_t* vin = (p & 1) ? sv1 : sv0;  // pass 0: valid0 -> valid1, then back
+    uint32_t* vout = (p & 1) ? sv0 : sv1;
     const int m = p + 1;
     const int lo_x = max(m, -ox), hi_x = min(XR_W - m, cx1), lo_y = max(m, -oy), hi_y = min(XR_H - m, cy1);
     for (int ly = lo_y + ty; ly < hi_y; ly += 16)
@@ -981,7 +981,7 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
 // one launch
 __global__ __launch_bounds__(XNT) void k_extrapolate_fused(Batch b) {
   __shared__ float sf[XR_N];
-  __shared__ uint8_t sv[2][XR_N];
+  __shared__ uint32_t sv[2][XR_N];  // 32-bit: no two lanes write one LDS word
   __shared__ float sl[XL_N];
   const int blk = xcd_tile((int)blockIdx.x, b.nblocks);
   const int i = slot_of(b.start, b.n, blk);
@@ -1170,7 +1170,7 @@ __device__ __forceinline__ float eikonal_lds(const float* sp, int c, bool l, boo
 
 __global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
   __shared__ float sp[2][FR_N];
-  __shared__ uint8_t sl[2][FR_N];
+  __shared__ uint32_t sl[2][FR_N];  // 32-bit: no two lanes write one LDS word
   const int blk = xcd_tile((int)blockIdx.x, b.nblocks);
   const int i = slot_of(b.start, b.n, blk);
   const int lb = blk - b.start[i];
@@ -1186,10 +1186,10 @@ __global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
   for (int c = threadIdx.x; c < FR_N; c += FNT) {
     const int x = ox + c % FR_W, y = oy + c / FR_W;
     float p = 0.0f;
-    uint8_t l = (uint8_t)L_NONE;
+    uint32_t l = L_NONE;
     if (inb(x, y, W, H)) {
       p = pin[(size_t)y * W + x];
-      l = (uint8_t)lin[(size_t)y * W + x];
+      l = lin[(size_t)y * W + x];
     }
     sp[0][c] = p;
     sl[0][c] = l;
@@ -1209,11 +1209,11 @@ __global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
       const float p = sp[0][c];
       if (l != L_ACTIVE || x >= cw || y >= ch) {
         sp[1][c] = p;
-        sl[1][c] = (uint8_t)l;
+        sl[1][c] = l;
       } else {
         const float q = eikonal_lds(sp[0], c, x > 0, x < W - 1, y > 0, y < H - 1) * sgn(p);
         sp[1][c] = q;
-        sl[1][c] = (uint8_t)((fabsf(p - q) > 1e-6f) ? L_ACTIVE : L_CONVERGED);
+        sl[1][c] = (fabsf(p - q) > 1e-6f) ? L_ACTIVE : L_CONVERGED;
       }
     }
     __syncthreads();
@@ -1226,7 +1226,7 @@ __global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
       }
       const uint32_t l = sl[1][c];
       if (l == L_CONVERGED) {
-        sl[0][c] = (uint8_t)L_NONE;
+        sl[0][c] = L_NONE;
         sp[0][c] = sp[1][c];
         continue;
       }
@@ -1247,7 +1247,7 @@ __global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
       const float qn = eikonal_lds(sp[1], c, inn[0], inn[1], inn[2], inn[3]);
       if (fabsf(pa) > qn) {
         sp[0][c] = qn * sgn(sp[1][c + nb[trig]]);
-        sl[0][c] = (uint8_t)L_ACTIVE;
+        sl[0][c] = L_ACTIVE;
       } else {
         sp[0][c] = pa;
       }
