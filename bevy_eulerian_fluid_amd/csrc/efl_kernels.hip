@@ -877,8 +877,23 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
   float* F = COMP == 0 ? I->u0 : I->v0;
   int32_t* V0 = COMP == 0 ? I->uval0 : I->vval0;
   int32_t* V1 = COMP == 0 ? I->uval1 : I->vval1;
-  // level-set cells [ox - 1, ox + XR_W + 1) x [oy - 1, oy + XR_H + 1)
-  for (int q = threadIdx.x; q < XL_N; q += XNT) sl[q] = ldz(I->ls0, W, H, ox - 1 + q % XL_W, oy - 1 + q / XL_W);
+  // level-set cells [ox - 1, ox + XR_W + 1) x [oy - 1, oy + XR_H + 1), every
+  // load issued before any is stored (a cell outside the grid reads index 0
+  // and is stored as 0)
+  {
+    constexpr int LIT = (XL_N + XNT - 1) / XNT;
+    float lv[LIT];
+#pragma unroll
+    for (int k = 0; k < LIT; ++k) {
+      const int q = (int)threadIdx.x + k * XNT, x = ox - 1 + q % XL_W, y = oy - 1 + q / XL_W;
+      lv[k] = I->ls0[q < XL_N && inb(x, y, W, H) ? (size_t)y * W + x : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < LIT; ++k) {
+      const int q = (int)threadIdx.x + k * XNT, x = ox - 1 + q % XL_W, y = oy - 1 + q / XL_W;
+      if (q < XL_N) sl[q] = inb(x, y, W, H) ? lv[k] : 0.0f;
+    }
+  }
   __syncthreads();
   // Validity of the region: recomputed on covered faces, staged (both
   // planes) elsewhere.  A covered face's second plane is written by pass 0
@@ -931,9 +946,19 @@ __device__ __forceinline__ void extrapolate_tile(const Batch& b, int i, int lb, 
     }
     return;
   }
-  for (int q = threadIdx.x; q < XR_N; q += XNT) {
-    const int x = ox + q % XR_W, y = oy + q / XR_W;
-    sf[q] = inb(x, y, w, h) ? F[(size_t)y * w + x] : 0.0f;
+  {
+    constexpr int FIT = (XR_N + XNT - 1) / XNT;
+    float fv[FIT];
+#pragma unroll
+    for (int k = 0; k < FIT; ++k) {
+      const int q = (int)threadIdx.x + k * XNT, x = ox + q % XR_W, y = oy + q / XR_W;
+      fv[k] = F[q < XR_N && inb(x, y, w, h) ? (size_t)y * w + x : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < FIT; ++k) {
+      const int q = (int)threadIdx.x + k * XNT, x = ox + q % XR_W, y = oy + q / XR_W;
+      if (q < XR_N) sf[q] = inb(x, y, w, h) ? fv[k] : 0.0f;
+    }
   }
   __syncthreads();
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 16 threads walk the window
@@ -1182,15 +1207,29 @@ __global__ __launch_bounds__(FNT) void k_fim_fused(Batch b, int src) {
   const uint32_t* lin = src ? I->lab_tmp : I->lab;
   float* pout = src ? I->ls0 : I->phi_tmp;
   uint32_t* lout = src ? I->lab : I->lab_tmp;
+  // staging: every load of the thread's cells issued before any is stored,
+  // so their latencies overlap (a cell outside the grid reads index 0 and
+  // is stored as 0 / NONE)
+  constexpr int FIT = (FR_N + FNT - 1) / FNT;
   bool active = false;
-  for (int c = threadIdx.x; c < FR_N; c += FNT) {
+  float pv[FIT];
+  uint32_t lv[FIT];
+#pragma unroll
+  for (int k = 0; k < FIT; ++k) {
+    const int c = (int)threadIdx.x + k * FNT;
     const int x = ox + c % FR_W, y = oy + c / FR_W;
-    float p = 0.0f;
-    uint32_t l = L_NONE;
-    if (inb(x, y, W, H)) {
-      p = pin[(size_t)y * W + x];
-      l = lin[(size_t)y * W + x];
-    }
+    const size_t id = c < FR_N && inb(x, y, W, H) ? (size_t)y * W + x : 0;
+    pv[k] = pin[id];
+    lv[k] = lin[id];
+  }
+#pragma unroll
+  for (int k = 0; k < FIT; ++k) {
+    const int c = (int)threadIdx.x + k * FNT;
+    if (c >= FR_N) continue;
+    const int x = ox + c % FR_W, y = oy + c / FR_W;
+    const bool in = inb(x, y, W, H);
+    const float p = in ? pv[k] : 0.0f;
+    const uint32_t l = in ? lv[k] : L_NONE;
     sp[0][c] = p;
     sl[0][c] = l;
     active |= l == L_ACTIVE && x < cw && y < ch;
