@@ -1128,43 +1128,70 @@ constexpr float FIM_FAR = 1e30f;
 // 8 floor(H / 8)) keeps the label it had, which no thread of this launch
 // writes.  So the results are those of the initialisation launch followed
 // by the labelling launch.
-__device__ __forceinline__ bool fim_source(const float* ls1, int W, int H, int x, int y, float level) {
-  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
-  bool src = false;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int nx = x + ox[k], ny = y + oy[k];
-    if (!src && inb(nx, ny, W, H) && level * ls1[(size_t)ny * W + nx] <= 0.0f) src = true;
-  }
-  return src;
-}
 __global__ __launch_bounds__(NT) void k_fim_init_active(Batch b) {
+  // the block's 64 x 4 tile: the level set over it and a two-cell ring in
+  // LDS, then the SOURCE label of every cell of the tile and a one-cell ring
+  // (computed for covered cells, the stored one for uncovered cells), then
+  // each cell's own initialisation and label from LDS
+  constexpr int SW = TX + 4, SH = TY + 4, SN = SW * SH, RW1 = TX + 2, RN1 = RW1 * (TY + 2);
+  __shared__ float sls[SN];
+  __shared__ uint32_t ssrc[RN1];
   Site s = locate(b);
-  if (!s.ok) return;
   CDevInst* I = cdev(s.I);
-  const int W = I->W, H = I->H, x = s.x, y = s.y;
+  const int W = I->W, H = I->H;
+  const int x0 = s.x - (int)(threadIdx.x % TX), y0 = s.y - (int)(threadIdx.x / TX);
+  constexpr int SIT = (SN + NT - 1) / NT;
+  float lv[SIT];
+#pragma unroll
+  for (int k = 0; k < SIT; ++k) {  // every load first (outside the grid: index 0, unused)
+    const int q = (int)threadIdx.x + k * NT, x = x0 - 2 + q % SW, y = y0 - 2 + q / SW;
+    lv[k] = I->ls1[q < SN && inb(x, y, W, H) ? (size_t)y * W + x : 0];
+  }
+#pragma unroll
+  for (int k = 0; k < SIT; ++k) {
+    const int q = (int)threadIdx.x + k * NT;
+    if (q < SN) sls[q] = lv[k];
+  }
+  __syncthreads();
+  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
+  for (int q = (int)threadIdx.x; q < RN1; q += NT) {
+    const int rx = q % RW1, ry = q / RW1, x = x0 - 1 + rx, y = y0 - 1 + ry;
+    uint32_t src = 0;
+    if (inb(x, y, W, H)) {
+      if (x < s.ex && y < s.ey) {  // covered: the source test
+        const int c = (ry + 1) * SW + rx + 1;
+        const float level = sls[c];
+        bool t = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (!t && inb(x + ox[k], y + oy[k], W, H) && level * sls[c + ox[k] + oy[k] * SW] <= 0.0f) t = true;
+        src = t ? 1u : 0u;
+      } else {  // outside the cover: the label it had
+        src = I->lab0[(size_t)y * W + x] == L_SOURCE ? 1u : 0u;
+      }
+    }
+    ssrc[q] = src;
+  }
+  __syncthreads();
+  if (!s.ok) return;
+  const int x = s.x, y = s.y;
   const size_t id = (size_t)y * W + x;
-  const float level = I->ls1[id];
-  const bool src = fim_source(I->ls1, W, H, x, y, level);
+  const int r = (y - y0 + 1) * RW1 + (x - x0 + 1);
+  const float level = sls[(y - y0 + 2) * SW + (x - x0 + 2)];
+  const bool src = ssrc[r] != 0u;
   I->ls0[id] = src ? level : sgn(level) * FIM_FAR;
   I->lab0[id] = src ? L_SOURCE : L_NONE;
   if (src) {
     I->lab[id] = L_SOURCE;
     return;
   }
-  const int ox[4] = {-1, 1, 0, 0}, oy[4] = {0, 0, -1, 1};
   uint32_t out = L_NONE;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int nx = x + ox[k], ny = y + oy[k];
-    if (!inb(nx, ny, W, H)) continue;
-    const size_t nid = (size_t)ny * W + nx;
-    const bool nsrc = (nx < s.ex && ny < s.ey) ? fim_source(I->ls1, W, H, nx, ny, I->ls1[nid])
-                                              : I->lab0[nid] == L_SOURCE;
-    if (nsrc) out = L_ACTIVE;
-  }
+  for (int k = 0; k < 4; ++k)
+    if (inb(x + ox[k], y + oy[k], W, H) && ssrc[r + ox[k] + oy[k] * RW1] != 0u) out = L_ACTIVE;
   I->lab[id] = out;
 }
+
 // The iterations, temporally blocked: a 64 x 16 tile with a halo of 2 FK
 // cells runs up to FK iterations (phases A and B, A converging a cell when
 // it moved by at most 1e-6) in LDS.  Each phase reads only the previous
