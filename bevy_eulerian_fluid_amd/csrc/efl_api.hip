@@ -1329,9 +1329,7 @@ void stage_extrapolate(hipStream_t st, const IV& v) {
 }
 
 void stage_fluid_to_solid(hipStream_t st, const IV& v) {  // render_node.rs:610-641
-  Batch per;
-  std::memset(&per, 0, sizeof(per));
-  per = make_batch(v, EX_CENTER);
+  const Batch per = make_batch(v, EX_CENTER, 0, nullptr, efl::tile_w(), efl::sample_forces_tile_h());
   efl::launch_clear_bins(st, per);
   efl::launch_sample_forces(st, per);
   efl::launch_accumulate_forces(st, per);

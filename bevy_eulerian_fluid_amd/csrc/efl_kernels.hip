@@ -29,6 +29,7 @@ namespace efl {
 constexpr int TX = 64;
 constexpr int TY = 4;
 constexpr int NT = TX * TY;
+constexpr int SF_TH = 32;  // k_sample_forces tile height
 static_assert(NT == RED_NT, "reduction kernels use RED_NT threads per block");
 
 // ---------------------------------------------------------------------------
@@ -1355,70 +1356,163 @@ __global__ void k_clear_bins(Batch b) {
   for (int i = threadIdx.x; i < 4 * EFL_MAX_SOLIDS; i += blockDim.x) I->bins[i] = 0;
 }
 __device__ __forceinline__ int fixed_point(float v) { return to_i32(v * (1.0f / 0.1f)); }
+// Centre of body k about which its torques are taken: the triangle's
+// centroid, a circle's, rectangle's or capsule's translation, else the origin.
+// The obstacle's first 96 bytes (ids, values, transform) are loaded as six
+// 16-byte words before the shape selects the result, in one round.
+static_assert(offsetof(efl_obstacle, values) == 8 && offsetof(efl_obstacle, transform) == 32 &&
+                  sizeof(efl_obstacle) % 16 == 0,
+              "efl_obstacle layout");
+__device__ __forceinline__ float2 body_centre(CDevInst* I, int k) {
+  float2 c = make_float2(0.0f, 0.0f);
+  if ((uint32_t)k < I->nobs) {
+    const float4* q = reinterpret_cast<const float4*>(&I->obs[k]);
+    const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
+    // keeps the compiler from sinking the triangle-only loads behind the test
+    asm volatile("" ::"v"(q2.x), "v"(q2.y), "v"(q3.x), "v"(q3.y), "v"(q4.x), "v"(q4.y));
+    const uint32_t shape = __float_as_uint(q0.y);
+    // values: q0.z q0.w q1.x q1.y q1.z q1.w; transform: q2 (0-3), q3 (4-7), q4 (8-11), q5 (12-15)
+    const float lx = ((q0.z + q1.x) + q1.z) / 3.0f, ly = ((q0.w + q1.y) + q1.w) / 3.0f;
+    // xform(transform, lx, ly)
+    const float tx = ((q2.x * lx + q3.x * ly) + q4.x * 0.0f) + q5.x * 1.0f;
+    const float ty = ((q2.y * lx + q3.y * ly) + q4.y * 0.0f) + q5.y * 1.0f;
+    c.x = shape == 4u ? tx : (shape <= 2u ? q5.x : 0.0f);
+    c.y = shape == 4u ? ty : (shape <= 2u ? q5.y : 0.0f);
+  }
+  return c;
+}
+// A block sums a TX x SF_TH tile (batch tiles of that size).  Every load is
+// issued before any result is needed: the tile's solid level set with a
+// one-cell ring (zero outside the grid) into LDS, each thread's column of
+// solid ids and pressures into registers, every body's centre into LDS.
+// Then each thread sums its column's cells (runs of one body in registers),
+// each wavefront its lanes, the block in LDS bins, and the block adds its
+// nonzero bins to the instance's, one global atomic each.  The sums wrap as
+// i32 (as atomicAdd) and are exact in any order.
 __global__ __launch_bounds__(NT) void k_sample_forces(Batch b) {
-  Site s = locate(b);
-  CDevInst* I = cdev(s.I);
-  int sid = -1;
-  int q[4] = {0, 0, 0, 0};
-  if (s.ok) {
-    int W = I->W, H = I->H, x = s.x, y = s.y;
-    size_t id = (size_t)y * W + x;
-    // a cell without a solid id contributes nothing whatever its fractions
-    int cand = I->sid[id];
-    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (cand >= 0 && cand < EFL_MAX_SOLIDS) {
-      f = fractions_at(I->lss, W, H, x, y);
-      if (all_closed(f)) cand = -1;
+  constexpr int RW = TX + 2, RH = SF_TH + 2, ROWS = SF_TH / TY;
+  __shared__ int sbin[4 * EFL_MAX_SOLIDS];
+  __shared__ float sls[RW * RH];
+  __shared__ float2 scen[EFL_MAX_SOLIDS];
+  const int blk = blockIdx.x;
+  const int i = batch_slot(b, blk);
+  CDevInst* I = cdev(b.inst[i]);
+  const int lb = blk - b.start[i], tw = b.tiles_x[i], ex = b.ex[i], ey = b.ey[i];
+  const int W = I->W, H = I->H;
+  const int ox = (lb % tw) * TX, oy = (lb / tw) * SF_TH;
+  const int tx = (int)(threadIdx.x % TX), ty = (int)(threadIdx.x / TX);
+  const int x = ox + tx;
+  const int32_t* sidp = I->sid;
+  const float *p0p = I->p0, *lssp = I->lss;
+  int sid[ROWS];
+  float pr[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    const int y = oy + ty + r * TY;
+    const bool in = x < ex && y < ey;
+    const size_t id = (size_t)y * W + x;
+    sid[r] = in ? sidp[id] : -1;
+    pr[r] = in ? p0p[id] : 0.0f;
+  }
+  float stage[(RW * RH + NT - 1) / NT];
+#pragma unroll
+  for (int k = 0; k < (RW * RH + NT - 1) / NT; ++k) {
+    const int t = (int)threadIdx.x + k * NT;
+    stage[k] = t < RW * RH ? ldz(lssp, W, H, ox - 1 + t % RW, oy - 1 + t / RW) : 0.0f;
+  }
+  // the instance's scalars, loaded with the rest rather than behind the
+  // barrier where they are first used
+  const float dxv = I->dx, sx = I->sizex, sy = I->sizey;
+  const float T[16] = {I->T[0], I->T[1], 0.f, 0.f, I->T[4], I->T[5], 0.f, 0.f,
+                       I->T[8], I->T[9], 0.f, 0.f, I->T[12], I->T[13], 0.f, 0.f};
+  asm volatile("" ::"s"(dxv), "s"(sx), "s"(sy), "s"(T[0]), "s"(T[1]), "s"(T[4]), "s"(T[5]), "s"(T[8]),
+               "s"(T[9]), "s"(T[12]), "s"(T[13]));
+  static_assert(EFL_MAX_SOLIDS == NT, "one body centre per thread");
+  const float2 cen = body_centre(I, (int)threadIdx.x);
+  for (int j = threadIdx.x; j < 4 * EFL_MAX_SOLIDS; j += NT) sbin[j] = 0;
+  scen[threadIdx.x] = cen;
+#pragma unroll
+  for (int k = 0; k < (RW * RH + NT - 1) / NT; ++k) {
+    const int t = (int)threadIdx.x + k * NT;
+    if (t < RW * RH) sls[t] = stage[k];
+  }
+  __syncthreads();
+  int cs = -1;
+  unsigned acc[4] = {0u, 0u, 0u, 0u};
+  // one copy of the cell body, the row's values picked from registers (an
+  // eighth of the unrolled code)
+#pragma unroll 1
+  for (int r = 0; r < ROWS; ++r) {
+    int cand = sid[0];
+    float prr = pr[0];
+#pragma unroll
+    for (int k = 1; k < ROWS; ++k) {
+      cand = r == k ? sid[k] : cand;
+      prr = r == k ? pr[k] : prr;
     }
-    if (cand >= 0 && cand < EFL_MAX_SOLIDS) {
-      float p = I->p0[id];
-      float fx = ((-(f.y - f.x)) * p) * I->dx;
-      float fy = ((-(f.w - f.z)) * p) * I->dx;
-      float cx = 0.0f, cy = 0.0f;
-      if ((uint32_t)cand < I->nobs) {
-        const efl_obstacle& o = I->obs[cand];
-        if (o.shape == 4u) {
-          float lx = ((o.values[0] + o.values[2]) + o.values[4]) / 3.0f;
-          float ly = ((o.values[1] + o.values[3]) + o.values[5]) / 3.0f;
-          float2 c = xform(o.transform, lx, ly);
-          cx = c.x; cy = c.y;
-        } else if (o.shape <= 2u) {
-          cx = o.transform[12]; cy = o.transform[13];
-        }
+    // a cell without a solid id contributes nothing whatever its fractions
+    if (cand < 0 || cand >= EFL_MAX_SOLIDS) continue;
+    const int ly = ty + r * TY + 1, lx = tx + 1;
+    const float* c = &sls[ly * RW + lx];
+    // fractions_at from the staged level set (same sums, same order)
+    float a0 = c[-RW - 1], a1 = c[-RW], a2 = c[-RW + 1];
+    float a3 = c[-1], a4 = c[0], a5 = c[1];
+    float a6 = c[RW - 1], a7 = c[RW], a8 = c[RW + 1];
+    float mm = 0.25f * (((a0 + a1) + a3) + a4);
+    float pm = 0.25f * (((a1 + a2) + a4) + a5);
+    float mp = 0.25f * (((a3 + a4) + a6) + a7);
+    float pp = 0.25f * (((a4 + a5) + a7) + a8);
+    // face_open(a, b) is 0 when max(a, b) <= 0 (a negative or zero quotient,
+    // or 0 / 0), so a cell inside a body is closed without the divisions
+    if (mm <= 0.0f && pm <= 0.0f && mp <= 0.0f && pp <= 0.0f) continue;
+    const float4 f = make_float4(face_open(mm, mp), face_open(pm, pp), face_open(mm, pm), face_open(mp, pp));
+    if (all_closed(f)) continue;
+    const int y = oy + ty + r * TY;
+    float fx = ((-(f.y - f.x)) * prr) * dxv;
+    float fy = ((-(f.w - f.z)) * prr) * dxv;
+    const float2 cc = scen[cand];
+    float ux = (float)x / sx, uy = (float)y / sy;
+    float2 w = xform(T, (ux - 0.5f) * sx, (-uy + 0.5f) * sy);
+    float rx = w.x - cc.x, ry = w.y - cc.y;
+    if (cand != cs) {
+      if (cs >= 0)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) atomicAdd(&sbin[k * EFL_MAX_SOLIDS + cs], (int)acc[k]);
+      cs = cand;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[k] = 0u;
+    }
+    acc[0] += (unsigned)fixed_point(fx);
+    acc[1] += (unsigned)fixed_point(fy);
+    acc[2] += (unsigned)fixed_point(rx * fy - ry * fx);
+    acc[3] += (unsigned)fixed_point(-rx * fy - ry * fx);
+  }
+  // when every holding lane of the wavefront holds the same body, fold the
+  // wave first and add once
+  const int lane = threadIdx.x & 63;
+  const unsigned long long act = __ballot(cs >= 0);
+  if (act != 0ull) {
+    const int first = __ffsll((long long)act) - 1;
+    const int cs0 = __shfl(cs, first, 64);
+    if (__all(cs < 0 || cs == cs0)) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        unsigned v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += (unsigned)__shfl_xor((int)v, off, 64);
+        acc[k] = v;
       }
-      float ux = (float)x / I->sizex, uy = (float)y / I->sizey;
-      float2 w = xform(I->T, (ux - 0.5f) * I->sizex, (-uy + 0.5f) * I->sizey);
-      float rx = w.x - cx, ry = w.y - cy;
-      float tq = rx * fy - ry * fx;
-      float tw = -rx * fy - ry * fx;
-      sid = cand;
-      q[0] = fixed_point(fx);
-      q[1] = fixed_point(fy);
-      q[2] = fixed_point(tq);
-      q[3] = fixed_point(tw);
+      if (lane == first)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) atomicAdd(&sbin[k * EFL_MAX_SOLIDS + cs0], (int)acc[k]);
+    } else if (cs >= 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) atomicAdd(&sbin[k * EFL_MAX_SOLIDS + cs], (int)acc[k]);
     }
   }
-  // Integer sums are exact in any order: when every contributing lane of the
-  // wavefront hits the same body, fold the wave first and add once.
-  int lane = threadIdx.x & 63;
-  unsigned long long act = __ballot(sid >= 0);
-  if (act == 0ull) return;
-  int first = __ffsll((long long)act) - 1;
-  int sid0 = __shfl(sid, first, 64);
-  bool uniform = __all(sid < 0 || sid == sid0);
-  if (uniform) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      unsigned v = (unsigned)q[k];  // wrapping i32 sums, as atomicAdd
-      for (int off = 32; off > 0; off >>= 1) v += (unsigned)__shfl_xor((int)v, off, 64);
-      q[k] = (int)v;
-    }
-    if (lane == first)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) atomicAdd(&I->bins[k * EFL_MAX_SOLIDS + sid0], q[k]);
-  } else if (sid >= 0) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) atomicAdd(&I->bins[k * EFL_MAX_SOLIDS + sid], q[k]);
+  __syncthreads();
+  for (int j = threadIdx.x; j < 4 * EFL_MAX_SOLIDS; j += NT) {
+    const int v = sbin[j];
+    if (v != 0) atomicAdd(&I->bins[j], v);
   }
 }
 __global__ void k_accumulate_forces(Batch b) {
@@ -2143,5 +2237,6 @@ void launch_max_divergence(hipStream_t st, const Batch& b) {
 int tile_w() { return TX; }
 int tile_h() { return TY; }
 int cg_tile_h() { return CG_TH; }
+int sample_forces_tile_h() { return SF_TH; }
 
 }  // namespace efl

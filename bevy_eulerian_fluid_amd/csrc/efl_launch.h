@@ -16,6 +16,7 @@ const char* take_launch_error();  // nullptr when none; clears it
 int tile_w();
 int tile_h();
 int cg_tile_h();
+int sample_forces_tile_h();  // k_sample_forces: tile_w() x this per block
 
 void launch_init_center(hipStream_t st, const Batch& b);
 void launch_init_edge(hipStream_t st, const Batch& b);

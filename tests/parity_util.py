@@ -64,13 +64,18 @@ def compare_all(g, o, fields=ALL_FIELDS, tol=None):
     return errs
 
 
-def small_scene(W=64, H=64, bodies=True, seed=7, proj=None, rein=None, forces=True):
+def small_scene(W=64, H=64, bodies=True, seed=7, proj=None, rein=None, forces=True, submerged=False):
     """A compact scene exercising every stage: a fluid box, one source and one
-    sink, and (optionally) one body of each shape and a few local forces."""
+    sink, and (optionally) one body of each shape and a few local forces.
+    submerged: the fluid box fills the domain, so that every body is under
+    water."""
     rng = scenes.SplitMix64(seed, 0)
     s = scenes.settings(W, H, proj=proj or scenes.projection("gauss_seidel", gs_iterations=20),
                         rein=rein or scenes.reinit("fast_iterative", 10))
-    init = [scenes.source((0.0, -0.2 * H), "aabb", half_size=(0.45 * W, 0.28 * H))]
+    if submerged:
+        init = [scenes.source((0.0, 0.0), "aabb", half_size=(0.49 * W, 0.49 * H))]
+    else:
+        init = [scenes.source((0.0, -0.2 * H), "aabb", half_size=(0.45 * W, 0.28 * H))]
     srcs = [scenes.source((-0.3 * W, 0.3 * H), "circle", radius=4.0, velocity=(20.0, -5.0)),
             scenes.source((0.35 * W, 0.1 * H), "aabb", half_size=(3.0, 3.0), sink=True)]
     obs = []

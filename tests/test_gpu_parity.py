@@ -51,9 +51,18 @@ def test_stage_parity_multiwave(gpu_ctx, size, stage):
     _stage_case(gpu_ctx, size, stage)
 
 
-def _stage_case(gpu_ctx, size, stage):
+@pytest.mark.parametrize("size", [(64, 64), (200, 136), (1000, 700)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_fluid_to_solid_parity_submerged(gpu_ctx, size):
+    """Every body (circle, rectangle, capsule, triangle) under water, so that
+    each one's bins hold forces and torques (about the triangle's centroid,
+    the others' translations); 1000 x 700 leaves partial tiles on two edges."""
+    bins = _stage_case(gpu_ctx, size, "FLUID_TO_SOLID", small_scene(*size, submerged=True))
+    assert (bins[:, :4] != 0).all(), bins[:, :4]
+
+
+def _stage_case(gpu_ctx, size, stage, sc=None):
     _say(f"stage parity {stage} {size}")
-    sc = small_scene(*size)
+    sc = sc or small_scene(*size)
     g, o = make_pair(gpu_ctx, sc, OracleInstance)
     try:
         _warm_oracle(o, sc)
@@ -69,6 +78,7 @@ def _stage_case(gpu_ctx, size, stage):
             tol = {"U1": (2e-6, 1e-6), "V1": (2e-6, 1e-6)}
         errs = compare_all(g, o, tol=tol)
         assert not errs, "\n".join(errs)
+        return g.get("SOLID_BINS")
     finally:
         g.close()
         o.close()
