@@ -1330,7 +1330,6 @@ void stage_extrapolate(hipStream_t st, const IV& v) {
 
 void stage_fluid_to_solid(hipStream_t st, const IV& v) {  // render_node.rs:610-641
   const Batch per = make_batch(v, EX_CENTER, 0, nullptr, efl::tile_w(), efl::sample_forces_tile_h());
-  efl::launch_clear_bins(st, per);
   efl::launch_sample_forces(st, per);
   efl::launch_accumulate_forces(st, per);
 }
@@ -1658,7 +1657,7 @@ FieldDesc field_desc(efl_instance* in, int f) {
     case EFL_FIELD_FIM_LABELS: return {h.lab, (size_t)W * H, W, H, 1};
     case EFL_FIELD_JFA_SEEDS0: return {h.seed0, (size_t)W * H * 2, W, H, 2};
     case EFL_FIELD_JFA_SEEDS1: return {h.seed1, (size_t)W * H * 2, W, H, 2};
-    case EFL_FIELD_SOLID_BINS: return {h.bins, (size_t)4 * EFL_MAX_SOLIDS, EFL_MAX_SOLIDS, 4, 1};
+    case EFL_FIELD_SOLID_BINS: return {h.bins_last, (size_t)4 * EFL_MAX_SOLIDS, EFL_MAX_SOLIDS, 4, 1};
     case EFL_FIELD_SOLID_FORCES: return {h.fts, (size_t)4 * EFL_MAX_SOLIDS, EFL_MAX_SOLIDS, 1, 4};
     case EFL_FIELD_MG_R0: return {h.lev[0].r, (size_t)W * H, W, H, 1};
     default: return {nullptr, 0, 0, 0, 0};
@@ -1888,7 +1887,8 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   size_t o_ls0 = take(n * 4), o_ls1 = take(n * 4), o_lss = take(n * 4), o_phit = take(n * 4);
   size_t o_lab0 = take(n * 4), o_lab = take(n * 4), o_labt = take(n * 4);
   size_t o_grad = take(n * 8), o_s0 = take(n * 8), o_s1 = take(n * 8), o_afs = take(n * 16);
-  size_t o_bins = take(4 * EFL_MAX_SOLIDS * 4), o_fts = take(EFL_MAX_SOLIDS * 16);
+  size_t o_bins = take(4 * EFL_MAX_SOLIDS * 4), o_binsl = take(4 * EFL_MAX_SOLIDS * 4);
+  size_t o_fts = take(EFL_MAX_SOLIDS * 16);
   size_t o_r0 = take(n * 4), o_xs0 = take(n * 4);
   std::vector<size_t> o_lx(nl), o_lb(nl), o_lls(nl), o_laf(nl), o_lr(nl), o_lxs(nl), o_lcf(nl), o_lfl(nl);
   for (int l = 0; l < nl; ++l) {
@@ -1940,7 +1940,8 @@ int efl_instance_create(efl_ctx* c, const efl_settings* s, efl_instance** out) {
   h.lab0 = (uint32_t*)(base + o_lab0); h.lab = (uint32_t*)(base + o_lab); h.lab_tmp = (uint32_t*)(base + o_labt);
   h.grad = (float2*)(base + o_grad); h.seed0 = (float2*)(base + o_s0); h.seed1 = (float2*)(base + o_s1);
   h.afs = (float4*)(base + o_afs);
-  h.bins = (int32_t*)(base + o_bins); h.fts = (float4*)(base + o_fts);
+  h.bins = (int32_t*)(base + o_bins); h.bins_last = (int32_t*)(base + o_binsl);
+  h.fts = (float4*)(base + o_fts);
   h.nlev = nl;
   h.lev[0] = {(int)W, (int)H, h.p0, h.div, h.ls0, h.afs, (float*)(base + o_r0), (float*)(base + o_xs0),
               (float4*)(base + o_lcf[0]), (int32_t*)(base + o_lfl[0])};

@@ -57,7 +57,8 @@ struct DevInst {
   float *phi_tmp;      // FIM phase buffer
   uint32_t* lab_tmp;   // FIM phase labels
   float2 *seed0, *seed1;
-  int32_t* bins;       // 3 x 256
+  int32_t* bins;       // 4 x 256 accumulators, zero between passes
+  int32_t* bins_last;  // 4 x 256, the last pass's sums (the SOLID_BINS field)
   float4* fts;         // 256 x (fx, fy, torque, 0)
   // sources (uniform-sized), forces, obstacles
   uint32_t nsrc_init, nsrc;

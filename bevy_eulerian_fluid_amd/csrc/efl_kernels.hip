@@ -1351,10 +1351,6 @@ __global__ __launch_bounds__(NT) void k_ls_gradient(Batch b) {
 // three), plus a fourth, the torque about the centre of mass in the world
 // frame (-rx * fy - ry * fx with f in grid axes), which has no counterpart and
 // feeds the rigid-body integrator.
-__global__ void k_clear_bins(Batch b) {
-  const DevInst* I = b.inst[blockIdx.x];
-  for (int i = threadIdx.x; i < 4 * EFL_MAX_SOLIDS; i += blockDim.x) I->bins[i] = 0;
-}
 __device__ __forceinline__ int fixed_point(float v) { return to_i32(v * (1.0f / 0.1f)); }
 // Centre of body k about which its torques are taken: the triangle's
 // centroid, a circle's, rectangle's or capsule's translation, else the origin.
@@ -1515,12 +1511,20 @@ __global__ __launch_bounds__(NT) void k_sample_forces(Batch b) {
     if (v != 0) atomicAdd(&I->bins[j], v);
   }
 }
+// The sums become forces and torques (fts) and the last pass's bins, and the
+// accumulators return to zero for the next pass (instances start zeroed).
 __global__ void k_accumulate_forces(Batch b) {
   const DevInst* I = b.inst[blockIdx.x];
-  for (int i = threadIdx.x; i < EFL_MAX_SOLIDS; i += blockDim.x)
-    I->fts[i] = make_float4((float)I->bins[i] * 0.1f, (float)I->bins[EFL_MAX_SOLIDS + i] * 0.1f,
-                            (float)I->bins[2 * EFL_MAX_SOLIDS + i] * 0.1f,
-                            (float)I->bins[3 * EFL_MAX_SOLIDS + i] * 0.1f);
+  for (int i = threadIdx.x; i < EFL_MAX_SOLIDS; i += blockDim.x) {
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = I->bins[k * EFL_MAX_SOLIDS + i];
+      I->bins_last[k * EFL_MAX_SOLIDS + i] = v[k];
+      I->bins[k * EFL_MAX_SOLIDS + i] = 0;
+    }
+    I->fts[i] = make_float4((float)v[0] * 0.1f, (float)v[1] * 0.1f, (float)v[2] * 0.1f, (float)v[3] * 0.1f);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2192,7 +2196,6 @@ int fim_tile_w() { return FT_W; }
 int fim_tile_h() { return FT_H; }
 int fim_iterations_per_launch() { return FK; }
 void launch_ls_gradient(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_ls_gradient, b, st); }
-void launch_clear_bins(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_clear_bins, b, st); }
 void launch_sample_forces(hipStream_t st, const Batch& b) { EFL_LAUNCH(k_sample_forces, b, st); }
 void launch_accumulate_forces(hipStream_t st, const Batch& b) { EFL_LAUNCH_PER_INST(k_accumulate_forces, b, st); }
 void launch_residual_norm(hipStream_t st, const Batch& b) {

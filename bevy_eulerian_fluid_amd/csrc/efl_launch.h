@@ -49,7 +49,6 @@ int fim_tile_w();
 int fim_tile_h();
 int fim_iterations_per_launch();
 void launch_ls_gradient(hipStream_t st, const Batch& b);
-void launch_clear_bins(hipStream_t st, const Batch& b);
 void launch_sample_forces(hipStream_t st, const Batch& b);
 void launch_accumulate_forces(hipStream_t st, const Batch& b);
 void launch_residual_norm(hipStream_t st, const Batch& b);
