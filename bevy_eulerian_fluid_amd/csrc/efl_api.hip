@@ -416,17 +416,41 @@ void mg_flip(const IV& v, int lvl) {
 }
 
 // Stencils of level 0 (all: of every level, with the level sets and area
-// fractions of the coarse levels restricted on the way), one launch per
-// level; the inputs are fixed for the rest of the projection.
+// fractions of the coarse levels restricted on the way), one tiled launch per
+// level down to the first level above level 0 of at most
+// mg_setup_coarse_cells() cells, then one launch for every instance's
+// remaining levels; the inputs are fixed for the rest of the projection.
+int first_coarse_setup_level(const efl_instance* i) {
+  const int nlev = i->h.nlev;
+  auto cells = [&](int l) { return (int64_t)i->h.lev[l].w * i->h.lev[l].h; };
+  for (int l0 = 1; l0 < nlev; ++l0) {
+    if (cells(l0) > efl::mg_setup_coarse_cells()) continue;
+    for (int l = l0 + 1; l < nlev; ++l)
+      if (cells(l) > efl::mg_setup_coarse_lds_cells(l - l0)) return nlev;  // a level would not fit in LDS
+    return l0;
+  }
+  return nlev;
+}
 void mg_setup(hipStream_t st, const IV& v, bool all) {
   int top = 0;
-  for (auto* i : v) top = std::max(top, all ? i->h.nlev : 1);
+  for (auto* i : v) top = std::max(top, all ? first_coarse_setup_level(i) : 1);
   for (int l = 0; l < top; ++l) {
-    IV part = filter(v, [&](efl_instance* i) { return l < (all ? i->h.nlev : 1); });
+    IV part = filter(v, [&](efl_instance* i) { return l < (all ? first_coarse_setup_level(i) : 1); });
     Batch b = make_batch(part, EX_LEVEL, l, nullptr, efl::mg_setup_tile_w(), efl::mg_setup_tile_h());
     for (int n = 0; n < b.n; ++n) b.sel[n] = (all && l + 1 < part[n]->h.nlev) ? 1 : 0;
     efl::launch_mg_setup(st, b);
   }
+  if (!all) return;
+  Batch c;
+  std::memset(&c, 0, sizeof(c));
+  for (auto* i : v) {
+    const int l0 = first_coarse_setup_level(i);
+    if (l0 >= i->h.nlev) continue;
+    c.inst[c.n] = i->d;
+    c.level[c.n] = l0;
+    ++c.n;
+  }
+  efl::launch_mg_setup_coarse(st, c);
 }
 
 // Bring level 0 back into its primary buffer (the pressure, or z for the CG

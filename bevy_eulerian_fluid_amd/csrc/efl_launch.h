@@ -86,6 +86,13 @@ int bottom_cells_enable(int device, int cells);
 int mg_setup_tile_w();
 int mg_setup_tile_h();
 void launch_mg_setup(hipStream_t st, const Batch& b);
+// levels b.level[n] .. nlev - 1 of each instance in one workgroup (levels of at
+// most mg_setup_coarse_cells() cells)
+void launch_mg_setup_coarse(hipStream_t st, const Batch& b);
+int mg_setup_coarse_cells();
+// capacity in cells of the LDS buffer of the level `offset` levels below the
+// first one of k_mg_setup_coarse
+int mg_setup_coarse_lds_cells(int offset);
 size_t bottom_lds_bytes(int cells);
 // th: tile height, 16 (64 x 16 tiles; the only one with CG reductions) or 8
 void launch_mg_down(hipStream_t st, const Batch& b, int pcg, int nsweeps, int first, float weight, int restrict_,

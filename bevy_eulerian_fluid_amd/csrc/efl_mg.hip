@@ -189,19 +189,47 @@ __device__ __forceinline__ float res_upd(float b, float4 c, int fl, float xc, fl
 // ---------------------------------------------------------------------------
 constexpr int SU_W = 64, SU_H = 4;
 
-__global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
-  int blk = blockIdx.x;
-  const int i = slot_of(b.start, b.n, blk);
-  const int lb = blk - b.start[i];
-  const DevInst* I = b.inst[i];
-  const int l = b.level[i];
-  const Level L = I->lev[l];
-  const int gx = (lb % b.tiles_x[i]) * SU_W + (int)(threadIdx.x % SU_W);
-  const int gy = (lb / b.tiles_x[i]) * SU_H + (int)(threadIdx.x / SU_W);
-  if (gx >= L.w || gy >= L.h) return;
+// Where a level's level set and area fractions are read (global memory or
+// LDS), and where the next level's are written (global memory, or global
+// memory and LDS).
+struct SetupGlobalSrc {
+  const float* ls;
+  const float4* af;
+  __device__ float l(size_t j) const { return ldg(ls, j); }
+  __device__ float4 a(size_t j) const { return ldg(af, j); }
+};
+struct SetupLdsSrc {
+  const float* ls;
+  const float4* af;
+  __device__ float l(size_t j) const { return ls[j]; }
+  __device__ float4 a(size_t j) const { return af[j]; }
+};
+struct SetupGlobalDst {
+  __device__ void put(const Level& C, size_t cid, float pr, float4 a) const {
+    stg(C.ls, cid, pr);
+    stg(C.af, cid, a);
+  }
+};
+struct SetupLdsDst {
+  float* ls;
+  float4* af;
+  __device__ void put(const Level& C, size_t cid, float pr, float4 a) const {
+    stg(C.ls, cid, pr);
+    stg(C.af, cid, a);
+    ls[cid] = pr;
+    af[cid] = a;
+  }
+};
+
+// One cell (gx, gy) of level l: its stencil and flags, and, when `coarse`
+// and the cell is a (0, 0) child, the next level's level set and area
+// fractions.
+template <class Src, class Dst>
+__device__ __forceinline__ void setup_cell(const DevInst* I, int l, const Level& L, int gx, int gy, bool coarse,
+                                           const Src& src, const Dst& dst) {
   const size_t id = (size_t)gy * L.w + gx;
-  const float lc = ldg(L.ls, id);
-  const float4 f = ldg(L.af, id);
+  const float lc = src.l(id);
+  const float4 f = src.a(id);
   const bool in[4] = {gx > 0, gx + 1 < L.w, gy > 0, gy + 1 < L.h};
   const size_t nb[4] = {id - 1, id + 1, id - L.w, id + L.w};
   const float fv[4] = {f.x, f.y, f.z, f.w};
@@ -212,7 +240,7 @@ __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
   for (int k = 0; k < 4; ++k) {
     cv[k] = 0.0f;
     if (in[k]) {
-      const float lv = ldg(L.ls, nb[k]);
+      const float lv = src.l(nb[k]);
       const bool fluid = lv < 0.0f;
       cv[k] = fluid ? fv[k] : fv[k] * (1.0f - lv / lc);
       denom += cv[k];
@@ -229,7 +257,7 @@ __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
   stg(L.fl, id, fl);
   // restriction.wgsl:10-60 of the level set and the area fractions, by the
   // thread of the coarse cell's (0, 0) child
-  if (!(b.sel[i] & 1) || (gx & 1) || (gy & 1)) return;
+  if (!coarse || (gx & 1) || (gy & 1)) return;
   const Level C = I->lev[l + 1];
   const int cx = gx >> 1, cy = gy >> 1;
   if (cx >= C.w || cy >= C.h) return;
@@ -241,18 +269,61 @@ __global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
     const int x = gx + dxs[k], y = gy + dys[k];
     const bool inl = mg_inb(x, y, L.w, L.h);
     const size_t j = (size_t)y * L.w + x;
-    l4[k] = inl ? ldg(L.ls, j) : 0.0f;
-    a4[k] = inl ? ldg(L.af, j) : make_float4(0.f, 0.f, 0.f, 0.f);
+    l4[k] = inl ? src.l(j) : 0.0f;
+    a4[k] = inl ? src.a(j) : make_float4(0.f, 0.f, 0.f, 0.f);
   }
   float pr = l4[0];
   if (fabsf(l4[1]) < fabsf(pr)) pr = l4[1];
   if (fabsf(l4[2]) < fabsf(pr)) pr = l4[2];
   if (fabsf(l4[3]) < fabsf(pr)) pr = l4[3];
   const size_t cid = (size_t)cy * C.w + cx;
-  stg(C.ls, cid, pr);
-  stg(C.af, cid,
-      make_float4(0.5f * (a4[0].x + a4[1].x), 0.5f * (a4[2].y + a4[3].y), 0.5f * (a4[0].z + a4[2].z),
-                  0.5f * (a4[1].w + a4[3].w)));
+  dst.put(C, cid, pr,
+          make_float4(0.5f * (a4[0].x + a4[1].x), 0.5f * (a4[2].y + a4[3].y), 0.5f * (a4[0].z + a4[2].z),
+                      0.5f * (a4[1].w + a4[3].w)));
+}
+
+__global__ __launch_bounds__(SU_W* SU_H) void k_mg_setup(Batch b) {
+  int blk = blockIdx.x;
+  const int i = slot_of(b.start, b.n, blk);
+  const int lb = blk - b.start[i];
+  const DevInst* I = b.inst[i];
+  const int l = b.level[i];
+  const Level L = I->lev[l];
+  const int gx = (lb % b.tiles_x[i]) * SU_W + (int)(threadIdx.x % SU_W);
+  const int gy = (lb / b.tiles_x[i]) * SU_H + (int)(threadIdx.x / SU_W);
+  if (gx >= L.w || gy >= L.h) return;
+  setup_cell(I, l, L, gx, gy, (b.sel[i] & 1) != 0, SetupGlobalSrc{L.ls, L.af}, SetupGlobalDst{});
+}
+
+// Levels b.level[n] .. nlev - 1 of instance n in one workgroup, level after
+// level.  The first level reads its inputs from global memory; each level's
+// restriction writes the next level's to global memory and to LDS, where the
+// next level reads them after the barrier (levels at odd offsets from the
+// first in the SC_B buffers, at even offsets in the SC_A ones, which the host
+// checks they fit).  For the coarse levels, where a tiled launch per level
+// costs more than the work.
+constexpr int SC_NT = 1024, SC_B = 2048, SC_A = 512;
+__global__ __launch_bounds__(SC_NT) void k_mg_setup_coarse(Batch b) {
+  __shared__ float lsB[SC_B], lsA[SC_A];
+  __shared__ float4 afB[SC_B], afA[SC_A];
+  const DevInst* I = b.inst[blockIdx.x];
+  const int nlev = I->nlev, l0 = b.level[blockIdx.x];
+  {
+    const Level L = I->lev[l0];
+    const int n = L.w * L.h;
+    for (int c = threadIdx.x; c < n; c += SC_NT)
+      setup_cell(I, l0, L, c % L.w, c / L.w, l0 + 1 < nlev, SetupGlobalSrc{L.ls, L.af}, SetupLdsDst{lsB, afB});
+    __syncthreads();
+  }
+  for (int l = l0 + 1; l < nlev; ++l) {
+    const Level L = I->lev[l];
+    const int n = L.w * L.h;
+    const bool odd = ((l - l0) & 1) != 0;
+    const SetupLdsSrc src = odd ? SetupLdsSrc{lsB, afB} : SetupLdsSrc{lsA, afA};
+    const SetupLdsDst dst = odd ? SetupLdsDst{lsA, afA} : SetupLdsDst{lsB, afB};
+    for (int c = threadIdx.x; c < n; c += SC_NT) setup_cell(I, l, L, c % L.w, c / L.w, l + 1 < nlev, src, dst);
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1593,6 +1664,11 @@ __global__ __launch_bounds__(MNT) void k_mg_coarse(CoarseBatch b, int pcg, int s
 // ---------------------------------------------------------------------------
 #define MG_LAUNCH(kern, nb, st, ...) hipLaunchKernelGGL(kern, dim3(nb), dim3(MNT), 0, st, __VA_ARGS__)
 
+void launch_mg_setup_coarse(hipStream_t st, const Batch& b) {
+  if (b.n > 0) hipLaunchKernelGGL(k_mg_setup_coarse, dim3(b.n), dim3(SC_NT), 0, st, b);
+}
+int mg_setup_coarse_cells() { return 64 * 64; }
+int mg_setup_coarse_lds_cells(int offset) { return (offset & 1) ? SC_B : SC_A; }
 void launch_mg_setup(hipStream_t st, const Batch& b) {
   int nb = b.start[b.n];
   if (nb <= 0) return;
